@@ -1,0 +1,124 @@
+/*
+ * swd_oracle.h -- CPU oracle for the 2D acoustic staggered-grid CPML FDTD / RTM path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates, as plain C host loops, the
+ * per-cell equations of the reference program qingchenzhang/Simple-wavenumber-
+ * deconvolution (files acoustic_HLVSPRTMC.cpp = "RC", acoustic_HLVSPRTMLc.cu = "RK",
+ * forward/acoustic_hilbert_fftkz_HLForC.cpp = "FC",
+ * forward/acoustic_hilbert_fftkz_HLForLc.cu = "FK").  It is the checker the HIP
+ * path is compared against; it is never linked into, called by, or shipped with
+ * the product library.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.
+ *
+ * Two twins of every propagation routine exist: *_f32 does all arithmetic in
+ * float (the parity reference), *_f64 does the identical sequence in double (the
+ * accuracy oracle).  Both take the same float inputs.  The arithmetic contract
+ * shared with the HIP kernels ("canonical order") is:
+ *   derivative   d = 0; for ic<Lc: d = fma(rc[ic]*(f[+]-f[-]), 1/dx, d)
+ *   memory var   psi = fma(b, psi, a*d)
+ *   velocity     v = fma(B, d + psi, v),           B = dt * (float)(1/(0.5(rho+rho')))
+ *   pressure     p = fma(K, ((dvx+psi_x)+dvz)+psi_z, p),  K = ((dt*rho)*vp)*vp
+ * which is the reference expression (RK:55-64, 104-113, 162-171) with the
+ * multiply-add contracted the way nvcc contracts it by default.
+ *
+ * Grid convention (RC:78-83, RK:45): pmlc = pml+Lc, ntx = nx+2*pmlc,
+ * ntz = nz+2*pmlc, cell (ix,iz) lives at ip = ix*ntz+iz (z contiguous).
+ */
+#ifndef SWD_ORACLE_H
+#define SWD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_grid {
+    int nx, nz;      /* interior cells */
+    int pml, Lc;     /* CPML cells (parameter.txt "pml"), half stencil order */
+    float dx, dz, dt;
+    int nt;          /* itmax */
+} or_grid;
+
+typedef struct or_coefs {
+    const float *rc;                        /* [Lc]  */
+    const float *a_x, *a_x_half, *b_x, *b_x_half;  /* [ntx] */
+    const float *a_z, *a_z_half, *b_z, *b_z_half;  /* [ntz] */
+} or_coefs;
+
+typedef struct or_shot {
+    int s_ix, s_iz;          /* padded-grid source node */
+    int nrec;                /* traces */
+    const int *rec_ix;       /* [nrec] padded-grid receiver column */
+    const int *rec_iz;       /* [nrec] padded-grid receiver row */
+} or_shot;
+
+/* ---- setup restatements (mixed precision exactly as the reference) ---- */
+void or_cal_xishu(int Lc, float *rc);                                   /* RC:1858-1878 */
+void or_ricker(float *rick, int nt, float f0, float t0, float dt);      /* RC:1284-1299 */
+void or_cpml(int ntx, int ntz, int pml_arg, float dx, float dz, float dt,
+             float *a_x, float *a_x_half, float *b_x, float *b_x_half,
+             float *a_z, float *a_z_half, float *b_z, float *b_z_half); /* RC:891-1129 */
+void or_model_in_pml(float *vp, int ntz, int ntx, int pml);             /* RC:1428-1468 */
+void or_smooth(float *vp, int ntz, int ntx, int pml, int window);       /* RC:1473-1569 */
+int  or_nfft(int n);                                                    /* FK:595-596 */
+int  or_kz_taper(float *coef, int nfftz, float dz);                     /* FC:354-377 */
+void or_hilbert1d(const float *rick, float *hrick, int nt);             /* FK:1257-1324 */
+int  or_geometry(int nx, int nz, int pml, int Lc, float dx, float dz,
+                 float sx0, float shotdx, float shotdep, float offsetz, int is,
+                 int *s_ix, int *s_iz, int *r_iz, int max_rn);          /* RC:150-200 */
+void or_adjoint_source(const float *d, float *rms, int ntr, int nt, float dt); /* RC:603-637 */
+void or_scatterers(float *rho, const float *vp, int ntx, int ntz, int pmlc, int swin); /* FC:697-705 */
+void or_image_normalize(float *image, const float *illum, int ntp);     /* RC:663-679 */
+void or_image_laplacian(const float *image, const float *vp, float *out, float *acc,
+                        int ntx, int ntz, int pmlc, float dx, float dz); /* RC:722-735 */
+void or_edge_zero(float *mig, int nx, int nz, int Lc);                  /* RC:754-768 */
+
+/* ---- propagation, FP32 / FP64 twins ---- */
+/* Forward CPML modelling of one shot (RK:239-432 with RK:24-221).  gather is
+ * [nrec][nt]; strips (optional) receives the Lc-wide interior frame of p after
+ * every step (RK:200-219), layout [nt][nframe] with or_frame_size(); final
+ * (optional) receives vx, vz, p at it = nt-1, [3][ntp] (RK:403-415). */
+int or_forward_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                   const or_shot *s, const float *wavelet, float *gather, float *strips, float *final_state);
+int or_forward_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                   const or_shot *s, const float *wavelet, double *gather, double *strips, double *final_state);
+
+/* RTM backward (RK:946-1215): source reconstruction from final state + strips,
+ * time-reversed receiver propagation with hard replacement of adj[nrec][nt],
+ * cross-correlation imaging.  image/illum_s/illum_r are [ntp]. */
+int or_rtm_backward_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                        const or_shot *s, const float *wavelet, const float *final_state,
+                        const float *strips, const float *adj,
+                        float *image, float *illum_s, float *illum_r);
+int or_rtm_backward_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                        const or_shot *s, const float *wavelet, const double *final_state,
+                        const double *strips, const float *adj,
+                        double *image, double *illum_s, double *illum_r);
+
+/* AFor forward (FK:576-863): p driven by wavelet and htp driven by hwavelet,
+ * gather of p, and per-step kz up/down separation of p + i*htp on the receiver
+ * columns (FK:705, 762-777, 52-138).  coef is the kz taper [nfftz]. */
+int or_afor_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                const or_shot *s, const float *wavelet, const float *hwavelet,
+                const float *kzcoef, int nfftz, float *gather, float *up, float *down);
+int or_afor_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                const or_shot *s, const float *wavelet, const float *hwavelet,
+                const float *kzcoef, int nfftz, double *gather, double *up, double *down);
+
+/* Born forward d = L m and its exact discrete adjoint m = L^T d (the linearised
+ * CPML scheme with K -> K(1+m) on the interior; see DESIGN notes in the source). */
+int or_born_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                const or_shot *s, const float *wavelet, const float *m, float *d);
+int or_born_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                const or_shot *s, const float *wavelet, const float *m, double *d);
+int or_born_adjoint_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                        const or_shot *s, const float *wavelet, const float *d, float *mgrad);
+int or_born_adjoint_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                        const or_shot *s, const float *wavelet, const float *d, double *mgrad);
+
+int or_frame_size(const or_grid *g);
+void or_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,266 @@
+"""ctypes bindings of the CPU oracle (oracle/libswd_oracle.so).
+
+Test infrastructure: the oracle is the checker the HIP path is compared with.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "_build", "libswd_oracle.so")
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("pml", C.c_int), ("Lc", C.c_int),
+                ("dx", C.c_float), ("dz", C.c_float), ("dt", C.c_float), ("nt", C.c_int)]
+
+
+class Coefs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("rc", "a_x", "a_x_half", "b_x", "b_x_half",
+                                          "a_z", "a_z_half", "b_z", "b_z_half")]
+
+
+class Shot(C.Structure):
+    _fields_ = [("s_ix", C.c_int), ("s_iz", C.c_int), ("nrec", C.c_int),
+                ("rec_ix", C.c_void_p), ("rec_iz", C.c_void_p)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.or_cal_xishu.argtypes = [C.c_int, f32p]
+        L.or_ricker.argtypes = [f32p, C.c_int, C.c_float, C.c_float, C.c_float]
+        L.or_cpml.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float] + [f32p] * 8
+        L.or_model_in_pml.argtypes = [f32p, C.c_int, C.c_int, C.c_int]
+        L.or_smooth.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_nfft.argtypes = [C.c_int]
+        L.or_nfft.restype = C.c_int
+        L.or_kz_taper.argtypes = [f32p, C.c_int, C.c_float]
+        L.or_kz_taper.restype = C.c_int
+        L.or_hilbert1d.argtypes = [f32p, f32p, C.c_int]
+        L.or_geometry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_int]
+        L.or_geometry.restype = C.c_int
+        L.or_adjoint_source.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_float]
+        L.or_scatterers.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_image_normalize.argtypes = [f32p, f32p, C.c_int]
+        L.or_image_laplacian.argtypes = [f32p, f32p, f32p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                         C.c_float, C.c_float]
+        L.or_edge_zero.argtypes = [f32p, C.c_int, C.c_int, C.c_int]
+        L.or_frame_size.argtypes = [C.POINTER(Grid)]
+        L.or_frame_size.restype = C.c_int
+        L.or_set_threads.argtypes = [C.c_int]
+        for name in ("or_forward_f32", "or_forward_f64", "or_rtm_backward_f32", "or_rtm_backward_f64",
+                     "or_afor_f32", "or_afor_f64", "or_born_f32", "or_born_f64",
+                     "or_born_adjoint_f32", "or_born_adjoint_f64"):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Problem:
+    """A padded-grid problem: grid, model, coefficients and one shot."""
+
+    def __init__(self, nx, nz, pml, Lc, dx, dz, dt, nt, vp, rho, s_ix, s_iz, rec_ix, rec_iz,
+                 wavelet=None, f0=None, coefs=None):
+        self.g = Grid(nx, nz, pml, Lc, dx, dz, dt, nt)
+        self.pmlc = pml + Lc
+        self.ntx = nx + 2 * self.pmlc
+        self.ntz = nz + 2 * self.pmlc
+        self.ntp = self.ntx * self.ntz
+        self.vp = np.ascontiguousarray(vp, np.float32).reshape(-1)
+        self.rho = np.ascontiguousarray(rho, np.float32).reshape(-1)
+        assert self.vp.size == self.ntp and self.rho.size == self.ntp
+        if coefs is None:
+            coefs = setup_coefs(self.ntx, self.ntz, self.pmlc, Lc, dx, dz, dt)
+        self.coefs = coefs
+        self.c = Coefs(*[ptr(coefs[k]) for k in ("rc", "a_x", "a_x_half", "b_x", "b_x_half",
+                                                 "a_z", "a_z_half", "b_z", "b_z_half")])
+        self.rec_ix = np.ascontiguousarray(rec_ix, np.int32)
+        self.rec_iz = np.ascontiguousarray(rec_iz, np.int32)
+        self.nrec = self.rec_ix.size
+        self.shot = Shot(int(s_ix), int(s_iz), self.nrec, ptr(self.rec_ix), ptr(self.rec_iz))
+        if wavelet is None:
+            wavelet = ricker(nt, f0, 1.0 / f0, dt)
+        self.wavelet = np.ascontiguousarray(wavelet, np.float32)
+        self.nt = nt
+
+    def frame_size(self):
+        return lib().or_frame_size(C.byref(self.g))
+
+    def _args(self):
+        return (C.byref(self.g), ptr(self.vp), ptr(self.rho), C.byref(self.c), C.byref(self.shot))
+
+    def forward(self, dtype=np.float32, strips=False, final=False):
+        L = lib()
+        gather = np.zeros((self.nrec, self.nt), dtype)
+        st = np.zeros((self.nt, self.frame_size()), dtype) if strips else None
+        fs = np.zeros((3, self.ntp), dtype) if final else None
+        fn = L.or_forward_f32 if dtype == np.float32 else L.or_forward_f64
+        rc = fn(*self._args(), ptr(self.wavelet), ptr(gather), ptr(st), ptr(fs))
+        assert rc == 0, rc
+        return gather, st, fs
+
+    def rtm_backward(self, final_state, strips, adj, dtype=np.float32):
+        L = lib()
+        adj = np.ascontiguousarray(adj, np.float32)
+        out = [np.zeros(self.ntp, dtype) for _ in range(3)]
+        fn = L.or_rtm_backward_f32 if dtype == np.float32 else L.or_rtm_backward_f64
+        fs = np.ascontiguousarray(final_state, dtype)
+        st = np.ascontiguousarray(strips, dtype)
+        rc = fn(*self._args(), ptr(self.wavelet), ptr(fs), ptr(st), ptr(adj), *[ptr(o) for o in out])
+        assert rc == 0, rc
+        return out
+
+    def afor(self, hwavelet, kzcoef, dtype=np.float32):
+        L = lib()
+        nfftz = kzcoef.size
+        outs = [np.zeros((self.nrec, self.nt), dtype) for _ in range(3)]
+        fn = L.or_afor_f32 if dtype == np.float32 else L.or_afor_f64
+        rc = fn(*self._args(), ptr(self.wavelet), ptr(np.ascontiguousarray(hwavelet, np.float32)),
+                ptr(np.ascontiguousarray(kzcoef, np.float32)), nfftz, *[ptr(o) for o in outs])
+        assert rc == 0, rc
+        return outs
+
+    def born(self, m, dtype=np.float32):
+        L = lib()
+        m = np.ascontiguousarray(m, np.float32).reshape(-1)
+        d = np.zeros((self.nrec, self.nt), dtype)
+        fn = L.or_born_f32 if dtype == np.float32 else L.or_born_f64
+        rc = fn(*self._args(), ptr(self.wavelet), ptr(m), ptr(d))
+        assert rc == 0, rc
+        return d
+
+    def born_adjoint(self, d, dtype=np.float32):
+        L = lib()
+        d = np.ascontiguousarray(d, np.float32)
+        g = np.zeros(self.g.nx * self.g.nz, dtype)
+        fn = L.or_born_adjoint_f32 if dtype == np.float32 else L.or_born_adjoint_f64
+        rc = fn(*self._args(), ptr(self.wavelet), ptr(d), ptr(g))
+        assert rc == 0, rc
+        return g
+
+
+def cal_xishu(Lc):
+    rc = np.zeros(Lc, np.float32)
+    lib().or_cal_xishu(Lc, rc)
+    return rc
+
+
+def ricker(nt, f0, t0, dt):
+    r = np.zeros(nt, np.float32)
+    lib().or_ricker(r, nt, f0, t0, dt)
+    return r
+
+
+def cpml(ntx, ntz, pml_arg, dx, dz, dt):
+    arrs = [np.zeros(ntx, np.float32) for _ in range(4)] + [np.zeros(ntz, np.float32) for _ in range(4)]
+    lib().or_cpml(ntx, ntz, pml_arg, dx, dz, dt, *arrs)
+    keys = ("a_x", "a_x_half", "b_x", "b_x_half", "a_z", "a_z_half", "b_z", "b_z_half")
+    return dict(zip(keys, arrs))
+
+
+def setup_coefs(ntx, ntz, pmlc, Lc, dx, dz, dt):
+    c = cpml(ntx, ntz, pmlc, dx, dz, dt)
+    c["rc"] = cal_xishu(Lc)
+    return c
+
+
+def smooth(vp_padded, ntz, ntx, pmlc, window):
+    v = np.ascontiguousarray(vp_padded, np.float32).reshape(-1).copy()
+    lib().or_smooth(v, ntz, ntx, pmlc, window)
+    return v
+
+
+def model_in_pml(vp_padded, ntz, ntx, pmlc):
+    v = np.ascontiguousarray(vp_padded, np.float32).reshape(-1).copy()
+    lib().or_model_in_pml(v, ntz, ntx, pmlc)
+    return v
+
+
+def nfft(n):
+    return lib().or_nfft(n)
+
+
+def kz_taper(nfftz, dz):
+    c = np.zeros(nfftz, np.float32)
+    layerz = lib().or_kz_taper(c, nfftz, dz)
+    return c, layerz
+
+
+def hilbert1d(rick):
+    rick = np.ascontiguousarray(rick, np.float32)
+    h = np.zeros_like(rick)
+    lib().or_hilbert1d(rick, h, rick.size)
+    return h
+
+
+def geometry(nx, nz, pml, Lc, dx, dz, sx0, shotdx, shotdep, offsetz, is_, max_rn=1 << 20):
+    s_ix, s_iz = C.c_int(), C.c_int()
+    rn = lib().or_geometry(nx, nz, pml, Lc, dx, dz, sx0, shotdx, shotdep, offsetz, is_,
+                           C.byref(s_ix), C.byref(s_iz), None, 0)
+    r_iz = np.zeros(max(rn, 1), np.int32)
+    lib().or_geometry(nx, nz, pml, Lc, dx, dz, sx0, shotdx, shotdep, offsetz, is_,
+                      C.byref(s_ix), C.byref(s_iz), ptr(r_iz), rn)
+    return s_ix.value, s_iz.value, r_iz[:rn]
+
+
+def adjoint_source(d, dt):
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.zeros_like(d)
+    lib().or_adjoint_source(d, out, d.shape[0], d.shape[1], dt)
+    return out
+
+
+def scatterers(rho, vp, ntx, ntz, pmlc, swin):
+    r = np.ascontiguousarray(rho, np.float32).reshape(-1).copy()
+    lib().or_scatterers(r, np.ascontiguousarray(vp, np.float32).reshape(-1), ntx, ntz, pmlc, swin)
+    return r
+
+
+def image_normalize(image, illum):
+    im = np.ascontiguousarray(image, np.float32).reshape(-1).copy()
+    lib().or_image_normalize(im, np.ascontiguousarray(illum, np.float32).reshape(-1), im.size)
+    return im
+
+
+def image_laplacian(image, vp, ntx, ntz, pmlc, dx, dz, acc=None):
+    nx, nz = ntx - 2 * pmlc, ntz - 2 * pmlc
+    out = np.zeros(nx * nz, np.float32)
+    lib().or_image_laplacian(np.ascontiguousarray(image, np.float32).reshape(-1),
+                             np.ascontiguousarray(vp, np.float32).reshape(-1), out,
+                             ptr(acc), ntx, ntz, pmlc, dx, dz)
+    return out
+
+
+def edge_zero(mig, nx, nz, Lc):
+    m = np.ascontiguousarray(mig, np.float32).reshape(-1).copy()
+    lib().or_edge_zero(m, nx, nz, Lc)
+    return m
+
+
+def set_threads(n):
+    lib().or_set_threads(n)
