@@ -1,0 +1,171 @@
+/*
+ * swd.h -- C-ABI of the MI355X (gfx950) 2D acoustic staggered-grid CPML FDTD /
+ * RTM library (libswd.so).
+ *
+ * This is the drop-in boundary for the reference's extern "C" seam between its
+ * host drivers and its device code.  Each entry point names the reference
+ * function it replaces (RC = acoustic_HLVSPRTMC.cpp, RK = acoustic_HLVSPRTMLc.cu,
+ * FC = forward/acoustic_hilbert_fftkz_HLForC.cpp,
+ * FK = forward/acoustic_hilbert_fftkz_HLForLc.cu, H = headrtm.h).
+ *
+ * Conventions
+ *  - Every call returns an int status: 0 = ok, < 0 = error (see SWD_E_*); the
+ *    message of the last error of the calling thread is in swd_last_error().
+ *  - Plain C types only.  Host buffers are caller-owned and touched only during
+ *    the call.  *_dev entry points take device pointers (HIP device memory on the
+ *    plan's device) and run on the plan's stream; they return when the results
+ *    are complete.
+ *  - Grid layout (RC:78-83, RK:45): pmlc = pml+Lc, ntx = nx+2*pmlc,
+ *    ntz = nz+2*pmlc; padded arrays are [ntx][ntz] floats (z contiguous, cell
+ *    (ix,iz) at ix*ntz+iz); images are padded [ntx][ntz] too.  Interior .bin
+ *    files are [nx][nz].  Gathers are trace-major [ntrace][nt].
+ *  - Source/receiver coordinates are padded-grid indices, as Source.s_ix/s_iz
+ *    and r_iz in the reference (H:9-13).
+ *  - A plan is bound to one device and one stream; it is not thread-safe, but
+ *    independent plans may be used from independent threads.
+ */
+#ifndef SWD_H
+#define SWD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWD_OK 0
+#define SWD_E_ARG (-1)      /* invalid argument / shape */
+#define SWD_E_DEVICE (-2)   /* no device or HIP runtime error */
+#define SWD_E_NOMEM (-3)    /* allocation failed */
+#define SWD_E_IO (-4)       /* file missing, short or unreadable */
+#define SWD_E_STATE (-5)    /* call order (e.g. model not set) */
+#define SWD_E_RANGE (-6)    /* configuration outside reference limits (layerz, CFL, ...) */
+
+#define SWD_MAX_LC 8
+
+typedef struct swd_grid {
+    int nx, nz;        /* interior cells */
+    int pml, Lc;       /* CPML cells as in parameter.txt, half stencil order (1..8) */
+    float dx, dz, dt;
+    int nt;            /* itmax */
+} swd_grid;
+
+/* FD and CPML coefficient vectors (host arrays), as produced by swd_cal_xishu
+ * and swd_cpml: rc[Lc], *_x[ntx], *_z[ntz]. */
+typedef struct swd_coefs {
+    const float *rc;
+    const float *a_x, *a_x_half, *b_x, *b_x_half;
+    const float *a_z, *a_z_half, *b_z, *b_z_half;
+} swd_coefs;
+
+/* One shot: source node and receiver list (padded-grid indices), replacing
+ * struct Source (H:9-13).  The reference's two-borehole geometry is the list
+ * {(pmlc, r_iz[i])} followed by {(ntx-pmlc-1, r_iz[i])}. */
+typedef struct swd_shot {
+    int s_ix, s_iz;
+    int nrec;
+    const int *rec_ix;
+    const int *rec_iz;
+} swd_shot;
+
+/* parameter.txt contents, in file order (RC:1931-2035). */
+typedef struct swd_params {
+    int nx, nz, pml, Lc;
+    float dx, dz;
+    int itmax;
+    float dt, f0;
+    int ns;
+    float sx0, shotdx, shotdep, recdix, offsetz;
+    int scatter_flag, swin, window;
+} swd_params;
+
+/* Execution path of a plan. */
+#define SWD_PATH_PLAIN 0   /* one kernel per sub-step (vx, vz, p), one shot per launch */
+#define SWD_PATH_FUSED 1   /* fused batched kernels (default) */
+
+typedef struct swd_stats {
+    double fwd_ms;          /* event time of the last forward time loop (all shots of the call) */
+    double bwd_ms;          /* event time of the last backward time loop */
+    double fft_ms;          /* event time of the kz separation kernels */
+    long long fwd_launches; /* kernel launches inside the last forward loop */
+    long long bwd_launches;
+    long long fwd_cells;    /* cell-updates of the last forward loop (per wavefield, padded grid) */
+    long long bwd_cells;
+} swd_stats;
+
+/* ---------------- errors and devices ---------------- */
+int swd_last_error(char *buf, size_t n);
+int swd_get_device_count(int *n);                              /* getdevice (RK:1839) */
+
+/* ---------------- host setup, reference semantics ---------------- */
+int swd_read_parameters(const char *path, swd_params *out);   /* input_parameters (RC:1931-2035) */
+int swd_cal_xishu(int Lc, float *rc);                          /* cal_xishu (RC:1858-1878) */
+int swd_ricker(float *rick, int nt, float f0, float t0, float dt); /* ricker_wave flag 2 (RC:1284-1299) */
+int swd_cpml(int ntx, int ntz, int pml_arg, float dx, float dz, float dt,
+             float *a_x, float *a_x_half, float *b_x, float *b_x_half,
+             float *a_z, float *a_z_half, float *b_z, float *b_z_half); /* get_absorbing_parameters (RC:891-1129) */
+int swd_model_in_pml(float *vp, int ntz, int ntx, int pml);    /* model_in_pml (RC:1428-1468) */
+int swd_read_model(const char *path, float *vp, float *rho, int ntx, int ntz, int pml); /* get_acc_model (RC:1136-1204) */
+int swd_smooth_model(float *vp, int ntz, int ntx, int pml, int window); /* ini_model_mine (RC:1473-1569) */
+int swd_nfft(int n);                                           /* 2^ceil(log2 n) (FK:595-596) */
+int swd_kz_taper(float *coef, int nfftz, float dz);            /* kz taper (FC:354-377); returns layerz */
+int swd_hilbert1d(const float *rick, float *hrick, int nt);    /* hilbert_transform1d (FK:1257-1324) */
+int swd_geometry(int nx, int nz, int pml, int Lc, float dx, float dz, float sx0, float shotdx,
+                 float shotdep, float offsetz, int is, int *s_ix, int *s_iz, int *r_iz, int max_rn);
+                                                               /* shot/receiver geometry (RC:150-200); returns r_n */
+int swd_adjoint_source(const float *d, float *rms, int ntr, int nt, float dt); /* RC:603-637 */
+int swd_scatterers(float *rho, const float *vp, int ntx, int ntz, int pmlc, int swin); /* FC:697-705 */
+int swd_image_normalize(float *image, const float *illum, int ntp);              /* RC:663-679 */
+int swd_image_laplacian(const float *image, const float *vp, float *out, float *acc,
+                        int ntx, int ntz, int pmlc, float dx, float dz);          /* RC:722-735 */
+int swd_edge_zero(float *mig, int nx, int nz, int Lc);                           /* RC:754-768 */
+
+/* ---------------- plans ---------------- */
+int swd_plan_create(const swd_grid *g, int device, void **plan); /* variables_malloc (RK:1613, FK:1425) */
+int swd_plan_destroy(void *plan);                                /* variables_free (RK:1732, FK:1565) */
+int swd_plan_set_coefs(void *plan, const swd_coefs *c);          /* H2D of rc and CPML vectors (RK:278-297) */
+int swd_plan_set_model(void *plan, const float *vp, const float *rho); /* H2D of vp, rho (RK:281-282) */
+int swd_plan_set_path(void *plan, int path);
+int swd_plan_get_stats(void *plan, swd_stats *st);
+
+/* ---------------- operators ----------------
+ * nshots shots of one plan run as one batch; wavelets are shared [nt];
+ * gathers of the batch are concatenated in shot order ([sum nrec][nt]). */
+
+/* Forward modelling, pressure gather (fdtd_2d_GPU_forward with inv_flag=0, RK:239-432). */
+int swd_forward(void *plan, int nshots, const swd_shot *shots, const float *wavelet, float *gather);
+
+/* AFor forward: p (wavelet) and htp (hwavelet) wavefields, gather of p and kz
+ * up/down separated gathers (FK:576-863 with FK:52-138, cuFFT FK:762-769).
+ * kzcoef is the taper [nfftz], nfftz a power of two >= ntz. */
+int swd_afor(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+             const float *hwavelet, const float *kzcoef, int nfftz,
+             float *gather, float *up, float *down);
+
+/* RTM / PSF imaging of a batch: forward with boundary strips (inv_flag=1,
+ * RK:239-432), then source reconstruction, receiver back-propagation of
+ * adj[sum nrec][nt] and cross-correlation imaging (fdtd_2d_GPU_backward,
+ * RK:946-1215).  image/illum_src/illum_rcv are [nshots][ntx*ntz];
+ * gather_syn (optional) receives the forward gathers. */
+int swd_rtm(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+            const float *adj, float *image, float *illum_src, float *illum_rcv, float *gather_syn);
+
+/* Same as swd_rtm with device pointers (adj, image, illum_*, gather_syn on the
+ * plan's device; gather_syn may be NULL). */
+int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+                const float *adj, float *image, float *illum_src, float *illum_rcv, float *gather_syn);
+
+/* Born modelling d = L m, m = dK/K on the interior [nx][nz] (no reference
+ * counterpart: linearisation of the RK:24-221 scheme), and its exact discrete
+ * adjoint m_grad = L^T d with sqrt(nt) checkpointing (nsnap <= 0: automatic).
+ * d is [sum nrec][nt]; m and m_grad are [nx][nz] per shot, m_grad summed over
+ * the batch. */
+int swd_born(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+             const float *m, float *d);
+int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+                     const float *d, float *m_grad, int nsnap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

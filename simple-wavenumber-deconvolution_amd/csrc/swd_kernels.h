@@ -1,0 +1,47 @@
+// swd_kernels.h -- kernel-side data structures shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "swd.h"
+
+// Grid constants of a plan, passed by value to every kernel.
+struct KGrid {
+    int ntx, ntz, ldz;      // padded grid; ldz = column stride in floats (>= ntz, multiple of 32)
+    int Lc, pmlc, nx, nz, nt, nframe;
+    float idx, idz;         // float(1/dx), float(1/dz) as RK:47
+    float rc[SWD_MAX_LC];
+    const float *ax, *axh, *bx, *bxh;   // [ntx]
+    const float *az, *azh, *bz, *bzh;   // [ntz]
+    const float *Bx, *Bz, *K, *K0;      // [ntx][ldz]: dt*buoyancy (x, z half points), dt*rho*vp^2, rho*vp^2
+};
+
+// One propagated wavefield ("instance") of a batch and what the kernels do with it.
+struct KInst {
+    float *p, *vx, *vz;          // state [ntx][ldz]
+    float *psx, *psz;            // CPML memory of dp/dx, dp/dz (on vx, vz)
+    float *pvx, *pvz;            // CPML memory of dvx/dx, dvz/dz (on p)
+    const float *wavelet;        // additive source [nt] (forward) or subtracted (reconstruction); may be null
+    int s_ix, s_iz;
+    const int *colmap;           // [ntx]: offset of the column's row map in rowmap, or -1
+    const int *rowmap;           // per receiver column [ntz]: trace index or -1
+    float *gather;               // record [ntr][nt] (forward) or null
+    const float *adj;            // hard replacement [ntr][nt] (receiver back-propagation) or null
+    float *strips;               // interior frame [nt][nframe] (save in forward, restore in reconstruction)
+    float2 *capture;             // AFor column capture [nt][ncap][nfftz] or null
+    const int *colcap;           // [ntx]: capture column index or -1
+    int capcomp, ncap, nfftz;
+    const float *bvx, *bvz;      // Born: background velocities of the same shot
+    const float *born_m;         // Born: dK/K on the interior [nx][nz]
+    float *dpdt;                 // reconstruction: written; receiver side: read for imaging
+    float *image, *illum_s, *illum_r;
+    const float *s0;             // Born adjoint: background sums [slot][nx][nz]
+    float *s0_out;               // Born adjoint recompute: store background sums [slot][nx][nz]
+    int s0_t0;                   // time step of slot 0 of s0 / s0_out
+    float *grad;                 // Born adjoint: gradient accumulator [nx][nz]
+    float *ex, *ez;              // Born adjoint scratch [ntx][ldz]
+    const float *dsum;           // Born adjoint: data summed per receiver node [ntr][nt]
+};
+
+// Column descriptor of the kz separation: one receiver column of one shot.
+struct KCapCol {
+    int tstart, tcount;          // range in the trace list
+};

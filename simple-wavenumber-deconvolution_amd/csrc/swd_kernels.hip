@@ -1,0 +1,520 @@
+// swd_kernels.hip -- gfx950 kernels of the 2D acoustic staggered-grid CPML
+// FDTD / RTM path: one kernel per sub-step ("plain" kernels), multi-instance.
+//
+// Layout: every field is [ntx][ldz] floats, z contiguous (the reference's
+// ip = ix*ntz+iz, RK:45, with the column stride padded to a multiple of 32
+// floats so each column starts on a 128-byte line).  A workgroup is 64 lanes
+// along z (one wavefront reads one 256-byte column segment) times 4 columns;
+// blockIdx.z selects the instance (shot / wavefield) of a batch.
+//
+// Arithmetic follows the reference per cell (RK:24-221, 434-817) in the
+// canonical order d = fma(rc*(f+ - f-), 1/dx, d), psi = fma(b, psi, a*d),
+// v = fma(B, d+psi, v), p = fma(K, ((dvx+psi)+dvz)+psi, p).  The library is
+// built with -ffp-contract=off, so only the fmas written here are fused.
+// CPML memory variables are read and written only where the damping
+// coefficient a is non-zero: elsewhere they stay exactly zero (psi starts at 0
+// and psi' = b*0 + 0*d = 0), so skipping them leaves every result unchanged.
+#include "swd_kernels.h"
+#include "swd_launch.h"
+
+namespace {
+
+__device__ __forceinline__ int frame_index(const KGrid &g, int ix, int iz)
+{
+    const int jx = ix - g.pmlc, jz = iz - g.pmlc, Lc = g.Lc, nx = g.nx, nz = g.nz;
+    if (jx < 0 || jx >= nx || jz < 0 || jz >= nz) return -1;
+    if (jx < Lc) return jx * nz + jz;
+    if (jx >= nx - Lc) return (jx - (nx - 2 * Lc)) * nz + jz;
+    if (jz < Lc) return 2 * Lc * nz + (jx - Lc) * 2 * Lc + jz;
+    if (jz >= nz - Lc) return 2 * Lc * nz + (jx - Lc) * 2 * Lc + Lc + (jz - (nz - Lc));
+    return -1;
+}
+
+__device__ __forceinline__ bool interior(const KGrid &g, int ix, int iz)
+{
+    return ix >= g.pmlc && ix < g.ntx - g.pmlc && iz >= g.pmlc && iz < g.ntz - g.pmlc;
+}
+
+// Forward difference from the integer grid to the half grid: the derivative
+// the velocity update uses (RK:55-58, RK:104-107).
+template <int LC>
+__device__ __forceinline__ float dplus(const float *__restrict__ f, size_t ip, size_t s, const float *rc, float inv)
+{
+    float d = 0.f;
+#pragma unroll
+    for (int ic = 0; ic < LC; ic++) d = __builtin_fmaf(rc[ic] * (f[ip + (ic + 1) * s] - f[ip - ic * s]), inv, d);
+    return d;
+}
+
+// Backward difference from the half grid to the integer grid: the derivative
+// the pressure update uses (RK:162-166).
+template <int LC>
+__device__ __forceinline__ float dminus(const float *__restrict__ f, size_t ip, size_t s, const float *rc, float inv)
+{
+    float d = 0.f;
+#pragma unroll
+    for (int ic = 0; ic < LC; ic++) d = __builtin_fmaf(rc[ic] * (f[ip + ic * s] - f[ip - (ic + 1) * s]), inv, d);
+    return d;
+}
+
+#define CELL_INDEX                                            \
+    const int iz = blockIdx.x * 64 + threadIdx.x;             \
+    const int ix = blockIdx.y * 4 + threadIdx.y;              \
+    if (ix >= g.ntx || iz >= g.ntz) return;                   \
+    const KInst &I = insts[blockIdx.z];                       \
+    const size_t ip = static_cast<size_t>(ix) * g.ldz + iz;   \
+    const size_t sx = g.ldz;
+
+// Velocity update with CPML (RK:24-118).  NEG: receiver back-propagation,
+// called by the reference with (-dx, -dz, +dt) and widened domains
+// (RK:623-715, 1111-1127).  COMP: 1 = vx, 2 = vz, 3 = both.
+template <int LC, bool NEG, int COMP>
+__global__ __launch_bounds__(256) void k_v(KGrid g, const KInst *__restrict__ insts)
+{
+    CELL_INDEX
+    const float *__restrict__ p = I.p;
+    if (COMP & 1) {
+        const bool in = ix >= LC - 1 && ix <= g.ntx - LC - 1 && (NEG || (iz >= LC && iz <= g.ntz - LC));
+        if (in) {
+            const float d = dplus<LC>(p, ip, sx, g.rc, NEG ? -g.idx : g.idx);
+            const float a = g.axh[ix];
+            float ps = 0.f;
+            if (a != 0.f) {
+                ps = __builtin_fmaf(g.bxh[ix], I.psx[ip], a * d);
+                I.psx[ip] = ps;
+            }
+            I.vx[ip] = __builtin_fmaf(g.Bx[ip], d + ps, I.vx[ip]);
+        }
+    }
+    if (COMP & 2) {
+        const bool in = iz >= LC - 1 && iz <= g.ntz - LC - 1 && (NEG || (ix >= LC && ix <= g.ntx - LC));
+        if (in) {
+            const float d = dplus<LC>(p, ip, 1, g.rc, NEG ? -g.idz : g.idz);
+            const float a = g.azh[iz];
+            float ps = 0.f;
+            if (a != 0.f) {
+                ps = __builtin_fmaf(g.bzh[iz], I.psz[ip], a * d);
+                I.psz[ip] = ps;
+            }
+            I.vz[ip] = __builtin_fmaf(g.Bz[ip], d + ps, I.vz[ip]);
+        }
+    }
+}
+
+// Pressure update with CPML (RK:121-221; receiver side RK:717-782 when NEG).
+// Forward: additive source, receiver recording, boundary-frame saving, AFor
+// column capture, Born scattering source, background-sum saving.  Receiver
+// side: hard replacement by the adjoint source, then the cross-correlation
+// imaging condition with the reconstruction's dp/dt (RK:785-817).
+template <int LC, bool NEG>
+__global__ __launch_bounds__(256) void k_p(KGrid g, const KInst *__restrict__ insts, int it)
+{
+    CELL_INDEX
+    float pn = I.p[ip];
+    bool wr = false;
+    if (ix >= LC && ix <= g.ntx - LC && iz >= LC && iz <= g.ntz - LC) {
+        const float dvx = dminus<LC>(I.vx, ip, sx, g.rc, NEG ? -g.idx : g.idx);
+        const float dvz = dminus<LC>(I.vz, ip, 1, g.rc, NEG ? -g.idz : g.idz);
+        const float ax = g.ax[ix], az = g.az[iz];
+        float qx = 0.f, qz = 0.f;
+        if (ax != 0.f) {
+            qx = __builtin_fmaf(g.bx[ix], I.pvx[ip], ax * dvx);
+            I.pvx[ip] = qx;
+        }
+        if (az != 0.f) {
+            qz = __builtin_fmaf(g.bz[iz], I.pvz[ip], az * dvz);
+            I.pvz[ip] = qz;
+        }
+        float S = ((dvx + qx) + dvz) + qz;
+        if (!NEG && (I.born_m || I.s0_out) && interior(g, ix, iz)) {
+            const int im = (ix - g.pmlc) * g.nz + (iz - g.pmlc);
+            if (I.s0_out) I.s0_out[static_cast<size_t>(it - I.s0_t0) * g.nx * g.nz + im] = S;
+            if (I.born_m) {
+                // background sum of the same step; psi is zero on the interior
+                const float s0 = dminus<LC>(I.bvx, ip, sx, g.rc, g.idx) + dminus<LC>(I.bvz, ip, 1, g.rc, g.idz);
+                S = __builtin_fmaf(I.born_m[im], s0, S);
+            }
+        }
+        pn = __builtin_fmaf(g.K[ip], S, pn);
+        wr = true;
+    }
+    if (!NEG && I.wavelet && ix == I.s_ix && iz == I.s_iz) {
+        pn += I.wavelet[it];   // RK:175-178
+        wr = true;
+    }
+    if (I.colmap) {
+        const int cm = I.colmap[ix];
+        if (cm >= 0) {
+            const int tr = I.rowmap[cm + iz];
+            if (tr >= 0) {
+                const size_t o = static_cast<size_t>(tr) * g.nt + it;
+                if (NEG) {
+                    pn = I.adj[o];   // hard replacement (RK:765-780)
+                    wr = true;
+                } else if (I.gather) {
+                    I.gather[o] = pn;   // RK:182-197
+                }
+            }
+        }
+    }
+    if (wr) I.p[ip] = pn;
+    if (!NEG) {
+        if (I.strips) {
+            const int f = frame_index(g, ix, iz);
+            if (f >= 0) I.strips[static_cast<size_t>(it) * g.nframe + f] = pn;   // RK:200-219
+        }
+        if (I.capture) {
+            const int c = I.colcap[ix];
+            if (c >= 0) {
+                float *dst = reinterpret_cast<float *>(I.capture + (static_cast<size_t>(it) * I.ncap + c) * I.nfftz + iz);
+                dst[I.capcomp] = pn;   // temp[ix*nfftz+iz].x / .y (FK:324-327, 529-532)
+            }
+        }
+    } else if (I.image && interior(g, ix, iz)) {
+        const float dp = I.dpdt[ip];
+        I.image[ip] = __builtin_fmaf(dp, pn, I.image[ip]);
+        I.illum_s[ip] = __builtin_fmaf(dp, dp, I.illum_s[ip]);
+        I.illum_r[ip] = __builtin_fmaf(pn, pn, I.illum_r[ip]);
+    }
+}
+
+// Source-wavefield reconstruction, pressure (RK:434-485 called with -dt), the
+// wavelet subtraction at it+1 (RK:478-481) and the frame restore (RK:575-620).
+template <int LC>
+__global__ __launch_bounds__(256) void k_src_p(KGrid g, const KInst *__restrict__ insts, int it)
+{
+    CELL_INDEX
+    float pn = 0.f;
+    bool wr = false, have = false;
+    if (interior(g, ix, iz)) {
+        const float S = dminus<LC>(I.vx, ip, sx, g.rc, g.idx) + dminus<LC>(I.vz, ip, 1, g.rc, g.idz);
+        pn = __builtin_fmaf(-g.K[ip], S, I.p[ip]);
+        I.dpdt[ip] = g.K0[ip] * S;
+        wr = have = true;
+    }
+    if (ix == I.s_ix && iz == I.s_iz) {
+        if (!have) pn = I.p[ip];
+        pn -= I.wavelet[it + 1];
+        wr = true;
+    }
+    const int f = frame_index(g, ix, iz);
+    if (f >= 0) {
+        pn = I.strips[static_cast<size_t>(it) * g.nframe + f];
+        wr = true;
+    }
+    if (wr) I.p[ip] = pn;
+}
+
+// Source-wavefield reconstruction, velocities (RK:488-573 called with -dt).
+template <int LC, int COMP>
+__global__ __launch_bounds__(256) void k_src_v(KGrid g, const KInst *__restrict__ insts)
+{
+    CELL_INDEX
+    if (!interior(g, ix, iz)) return;
+    if (COMP & 1) I.vx[ip] = __builtin_fmaf(-g.Bx[ip], dplus<LC>(I.p, ip, sx, g.rc, g.idx), I.vx[ip]);
+    if (COMP & 2) I.vz[ip] = __builtin_fmaf(-g.Bz[ip], dplus<LC>(I.p, ip, 1, g.rc, g.idz), I.vz[ip]);
+}
+
+// ---- exact adjoint of one forward step (Born adjoint) ----
+// A_p^T, first half: ex/ez = adjoint of dvx/dvz on the p domain, psi-hat update.
+template <int LC>
+__global__ __launch_bounds__(256) void k_adj_p1(KGrid g, const KInst *__restrict__ insts)
+{
+    CELL_INDEX
+    float ex = 0.f, ez = 0.f;
+    if (ix >= LC && ix <= g.ntx - LC && iz >= LC && iz <= g.ntz - LC) {
+        const float t = g.K[ip] * I.p[ip];
+        const float ax = g.ax[ix], az = g.az[iz];
+        ex = t;
+        ez = t;
+        if (ax != 0.f) {
+            const float u = t + I.pvx[ip];
+            ex = __builtin_fmaf(ax, u, t);
+            I.pvx[ip] = g.bx[ix] * u;
+        }
+        if (az != 0.f) {
+            const float u = t + I.pvz[ip];
+            ez = __builtin_fmaf(az, u, t);
+            I.pvz[ip] = g.bz[iz] * u;
+        }
+    }
+    I.ex[ip] = ex;
+    I.ez[ip] = ez;
+}
+
+// A_p^T, second half: vhat -= D+ e (transpose of the backward difference).
+template <int LC>
+__global__ __launch_bounds__(256) void k_adj_v1(KGrid g, const KInst *__restrict__ insts)
+{
+    CELL_INDEX
+    float gx = 0.f, gz = 0.f;
+#pragma unroll
+    for (int ic = 0; ic < LC; ic++) {
+        const float e1 = ix + ic + 1 < g.ntx ? I.ex[ip + (ic + 1) * sx] : 0.f;
+        const float e0 = ix - ic >= 0 ? I.ex[ip - ic * sx] : 0.f;
+        const float f1 = iz + ic + 1 < g.ntz ? I.ez[ip + ic + 1] : 0.f;
+        const float f0 = iz - ic >= 0 ? I.ez[ip - ic] : 0.f;
+        gx = __builtin_fmaf(g.rc[ic] * (e1 - e0), g.idx, gx);
+        gz = __builtin_fmaf(g.rc[ic] * (f1 - f0), g.idz, gz);
+    }
+    I.vx[ip] -= gx;
+    I.vz[ip] -= gz;
+}
+
+// A_v^T, first half: adjoint of dp/dx, dp/dz on the velocity domains.
+template <int LC>
+__global__ __launch_bounds__(256) void k_adj_v2(KGrid g, const KInst *__restrict__ insts)
+{
+    CELL_INDEX
+    float ex = 0.f, ez = 0.f;
+    if (ix >= LC - 1 && ix <= g.ntx - LC - 1 && iz >= LC && iz <= g.ntz - LC) {
+        const float bv = g.Bx[ip] * I.vx[ip];
+        ex = bv;
+        const float a = g.axh[ix];
+        if (a != 0.f) {
+            const float u = bv + I.psx[ip];
+            ex = __builtin_fmaf(a, u, bv);
+            I.psx[ip] = g.bxh[ix] * u;
+        }
+    }
+    if (ix >= LC && ix <= g.ntx - LC && iz >= LC - 1 && iz <= g.ntz - LC - 1) {
+        const float bv = g.Bz[ip] * I.vz[ip];
+        ez = bv;
+        const float a = g.azh[iz];
+        if (a != 0.f) {
+            const float u = bv + I.psz[ip];
+            ez = __builtin_fmaf(a, u, bv);
+            I.psz[ip] = g.bzh[iz] * u;
+        }
+    }
+    I.ex[ip] = ex;
+    I.ez[ip] = ez;
+}
+
+// A_v^T, second half: phat -= D- e; then the data injection R^T d and the
+// gradient sum grad += (K*S0)*phat on the interior.
+template <int LC>
+__global__ __launch_bounds__(256) void k_adj_p2(KGrid g, const KInst *__restrict__ insts, int it, int first)
+{
+    CELL_INDEX
+    float ph = I.p[ip];
+    if (!first) {
+        float gx = 0.f, gz = 0.f;
+#pragma unroll
+        for (int ic = 0; ic < LC; ic++) {
+            const float e1 = ix + ic < g.ntx ? I.ex[ip + ic * sx] : 0.f;
+            const float e0 = ix - ic - 1 >= 0 ? I.ex[ip - (ic + 1) * sx] : 0.f;
+            const float f1 = iz + ic < g.ntz ? I.ez[ip + ic] : 0.f;
+            const float f0 = iz - ic - 1 >= 0 ? I.ez[ip - (ic + 1)] : 0.f;
+            gx = __builtin_fmaf(g.rc[ic] * (e1 - e0), g.idx, gx);
+            gz = __builtin_fmaf(g.rc[ic] * (f1 - f0), g.idz, gz);
+        }
+        ph -= gx + gz;
+    }
+    if (I.colmap) {
+        const int cm = I.colmap[ix];
+        if (cm >= 0) {
+            const int tr = I.rowmap[cm + iz];
+            if (tr >= 0) ph += I.dsum[static_cast<size_t>(tr) * g.nt + it];
+        }
+    }
+    I.p[ip] = ph;
+    if (interior(g, ix, iz)) {
+        const int im = (ix - g.pmlc) * g.nz + (iz - g.pmlc);
+        const float s0 = I.s0[static_cast<size_t>(it - I.s0_t0) * g.nx * g.nz + im];
+        I.grad[im] = __builtin_fmaf(g.K[ip] * s0, ph, I.grad[im]);
+    }
+}
+
+// Copy gather rows of traces that share a node with a later trace.
+__global__ void k_copy_rows(float *__restrict__ gather, const int2 *__restrict__ pairs, int npairs, int nt)
+{
+    const int k = blockIdx.y;
+    if (k >= npairs) return;
+    const int2 pr = pairs[k];   // (dst, src)
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < nt; it += gridDim.x * blockDim.x)
+        gather[static_cast<size_t>(pr.x) * nt + it] = gather[static_cast<size_t>(pr.y) * nt + it];
+}
+
+// Sum rows of traces sharing a node into the row of the node's last trace.
+__global__ void k_sum_rows(float *__restrict__ dst, const float *__restrict__ src, const int2 *__restrict__ pairs,
+                           int npairs, int nt)
+{
+    // pairs are (dup, primary), grouped by primary, applied sequentially per primary
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < nt; it += gridDim.x * blockDim.x)
+        for (int k = 0; k < npairs; k++) {
+            const int2 pr = pairs[k];
+            dst[static_cast<size_t>(pr.y) * nt + it] += src[static_cast<size_t>(pr.x) * nt + it];
+        }
+}
+
+// ---- kz up/down separation on receiver columns (FK:705, 762-777, 52-138) ----
+// One workgroup per (time step, receiver column): the captured column
+// p + i*htp, zero padded to nfftz, is transformed along z in LDS (radix-2
+// decimation in time, twiddles from an FP64-precomputed table), multiplied by
+// the taper and split at nfftz/2 (Nyquist goes up), inverse transformed
+// (unnormalised, as cuFFT INVERSE), and the real parts at the receiver rows
+// are written to the up/down gathers.
+__device__ __forceinline__ void lds_fft(float2 *a, float2 *b, int n, int logn, const float2 *__restrict__ tw, int twn,
+                                        bool inverse, bool two)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int j = __brev(static_cast<unsigned>(i)) >> (32 - logn);
+        if (i < j) {
+            float2 t = a[i]; a[i] = a[j]; a[j] = t;
+            if (two) { t = b[i]; b[i] = b[j]; b[j] = t; }
+        }
+    }
+    __syncthreads();
+    for (int len = 2, stride = twn / 2; len <= n; len <<= 1, stride >>= 1) {
+        const int half = len >> 1;
+        for (int q = threadIdx.x; q < n / 2; q += blockDim.x) {
+            const int k = q & (half - 1);
+            const int i0 = (q / half) * len + k, i1 = i0 + half;
+            float2 w = tw[k * stride];
+            if (inverse) w.y = -w.y;
+            {
+                const float2 x1 = a[i1], x0 = a[i0];
+                const float tr = x1.x * w.x - x1.y * w.y;
+                const float ti = x1.x * w.y + x1.y * w.x;
+                a[i1] = make_float2(x0.x - tr, x0.y - ti);
+                a[i0] = make_float2(x0.x + tr, x0.y + ti);
+            }
+            if (two) {
+                const float2 x1 = b[i1], x0 = b[i0];
+                const float tr = x1.x * w.x - x1.y * w.y;
+                const float ti = x1.x * w.y + x1.y * w.x;
+                b[i1] = make_float2(x0.x - tr, x0.y - ti);
+                b[i0] = make_float2(x0.x + tr, x0.y + ti);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_kzsep(const float2 *__restrict__ capture, int ncap, int nfftz, int logn,
+                                               const float2 *__restrict__ tw, const float *__restrict__ coef,
+                                               const KCapCol *__restrict__ cols, const int2 *__restrict__ traces,
+                                               float *__restrict__ up, float *__restrict__ down, int nt)
+{
+    extern __shared__ float2 lds[];
+    float2 *a = lds, *b = lds + nfftz;
+    const int c = blockIdx.x, it = blockIdx.y;
+    const float2 *src = capture + (static_cast<size_t>(it) * ncap + c) * nfftz;
+    for (int i = threadIdx.x; i < nfftz; i += blockDim.x) a[i] = src[i];
+    __syncthreads();
+    lds_fft(a, b, nfftz, logn, tw, nfftz, false, false);
+    const int halfz = nfftz / 2;
+    for (int i = threadIdx.x; i < nfftz; i += blockDim.x) {
+        const float cf = coef[i];
+        const float2 t = a[i];
+        const float2 s = make_float2(t.x * cf, t.y * cf);
+        if (i <= halfz) { a[i] = s; b[i] = make_float2(0.f, 0.f); }
+        else { b[i] = s; a[i] = make_float2(0.f, 0.f); }
+    }
+    __syncthreads();
+    lds_fft(a, b, nfftz, logn, tw, nfftz, true, true);
+    const KCapCol cc = cols[c];
+    for (int k = threadIdx.x; k < cc.tcount; k += blockDim.x) {
+        const int2 t = traces[cc.tstart + k];   // (gather row, iz)
+        up[static_cast<size_t>(t.x) * nt + it] = a[t.y].x;
+        down[static_cast<size_t>(t.x) * nt + it] = b[t.y].x;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+static inline dim3 cell_grid(const KGrid &g, int ninst) { return dim3((g.ntz + 63) / 64, (g.ntx + 3) / 4, ninst); }
+static const dim3 kCellBlock(64, 4, 1);
+
+#define DISPATCH_LC(LC, CALL)                          \
+    switch (LC) {                                      \
+    case 1: { constexpr int L = 1; CALL; } break;      \
+    case 2: { constexpr int L = 2; CALL; } break;      \
+    case 3: { constexpr int L = 3; CALL; } break;      \
+    case 4: { constexpr int L = 4; CALL; } break;      \
+    case 5: { constexpr int L = 5; CALL; } break;      \
+    case 6: { constexpr int L = 6; CALL; } break;      \
+    case 7: { constexpr int L = 7; CALL; } break;      \
+    case 8: { constexpr int L = 8; CALL; } break;      \
+    default: return hipErrorInvalidValue;              \
+    }
+
+hipError_t launch_v(const KGrid &g, const KInst *insts, int ninst, bool neg, int comp, hipStream_t s)
+{
+    const dim3 gr = cell_grid(g, ninst);
+#define LV(NEGV, C) hipLaunchKernelGGL((k_v<L, NEGV, C>), gr, kCellBlock, 0, s, g, insts)
+    if (!neg) {
+        if (comp == 1) { DISPATCH_LC(g.Lc, LV(false, 1)); }
+        else if (comp == 2) { DISPATCH_LC(g.Lc, LV(false, 2)); }
+        else { DISPATCH_LC(g.Lc, LV(false, 3)); }
+    } else {
+        if (comp == 1) { DISPATCH_LC(g.Lc, LV(true, 1)); }
+        else if (comp == 2) { DISPATCH_LC(g.Lc, LV(true, 2)); }
+        else { DISPATCH_LC(g.Lc, LV(true, 3)); }
+    }
+#undef LV
+    return hipGetLastError();
+}
+
+hipError_t launch_p(const KGrid &g, const KInst *insts, int ninst, bool neg, int it, hipStream_t s)
+{
+    const dim3 gr = cell_grid(g, ninst);
+    if (!neg) { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_p<L, false>), gr, kCellBlock, 0, s, g, insts, it)); }
+    else { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_p<L, true>), gr, kCellBlock, 0, s, g, insts, it)); }
+    return hipGetLastError();
+}
+
+hipError_t launch_src_p(const KGrid &g, const KInst *insts, int ninst, int it, hipStream_t s)
+{
+    const dim3 gr = cell_grid(g, ninst);
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_p<L>), gr, kCellBlock, 0, s, g, insts, it));
+    return hipGetLastError();
+}
+
+hipError_t launch_src_v(const KGrid &g, const KInst *insts, int ninst, int comp, hipStream_t s)
+{
+    const dim3 gr = cell_grid(g, ninst);
+    if (comp == 1) { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_v<L, 1>), gr, kCellBlock, 0, s, g, insts)); }
+    else if (comp == 2) { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_v<L, 2>), gr, kCellBlock, 0, s, g, insts)); }
+    else { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_v<L, 3>), gr, kCellBlock, 0, s, g, insts)); }
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_step(const KGrid &g, const KInst *insts, int ninst, int it, bool first, hipStream_t s)
+{
+    const dim3 gr = cell_grid(g, ninst);
+    if (!first) {
+        DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_adj_p1<L>), gr, kCellBlock, 0, s, g, insts));
+        DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_adj_v1<L>), gr, kCellBlock, 0, s, g, insts));
+        DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_adj_v2<L>), gr, kCellBlock, 0, s, g, insts));
+    }
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_adj_p2<L>), gr, kCellBlock, 0, s, g, insts, it, first ? 1 : 0));
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_rows(float *gather, const int2 *pairs, int npairs, int nt, hipStream_t s)
+{
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_rows, dim3((nt + 255) / 256, npairs), dim3(256), 0, s, gather, pairs, npairs, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_rows(float *dst, const float *src, const int2 *pairs, int npairs, int nt, hipStream_t s)
+{
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_rows, dim3((nt + 255) / 256), dim3(256), 0, s, dst, src, pairs, npairs, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2 *tw, const float *coef,
+                        const KCapCol *cols, const int2 *traces, float *up, float *down, int nt, hipStream_t s)
+{
+    int logn = 0;
+    while ((1 << logn) < nfftz) logn++;
+    const size_t lds = sizeof(float2) * 2 * nfftz;
+    hipLaunchKernelGGL(k_kzsep, dim3(ncap, nt), dim3(256), lds, s, capture, ncap, nfftz, logn, tw, coef, cols, traces,
+                       up, down, nt);
+    return hipGetLastError();
+}

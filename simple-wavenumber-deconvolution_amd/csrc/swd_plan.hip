@@ -1,0 +1,824 @@
+// swd_plan.hip -- plans, device memory and the operators of libswd's C-ABI.
+//
+// A plan owns one HIP stream (default priority) on one device, the model in
+// the kernels' form (dt*buoyancy at the two half points, dt*rho*vp^2 and
+// rho*vp^2, each rounded exactly as the reference rounds them per cell), the
+// CPML vectors, and a growing workspace from which every call carves its
+// wavefields, strips, gathers and scratch.  Operators run a batch of shots;
+// on SWD_PATH_FUSED the batch shares each launch (one wavefield per
+// blockIdx.z), on SWD_PATH_PLAIN shots run one after another with one kernel
+// per sub-step, which is the single-shot baseline.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "swd.h"
+#include "swd_internal.h"
+#include "swd_kernels.h"
+#include "swd_launch.h"
+#include "swd_fused.h"
+
+// ------------------------------------------------------------------ errors
+static thread_local char g_last_error[1024] = "";
+
+int swd_fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_last_error, sizeof g_last_error, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" int swd_last_error(char *buf, size_t n)
+{
+    if (!buf || n == 0) return SWD_E_ARG;
+    std::snprintf(buf, n, "%s", g_last_error);
+    return SWD_OK;
+}
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return swd_fail(e_ == hipErrorOutOfMemory ? SWD_E_NOMEM : SWD_E_DEVICE, "%s failed: %s (%s:%d)", \
+                            #expr, hipGetErrorString(e_), __FILE__, __LINE__);                            \
+    } while (0)
+
+#define SWD_TRY(expr)              \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != SWD_OK) return r_; \
+    } while (0)
+
+extern "C" int swd_get_device_count(int *n)
+{
+    if (!n) return swd_fail(SWD_E_ARG, "swd_get_device_count: null argument");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        *n = 0;
+        return swd_fail(SWD_E_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *n = c;
+    return SWD_OK;
+}
+
+// ------------------------------------------------------------------ plan
+namespace {
+
+struct Plan {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    swd_grid g{};
+    int pmlc = 0, ntx = 0, ntz = 0, ldz = 0, nframe = 0;
+    size_t nfield = 0;   // ntx*ldz
+    int path = SWD_PATH_FUSED;
+    bool have_coefs = false, have_model = false;
+    float rc[SWD_MAX_LC] = {};
+    float *d_coef = nullptr;    // 4*ntx + 4*ntz
+    float *d_model = nullptr;   // 4*nfield
+    char *ws = nullptr;
+    size_t ws_cap = 0;
+    hipEvent_t ev[4] = {};
+    swd_stats stats{};
+    KGrid kg{};
+};
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Sequential carving of one workspace block.
+struct Carver {
+    size_t off = 0;
+    template <class T>
+    size_t add(size_t count)
+    {
+        const size_t o = align_up(off, 256);
+        off = o + count * sizeof(T);
+        return o;
+    }
+};
+
+int ensure_ws(Plan &P, size_t bytes)
+{
+    if (bytes <= P.ws_cap) return SWD_OK;
+    if (P.ws) {
+        HIP_TRY(hipStreamSynchronize(P.stream));
+        HIP_TRY(hipFree(P.ws));
+        P.ws = nullptr;
+        P.ws_cap = 0;
+    }
+    const size_t cap = align_up(bytes + bytes / 8, 1 << 20);
+    HIP_TRY(hipMalloc(&P.ws, cap));
+    P.ws_cap = cap;
+    return SWD_OK;
+}
+
+template <class T>
+T *at(Plan &P, size_t off) { return reinterpret_cast<T *>(P.ws + off); }
+
+int check_plan(void *plan, Plan *&P)
+{
+    if (!plan) return swd_fail(SWD_E_ARG, "null plan");
+    P = static_cast<Plan *>(plan);
+    if (!P->have_coefs || !P->have_model)
+        return swd_fail(SWD_E_STATE, "plan needs swd_plan_set_coefs and swd_plan_set_model before running");
+    HIP_TRY(hipSetDevice(P->dev));
+    return SWD_OK;
+}
+
+// Receiver lookup of one shot: colmap[ntx] -> row map offset, rowmap -> trace
+// of the node (the last trace on a node wins, as the reference's receiver
+// loops do), dup = traces that share a node with a later trace.
+struct RecvMap {
+    std::vector<int> colmap, rowmap;
+    std::vector<int> prim;   // primary trace of each trace
+};
+
+int build_recv(const Plan &P, const swd_shot &s, RecvMap &m)
+{
+    if (s.s_ix < 0 || s.s_ix >= P.ntx || s.s_iz < 0 || s.s_iz >= P.ntz)
+        return swd_fail(SWD_E_ARG, "source (%d,%d) outside the padded grid %dx%d", s.s_ix, s.s_iz, P.ntx, P.ntz);
+    if (s.nrec < 0 || (s.nrec > 0 && (!s.rec_ix || !s.rec_iz))) return swd_fail(SWD_E_ARG, "bad receiver list");
+    m.colmap.assign(P.ntx, -1);
+    m.rowmap.clear();
+    for (int tr = 0; tr < s.nrec; tr++) {
+        const int ix = s.rec_ix[tr], iz = s.rec_iz[tr];
+        if (ix < 0 || ix >= P.ntx || iz < 0 || iz >= P.ntz)
+            return swd_fail(SWD_E_ARG, "receiver %d at (%d,%d) outside the padded grid", tr, ix, iz);
+        if (m.colmap[ix] < 0) {
+            m.colmap[ix] = static_cast<int>(m.rowmap.size());
+            m.rowmap.resize(m.rowmap.size() + P.ntz, -1);
+        }
+        m.rowmap[m.colmap[ix] + iz] = tr;
+    }
+    m.prim.resize(s.nrec);
+    for (int tr = 0; tr < s.nrec; tr++) m.prim[tr] = m.rowmap[m.colmap[s.rec_ix[tr]] + s.rec_iz[tr]];
+    return SWD_OK;
+}
+
+// Host-side description of a batch of shots, uploaded in one block.
+struct Batch {
+    int n = 0;
+    std::vector<RecvMap> maps;
+    std::vector<int> toff;         // first gather row of each shot
+    int ntr = 0;
+    std::vector<int> ints;         // all colmaps and rowmaps, concatenated
+    std::vector<size_t> col_off, row_off;
+    std::vector<int2> dup_pairs;   // (dup row, primary row), global rows
+};
+
+int build_batch(const Plan &P, int nshots, const swd_shot *shots, Batch &b)
+{
+    if (nshots < 1 || !shots) return swd_fail(SWD_E_ARG, "need at least one shot");
+    b.n = nshots;
+    b.maps.resize(nshots);
+    b.toff.resize(nshots);
+    b.col_off.resize(nshots);
+    b.row_off.resize(nshots);
+    b.ntr = 0;
+    b.ints.clear();
+    b.dup_pairs.clear();
+    for (int k = 0; k < nshots; k++) {
+        SWD_TRY(build_recv(P, shots[k], b.maps[k]));
+        b.toff[k] = b.ntr;
+        b.col_off[k] = b.ints.size();
+        b.ints.insert(b.ints.end(), b.maps[k].colmap.begin(), b.maps[k].colmap.end());
+        b.row_off[k] = b.ints.size();
+        b.ints.insert(b.ints.end(), b.maps[k].rowmap.begin(), b.maps[k].rowmap.end());
+        for (int tr = 0; tr < shots[k].nrec; tr++)
+            if (b.maps[k].prim[tr] != tr) b.dup_pairs.push_back(make_int2(b.ntr + tr, b.ntr + b.maps[k].prim[tr]));
+        b.ntr += shots[k].nrec;
+    }
+    if (b.ints.empty()) b.ints.push_back(-1);
+    return SWD_OK;
+}
+
+void stats_begin(Plan &P, int which) { (void)hipEventRecord(P.ev[2 * which], P.stream); }
+double stats_end(Plan &P, int which)
+{
+    (void)hipEventRecord(P.ev[2 * which + 1], P.stream);
+    (void)hipEventSynchronize(P.ev[2 * which + 1]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, P.ev[2 * which], P.ev[2 * which + 1]);
+    return ms;
+}
+
+}  // namespace
+
+extern "C" int swd_plan_create(const swd_grid *g, int device, void **out)
+{
+    if (!g || !out) return swd_fail(SWD_E_ARG, "swd_plan_create: null argument");
+    *out = nullptr;
+    if (g->Lc < 1 || g->Lc > SWD_MAX_LC) return swd_fail(SWD_E_ARG, "Lc=%d outside 1..%d", g->Lc, SWD_MAX_LC);
+    if (g->nx < 2 * g->Lc + 1 || g->nz < 2 * g->Lc + 1 || g->pml < 0 || g->nt < 2 || !(g->dx > 0) ||
+        !(g->dz > 0) || !(g->dt > 0))
+        return swd_fail(SWD_E_ARG, "invalid grid (nx=%d nz=%d pml=%d Lc=%d nt=%d)", g->nx, g->nz, g->pml, g->Lc, g->nt);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return swd_fail(SWD_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return swd_fail(SWD_E_ARG, "device %d outside 0..%d", device, ndev - 1);
+    Plan *P = new Plan();
+    P->dev = device;
+    P->g = *g;
+    P->pmlc = g->pml + g->Lc;
+    P->ntx = g->nx + 2 * P->pmlc;
+    P->ntz = g->nz + 2 * P->pmlc;
+    P->ldz = static_cast<int>(align_up(P->ntz, 32));
+    P->nfield = static_cast<size_t>(P->ntx) * P->ldz;
+    P->nframe = 2 * g->Lc * (g->nx + g->nz - 2 * g->Lc);
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreate(&P->stream);
+    for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&P->ev[i]);
+    if (e == hipSuccess) e = hipMalloc(&P->d_coef, sizeof(float) * 4 * (P->ntx + P->ntz));
+    if (e == hipSuccess) e = hipMalloc(&P->d_model, sizeof(float) * 4 * P->nfield);
+    if (e == hipSuccess) e = hipMemset(P->d_model, 0, sizeof(float) * 4 * P->nfield);
+    if (e != hipSuccess) {
+        swd_plan_destroy(P);
+        return swd_fail(SWD_E_DEVICE, "swd_plan_create: %s", hipGetErrorString(e));
+    }
+    KGrid &k = P->kg;
+    k.ntx = P->ntx; k.ntz = P->ntz; k.ldz = P->ldz;
+    k.Lc = g->Lc; k.pmlc = P->pmlc; k.nx = g->nx; k.nz = g->nz; k.nt = g->nt; k.nframe = P->nframe;
+    k.idx = static_cast<float>(1.0 / g->dx);   // float one_over_dx = 1.0/dx (RK:47)
+    k.idz = static_cast<float>(1.0 / g->dz);
+    float *c = P->d_coef;
+    k.ax = c; k.axh = c + P->ntx; k.bx = c + 2 * P->ntx; k.bxh = c + 3 * P->ntx;
+    float *cz = c + 4 * P->ntx;
+    k.az = cz; k.azh = cz + P->ntz; k.bz = cz + 2 * P->ntz; k.bzh = cz + 3 * P->ntz;
+    k.Bx = P->d_model; k.Bz = P->d_model + P->nfield; k.K = P->d_model + 2 * P->nfield; k.K0 = P->d_model + 3 * P->nfield;
+    *out = P;
+    return SWD_OK;
+}
+
+extern "C" int swd_plan_destroy(void *plan)
+{
+    if (!plan) return SWD_OK;
+    Plan *P = static_cast<Plan *>(plan);
+    (void)hipSetDevice(P->dev);
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    if (P->ws) (void)hipFree(P->ws);
+    if (P->d_coef) (void)hipFree(P->d_coef);
+    if (P->d_model) (void)hipFree(P->d_model);
+    for (auto &e : P->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (P->stream) (void)hipStreamDestroy(P->stream);
+    delete P;
+    return SWD_OK;
+}
+
+extern "C" int swd_plan_set_coefs(void *plan, const swd_coefs *c)
+{
+    if (!plan || !c || !c->rc || !c->a_x || !c->a_x_half || !c->b_x || !c->b_x_half || !c->a_z || !c->a_z_half ||
+        !c->b_z || !c->b_z_half)
+        return swd_fail(SWD_E_ARG, "swd_plan_set_coefs: null argument");
+    Plan *P = static_cast<Plan *>(plan);
+    HIP_TRY(hipSetDevice(P->dev));
+    std::vector<float> h(4 * (P->ntx + P->ntz));
+    const float *xs[4] = {c->a_x, c->a_x_half, c->b_x, c->b_x_half};
+    const float *zs[4] = {c->a_z, c->a_z_half, c->b_z, c->b_z_half};
+    for (int i = 0; i < 4; i++) {
+        std::copy(xs[i], xs[i] + P->ntx, h.begin() + i * P->ntx);
+        std::copy(zs[i], zs[i] + P->ntz, h.begin() + 4 * P->ntx + i * P->ntz);
+    }
+    HIP_TRY(hipMemcpy(P->d_coef, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    for (int i = 0; i < P->g.Lc; i++) P->kg.rc[i] = c->rc[i];
+    P->have_coefs = true;
+    return SWD_OK;
+}
+
+extern "C" int swd_plan_set_model(void *plan, const float *vp, const float *rho)
+{
+    if (!plan || !vp || !rho) return swd_fail(SWD_E_ARG, "swd_plan_set_model: null argument");
+    Plan *P = static_cast<Plan *>(plan);
+    HIP_TRY(hipSetDevice(P->dev));
+    const int ntx = P->ntx, ntz = P->ntz, ldz = P->ldz;
+    const float dt = P->g.dt;
+    std::vector<float> h(4 * P->nfield, 0.f);
+    float *Bx = h.data(), *Bz = Bx + P->nfield, *K = Bz + P->nfield, *K0 = K + P->nfield;
+    for (int ix = 0; ix < ntx; ix++)
+        for (int iz = 0; iz < ntz; iz++) {
+            const size_t ip = static_cast<size_t>(ix) * ntz + iz, o = static_cast<size_t>(ix) * ldz + iz;
+            if (!(vp[ip] > 0.f) || !(rho[ip] > 0.f))
+                return swd_fail(SWD_E_ARG, "model must be positive (vp=%g rho=%g at ix=%d iz=%d)", vp[ip], rho[ip], ix, iz);
+            // 1/(0.5*(rho+rho')) in double, stored as float, times dt (RK:62-64, RK:111-113)
+            if (ix + 1 < ntx) Bx[o] = dt * static_cast<float>(1.0 / (0.5 * (static_cast<double>(rho[ip]) + rho[ip + ntz])));
+            if (iz + 1 < ntz) Bz[o] = dt * static_cast<float>(1.0 / (0.5 * (static_cast<double>(rho[ip]) + rho[ip + 1])));
+            K[o] = dt * rho[ip] * vp[ip] * vp[ip];   // RK:171, left to right in float
+            K0[o] = rho[ip] * vp[ip] * vp[ip];       // RK:475
+        }
+    HIP_TRY(hipMemcpy(P->d_model, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    P->have_model = true;
+    return SWD_OK;
+}
+
+extern "C" int swd_plan_set_path(void *plan, int path)
+{
+    if (!plan || (path != SWD_PATH_PLAIN && path != SWD_PATH_FUSED)) return swd_fail(SWD_E_ARG, "swd_plan_set_path: bad path");
+    static_cast<Plan *>(plan)->path = path;
+    return SWD_OK;
+}
+
+extern "C" int swd_plan_get_stats(void *plan, swd_stats *st)
+{
+    if (!plan || !st) return swd_fail(SWD_E_ARG, "swd_plan_get_stats: null argument");
+    *st = static_cast<Plan *>(plan)->stats;
+    return SWD_OK;
+}
+
+// ------------------------------------------------------------------ time loops
+namespace {
+
+// Upload of the batch's integer maps and instance table into the workspace.
+int upload_batch(Plan &P, const Batch &b, size_t o_ints, size_t o_pairs)
+{
+    HIP_TRY(hipMemcpyAsync(P.ws + o_ints, b.ints.data(), sizeof(int) * b.ints.size(), hipMemcpyHostToDevice, P.stream));
+    if (!b.dup_pairs.empty())
+        HIP_TRY(hipMemcpyAsync(P.ws + o_pairs, b.dup_pairs.data(), sizeof(int2) * b.dup_pairs.size(),
+                               hipMemcpyHostToDevice, P.stream));
+    return SWD_OK;
+}
+
+int upload_insts(Plan &P, const std::vector<KInst> &h, size_t o_inst)
+{
+    HIP_TRY(hipMemcpyAsync(P.ws + o_inst, h.data(), sizeof(KInst) * h.size(), hipMemcpyHostToDevice, P.stream));
+    return SWD_OK;
+}
+
+// Forward time loop over instance groups [first, first+count) of d_inst.
+// Groups of one shot each on the plain path, one group on the fused path.
+int forward_loop(Plan &P, const KInst *d_inst, const std::vector<KInst> &h_inst, int per_shot, int nshots)
+{
+    const KGrid &g = P.kg;
+    long long launches = 0;
+    stats_begin(P, 0);
+    if (P.path == SWD_PATH_PLAIN) {
+        for (int s = 0; s < nshots; s++) {
+            const KInst *di = d_inst + s * per_shot;
+            for (int it = 0; it < g.nt; it++) {
+                HIP_TRY(launch_v(g, di, per_shot, false, 1, P.stream));
+                HIP_TRY(launch_v(g, di, per_shot, false, 2, P.stream));
+                HIP_TRY(launch_p(g, di, per_shot, false, it, P.stream));
+                launches += 3;
+            }
+        }
+    } else {
+        const int n = per_shot * nshots;
+        SWD_TRY(fused_forward_loop(P.kg, d_inst, h_inst.data(), n, P.stream, &launches));
+    }
+    P.stats.fwd_ms = stats_end(P, 0);
+    P.stats.fwd_launches = launches;
+    P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * g.nt * per_shot * nshots;
+    return SWD_OK;
+}
+
+void zero_fields(Plan &P, float *base, size_t count) { (void)hipMemsetAsync(base, 0, sizeof(float) * count, P.stream); }
+
+void bind_fields(KInst &I, float *f, size_t nfield)
+{
+    I.p = f; I.vx = f + nfield; I.vz = f + 2 * nfield;
+    I.psx = f + 3 * nfield; I.psz = f + 4 * nfield; I.pvx = f + 5 * nfield; I.pvz = f + 6 * nfield;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ operators
+namespace {
+
+// Forward modelling of a batch; gathers stay on the device at d_gather.
+int forward_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet, float *d_gather)
+{
+    Batch b;
+    SWD_TRY(build_batch(P, nshots, shots, b));
+    const int nt = P.g.nt;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KInst>(nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_f = c.add<float>(7 * P.nfield * nshots);
+    SWD_TRY(ensure_ws(P, c.off));
+    SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
+    zero_fields(P, at<float>(P, o_f), 7 * P.nfield * nshots);
+    std::vector<KInst> h(nshots);
+    for (int k = 0; k < nshots; k++) {
+        KInst &I = h[k];
+        std::memset(&I, 0, sizeof I);
+        bind_fields(I, at<float>(P, o_f) + 7 * P.nfield * k, P.nfield);
+        I.wavelet = at<float>(P, o_wav);
+        I.s_ix = shots[k].s_ix; I.s_iz = shots[k].s_iz;
+        if (shots[k].nrec > 0) {
+            I.colmap = at<int>(P, o_ints) + b.col_off[k];
+            I.rowmap = at<int>(P, o_ints) + b.row_off[k];
+            I.gather = d_gather + static_cast<size_t>(b.toff[k]) * nt;
+        }
+    }
+    SWD_TRY(upload_insts(P, h, o_inst));
+    SWD_TRY(forward_loop(P, at<KInst>(P, o_inst), h, 1, nshots));
+    HIP_TRY(launch_copy_rows(d_gather, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
+    HIP_TRY(hipStreamSynchronize(P.stream));
+    return SWD_OK;
+}
+
+int total_traces(int nshots, const swd_shot *shots)
+{
+    int n = 0;
+    for (int k = 0; k < nshots; k++) n += shots[k].nrec > 0 ? shots[k].nrec : 0;
+    return n;
+}
+
+// Device scratch for host-buffer wrappers (separate from the plan workspace).
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t bytes)
+    {
+        if (bytes == 0) bytes = 256;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return swd_fail(SWD_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        return SWD_OK;
+    }
+    template <class T> T *as() { return static_cast<T *>(p); }
+};
+
+}  // namespace
+
+extern "C" int swd_forward(void *plan, int nshots, const swd_shot *shots, const float *wavelet, float *gather)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !gather || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_forward: null argument");
+    const size_t n = static_cast<size_t>(total_traces(nshots, shots)) * P->g.nt;
+    DevBuf d;
+    SWD_TRY(d.alloc(sizeof(float) * n));
+    SWD_TRY(forward_dev(*P, nshots, shots, wavelet, d.as<float>()));
+    HIP_TRY(hipMemcpy(gather, d.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return SWD_OK;
+}
+
+extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *hwavelet,
+                        const float *kzcoef, int nfftz, float *gather, float *up, float *down)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !hwavelet || !kzcoef || !gather || !up || !down || !shots || nshots < 1)
+        return swd_fail(SWD_E_ARG, "swd_afor: null argument");
+    if (nfftz < P->ntz || (nfftz & (nfftz - 1)) || nfftz > 8192)
+        return swd_fail(SWD_E_ARG, "nfftz=%d must be a power of two in [ntz=%d, 8192]", nfftz, P->ntz);
+    Batch b;
+    SWD_TRY(build_batch(*P, nshots, shots, b));
+    const int nt = P->g.nt;
+    // receiver columns of each shot and their trace lists (gather row, iz)
+    std::vector<int> colcap(static_cast<size_t>(P->ntx) * nshots, -1);
+    std::vector<KCapCol> cols;
+    std::vector<int2> traces;
+    std::vector<int> cap0(nshots);
+    for (int k = 0; k < nshots; k++) {
+        cap0[k] = static_cast<int>(cols.size());
+        std::vector<std::vector<int2>> per(P->ntx);
+        for (int tr = 0; tr < shots[k].nrec; tr++)
+            per[shots[k].rec_ix[tr]].push_back(make_int2(b.toff[k] + tr, shots[k].rec_iz[tr]));
+        for (int ix = 0; ix < P->ntx; ix++)
+            if (!per[ix].empty()) {
+                colcap[static_cast<size_t>(k) * P->ntx + ix] = static_cast<int>(cols.size()) - cap0[k];
+                cols.push_back(KCapCol{static_cast<int>(traces.size()), static_cast<int>(per[ix].size())});
+                traces.insert(traces.end(), per[ix].begin(), per[ix].end());
+            }
+    }
+    const int ncap = std::max<int>(1, static_cast<int>(cols.size()));
+    // twiddles w_N^k = exp(-2 pi i k / N) for k < N/2, evaluated in double
+    std::vector<float2> tw(nfftz / 2);
+    for (int k = 0; k < nfftz / 2; k++) {
+        const double ang = -1 * 2.0 * 3.14159265358979323846 * k / nfftz;
+        tw[k] = make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
+    }
+    const size_t ng = static_cast<size_t>(b.ntr) * nt;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KInst>(2 * nshots);
+    const size_t o_wav = c.add<float>(2 * nt);
+    const size_t o_colcap = c.add<int>(colcap.size());
+    const size_t o_cols = c.add<KCapCol>(ncap);
+    const size_t o_tr = c.add<int2>(std::max<size_t>(1, traces.size()));
+    const size_t o_tw = c.add<float2>(nfftz / 2);
+    const size_t o_coef = c.add<float>(nfftz);
+    const size_t o_gat = c.add<float>(3 * std::max<size_t>(ng, 1));
+    const size_t o_cap = c.add<float2>(static_cast<size_t>(nt) * ncap * nfftz);
+    const size_t o_f = c.add<float>(7 * P->nfield * 2 * nshots);
+    SWD_TRY(ensure_ws(*P, c.off));
+    SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P->ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->ws + o_wav + sizeof(float) * nt, hwavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->ws + o_colcap, colcap.data(), sizeof(int) * colcap.size(), hipMemcpyHostToDevice, P->stream));
+    if (!cols.empty()) {
+        HIP_TRY(hipMemcpyAsync(P->ws + o_cols, cols.data(), sizeof(KCapCol) * cols.size(), hipMemcpyHostToDevice, P->stream));
+        HIP_TRY(hipMemcpyAsync(P->ws + o_tr, traces.data(), sizeof(int2) * traces.size(), hipMemcpyHostToDevice, P->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(P->ws + o_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->ws + o_coef, kzcoef, sizeof(float) * nfftz, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemsetAsync(P->ws + o_cap, 0, sizeof(float2) * static_cast<size_t>(nt) * ncap * nfftz, P->stream));
+    zero_fields(*P, at<float>(*P, o_f), 7 * P->nfield * 2 * nshots);
+    float *d_gather = at<float>(*P, o_gat), *d_up = d_gather + ng, *d_down = d_up + ng;
+    std::vector<KInst> h(2 * nshots);
+    for (int k = 0; k < nshots; k++)
+        for (int w = 0; w < 2; w++) {
+            KInst &I = h[2 * k + w];
+            std::memset(&I, 0, sizeof I);
+            bind_fields(I, at<float>(*P, o_f) + 7 * P->nfield * (2 * k + w), P->nfield);
+            I.wavelet = at<float>(*P, o_wav) + w * nt;   // p: rick, htp: hrick (FK:707, 747)
+            I.s_ix = shots[k].s_ix; I.s_iz = shots[k].s_iz;
+            if (w == 0 && shots[k].nrec > 0) {
+                I.colmap = at<int>(*P, o_ints) + b.col_off[k];
+                I.rowmap = at<int>(*P, o_ints) + b.row_off[k];
+                I.gather = d_gather + static_cast<size_t>(b.toff[k]) * nt;
+            }
+            if (shots[k].nrec > 0) {
+                I.capture = at<float2>(*P, o_cap) + cap0[k] * static_cast<size_t>(nfftz);
+                I.colcap = at<int>(*P, o_colcap) + static_cast<size_t>(k) * P->ntx;
+                I.capcomp = w;
+                I.ncap = ncap;
+                I.nfftz = nfftz;
+            }
+        }
+    SWD_TRY(upload_insts(*P, h, o_inst));
+    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), h, 2, nshots));
+    HIP_TRY(launch_copy_rows(d_gather, at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
+    if (!cols.empty()) {
+        stats_begin(*P, 1);
+        HIP_TRY(launch_kzsep(at<float2>(*P, o_cap), ncap, nfftz, at<float2>(*P, o_tw), at<float>(*P, o_coef),
+                             at<KCapCol>(*P, o_cols), at<int2>(*P, o_tr), d_up, d_down, nt, P->stream));
+        P->stats.fft_ms = stats_end(*P, 1);
+    }
+    HIP_TRY(hipMemcpyAsync(gather, d_gather, sizeof(float) * ng, hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipMemcpyAsync(up, d_up, sizeof(float) * ng, hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipMemcpyAsync(down, d_down, sizeof(float) * ng, hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return SWD_OK;
+}
+
+extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
+                           float *image, float *illum_src, float *illum_rcv, float *gather_syn)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !adj || !image || !illum_src || !illum_rcv || !shots || nshots < 1)
+        return swd_fail(SWD_E_ARG, "swd_rtm: null argument");
+    Batch b;
+    SWD_TRY(build_batch(*P, nshots, shots, b));
+    const int nt = P->g.nt;
+    const size_t nf = P->nfield;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KInst>(2 * nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_strips = c.add<float>(static_cast<size_t>(nt) * P->nframe * nshots);
+    const size_t o_img = c.add<float>(4 * nf * nshots);        // image, illum_s, illum_r, dpdt
+    const size_t o_f = c.add<float>(7 * nf * 2 * nshots);       // source-side then receiver-side fields
+    SWD_TRY(ensure_ws(*P, c.off));
+    SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P->ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
+    zero_fields(*P, at<float>(*P, o_f), 7 * nf * 2 * nshots);
+    zero_fields(*P, at<float>(*P, o_img), 4 * nf * nshots);
+    std::vector<KInst> h(2 * nshots);
+    float *fields = at<float>(*P, o_f), *imgs = at<float>(*P, o_img);
+    for (int k = 0; k < nshots; k++) {
+        KInst &S = h[k];        // source side: forward, then reconstruction
+        KInst &R = h[nshots + k];   // receiver side
+        std::memset(&S, 0, sizeof S);
+        std::memset(&R, 0, sizeof R);
+        bind_fields(S, fields + 7 * nf * k, nf);
+        bind_fields(R, fields + 7 * nf * (nshots + k), nf);
+        S.wavelet = at<float>(*P, o_wav);
+        S.s_ix = R.s_ix = shots[k].s_ix;
+        S.s_iz = R.s_iz = shots[k].s_iz;
+        S.strips = at<float>(*P, o_strips) + static_cast<size_t>(nt) * P->nframe * k;
+        if (shots[k].nrec > 0) {
+            S.colmap = R.colmap = at<int>(*P, o_ints) + b.col_off[k];
+            S.rowmap = R.rowmap = at<int>(*P, o_ints) + b.row_off[k];
+            if (gather_syn) S.gather = gather_syn + static_cast<size_t>(b.toff[k]) * nt;
+            R.adj = adj + static_cast<size_t>(b.toff[k]) * nt;
+        }
+        float *im = imgs + 4 * nf * k;
+        R.image = im; R.illum_s = im + nf; R.illum_r = im + 2 * nf;
+        S.dpdt = R.dpdt = im + 3 * nf;
+        R.wavelet = nullptr;
+    }
+    SWD_TRY(upload_insts(*P, h, o_inst));
+    const KInst *dS = at<KInst>(*P, o_inst), *dR = dS + nshots;
+    SWD_TRY(forward_loop(*P, dS, h, 1, nshots));
+    if (gather_syn)
+        HIP_TRY(launch_copy_rows(gather_syn, at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
+    // backward: it = nt-2 .. 0 (RK:1073)
+    const KGrid &g = P->kg;
+    long long launches = 0;
+    stats_begin(*P, 1);
+    if (P->path == SWD_PATH_PLAIN) {
+        for (int k = 0; k < nshots; k++)
+            for (int it = nt - 2; it >= 0; it--) {
+                HIP_TRY(launch_src_p(g, dS + k, 1, it, P->stream));
+                HIP_TRY(launch_src_v(g, dS + k, 1, 1, P->stream));
+                HIP_TRY(launch_src_v(g, dS + k, 1, 2, P->stream));
+                HIP_TRY(launch_v(g, dR + k, 1, true, 1, P->stream));
+                HIP_TRY(launch_v(g, dR + k, 1, true, 2, P->stream));
+                HIP_TRY(launch_p(g, dR + k, 1, true, it, P->stream));
+                launches += 6;
+            }
+    } else {
+        SWD_TRY(fused_backward_loop(P->kg, dS, dR, h.data(), h.data() + nshots, nshots, P->stream, &launches));
+    }
+    P->stats.bwd_ms = stats_end(*P, 1);
+    P->stats.bwd_launches = launches;
+    P->stats.bwd_cells = static_cast<long long>(nt - 1) * nshots *
+                         (static_cast<long long>(P->ntx) * P->ntz + static_cast<long long>(P->g.nx) * P->g.nz);
+    // compact [ntx][ldz] -> [ntx][ntz]
+    const size_t ntp = static_cast<size_t>(P->ntx) * P->ntz;
+    for (int k = 0; k < nshots; k++) {
+        float *outs[3] = {image + ntp * k, illum_src + ntp * k, illum_rcv + ntp * k};
+        for (int q = 0; q < 3; q++)
+            HIP_TRY(hipMemcpy2DAsync(outs[q], sizeof(float) * P->ntz, imgs + 4 * nf * k + q * nf, sizeof(float) * P->ldz,
+                                     sizeof(float) * P->ntz, P->ntx, hipMemcpyDeviceToDevice, P->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return SWD_OK;
+}
+
+extern "C" int swd_rtm(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
+                       float *image, float *illum_src, float *illum_rcv, float *gather_syn)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!adj || !image || !illum_src || !illum_rcv || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_rtm: null argument");
+    const size_t ng = static_cast<size_t>(total_traces(nshots, shots)) * P->g.nt;
+    const size_t ntp = static_cast<size_t>(P->ntx) * P->ntz;
+    DevBuf d;
+    SWD_TRY(d.alloc(sizeof(float) * (2 * ng + 3 * ntp * nshots)));
+    float *d_adj = d.as<float>(), *d_gs = d_adj + ng, *d_img = d_gs + ng;
+    HIP_TRY(hipMemcpy(d_adj, adj, sizeof(float) * ng, hipMemcpyHostToDevice));
+    SWD_TRY(swd_rtm_dev(plan, nshots, shots, wavelet, d_adj, d_img, d_img + ntp * nshots, d_img + 2 * ntp * nshots,
+                        gather_syn ? d_gs : nullptr));
+    HIP_TRY(hipMemcpy(image, d_img, sizeof(float) * ntp * nshots, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(illum_src, d_img + ntp * nshots, sizeof(float) * ntp * nshots, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(illum_rcv, d_img + 2 * ntp * nshots, sizeof(float) * ntp * nshots, hipMemcpyDeviceToHost));
+    if (gather_syn) HIP_TRY(hipMemcpy(gather_syn, d_gs, sizeof(float) * ng, hipMemcpyDeviceToHost));
+    return SWD_OK;
+}
+
+// ------------------------------------------------------------------ Born
+extern "C" int swd_born(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *m, float *d)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !m || !d || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_born: null argument");
+    Batch b;
+    SWD_TRY(build_batch(*P, nshots, shots, b));
+    const int nt = P->g.nt;
+    const size_t nf = P->nfield, nint = static_cast<size_t>(P->g.nx) * P->g.nz;
+    const size_t ng = static_cast<size_t>(b.ntr) * nt;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KInst>(2 * nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_m = c.add<float>(nint * nshots);
+    const size_t o_d = c.add<float>(std::max<size_t>(ng, 1));
+    const size_t o_f = c.add<float>(7 * nf * 2 * nshots);
+    SWD_TRY(ensure_ws(*P, c.off));
+    SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P->ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(P->ws + o_m, m, sizeof(float) * nint * nshots, hipMemcpyHostToDevice, P->stream));
+    zero_fields(*P, at<float>(*P, o_f), 7 * nf * 2 * nshots);
+    std::vector<KInst> h(2 * nshots);
+    for (int k = 0; k < nshots; k++) {
+        KInst &B = h[2 * k], &U = h[2 * k + 1];
+        std::memset(&B, 0, sizeof B);
+        std::memset(&U, 0, sizeof U);
+        bind_fields(B, at<float>(*P, o_f) + 7 * nf * 2 * k, nf);
+        bind_fields(U, at<float>(*P, o_f) + 7 * nf * (2 * k + 1), nf);
+        B.wavelet = at<float>(*P, o_wav);
+        B.s_ix = U.s_ix = shots[k].s_ix;
+        B.s_iz = U.s_iz = shots[k].s_iz;
+        U.bvx = B.vx; U.bvz = B.vz;
+        U.born_m = at<float>(*P, o_m) + nint * k;
+        if (shots[k].nrec > 0) {
+            U.colmap = at<int>(*P, o_ints) + b.col_off[k];
+            U.rowmap = at<int>(*P, o_ints) + b.row_off[k];
+            U.gather = at<float>(*P, o_d) + static_cast<size_t>(b.toff[k]) * nt;
+        }
+    }
+    SWD_TRY(upload_insts(*P, h, o_inst));
+    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), h, 2, nshots));
+    HIP_TRY(launch_copy_rows(at<float>(*P, o_d), at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
+    HIP_TRY(hipMemcpyAsync(d, at<float>(*P, o_d), sizeof(float) * ng, hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    return SWD_OK;
+}
+
+extern "C" int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *d,
+                                float *m_grad, int nsnap)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !m_grad || !d || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_born_adjoint: null argument");
+    Batch b;
+    SWD_TRY(build_batch(*P, nshots, shots, b));
+    const int nt = P->g.nt;
+    const size_t nf = P->nfield, nint = static_cast<size_t>(P->g.nx) * P->g.nz;
+    const size_t ng = static_cast<size_t>(b.ntr) * nt;
+    const int seg = nsnap > 0 ? (nt + nsnap - 1) / nsnap : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(nt))));
+    const int nseg = (nt + seg - 1) / seg;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KInst>(2 * nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_d = c.add<float>(2 * std::max<size_t>(ng, 1));
+    const size_t o_grad = c.add<float>(nint * nshots);
+    const size_t o_s0 = c.add<float>(nint * seg * nshots);
+    const size_t o_snap = c.add<float>(7 * nf * static_cast<size_t>(nseg) * nshots);
+    const size_t o_f = c.add<float>(9 * nf * 2 * nshots);   // background (7) + adjoint (7) + ex, ez
+    SWD_TRY(ensure_ws(*P, c.off));
+    SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P->ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
+    float *dd = at<float>(*P, o_d), *dsum = dd + ng;
+    HIP_TRY(hipMemcpyAsync(dd, d, sizeof(float) * ng, hipMemcpyHostToDevice, P->stream));
+    HIP_TRY(hipMemcpyAsync(dsum, dd, sizeof(float) * ng, hipMemcpyDeviceToDevice, P->stream));
+    HIP_TRY(launch_sum_rows(dsum, dd, at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
+    zero_fields(*P, at<float>(*P, o_f), 9 * nf * 2 * nshots);
+    zero_fields(*P, at<float>(*P, o_grad), nint * nshots);
+    std::vector<KInst> h(2 * nshots);
+    for (int k = 0; k < nshots; k++) {
+        KInst &B = h[k], &A = h[nshots + k];
+        std::memset(&B, 0, sizeof B);
+        std::memset(&A, 0, sizeof A);
+        bind_fields(B, at<float>(*P, o_f) + 9 * nf * k, nf);
+        bind_fields(A, at<float>(*P, o_f) + 9 * nf * (nshots + k), nf);
+        A.ex = A.p + 7 * nf;
+        A.ez = A.p + 8 * nf;
+        B.wavelet = at<float>(*P, o_wav);
+        B.s_ix = A.s_ix = shots[k].s_ix;
+        B.s_iz = A.s_iz = shots[k].s_iz;
+        if (shots[k].nrec > 0) {
+            A.colmap = at<int>(*P, o_ints) + b.col_off[k];
+            A.rowmap = at<int>(*P, o_ints) + b.row_off[k];
+            A.dsum = dsum + static_cast<size_t>(b.toff[k]) * nt;
+        }
+        A.grad = at<float>(*P, o_grad) + nint * k;
+        A.s0 = at<float>(*P, o_s0) + nint * seg * k;
+    }
+    const KInst *dB = at<KInst>(*P, o_inst), *dA = dB + nshots;
+    const KGrid &g = P->kg;
+    long long fl = 0, bl = 0;
+    // checkpoint pass
+    SWD_TRY(upload_insts(*P, h, o_inst));
+    stats_begin(*P, 0);
+    for (int it = 0; it < nt; it++) {
+        if (it % seg == 0)
+            for (int k = 0; k < nshots; k++)
+                HIP_TRY(hipMemcpyAsync(at<float>(*P, o_snap) + 7 * nf * (static_cast<size_t>(nseg) * k + it / seg), h[k].p,
+                                       sizeof(float) * 7 * nf, hipMemcpyDeviceToDevice, P->stream));
+        HIP_TRY(launch_v(g, dB, nshots, false, 3, P->stream));
+        HIP_TRY(launch_p(g, dB, nshots, false, it, P->stream));
+        fl += 2;
+    }
+    // reverse segments: recompute the background sums, then the adjoint steps
+    for (int s = nseg - 1; s >= 0; s--) {
+        const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
+        for (int k = 0; k < nshots; k++) {
+            HIP_TRY(hipMemcpyAsync(h[k].p, at<float>(*P, o_snap) + 7 * nf * (static_cast<size_t>(nseg) * k + s),
+                                   sizeof(float) * 7 * nf, hipMemcpyDeviceToDevice, P->stream));
+            h[k].s0_out = at<float>(*P, o_s0) + nint * seg * k;
+            h[k].s0_t0 = t0;
+            h[nshots + k].s0_t0 = t0;
+        }
+        SWD_TRY(upload_insts(*P, h, o_inst));
+        for (int it = t0; it < t1; it++) {
+            HIP_TRY(launch_v(g, dB, nshots, false, 3, P->stream));
+            HIP_TRY(launch_p(g, dB, nshots, false, it, P->stream));
+            fl += 2;
+        }
+        for (int it = t1 - 1; it >= t0; it--) {
+            HIP_TRY(launch_adjoint_step(g, dA, nshots, it, it == nt - 1, P->stream));
+            bl += it == nt - 1 ? 1 : 4;
+        }
+    }
+    P->stats.fwd_ms = stats_end(*P, 0);
+    P->stats.fwd_launches = fl;
+    P->stats.bwd_launches = bl;
+    std::vector<float> gr(nint * nshots);
+    HIP_TRY(hipMemcpyAsync(gr.data(), at<float>(*P, o_grad), sizeof(float) * nint * nshots, hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    for (size_t i = 0; i < nint; i++) {
+        float acc = gr[i];
+        for (int k = 1; k < nshots; k++) acc += gr[nint * k + i];
+        m_grad[i] = acc;
+    }
+    return SWD_OK;
+}

@@ -1,0 +1,341 @@
+"""Python harness over libswd's C-ABI (include/swd.h).
+
+Thin ctypes bindings: host numpy arrays in, numpy arrays out, plus device-pointer
+entry points for callers that keep data in HBM (torch tensors on ``cuda``).
+The compute path is the HIP library; there is no CPU fallback, and loading
+fails loudly when ``lib/libswd.so`` has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libswd.so")
+
+SWD_PATH_PLAIN = 0
+SWD_PATH_FUSED = 1
+ERRORS = {-1: "SWD_E_ARG", -2: "SWD_E_DEVICE", -3: "SWD_E_NOMEM", -4: "SWD_E_IO", -5: "SWD_E_STATE",
+          -6: "SWD_E_RANGE"}
+
+
+class SwdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("pml", C.c_int), ("Lc", C.c_int),
+                ("dx", C.c_float), ("dz", C.c_float), ("dt", C.c_float), ("nt", C.c_int)]
+
+    @property
+    def pmlc(self):
+        return self.pml + self.Lc
+
+    @property
+    def ntx(self):
+        return self.nx + 2 * self.pmlc
+
+    @property
+    def ntz(self):
+        return self.nz + 2 * self.pmlc
+
+
+class Coefs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("rc", "a_x", "a_x_half", "b_x", "b_x_half",
+                                          "a_z", "a_z_half", "b_z", "b_z_half")]
+
+
+class Shot(C.Structure):
+    _fields_ = [("s_ix", C.c_int), ("s_iz", C.c_int), ("nrec", C.c_int),
+                ("rec_ix", C.c_void_p), ("rec_iz", C.c_void_p)]
+
+
+class Params(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("pml", C.c_int), ("Lc", C.c_int),
+                ("dx", C.c_float), ("dz", C.c_float), ("itmax", C.c_int), ("dt", C.c_float),
+                ("f0", C.c_float), ("ns", C.c_int), ("sx0", C.c_float), ("shotdx", C.c_float),
+                ("shotdep", C.c_float), ("recdix", C.c_float), ("offsetz", C.c_float),
+                ("scatter_flag", C.c_int), ("swin", C.c_int), ("window", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("fwd_ms", C.c_double), ("bwd_ms", C.c_double), ("fft_ms", C.c_double),
+                ("fwd_launches", C.c_longlong), ("bwd_launches", C.c_longlong),
+                ("fwd_cells", C.c_longlong), ("bwd_cells", C.c_longlong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd_cal_xishu", "swd_ricker",
+           "swd_cpml", "swd_model_in_pml", "swd_read_model", "swd_smooth_model", "swd_nfft", "swd_kz_taper",
+           "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
+           "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
+           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_get_stats", "swd_forward", "swd_afor",
+           "swd_rtm", "swd_rtm_dev", "swd_born", "swd_born_adjoint")
+
+
+def load():
+    """Loads lib/libswd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libswd.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    load().swd_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc):
+    if rc < 0:
+        raise SwdError(rc, last_error())
+    return rc
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- setup (host)
+def device_count():
+    n = C.c_int(0)
+    rc = load().swd_get_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def read_parameters(path):
+    p = Params()
+    check(load().swd_read_parameters(path.encode(), C.byref(p)))
+    return {k: getattr(p, k) for k, _ in Params._fields_}
+
+
+def cal_xishu(Lc):
+    rc = np.zeros(Lc, np.float32)
+    check(load().swd_cal_xishu(Lc, _p(rc)))
+    return rc
+
+
+def ricker(nt, f0, t0, dt):
+    r = np.zeros(nt, np.float32)
+    check(load().swd_ricker(_p(r), nt, C.c_float(f0), C.c_float(t0), C.c_float(dt)))
+    return r
+
+
+def cpml(ntx, ntz, pml_arg, dx, dz, dt):
+    xs = [np.zeros(ntx, np.float32) for _ in range(4)]
+    zs = [np.zeros(ntz, np.float32) for _ in range(4)]
+    check(load().swd_cpml(ntx, ntz, pml_arg, C.c_float(dx), C.c_float(dz), C.c_float(dt),
+                          *[_p(a) for a in xs + zs]))
+    keys = ("a_x", "a_x_half", "b_x", "b_x_half", "a_z", "a_z_half", "b_z", "b_z_half")
+    return dict(zip(keys, xs + zs))
+
+
+def coefs_for(grid):
+    """FD + CPML coefficients of a grid as the reference driver computes them
+    (CPML thickness pml+Lc, RC:462-468)."""
+    c = cpml(grid.ntx, grid.ntz, grid.pmlc, grid.dx, grid.dz, grid.dt)
+    c["rc"] = cal_xishu(grid.Lc)
+    return c
+
+
+def model_in_pml(vp, ntz, ntx, pml):
+    v = f32(vp).reshape(-1).copy()
+    check(load().swd_model_in_pml(_p(v), ntz, ntx, pml))
+    return v
+
+
+def read_model(path, ntx, ntz, pml):
+    vp = np.zeros(ntx * ntz, np.float32)
+    rho = np.zeros(ntx * ntz, np.float32)
+    check(load().swd_read_model(path.encode(), _p(vp), _p(rho), ntx, ntz, pml))
+    return vp, rho
+
+
+def smooth_model(vp, ntz, ntx, pml, window):
+    v = f32(vp).reshape(-1).copy()
+    check(load().swd_smooth_model(_p(v), ntz, ntx, pml, window))
+    return v
+
+
+def nfft(n):
+    return check(load().swd_nfft(n))
+
+
+def kz_taper(nfftz, dz):
+    c = np.zeros(nfftz, np.float32)
+    layerz = check(load().swd_kz_taper(_p(c), nfftz, C.c_float(dz)))
+    return c, layerz
+
+
+def hilbert1d(rick):
+    r = f32(rick)
+    h = np.zeros_like(r)
+    check(load().swd_hilbert1d(_p(r), _p(h), r.size))
+    return h
+
+
+def geometry(nx, nz, pml, Lc, dx, dz, sx0, shotdx, shotdep, offsetz, is_):
+    L = load()
+    s_ix, s_iz = C.c_int(), C.c_int()
+    f = C.c_float
+    rn = check(L.swd_geometry(nx, nz, pml, Lc, f(dx), f(dz), f(sx0), f(shotdx), f(shotdep), f(offsetz), is_,
+                              C.byref(s_ix), C.byref(s_iz), None, 0))
+    r_iz = np.zeros(max(rn, 1), np.int32)
+    check(L.swd_geometry(nx, nz, pml, Lc, f(dx), f(dz), f(sx0), f(shotdx), f(shotdep), f(offsetz), is_,
+                         C.byref(s_ix), C.byref(s_iz), _p(r_iz), rn))
+    return s_ix.value, s_iz.value, r_iz[:rn]
+
+
+def adjoint_source(d, dt):
+    d = f32(d)
+    out = np.zeros_like(d)
+    check(load().swd_adjoint_source(_p(d), _p(out), d.shape[0], d.shape[1], C.c_float(dt)))
+    return out
+
+
+def scatterers(rho, vp, ntx, ntz, pmlc, swin):
+    r = f32(rho).reshape(-1).copy()
+    check(load().swd_scatterers(_p(r), _p(f32(vp).reshape(-1)), ntx, ntz, pmlc, swin))
+    return r
+
+
+def image_normalize(image, illum):
+    im = f32(image).reshape(-1).copy()
+    check(load().swd_image_normalize(_p(im), _p(f32(illum).reshape(-1)), im.size))
+    return im
+
+
+def image_laplacian(image, vp, ntx, ntz, pmlc, dx, dz, acc=None):
+    out = np.zeros((ntx - 2 * pmlc) * (ntz - 2 * pmlc), np.float32)
+    check(load().swd_image_laplacian(_p(f32(image).reshape(-1)), _p(f32(vp).reshape(-1)), _p(out), _p(acc),
+                                     ntx, ntz, pmlc, C.c_float(dx), C.c_float(dz)))
+    return out
+
+
+def edge_zero(mig, nx, nz, Lc):
+    m = f32(mig).reshape(-1).copy()
+    check(load().swd_edge_zero(_p(m), nx, nz, Lc))
+    return m
+
+
+# ---------------------------------------------------------------- shots
+class ShotSet:
+    """A batch of shots: source nodes and receiver lists in padded-grid indices."""
+
+    def __init__(self, shots):
+        self._keep = []
+        arr = (Shot * len(shots))()
+        for k, (s_ix, s_iz, rec_ix, rec_iz) in enumerate(shots):
+            rx, rz = i32(rec_ix).reshape(-1), i32(rec_iz).reshape(-1)
+            assert rx.size == rz.size
+            self._keep += [rx, rz]
+            arr[k] = Shot(int(s_ix), int(s_iz), rx.size, rx.ctypes.data if rx.size else None,
+                          rz.ctypes.data if rz.size else None)
+        self.arr = arr
+        self.n = len(shots)
+        self.ntr = sum(int(i32(s[2]).size) for s in shots)
+
+
+# ---------------------------------------------------------------- plan
+class Plan:
+    """A device plan: grid, coefficients and model resident on one GPU."""
+
+    def __init__(self, grid, coefs, vp, rho, device=0, path=SWD_PATH_FUSED):
+        self.L = load()
+        self.grid = grid
+        h = C.c_void_p()
+        check(self.L.swd_plan_create(C.byref(grid), device, C.byref(h)))
+        self.h = h
+        self._coefs = {k: f32(v) for k, v in coefs.items()}
+        c = Coefs(*[self._coefs[k].ctypes.data for k in ("rc", "a_x", "a_x_half", "b_x", "b_x_half",
+                                                          "a_z", "a_z_half", "b_z", "b_z_half")])
+        check(self.L.swd_plan_set_coefs(self.h, C.byref(c)))
+        self.set_model(vp, rho)
+        self.set_path(path)
+
+    def set_model(self, vp, rho):
+        vp, rho = f32(vp).reshape(-1), f32(rho).reshape(-1)
+        assert vp.size == self.grid.ntx * self.grid.ntz == rho.size
+        check(self.L.swd_plan_set_model(self.h, _p(vp), _p(rho)))
+
+    def set_path(self, path):
+        check(self.L.swd_plan_set_path(self.h, int(path)))
+
+    def close(self):
+        if self.h:
+            self.L.swd_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = Stats()
+        check(self.L.swd_plan_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def forward(self, shots, wavelet):
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        g = np.zeros((S.ntr, self.grid.nt), np.float32)
+        check(self.L.swd_forward(self.h, S.n, S.arr, _p(f32(wavelet)), _p(g)))
+        return g
+
+    def afor(self, shots, wavelet, hwavelet, kzcoef):
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        kz = f32(kzcoef)
+        outs = [np.zeros((S.ntr, self.grid.nt), np.float32) for _ in range(3)]
+        check(self.L.swd_afor(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(hwavelet)), _p(kz), kz.size,
+                              *[_p(o) for o in outs]))
+        return outs
+
+    def rtm(self, shots, wavelet, adj, want_gather=False):
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        ntp = self.grid.ntx * self.grid.ntz
+        outs = [np.zeros((S.n, ntp), np.float32) for _ in range(3)]
+        gs = np.zeros((S.ntr, self.grid.nt), np.float32) if want_gather else None
+        check(self.L.swd_rtm(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(adj)), *[_p(o) for o in outs], _p(gs)))
+        return (*outs, gs)
+
+    def rtm_dev(self, shots, wavelet, adj_ptr, image_ptr, illum_s_ptr, illum_r_ptr, gather_ptr=None):
+        """Device-pointer RTM (pointers are integers, e.g. torch tensor data_ptr())."""
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        vp = lambda x: None if x is None else C.c_void_p(int(x))
+        check(self.L.swd_rtm_dev(self.h, S.n, S.arr, _p(f32(wavelet)), vp(adj_ptr), vp(image_ptr),
+                                 vp(illum_s_ptr), vp(illum_r_ptr), vp(gather_ptr)))
+
+    def born(self, shots, wavelet, m):
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        m = f32(m)
+        assert m.size == S.n * self.grid.nx * self.grid.nz
+        d = np.zeros((S.ntr, self.grid.nt), np.float32)
+        check(self.L.swd_born(self.h, S.n, S.arr, _p(f32(wavelet)), _p(m), _p(d)))
+        return d
+
+    def born_adjoint(self, shots, wavelet, d, nsnap=0):
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        g = np.zeros(self.grid.nx * self.grid.nz, np.float32)
+        check(self.L.swd_born_adjoint(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(d)), _p(g), int(nsnap)))
+        return g

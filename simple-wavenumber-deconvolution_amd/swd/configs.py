@@ -1,0 +1,150 @@
+"""Seeded synthetic workloads of the FDTD/RTM path.
+
+Each builder returns a ``Case``: grid, padded model (vp, rho as [ntx][ntz]),
+FD/CPML coefficients from the library's own setup routines, wavelet and shots
+(padded-grid source node and receiver lists).  Seeds are fixed so every run
+(tests, bench, CPU baseline) sees the same inputs.
+
+Workloads (interior nz x nx):
+  c1  200 x 300, 10 m, dt 1 ms, nt 1000, pml 20, Lc 4: 5 flat layers 1500-3500 m/s
+      times (1 + 0.03 g), g a sigma=5-cell Gaussian-smoothed N(0,1) field scaled to
+      max|g| = 1 (seed 1); rho 1000; 25 Hz Ricker at (iz 2, ix 150); 91 receivers at
+      ix 100, iz 10..190 step 2.
+  c2  502 x 845, 10 m, dt 0.8 ms, nt 4000, pml 40, Lc 4: layered 1500-4500 m/s with
+      8 random dipping reflectors (seed 2); migration model = 15x15 box average;
+      20 Hz Ricker at the surface; 401 receivers at ix 420 spaced 5 m (nearest node).
+"""
+import dataclasses
+
+import numpy as np
+
+from . import Grid, cal_xishu, coefs_for, ricker
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    grid: Grid
+    vp: np.ndarray          # [ntx][ntz]
+    rho: np.ndarray
+    coefs: dict
+    wavelet: np.ndarray
+    shots: list             # [(s_ix, s_iz, rec_ix, rec_iz)]
+    f0: float
+    vp_true: np.ndarray = None
+
+    @property
+    def pmlc(self):
+        return self.grid.pmlc
+
+
+def _gauss_smooth(field, sigma):
+    """Separable Gaussian smoothing with reflecting edges (numpy only)."""
+    r = int(np.ceil(4 * sigma))
+    x = np.arange(-r, r + 1)
+    k = np.exp(-0.5 * (x / sigma) ** 2)
+    k /= k.sum()
+    out = field
+    for ax in (0, 1):
+        pad = [(0, 0), (0, 0)]
+        pad[ax] = (r, r)
+        f = np.pad(out, pad, mode="reflect")
+        out = np.apply_along_axis(lambda v: np.convolve(v, k, mode="valid"), ax, f)
+    return out
+
+
+def pad_model(interior, pmlc):
+    """Interior [nx][nz] -> padded [ntx][ntz] by edge replication (RC:1428-1468)."""
+    return np.pad(np.asarray(interior, np.float32), pmlc, mode="edge")
+
+
+def layered_c1(nx=300, nz=200, seed=1):
+    rng = np.random.default_rng(seed)
+    vel = np.array([1500, 2000, 2500, 3000, 3500], np.float64)
+    layer = np.minimum(np.arange(nz) * 5 // nz, 4)
+    v = np.tile(vel[layer], (nx, 1))
+    g = _gauss_smooth(rng.standard_normal((nx, nz)), 5.0)
+    g /= np.abs(g).max()
+    return (v * (1.0 + 0.03 * g)).astype(np.float32)
+
+
+def make_case(name, nx, nz, pml, Lc, dx, dz, dt, nt, vp_int, f0, shots_int, rho_val=1000.0):
+    """Builds a Case from an interior model and shots in interior coordinates
+    (s_ix, s_iz, rec_ix[], rec_iz[]) which are shifted by pmlc."""
+    g = Grid(nx, nz, pml, Lc, dx, dz, dt, nt)
+    pmlc = g.pmlc
+    vp = pad_model(vp_int, pmlc)
+    rho = np.full_like(vp, rho_val)
+    coefs = coefs_for(g)
+    wav = ricker(nt, f0, 1.0 / f0, dt)
+    shots = [(s_ix + pmlc, s_iz + pmlc, np.asarray(rx, np.int32) + pmlc, np.asarray(rz, np.int32) + pmlc)
+             for (s_ix, s_iz, rx, rz) in shots_int]
+    return Case(name, g, vp, rho, coefs, wav, shots, f0)
+
+
+def c1(nt=1000):
+    vp = layered_c1()
+    rz = np.arange(10, 191, 2)
+    return make_case("c1", 300, 200, 20, 4, 10.0, 10.0, 1e-3, nt, vp, 25.0,
+                     [(150, 2, np.full(rz.size, 100), rz)])
+
+
+def layered_c2(nx=845, nz=502, seed=2):
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz)[None, :] * 10.0
+    x = np.arange(nx)[:, None] * 10.0
+    # smooth background 1500 -> 4500 m/s with depth
+    v = 1500.0 + 3000.0 * (z / (nz * 10.0)) ** 0.8 + 0.0 * x
+    for _ in range(8):
+        z0 = rng.uniform(500, nz * 10.0 - 300)
+        dip = np.tan(np.deg2rad(rng.uniform(-15, 15)))
+        dv = rng.uniform(-250, 350)
+        v = v + dv * (z > z0 + dip * (x - nx * 5.0))
+    return np.clip(v, 1500, 4500).astype(np.float32)
+
+
+def box_smooth(v, w=15):
+    """w x w box average with edge replication (migration model of c2)."""
+    r = w // 2
+    p = np.pad(v.astype(np.float64), r, mode="edge")
+    c = p.cumsum(0).cumsum(1)
+    c = np.pad(c, ((1, 0), (1, 0)))
+    s = c[w:, w:] - c[:-w, w:] - c[w:, :-w] + c[:-w, :-w]
+    return (s / (w * w)).astype(np.float32)
+
+
+def c2_receivers(nrec=401, spacing=5.0, dz=10.0):
+    zr = np.arange(nrec) * spacing
+    return np.floor(zr / dz + 0.5).astype(np.int32)
+
+
+def c2(nt=4000, migration=True, shot_ix=(300,), nx=845, nz=502):
+    vt = layered_c2(nx, nz)
+    vm = box_smooth(vt) if migration else vt
+    rz = c2_receivers()
+    rz = rz[rz < nz]
+    shots = [(s, 0, np.full(rz.size, 420 if nx > 420 else nx // 2), rz) for s in shot_ix]
+    c = make_case("c2", nx, nz, 40, 4, 10.0, 10.0, 8e-4, nt, vm, 20.0, shots)
+    c.vp_true = pad_model(vt, c.pmlc)
+    return c
+
+
+def small(nx=120, nz=80, pml=10, Lc=4, nt=400, seed=7, f0=25.0, nshots=1, dup=False):
+    """Small heterogeneous case for parity tests (oracle runs in well under a second)."""
+    rng = np.random.default_rng(seed)
+    v = np.linspace(1800, 3200, nz)[None, :] * np.ones((nx, 1))
+    v = v * (1.0 + 0.05 * _gauss_smooth(rng.standard_normal((nx, nz)), 3.0) /
+             max(1e-9, np.abs(_gauss_smooth(rng.standard_normal((nx, nz)), 3.0)).max()))
+    shots = []
+    for k in range(nshots):
+        s_ix = int(nx * (k + 1) / (nshots + 1))
+        rz = np.arange(2, nz - 2, 3)
+        rx = np.full(rz.size, nx // 4 + k)
+        if dup:
+            rz = np.concatenate([rz, rz[:5]])
+            rx = np.concatenate([rx, rx[:5]])
+        shots.append((s_ix, 1 + k, rx, rz))
+    c = make_case("small", nx, nz, pml, Lc, 10.0, 10.0, 1e-3, nt, v.astype(np.float32), f0, shots)
+    c.rho = c.rho * (1.0 + 0.1 * (np.arange(c.rho.shape[1])[None, :] % 7 == 0))   # density contrasts too
+    c.rho = c.rho.astype(np.float32)
+    return c

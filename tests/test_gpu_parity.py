@@ -1,0 +1,120 @@
+"""GPU parity of the HIP path against the CPU oracle (fp32 host loops, fp64 twin).
+
+Bar: <= 1e-5 relative L2 against the fp32 restatement of the
+reference's per-cell equations; the fp64 error is printed for every result.
+Tolerances are written in each test.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import swd
+from swd import configs
+from helpers import oracle_problem, rel_l2, report
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def plan_of(case, path=swd.SWD_PATH_FUSED):
+    return swd.Plan(case.grid, case.coefs, case.vp, case.rho, path=path)
+
+
+@pytest.mark.parametrize("path", [swd.SWD_PATH_FUSED, swd.SWD_PATH_PLAIN])
+def test_forward_small(path):
+    c = configs.small()
+    print(f"case small forward path={path}", flush=True)
+    g = plan_of(c, path).forward(c.shots, c.wavelet)
+    P = oracle_problem(c)
+    o32, _, _ = P.forward()
+    o64, _, _ = P.forward(np.float64)
+    e32, _ = report("forward small", g, o32, o64)
+    assert e32 <= TOL
+
+
+@pytest.mark.parametrize("Lc", [1, 2, 5, 8])
+def test_forward_orders(Lc):
+    c = configs.small(Lc=Lc, nt=250)
+    print(f"case small forward Lc={Lc}", flush=True)
+    g = plan_of(c).forward(c.shots, c.wavelet)
+    o32, _, _ = oracle_problem(c).forward()
+    e32, _ = report(f"forward Lc={Lc}", g, o32)
+    assert e32 <= TOL
+
+
+def test_forward_batch_with_duplicate_receivers():
+    c = configs.small(nshots=3, dup=True, nt=300)
+    print("case small batch of 3 shots, duplicate receivers", flush=True)
+    g = plan_of(c).forward(c.shots, c.wavelet)
+    row = 0
+    for k in range(3):
+        o32, _, _ = oracle_problem(c, k).forward()
+        e32, _ = report(f"batch shot {k}", g[row:row + o32.shape[0]], o32)
+        assert e32 <= TOL
+        row += o32.shape[0]
+
+
+def test_forward_c1():
+    c = configs.c1()
+    print("case c1 forward 200x300 nt=1000", flush=True)
+    g = plan_of(c).forward(c.shots, c.wavelet)
+    P = oracle_problem(c)
+    o32, _, _ = P.forward()
+    o64, _, _ = P.forward(np.float64)
+    e32, e64 = report("forward c1", g, o32, o64)
+    assert g.shape == (91, 1000)
+    assert e32 <= TOL and e64 <= TOL
+
+
+@pytest.mark.parametrize("path", [swd.SWD_PATH_FUSED, swd.SWD_PATH_PLAIN])
+def test_rtm_small(path):
+    c = configs.small(nt=300)
+    print(f"case small rtm path={path}", flush=True)
+    P = oracle_problem(c)
+    d, st, fs = P.forward(strips=True, final=True)
+    adj = O.adjoint_source(d, c.grid.dt)
+    img, ils, ilr, gs = plan_of(c, path).rtm(c.shots, c.wavelet, adj, want_gather=True)
+    o = P.rtm_backward(fs, st, adj)
+    d64, st64, fs64 = P.forward(np.float64, strips=True, final=True)
+    o64 = P.rtm_backward(fs64, st64, adj, np.float64)
+    assert rel_l2(gs, d) <= TOL
+    for name, a, b, b64 in (("image", img[0], o[0], o64[0]), ("illum_src", ils[0], o[1], o64[1]),
+                            ("illum_rcv", ilr[0], o[2], o64[2])):
+        e32, _ = report(f"rtm {name}", a, b, b64)
+        assert e32 <= TOL
+
+
+def test_afor_small():
+    c = configs.small(nt=300)
+    print("case small afor", flush=True)
+    nfftz = swd.nfft(c.grid.ntz)
+    kz, _ = swd.kz_taper(nfftz, c.grid.dz)
+    hw = swd.hilbert1d(c.wavelet)
+    g, up, dn = plan_of(c).afor(c.shots, c.wavelet, hw, kz)
+    P = oracle_problem(c)
+    og, ou, od = P.afor(hw, kz)
+    og64, ou64, od64 = P.afor(hw, kz, np.float64)
+    for name, a, b, b64 in (("gather", g, og, og64), ("up", up, ou, ou64), ("down", dn, od, od64)):
+        e32, _ = report(f"afor {name}", a, b, b64)
+        assert e32 <= TOL
+
+
+def test_born_and_adjoint_small():
+    c = configs.small(nt=300)
+    print("case small born", flush=True)
+    rng = np.random.default_rng(3)
+    m = rng.standard_normal(c.grid.nx * c.grid.nz).astype(np.float32)
+    dd = rng.standard_normal((c.shots[0][2].size, c.grid.nt)).astype(np.float32)
+    plan = plan_of(c)
+    d = plan.born(c.shots, c.wavelet, m)
+    g = plan.born_adjoint(c.shots, c.wavelet, dd)
+    P = oracle_problem(c)
+    od = P.born(m)
+    og = P.born_adjoint(dd)
+    e1, _ = report("born", d, od, P.born(m, np.float64))
+    e2, _ = report("born adjoint", g, og, P.born_adjoint(dd, np.float64))
+    assert e1 <= TOL and e2 <= TOL
+    lhs = float(np.dot(d.ravel().astype(np.float64), dd.ravel()))
+    rhs = float(np.dot(m.astype(np.float64), g.astype(np.float64)))
+    print(f"dot test <Lm,d>={lhs:.9e} <m,L^T d>={rhs:.9e} rel {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
