@@ -1,0 +1,210 @@
+"""Benchmark of the RTM hot path on one MI355X.
+
+Workload: the c2 configuration (single-shot VSP RTM at reference scale:
+interior 502 x 845, 10 m, dt 0.8 ms, nt 4000, 40-cell CPML, Lc 4, 401 borehole
+receivers, smoothed migration model) run as a batch of shots from surface
+positions.  One step = one batched RTM pass (forward with boundary strips,
+source reconstruction, receiver back-propagation, imaging) over the batch, with
+the adjoint sources already in HBM.
+
+Prints one JSON line: Gcell-updates/s (cell-update = one leapfrog step of one
+wavefield at one padded node; RTM per shot = nt*N + (nt-1)*(N + N_int)),
+shots/s, the plain one-shot-per-launch path's shots/s, the roofline of the
+dominant kernel and a CPU baseline (the oracle's fp32 host loops with OpenMP,
+on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "simple-wavenumber-deconvolution_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def rtm_bytes_per_step(g, nrec, nframe):
+    """Algorithmic HBM bytes of one forward and one backward RTM step of one shot."""
+    N = g.ntx * g.ntz
+    Nint = g.nx * g.nz
+    P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
+    fwd = 32 * N + 16 * P + 4 * nrec + 4 * nframe
+    bwd = 24 * Nint + 32 * N + 16 * P + 24 * Nint + 4 * nframe + 4 * nrec
+    return fwd, bwd
+
+
+def rtm_cells(g):
+    N = g.ntx * g.ntz
+    return g.nt * N + (g.nt - 1) * (N + g.nx * g.nz)
+
+
+def build_case(nshots, nt):
+    from swd import configs
+
+    nx = 845
+    xs = [int(v) for v in np.linspace(60, nx - 60, nshots)]
+    return configs.c2(nt=nt, shot_ix=xs)
+
+
+def cpu_baseline(case, nt_sample, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    from helpers import oracle_problem
+    import swd
+
+    O.set_threads(threads)
+    g = case.grid
+    sub = swd.configs.Case(case.name, swd.Grid(g.nx, g.nz, g.pml, g.Lc, g.dx, g.dz, g.dt, nt_sample),
+                           case.vp, case.rho, case.coefs, case.wavelet[:nt_sample], case.shots[:1], case.f0)
+    P = oracle_problem(sub)
+    t0 = time.perf_counter()
+    d, st, fs = P.forward(strips=True, final=True)
+    adj = O.adjoint_source(d, g.dt)
+    P.rtm_backward(fs, st, adj)
+    dt = time.perf_counter() - t0
+    cells = rtm_cells(sub.grid)
+    return cells / dt / 1e9, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=16, help="shots per RTM step")
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-nt", type=int, default=600)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU shot sharding is "
+              "design-only in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import swd
+
+    log(f"bench: c2 RTM, batch {a.batch} shots, nt {a.nt}")
+    case = build_case(a.batch, a.nt)
+    g = case.grid
+    plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0)
+    shots = swd.ShotSet(case.shots)
+    # observed data on the true model, up-going integration as the adjoint source (RC:603-637)
+    plan_true = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0)
+    t0 = time.perf_counter()
+    obs = plan_true.forward(shots, case.wavelet)
+    plan_true.close()
+    adj = swd.adjoint_source(obs, g.dt)
+    log(f"bench: observed data {obs.shape} in {time.perf_counter() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    ntp = g.ntx * g.ntz
+    d_adj = torch.from_numpy(adj).to(dev)
+    d_img = torch.zeros((3, a.batch, ntp), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        plan.rtm_dev(shots, case.wavelet, d_adj.data_ptr(), d_img[0].data_ptr(), d_img[1].data_ptr(),
+                     d_img[2].data_ptr())
+
+    for i in range(a.warmup):
+        step()
+        log(f"bench: warmup {i + 1}/{a.warmup} done")
+    torch.cuda.synchronize()
+    fwd_ms = bwd_ms = 0.0
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        step()
+        st = plan.stats()
+        fwd_ms += st["fwd_ms"]
+        bwd_ms += st["bwd_ms"]
+        log(f"bench: step {i + 1}/{a.steps} fwd {st['fwd_ms']:.1f} ms bwd {st['bwd_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / a.steps
+    cells = rtm_cells(g) * a.batch
+    gcells = cells * a.steps / elapsed / 1e9
+    shots_per_s = a.batch * a.steps / elapsed
+    img = d_img[0].cpu().numpy()
+    finite = bool(np.isfinite(img).all())
+
+    # roofline of the dominant kernel (the backward step) from event timing on the plan's stream
+    nrec = len(case.shots[0][2])
+    nframe = 2 * g.Lc * (g.nx + g.nz - 2 * g.Lc)
+    bf, bb = rtm_bytes_per_step(g, nrec, nframe)
+    st = plan.stats()
+    bwd_launch_ms = bwd_ms / a.steps / max(1, st["bwd_launches"])
+    bwd_launches_per_step = st["bwd_launches"] / (g.nt - 1)
+    bytes_per_launch = bb * a.batch / bwd_launches_per_step
+    achieved = bytes_per_launch / (bwd_launch_ms * 1e-3) / 1e9
+    fwd_launch_ms = fwd_ms / a.steps / max(1, st["fwd_launches"])
+    fwd_launches_per_step = st["fwd_launches"] / g.nt
+    fwd_achieved = bf * a.batch / fwd_launches_per_step / (fwd_launch_ms * 1e-3) / 1e9
+
+    plain = None
+    if not a.no_plain:
+        one = swd.ShotSet(case.shots[:1])
+        plan.set_path(swd.SWD_PATH_PLAIN)
+        rows = len(case.shots[0][2])
+        d1 = torch.from_numpy(adj[:rows].copy()).to(dev)
+        im1 = torch.zeros((3, ntp), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plan.rtm_dev(one, case.wavelet, d1.data_ptr(), im1[0].data_ptr(), im1[1].data_ptr(), im1[2].data_ptr())
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t1
+        plain = {"shots_per_s": 1.0 / tp, "gcell_per_s": rtm_cells(g) / tp / 1e9}
+        rel = float(np.linalg.norm(im1[0].cpu().numpy() - img[0]) / max(1e-30, np.linalg.norm(img[0])))
+        plain["image_rel_l2_vs_batched"] = rel
+        plan.set_path(swd.SWD_PATH_FUSED)
+        log(f"bench: plain path one shot {tp:.2f}s, image rel diff vs batched {rel:.2e}")
+
+    cpu = None
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rate, dt = cpu_baseline(case, a.cpu_nt, threads)
+        cpu = {"value": rate, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"oracle fp32 host loops (OpenMP) on one c2 shot with nt={a.cpu_nt} "
+                         f"(forward with strips + backward + imaging), {dt:.1f}s"}
+        log(f"bench: cpu baseline {rate:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
+
+    out = {
+        "metric": "Gcell-updates/s and batched shots/s vs plain GPU path at <=1e-5 rel L2",
+        "value": gcells,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered model with dipping reflectors, c2 geometry)",
+        "config": {"workload": "c2 batched VSP RTM: 502x845 interior (590x933 padded), nt 4000, Lc 4, "
+                               "401 borehole receivers, forward+strips+reconstruction+imaging",
+                   "shots_per_step": a.batch, "nt": g.nt, "padded_grid": [g.ntx, g.ntz],
+                   "shots_per_s": shots_per_s,
+                   "plain_path": plain,
+                   "speedup_vs_plain_shots_per_s": (shots_per_s / plain["shots_per_s"]) if plain else None,
+                   "fwd_ms_per_step": fwd_ms / a.steps, "bwd_ms_per_step": bwd_ms / a.steps,
+                   "fwd_achieved_GBs": fwd_achieved, "image_finite": finite},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "RTM backward step", "bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": bwd_launch_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
