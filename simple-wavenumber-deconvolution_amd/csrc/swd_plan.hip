@@ -319,7 +319,7 @@ extern "C" int swd_plan_set_model(void *plan, const float *vp, const float *rho)
 
 extern "C" int swd_plan_set_path(void *plan, int path)
 {
-    if (!plan || (path != SWD_PATH_PLAIN && path != SWD_PATH_FUSED)) return swd_fail(SWD_E_ARG, "swd_plan_set_path: bad path");
+    if (!plan || (path != SWD_PATH_PLAIN && path != SWD_PATH_FUSED && path != SWD_PATH_BATCHED)) return swd_fail(SWD_E_ARG, "swd_plan_set_path: bad path");
     static_cast<Plan *>(plan)->path = path;
     return SWD_OK;
 }
@@ -350,9 +350,16 @@ int upload_insts(Plan &P, const std::vector<KInst> &h, size_t o_inst)
     return SWD_OK;
 }
 
-// Forward time loop over instance groups [first, first+count) of d_inst.
-// Groups of one shot each on the plain path, one group on the fused path.
-int forward_loop(Plan &P, const KInst *d_inst, const std::vector<KInst> &h_inst, int per_shot, int nshots)
+// Arrays per wavefield: p, vx, vz, psx, psz, pvx, pvz (primary set) and
+// p2, vx2, vz2, psx2, psz2 (second set of the double-buffered fused step).
+constexpr int kFieldArrays = 12;
+
+// Forward time loop of nshots shots with per_shot wavefields each.  Plain
+// path: shot after shot, one kernel per sub-step.  Fused path: every shot in
+// each launch, one launch per step.  *final_set tells which buffer set holds
+// the state after the last step (0 = primary, 1 = second).
+int forward_loop(Plan &P, const KInst *d_inst, const std::vector<KInst> &h_inst, int per_shot, int nshots,
+                 size_t o_tab, int *final_set)
 {
     const KGrid &g = P.kg;
     long long launches = 0;
@@ -367,9 +374,18 @@ int forward_loop(Plan &P, const KInst *d_inst, const std::vector<KInst> &h_inst,
                 launches += 3;
             }
         }
+        *final_set = 0;
+    } else if (P.path == SWD_PATH_BATCHED) {
+        for (int it = 0; it < g.nt; it++) {
+            HIP_TRY(launch_v(g, d_inst, per_shot * nshots, false, 3, P.stream));
+            HIP_TRY(launch_p(g, d_inst, per_shot * nshots, false, it, P.stream));
+            launches += 2;
+        }
+        *final_set = 0;
     } else {
-        const int n = per_shot * nshots;
-        SWD_TRY(fused_forward_loop(P.kg, d_inst, h_inst.data(), n, P.stream, &launches));
+        SWD_TRY(fused_build_table(h_inst.data(), per_shot * nshots, per_shot, P.ws + o_tab, P.stream));
+        SWD_TRY(fused_forward_steps(g, P.ws + o_tab, nshots, per_shot, 0, g.nt, P.stream, &launches));
+        *final_set = g.nt & 1;
     }
     P.stats.fwd_ms = stats_end(P, 0);
     P.stats.fwd_launches = launches;
@@ -383,6 +399,8 @@ void bind_fields(KInst &I, float *f, size_t nfield)
 {
     I.p = f; I.vx = f + nfield; I.vz = f + 2 * nfield;
     I.psx = f + 3 * nfield; I.psz = f + 4 * nfield; I.pvx = f + 5 * nfield; I.pvz = f + 6 * nfield;
+    I.p2 = f + 7 * nfield; I.vx2 = f + 8 * nfield; I.vz2 = f + 9 * nfield;
+    I.psx2 = f + 10 * nfield; I.psz2 = f + 11 * nfield;
 }
 
 }  // namespace
@@ -401,16 +419,17 @@ int forward_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet
     const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
     const size_t o_inst = c.add<KInst>(nshots);
     const size_t o_wav = c.add<float>(nt);
-    const size_t o_f = c.add<float>(7 * P.nfield * nshots);
+    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
+    const size_t o_f = c.add<float>(kFieldArrays * P.nfield * nshots);
     SWD_TRY(ensure_ws(P, c.off));
     SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
-    zero_fields(P, at<float>(P, o_f), 7 * P.nfield * nshots);
+    zero_fields(P, at<float>(P, o_f), kFieldArrays * P.nfield * nshots);
     std::vector<KInst> h(nshots);
     for (int k = 0; k < nshots; k++) {
         KInst &I = h[k];
         std::memset(&I, 0, sizeof I);
-        bind_fields(I, at<float>(P, o_f) + 7 * P.nfield * k, P.nfield);
+        bind_fields(I, at<float>(P, o_f) + kFieldArrays * P.nfield * k, P.nfield);
         I.wavelet = at<float>(P, o_wav);
         I.s_ix = shots[k].s_ix; I.s_iz = shots[k].s_iz;
         if (shots[k].nrec > 0) {
@@ -420,7 +439,8 @@ int forward_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet
         }
     }
     SWD_TRY(upload_insts(P, h, o_inst));
-    SWD_TRY(forward_loop(P, at<KInst>(P, o_inst), h, 1, nshots));
+    int fs = 0;
+    SWD_TRY(forward_loop(P, at<KInst>(P, o_inst), h, 1, nshots, o_tab, &fs));
     HIP_TRY(launch_copy_rows(d_gather, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
     HIP_TRY(hipStreamSynchronize(P.stream));
     return SWD_OK;
@@ -511,7 +531,8 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     const size_t o_coef = c.add<float>(nfftz);
     const size_t o_gat = c.add<float>(3 * std::max<size_t>(ng, 1));
     const size_t o_cap = c.add<float2>(static_cast<size_t>(nt) * ncap * nfftz);
-    const size_t o_f = c.add<float>(7 * P->nfield * 2 * nshots);
+    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
+    const size_t o_f = c.add<float>(kFieldArrays * P->nfield * 2 * nshots);
     SWD_TRY(ensure_ws(*P, c.off));
     SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P->ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
@@ -524,14 +545,14 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     HIP_TRY(hipMemcpyAsync(P->ws + o_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice, P->stream));
     HIP_TRY(hipMemcpyAsync(P->ws + o_coef, kzcoef, sizeof(float) * nfftz, hipMemcpyHostToDevice, P->stream));
     HIP_TRY(hipMemsetAsync(P->ws + o_cap, 0, sizeof(float2) * static_cast<size_t>(nt) * ncap * nfftz, P->stream));
-    zero_fields(*P, at<float>(*P, o_f), 7 * P->nfield * 2 * nshots);
+    zero_fields(*P, at<float>(*P, o_f), kFieldArrays * P->nfield * 2 * nshots);
     float *d_gather = at<float>(*P, o_gat), *d_up = d_gather + ng, *d_down = d_up + ng;
     std::vector<KInst> h(2 * nshots);
     for (int k = 0; k < nshots; k++)
         for (int w = 0; w < 2; w++) {
             KInst &I = h[2 * k + w];
             std::memset(&I, 0, sizeof I);
-            bind_fields(I, at<float>(*P, o_f) + 7 * P->nfield * (2 * k + w), P->nfield);
+            bind_fields(I, at<float>(*P, o_f) + kFieldArrays * P->nfield * (2 * k + w), P->nfield);
             I.wavelet = at<float>(*P, o_wav) + w * nt;   // p: rick, htp: hrick (FK:707, 747)
             I.s_ix = shots[k].s_ix; I.s_iz = shots[k].s_iz;
             if (w == 0 && shots[k].nrec > 0) {
@@ -548,7 +569,8 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
             }
         }
     SWD_TRY(upload_insts(*P, h, o_inst));
-    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), h, 2, nshots));
+    int fs = 0;
+    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), h, 2, nshots, o_tab, &fs));
     HIP_TRY(launch_copy_rows(d_gather, at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
     if (!cols.empty()) {
         stats_begin(*P, 1);
@@ -581,11 +603,12 @@ extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const 
     const size_t o_wav = c.add<float>(nt);
     const size_t o_strips = c.add<float>(static_cast<size_t>(nt) * P->nframe * nshots);
     const size_t o_img = c.add<float>(4 * nf * nshots);        // image, illum_s, illum_r, dpdt
-    const size_t o_f = c.add<float>(7 * nf * 2 * nshots);       // source-side then receiver-side fields
+    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
+    const size_t o_f = c.add<float>(kFieldArrays * nf * 2 * nshots);   // source-side then receiver-side fields
     SWD_TRY(ensure_ws(*P, c.off));
     SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P->ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
-    zero_fields(*P, at<float>(*P, o_f), 7 * nf * 2 * nshots);
+    zero_fields(*P, at<float>(*P, o_f), kFieldArrays * nf * 2 * nshots);
     zero_fields(*P, at<float>(*P, o_img), 4 * nf * nshots);
     std::vector<KInst> h(2 * nshots);
     float *fields = at<float>(*P, o_f), *imgs = at<float>(*P, o_img);
@@ -594,8 +617,8 @@ extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const 
         KInst &R = h[nshots + k];   // receiver side
         std::memset(&S, 0, sizeof S);
         std::memset(&R, 0, sizeof R);
-        bind_fields(S, fields + 7 * nf * k, nf);
-        bind_fields(R, fields + 7 * nf * (nshots + k), nf);
+        bind_fields(S, fields + kFieldArrays * nf * k, nf);
+        bind_fields(R, fields + kFieldArrays * nf * (nshots + k), nf);
         S.wavelet = at<float>(*P, o_wav);
         S.s_ix = R.s_ix = shots[k].s_ix;
         S.s_iz = R.s_iz = shots[k].s_iz;
@@ -613,7 +636,17 @@ extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const 
     }
     SWD_TRY(upload_insts(*P, h, o_inst));
     const KInst *dS = at<KInst>(*P, o_inst), *dR = dS + nshots;
-    SWD_TRY(forward_loop(*P, dS, h, 1, nshots));
+    int fs = 0;
+    SWD_TRY(forward_loop(*P, dS, h, 1, nshots, o_tab, &fs));
+    if (fs == 1) {
+        // the final forward state (RK:403-415) is in the second buffer set
+        for (int k = 0; k < nshots; k++) {
+            std::swap(h[k].p, h[k].p2);
+            std::swap(h[k].vx, h[k].vx2);
+            std::swap(h[k].vz, h[k].vz2);
+        }
+        SWD_TRY(upload_insts(*P, h, o_inst));
+    }
     if (gather_syn)
         HIP_TRY(launch_copy_rows(gather_syn, at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
     // backward: it = nt-2 .. 0 (RK:1073)
@@ -689,19 +722,20 @@ extern "C" int swd_born(void *plan, int nshots, const swd_shot *shots, const flo
     const size_t o_wav = c.add<float>(nt);
     const size_t o_m = c.add<float>(nint * nshots);
     const size_t o_d = c.add<float>(std::max<size_t>(ng, 1));
-    const size_t o_f = c.add<float>(7 * nf * 2 * nshots);
+    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
+    const size_t o_f = c.add<float>(kFieldArrays * nf * 2 * nshots);
     SWD_TRY(ensure_ws(*P, c.off));
     SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P->ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
     HIP_TRY(hipMemcpyAsync(P->ws + o_m, m, sizeof(float) * nint * nshots, hipMemcpyHostToDevice, P->stream));
-    zero_fields(*P, at<float>(*P, o_f), 7 * nf * 2 * nshots);
+    zero_fields(*P, at<float>(*P, o_f), kFieldArrays * nf * 2 * nshots);
     std::vector<KInst> h(2 * nshots);
     for (int k = 0; k < nshots; k++) {
         KInst &B = h[2 * k], &U = h[2 * k + 1];
         std::memset(&B, 0, sizeof B);
         std::memset(&U, 0, sizeof U);
-        bind_fields(B, at<float>(*P, o_f) + 7 * nf * 2 * k, nf);
-        bind_fields(U, at<float>(*P, o_f) + 7 * nf * (2 * k + 1), nf);
+        bind_fields(B, at<float>(*P, o_f) + kFieldArrays * nf * 2 * k, nf);
+        bind_fields(U, at<float>(*P, o_f) + kFieldArrays * nf * (2 * k + 1), nf);
         B.wavelet = at<float>(*P, o_wav);
         B.s_ix = U.s_ix = shots[k].s_ix;
         B.s_iz = U.s_iz = shots[k].s_iz;
@@ -714,7 +748,8 @@ extern "C" int swd_born(void *plan, int nshots, const swd_shot *shots, const flo
         }
     }
     SWD_TRY(upload_insts(*P, h, o_inst));
-    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), h, 2, nshots));
+    int fs = 0;
+    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), h, 2, nshots, o_tab, &fs));
     HIP_TRY(launch_copy_rows(at<float>(*P, o_d), at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
     HIP_TRY(hipMemcpyAsync(d, at<float>(*P, o_d), sizeof(float) * ng, hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
@@ -743,7 +778,9 @@ extern "C" int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, c
     const size_t o_grad = c.add<float>(nint * nshots);
     const size_t o_s0 = c.add<float>(nint * seg * nshots);
     const size_t o_snap = c.add<float>(7 * nf * static_cast<size_t>(nseg) * nshots);
-    const size_t o_f = c.add<float>(9 * nf * 2 * nshots);   // background (7) + adjoint (7) + ex, ez
+    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
+    const size_t per_a = kFieldArrays + 2;   // adjoint fields + ex, ez scratch
+    const size_t o_f = c.add<float>((kFieldArrays + per_a) * nf * nshots);
     SWD_TRY(ensure_ws(*P, c.off));
     SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P->ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
@@ -751,17 +788,18 @@ extern "C" int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, c
     HIP_TRY(hipMemcpyAsync(dd, d, sizeof(float) * ng, hipMemcpyHostToDevice, P->stream));
     HIP_TRY(hipMemcpyAsync(dsum, dd, sizeof(float) * ng, hipMemcpyDeviceToDevice, P->stream));
     HIP_TRY(launch_sum_rows(dsum, dd, at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
-    zero_fields(*P, at<float>(*P, o_f), 9 * nf * 2 * nshots);
+    zero_fields(*P, at<float>(*P, o_f), (kFieldArrays + per_a) * nf * nshots);
     zero_fields(*P, at<float>(*P, o_grad), nint * nshots);
     std::vector<KInst> h(2 * nshots);
+    float *fb = at<float>(*P, o_f), *fa = fb + kFieldArrays * nf * nshots;
     for (int k = 0; k < nshots; k++) {
         KInst &B = h[k], &A = h[nshots + k];
         std::memset(&B, 0, sizeof B);
         std::memset(&A, 0, sizeof A);
-        bind_fields(B, at<float>(*P, o_f) + 9 * nf * k, nf);
-        bind_fields(A, at<float>(*P, o_f) + 9 * nf * (nshots + k), nf);
-        A.ex = A.p + 7 * nf;
-        A.ez = A.p + 8 * nf;
+        bind_fields(B, fb + kFieldArrays * nf * k, nf);
+        bind_fields(A, fa + per_a * nf * k, nf);
+        A.ex = A.p + kFieldArrays * nf;
+        A.ez = A.ex + nf;
         B.wavelet = at<float>(*P, o_wav);
         B.s_ix = A.s_ix = shots[k].s_ix;
         B.s_iz = A.s_iz = shots[k].s_iz;
@@ -775,35 +813,49 @@ extern "C" int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, c
     }
     const KInst *dB = at<KInst>(*P, o_inst), *dA = dB + nshots;
     const KGrid &g = P->kg;
+    const bool fused = P->path == SWD_PATH_FUSED;
     long long fl = 0, bl = 0;
+    // the plain path runs the background of all shots in each launch here too
+    // background state before step it: buffer set it&1 on the fused path (p..psz),
+    // pvx/pvz in place; a snapshot holds p, vx, vz, psx, psz, pvx, pvz
+    auto set_of = [&](int k, int it) -> float * { return (fused && (it & 1)) ? h[k].p2 : h[k].p; };
+    auto snap = [&](int k, int s) -> float * { return at<float>(*P, o_snap) + 7 * nf * (static_cast<size_t>(nseg) * k + s); };
+    auto steps = [&](int it0, int it1) -> int {
+        if (fused) return fused_forward_steps(g, P->ws + o_tab, nshots, 1, it0, it1, P->stream, &fl);
+        for (int it = it0; it < it1; it++) {
+            HIP_TRY(launch_v(g, dB, nshots, false, 3, P->stream));
+            HIP_TRY(launch_p(g, dB, nshots, false, it, P->stream));
+            fl += 2;
+        }
+        return SWD_OK;
+    };
     // checkpoint pass
     SWD_TRY(upload_insts(*P, h, o_inst));
+    if (fused) SWD_TRY(fused_build_table(h.data(), nshots, 1, P->ws + o_tab, P->stream));
     stats_begin(*P, 0);
-    for (int it = 0; it < nt; it++) {
-        if (it % seg == 0)
-            for (int k = 0; k < nshots; k++)
-                HIP_TRY(hipMemcpyAsync(at<float>(*P, o_snap) + 7 * nf * (static_cast<size_t>(nseg) * k + it / seg), h[k].p,
-                                       sizeof(float) * 7 * nf, hipMemcpyDeviceToDevice, P->stream));
-        HIP_TRY(launch_v(g, dB, nshots, false, 3, P->stream));
-        HIP_TRY(launch_p(g, dB, nshots, false, it, P->stream));
-        fl += 2;
+    for (int s = 0; s < nseg; s++) {
+        const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
+        for (int k = 0; k < nshots; k++) {
+            HIP_TRY(hipMemcpyAsync(snap(k, s), set_of(k, t0), sizeof(float) * 5 * nf, hipMemcpyDeviceToDevice, P->stream));
+            HIP_TRY(hipMemcpyAsync(snap(k, s) + 5 * nf, h[k].pvx, sizeof(float) * 2 * nf, hipMemcpyDeviceToDevice,
+                                   P->stream));
+        }
+        SWD_TRY(steps(t0, t1));
     }
     // reverse segments: recompute the background sums, then the adjoint steps
     for (int s = nseg - 1; s >= 0; s--) {
         const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
         for (int k = 0; k < nshots; k++) {
-            HIP_TRY(hipMemcpyAsync(h[k].p, at<float>(*P, o_snap) + 7 * nf * (static_cast<size_t>(nseg) * k + s),
-                                   sizeof(float) * 7 * nf, hipMemcpyDeviceToDevice, P->stream));
+            HIP_TRY(hipMemcpyAsync(set_of(k, t0), snap(k, s), sizeof(float) * 5 * nf, hipMemcpyDeviceToDevice, P->stream));
+            HIP_TRY(hipMemcpyAsync(h[k].pvx, snap(k, s) + 5 * nf, sizeof(float) * 2 * nf, hipMemcpyDeviceToDevice,
+                                   P->stream));
             h[k].s0_out = at<float>(*P, o_s0) + nint * seg * k;
             h[k].s0_t0 = t0;
             h[nshots + k].s0_t0 = t0;
         }
         SWD_TRY(upload_insts(*P, h, o_inst));
-        for (int it = t0; it < t1; it++) {
-            HIP_TRY(launch_v(g, dB, nshots, false, 3, P->stream));
-            HIP_TRY(launch_p(g, dB, nshots, false, it, P->stream));
-            fl += 2;
-        }
+        if (fused) SWD_TRY(fused_build_table(h.data(), nshots, 1, P->ws + o_tab, P->stream));
+        SWD_TRY(steps(t0, t1));
         for (int it = t1 - 1; it >= t0; it--) {
             HIP_TRY(launch_adjoint_step(g, dA, nshots, it, it == nt - 1, P->stream));
             bl += it == nt - 1 ? 1 : 4;

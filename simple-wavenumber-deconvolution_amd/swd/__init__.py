@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libswd.so")
 
 SWD_PATH_PLAIN = 0
 SWD_PATH_FUSED = 1
+SWD_PATH_BATCHED = 2
 ERRORS = {-1: "SWD_E_ARG", -2: "SWD_E_DEVICE", -3: "SWD_E_NOMEM", -4: "SWD_E_IO", -5: "SWD_E_STATE",
           -6: "SWD_E_RANGE"}
 
