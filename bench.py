@@ -164,7 +164,7 @@ def main():
         plain = {"shots_per_s": 1.0 / tp, "gcell_per_s": rtm_cells(g) / tp / 1e9}
         rel = float(np.linalg.norm(im1[0].cpu().numpy() - img[0]) / max(1e-30, np.linalg.norm(img[0])))
         plain["image_rel_l2_vs_batched"] = rel
-        plan.set_path(swd.SWD_PATH_FUSED)
+        plan.set_path(swd.SWD_PATH_BATCHED)
         log(f"bench: plain path one shot {tp:.2f}s, image rel diff vs batched {rel:.2e}")
 
     cpu = None

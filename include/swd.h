@@ -81,8 +81,7 @@ typedef struct swd_params {
 
 /* Execution path of a plan. */
 #define SWD_PATH_PLAIN 0   /* one kernel per sub-step (vx, vz, p), one shot per launch */
-#define SWD_PATH_FUSED 1   /* one fused launch per time step for the whole batch (default) */
-#define SWD_PATH_BATCHED 2 /* one kernel per sub-step, whole batch per launch */
+#define SWD_PATH_BATCHED 1 /* velocity kernel + pressure kernel per step, whole batch per launch (default) */
 
 typedef struct swd_stats {
     double fwd_ms;          /* event time of the last forward time loop (all shots of the call) */

@@ -19,8 +19,6 @@ struct KInst {
     float *p, *vx, *vz;          // state [ntx][ldz]
     float *psx, *psz;            // CPML memory of dp/dx, dp/dz (on vx, vz)
     float *pvx, *pvz;            // CPML memory of dvx/dx, dvz/dz (on p)
-    float *p2, *vx2, *vz2;       // second buffer set of the fused (double-buffered) step
-    float *psx2, *psz2;
     const float *wavelet;        // additive source [nt] (forward) or subtracted (reconstruction); may be null
     int s_ix, s_iz;
     const int *colmap;           // [ntx]: offset of the column's row map in rowmap, or -1

@@ -57,46 +57,140 @@ __device__ __forceinline__ float dminus(const float *__restrict__ f, size_t ip, 
     return d;
 }
 
-#define CELL_INDEX                                            \
-    const int iz = blockIdx.x * 64 + threadIdx.x;             \
-    const int ix = blockIdx.y * 4 + threadIdx.y;              \
-    if (ix >= g.ntx || iz >= g.ntz) return;                   \
-    const KInst &I = insts[blockIdx.z];                       \
-    const size_t ip = static_cast<size_t>(ix) * g.ldz + iz;   \
-    const size_t sx = g.ldz;
+// ---- quad kernels: one thread updates 4 consecutive z cells of one column ----
+// A workgroup covers 64 rows x 16 columns (a wavefront = 4 columns x 64 rows,
+// each column read as one 256-byte segment with float4 accesses).  Workgroups
+// are ordered instance-fastest and handed to the XCDs in contiguous ranges, so
+// the shots of one tile share the model and the halo columns in one L2.
+typedef __attribute__((address_space(1))) float gfl;
+typedef __attribute__((address_space(1))) const float gcfl;
+
+__device__ __forceinline__ int xcd_block()
+{
+    const int b = blockIdx.x, per = gridDim.x >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const v4f gcv4;
+typedef __attribute__((address_space(1))) v4f gv4;
+__device__ __forceinline__ float4 ld4(gcfl *f, size_t o)
+{
+    const v4f v = *(gcv4 *)(f + o);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st4(gfl *f, size_t o, float4 v)
+{
+    v4f w = {v.x, v.y, v.z, v.w};
+    *(gv4 *)(f + o) = w;
+}
+__device__ __forceinline__ float &el(float4 &v, int k) { return reinterpret_cast<float *>(&v)[k]; }
+__device__ __forceinline__ float elc(const float4 &v, int k) { return reinterpret_cast<const float *>(&v)[k]; }
+
+#define QUAD_INDEX                                                         \
+    const int L = xcd_block();                                             \
+    if (L >= ninst * nzb * nxb) return;                                    \
+    const int inst = L % ninst;                                            \
+    const int zb = (L / ninst) % nzb, xb = L / (ninst * nzb);              \
+    const int x = xb * 16 + (threadIdx.x >> 4);                            \
+    const int z = zb * 64 + (threadIdx.x & 15) * 4;                        \
+    if (x >= g.ntx || z >= g.ntz) return;                                  \
+    const KInst &I = insts[inst];                                          \
+    const size_t ldz = g.ldz;                                              \
+    const size_t o = static_cast<size_t>(x) * ldz + z;
+
+// rows z-4R .. z+4R+3 of one column (R quads of halo on each side), zero outside
+template <int R>
+__device__ __forceinline__ void column_window(gcfl *f, size_t o, int z, int ldz, float (&a)[4 * (2 * R + 1)])
+{
+#pragma unroll
+    for (int q = -R; q <= R; q++) {
+        const int zq = z + 4 * q;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (zq >= 0 && zq < ldz) v = ld4(f, o + 4 * q);
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[4 * (q + R) + k] = elc(v, k);
+    }
+}
 
 // Velocity update with CPML (RK:24-118).  NEG: receiver back-propagation,
 // called by the reference with (-dx, -dz, +dt) and widened domains
 // (RK:623-715, 1111-1127).  COMP: 1 = vx, 2 = vz, 3 = both.
 template <int LC, bool NEG, int COMP>
-__global__ __launch_bounds__(256) void k_v(KGrid g, const KInst *__restrict__ insts)
+__global__ __launch_bounds__(256) void k_v(KGrid g, const KInst *__restrict__ insts, int ninst, int nzb, int nxb)
 {
-    CELL_INDEX
-    const float *__restrict__ p = I.p;
+    constexpr int R = (LC + 3) / 4;
+    QUAD_INDEX
+    gcfl *P = (gcfl *)I.p;
     if (COMP & 1) {
-        const bool in = ix >= LC - 1 && ix <= g.ntx - LC - 1 && (NEG || (iz >= LC && iz <= g.ntz - LC));
-        if (in) {
-            const float d = dplus<LC>(p, ip, sx, g.rc, NEG ? -g.idx : g.idx);
-            const float a = g.axh[ix];
-            float ps = 0.f;
-            if (a != 0.f) {
-                ps = __builtin_fmaf(g.bxh[ix], I.psx[ip], a * d);
-                I.psx[ip] = ps;
+        if (x >= LC - 1 && x <= g.ntx - LC - 1) {
+            const float idx = NEG ? -g.idx : g.idx;
+            float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int ic = 0; ic < LC; ic++) {
+                const float4 a = ld4(P, o + (ic + 1) * ldz), b = ld4(P, o - ic * ldz);
+#pragma unroll
+                for (int k = 0; k < 4; k++) el(d, k) = __builtin_fmaf(g.rc[ic] * (elc(a, k) - elc(b, k)), idx, el(d, k));
             }
-            I.vx[ip] = __builtin_fmaf(g.Bx[ip], d + ps, I.vx[ip]);
+            const float ah = g.axh[x], bh = g.bxh[x];
+            const float4 vo = ld4((gcfl *)I.vx, o), B = ld4((gcfl *)g.Bx, o);
+            float4 vn = vo;
+            float4 pso = make_float4(0.f, 0.f, 0.f, 0.f), psn = pso;
+            if (ah != 0.f) psn = pso = ld4((gcfl *)I.psx, o);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int zk = z + k;
+                const bool in = NEG ? zk < g.ntz : (zk >= LC && zk <= g.ntz - LC);
+                if (in) {
+                    float ps = 0.f;
+                    if (ah != 0.f) {
+                        ps = __builtin_fmaf(bh, elc(pso, k), ah * elc(d, k));
+                        el(psn, k) = ps;
+                    }
+                    el(vn, k) = __builtin_fmaf(elc(B, k), elc(d, k) + ps, elc(vo, k));
+                }
+            }
+            st4((gfl *)I.vx, o, vn);
+            if (ah != 0.f) st4((gfl *)I.psx, o, psn);
         }
     }
     if (COMP & 2) {
-        const bool in = iz >= LC - 1 && iz <= g.ntz - LC - 1 && (NEG || (ix >= LC && ix <= g.ntx - LC));
-        if (in) {
-            const float d = dplus<LC>(p, ip, 1, g.rc, NEG ? -g.idz : g.idz);
-            const float a = g.azh[iz];
-            float ps = 0.f;
-            if (a != 0.f) {
-                ps = __builtin_fmaf(g.bzh[iz], I.psz[ip], a * d);
-                I.psz[ip] = ps;
+        if (NEG || (x >= LC && x <= g.ntx - LC)) {
+            const float idz = NEG ? -g.idz : g.idz;
+            float a[4 * (2 * R + 1)];
+            column_window<R>(P, o, z, g.ldz, a);
+            const float4 vo = ld4((gcfl *)I.vz, o), B = ld4((gcfl *)g.Bz, o);
+            float4 vn = vo;
+            bool anyps = false;
+            float ahz[4], bhz[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int zc = z + k < g.ntz ? z + k : g.ntz - 1;
+                ahz[k] = g.azh[zc];
+                bhz[k] = g.bzh[zc];
+                anyps |= ahz[k] != 0.f;
             }
-            I.vz[ip] = __builtin_fmaf(g.Bz[ip], d + ps, I.vz[ip]);
+            float4 pso = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (anyps) pso = ld4((gcfl *)I.psz, o);
+            float4 psn = pso;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int zk = z + k;
+                if (zk >= LC - 1 && zk <= g.ntz - LC - 1) {
+                    float d = 0.f;
+#pragma unroll
+                    for (int ic = 0; ic < LC; ic++)
+                        d = __builtin_fmaf(g.rc[ic] * (a[4 * R + k + ic + 1] - a[4 * R + k - ic]), idz, d);
+                    float ps = 0.f;
+                    if (ahz[k] != 0.f) {
+                        ps = __builtin_fmaf(bhz[k], elc(pso, k), ahz[k] * d);
+                        el(psn, k) = ps;
+                    }
+                    el(vn, k) = __builtin_fmaf(elc(B, k), d + ps, elc(vo, k));
+                }
+            }
+            st4((gfl *)I.vz, o, vn);
+            if (anyps) st4((gfl *)I.psz, o, psn);
         }
     }
 }
@@ -107,113 +201,252 @@ __global__ __launch_bounds__(256) void k_v(KGrid g, const KInst *__restrict__ in
 // side: hard replacement by the adjoint source, then the cross-correlation
 // imaging condition with the reconstruction's dp/dt (RK:785-817).
 template <int LC, bool NEG>
-__global__ __launch_bounds__(256) void k_p(KGrid g, const KInst *__restrict__ insts, int it)
+__global__ __launch_bounds__(256) void k_p(KGrid g, const KInst *__restrict__ insts, int ninst, int nzb, int nxb, int it)
 {
-    CELL_INDEX
-    float pn = I.p[ip];
-    bool wr = false;
-    if (ix >= LC && ix <= g.ntx - LC && iz >= LC && iz <= g.ntz - LC) {
-        const float dvx = dminus<LC>(I.vx, ip, sx, g.rc, NEG ? -g.idx : g.idx);
-        const float dvz = dminus<LC>(I.vz, ip, 1, g.rc, NEG ? -g.idz : g.idz);
-        const float ax = g.ax[ix], az = g.az[iz];
-        float qx = 0.f, qz = 0.f;
-        if (ax != 0.f) {
-            qx = __builtin_fmaf(g.bx[ix], I.pvx[ip], ax * dvx);
-            I.pvx[ip] = qx;
+    constexpr int R = (LC + 3) / 4;
+    QUAD_INDEX
+    const float4 pold = ld4((gcfl *)I.p, o);
+    float4 pn = pold;
+    float4 Sv = make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool colin = x >= LC && x <= g.ntx - LC;
+    const bool colint = x >= g.pmlc && x < g.ntx - g.pmlc;
+    if (colin) {
+        const float idx = NEG ? -g.idx : g.idx, idz = NEG ? -g.idz : g.idz;
+        gcfl *VX = (gcfl *)I.vx;
+        float4 dvx = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int ic = 0; ic < LC; ic++) {
+            const float4 a = ld4(VX, o + ic * ldz), b = ld4(VX, o - (ic + 1) * ldz);
+#pragma unroll
+            for (int k = 0; k < 4; k++) el(dvx, k) = __builtin_fmaf(g.rc[ic] * (elc(a, k) - elc(b, k)), idx, el(dvx, k));
         }
-        if (az != 0.f) {
-            qz = __builtin_fmaf(g.bz[iz], I.pvz[ip], az * dvz);
-            I.pvz[ip] = qz;
+        float a[4 * (2 * R + 1)];
+        column_window<R>((gcfl *)I.vz, o, z, g.ldz, a);
+        const float ax = g.ax[x], bx = g.bx[x];
+        float azr[4], bzr[4];
+        bool anyz = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int zc = z + k < g.ntz ? z + k : g.ntz - 1;
+            azr[k] = g.az[zc];
+            bzr[k] = g.bz[zc];
+            anyz |= azr[k] != 0.f;
         }
-        float S = ((dvx + qx) + dvz) + qz;
-        if (!NEG && (I.born_m || I.s0_out) && interior(g, ix, iz)) {
-            const int im = (ix - g.pmlc) * g.nz + (iz - g.pmlc);
-            if (I.s0_out) I.s0_out[static_cast<size_t>(it - I.s0_t0) * g.nx * g.nz + im] = S;
-            if (I.born_m) {
-                // background sum of the same step; psi is zero on the interior
-                const float s0 = dminus<LC>(I.bvx, ip, sx, g.rc, g.idx) + dminus<LC>(I.bvz, ip, 1, g.rc, g.idz);
-                S = __builtin_fmaf(I.born_m[im], s0, S);
+        float4 qxo = make_float4(0.f, 0.f, 0.f, 0.f), qzo = qxo;
+        if (ax != 0.f) qxo = ld4((gcfl *)I.pvx, o);
+        if (anyz) qzo = ld4((gcfl *)I.pvz, o);
+        float4 qxn = qxo, qzn = qzo;
+        const float4 K = ld4((gcfl *)g.K, o);
+        float4 bm = make_float4(0.f, 0.f, 0.f, 0.f), s0 = bm;
+        const bool born = !NEG && I.born_m && colint;
+        if (born) {
+            // background sum of the same step; psi is zero on the interior
+            gcfl *BX = (gcfl *)I.bvx;
+            float4 bdx = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int ic = 0; ic < LC; ic++) {
+                const float4 u = ld4(BX, o + ic * ldz), v = ld4(BX, o - (ic + 1) * ldz);
+#pragma unroll
+                for (int k = 0; k < 4; k++) el(bdx, k) = __builtin_fmaf(g.rc[ic] * (elc(u, k) - elc(v, k)), g.idx, el(bdx, k));
+            }
+            float bz_[4 * (2 * R + 1)];
+            column_window<R>((gcfl *)I.bvz, o, z, g.ldz, bz_);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                float d = 0.f;
+#pragma unroll
+                for (int ic = 0; ic < LC; ic++)
+                    d = __builtin_fmaf(g.rc[ic] * (bz_[4 * R + k + ic] - bz_[4 * R + k - ic - 1]), g.idz, d);
+                el(s0, k) = elc(bdx, k) + d;
             }
         }
-        pn = __builtin_fmaf(g.K[ip], S, pn);
-        wr = true;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int zk = z + k;
+            if (zk >= LC && zk <= g.ntz - LC) {
+                float dvz = 0.f;
+#pragma unroll
+                for (int ic = 0; ic < LC; ic++)
+                    dvz = __builtin_fmaf(g.rc[ic] * (a[4 * R + k + ic] - a[4 * R + k - ic - 1]), idz, dvz);
+                float qx = 0.f, qz = 0.f;
+                if (ax != 0.f) {
+                    qx = __builtin_fmaf(bx, elc(qxo, k), ax * elc(dvx, k));
+                    el(qxn, k) = qx;
+                }
+                if (azr[k] != 0.f) {
+                    qz = __builtin_fmaf(bzr[k], elc(qzo, k), azr[k] * dvz);
+                    el(qzn, k) = qz;
+                }
+                float S = ((elc(dvx, k) + qx) + dvz) + qz;
+                el(Sv, k) = S;
+                const bool inter = colint && zk >= g.pmlc && zk < g.ntz - g.pmlc;
+                if (born && inter) {
+                    const int im = (x - g.pmlc) * g.nz + (zk - g.pmlc);
+                    S = __builtin_fmaf(I.born_m[im], elc(s0, k), S);
+                }
+                el(pn, k) = __builtin_fmaf(elc(K, k), S, elc(pold, k));
+            }
+        }
+        if (ax != 0.f) st4((gfl *)I.pvx, o, qxn);
+        if (anyz) st4((gfl *)I.pvz, o, qzn);
+        if (!NEG && I.s0_out && colint) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int zk = z + k;
+                if (zk >= g.pmlc && zk < g.ntz - g.pmlc)
+                    I.s0_out[static_cast<size_t>(it - I.s0_t0) * g.nx * g.nz + (x - g.pmlc) * g.nz + (zk - g.pmlc)] =
+                        elc(Sv, k);
+            }
+        }
     }
-    if (!NEG && I.wavelet && ix == I.s_ix && iz == I.s_iz) {
-        pn += I.wavelet[it];   // RK:175-178
-        wr = true;
-    }
+    if (!NEG && I.wavelet && x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4) el(pn, I.s_iz - z) += I.wavelet[it];   // RK:175-178
     if (I.colmap) {
-        const int cm = I.colmap[ix];
+        const int cm = I.colmap[x];
         if (cm >= 0) {
-            const int tr = I.rowmap[cm + iz];
-            if (tr >= 0) {
-                const size_t o = static_cast<size_t>(tr) * g.nt + it;
-                if (NEG) {
-                    pn = I.adj[o];   // hard replacement (RK:765-780)
-                    wr = true;
-                } else if (I.gather) {
-                    I.gather[o] = pn;   // RK:182-197
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (z + k >= g.ntz) break;
+                const int tr = I.rowmap[cm + z + k];
+                if (tr >= 0) {
+                    const size_t go = static_cast<size_t>(tr) * g.nt + it;
+                    if (NEG) el(pn, k) = I.adj[go];                 // hard replacement (RK:765-780)
+                    else if (I.gather) I.gather[go] = elc(pn, k);   // RK:182-197
                 }
             }
         }
     }
-    if (wr) I.p[ip] = pn;
+    st4((gfl *)I.p, o, pn);
     if (!NEG) {
         if (I.strips) {
-            const int f = frame_index(g, ix, iz);
-            if (f >= 0) I.strips[static_cast<size_t>(it) * g.nframe + f] = pn;   // RK:200-219
-        }
-        if (I.capture) {
-            const int c = I.colcap[ix];
-            if (c >= 0) {
-                float *dst = reinterpret_cast<float *>(I.capture + (static_cast<size_t>(it) * I.ncap + c) * I.nfftz + iz);
-                dst[I.capcomp] = pn;   // temp[ix*nfftz+iz].x / .y (FK:324-327, 529-532)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int f = frame_index(g, x, z + k);
+                if (f >= 0) I.strips[static_cast<size_t>(it) * g.nframe + f] = elc(pn, k);   // RK:200-219
             }
         }
-    } else if (I.image && interior(g, ix, iz)) {
-        const float dp = I.dpdt[ip];
-        I.image[ip] = __builtin_fmaf(dp, pn, I.image[ip]);
-        I.illum_s[ip] = __builtin_fmaf(dp, dp, I.illum_s[ip]);
-        I.illum_r[ip] = __builtin_fmaf(pn, pn, I.illum_r[ip]);
+        if (I.capture) {
+            const int c = I.colcap[x];
+            if (c >= 0) {
+                float *dst = reinterpret_cast<float *>(I.capture + (static_cast<size_t>(it) * I.ncap + c) * I.nfftz + z);
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (z + k < g.ntz) dst[2 * k + I.capcomp] = elc(pn, k);   // FK:324-327, 529-532
+            }
+        }
+    } else if (I.image && colint) {
+        const float4 dp = ld4((gcfl *)I.dpdt, o);
+        float4 im = ld4((gcfl *)I.image, o), is = ld4((gcfl *)I.illum_s, o), ir = ld4((gcfl *)I.illum_r, o);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int zk = z + k;
+            if (zk >= g.pmlc && zk < g.ntz - g.pmlc) {
+                el(im, k) = __builtin_fmaf(elc(dp, k), elc(pn, k), elc(im, k));
+                el(is, k) = __builtin_fmaf(elc(dp, k), elc(dp, k), elc(is, k));
+                el(ir, k) = __builtin_fmaf(elc(pn, k), elc(pn, k), elc(ir, k));
+            }
+        }
+        st4((gfl *)I.image, o, im);
+        st4((gfl *)I.illum_s, o, is);
+        st4((gfl *)I.illum_r, o, ir);
     }
 }
 
 // Source-wavefield reconstruction, pressure (RK:434-485 called with -dt), the
 // wavelet subtraction at it+1 (RK:478-481) and the frame restore (RK:575-620).
 template <int LC>
-__global__ __launch_bounds__(256) void k_src_p(KGrid g, const KInst *__restrict__ insts, int it)
+__global__ __launch_bounds__(256) void k_src_p(KGrid g, const KInst *__restrict__ insts, int ninst, int nzb, int nxb,
+                                               int it)
 {
-    CELL_INDEX
-    float pn = 0.f;
-    bool wr = false, have = false;
-    if (interior(g, ix, iz)) {
-        const float S = dminus<LC>(I.vx, ip, sx, g.rc, g.idx) + dminus<LC>(I.vz, ip, 1, g.rc, g.idz);
-        pn = __builtin_fmaf(-g.K[ip], S, I.p[ip]);
-        I.dpdt[ip] = g.K0[ip] * S;
-        wr = have = true;
+    constexpr int R = (LC + 3) / 4;
+    QUAD_INDEX
+    float4 pn = ld4((gcfl *)I.p, o);
+    const bool colint = x >= g.pmlc && x < g.ntx - g.pmlc;
+    if (colint) {
+        gcfl *VX = (gcfl *)I.vx;
+        float4 dvx = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int ic = 0; ic < LC; ic++) {
+            const float4 a = ld4(VX, o + ic * ldz), b = ld4(VX, o - (ic + 1) * ldz);
+#pragma unroll
+            for (int k = 0; k < 4; k++) el(dvx, k) = __builtin_fmaf(g.rc[ic] * (elc(a, k) - elc(b, k)), g.idx, el(dvx, k));
+        }
+        float a[4 * (2 * R + 1)];
+        column_window<R>((gcfl *)I.vz, o, z, g.ldz, a);
+        const float4 K = ld4((gcfl *)g.K, o), K0 = ld4((gcfl *)g.K0, o);
+        float4 dp = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int zk = z + k;
+            if (zk >= g.pmlc && zk < g.ntz - g.pmlc) {
+                float dvz = 0.f;
+#pragma unroll
+                for (int ic = 0; ic < LC; ic++)
+                    dvz = __builtin_fmaf(g.rc[ic] * (a[4 * R + k + ic] - a[4 * R + k - ic - 1]), g.idz, dvz);
+                const float S = elc(dvx, k) + dvz;
+                el(pn, k) = __builtin_fmaf(-elc(K, k), S, elc(pn, k));
+                el(dp, k) = elc(K0, k) * S;
+            }
+        }
+        st4((gfl *)I.dpdt, o, dp);
     }
-    if (ix == I.s_ix && iz == I.s_iz) {
-        if (!have) pn = I.p[ip];
-        pn -= I.wavelet[it + 1];
-        wr = true;
+    if (x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4) el(pn, I.s_iz - z) -= I.wavelet[it + 1];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int f = frame_index(g, x, z + k);
+        if (f >= 0) el(pn, k) = I.strips[static_cast<size_t>(it) * g.nframe + f];
     }
-    const int f = frame_index(g, ix, iz);
-    if (f >= 0) {
-        pn = I.strips[static_cast<size_t>(it) * g.nframe + f];
-        wr = true;
-    }
-    if (wr) I.p[ip] = pn;
+    st4((gfl *)I.p, o, pn);
 }
 
 // Source-wavefield reconstruction, velocities (RK:488-573 called with -dt).
 template <int LC, int COMP>
-__global__ __launch_bounds__(256) void k_src_v(KGrid g, const KInst *__restrict__ insts)
+__global__ __launch_bounds__(256) void k_src_v(KGrid g, const KInst *__restrict__ insts, int ninst, int nzb, int nxb)
 {
-    CELL_INDEX
-    if (!interior(g, ix, iz)) return;
-    if (COMP & 1) I.vx[ip] = __builtin_fmaf(-g.Bx[ip], dplus<LC>(I.p, ip, sx, g.rc, g.idx), I.vx[ip]);
-    if (COMP & 2) I.vz[ip] = __builtin_fmaf(-g.Bz[ip], dplus<LC>(I.p, ip, 1, g.rc, g.idz), I.vz[ip]);
+    constexpr int R = (LC + 3) / 4;
+    QUAD_INDEX
+    if (x < g.pmlc || x >= g.ntx - g.pmlc) return;
+    gcfl *P = (gcfl *)I.p;
+    if (COMP & 1) {
+        float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int ic = 0; ic < LC; ic++) {
+            const float4 a = ld4(P, o + (ic + 1) * ldz), b = ld4(P, o - ic * ldz);
+#pragma unroll
+            for (int k = 0; k < 4; k++) el(d, k) = __builtin_fmaf(g.rc[ic] * (elc(a, k) - elc(b, k)), g.idx, el(d, k));
+        }
+        float4 v = ld4((gcfl *)I.vx, o);
+        const float4 B = ld4((gcfl *)g.Bx, o);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (z + k >= g.pmlc && z + k < g.ntz - g.pmlc) el(v, k) = __builtin_fmaf(-elc(B, k), elc(d, k), elc(v, k));
+        st4((gfl *)I.vx, o, v);
+    }
+    if (COMP & 2) {
+        float a[4 * (2 * R + 1)];
+        column_window<R>(P, o, z, g.ldz, a);
+        float4 v = ld4((gcfl *)I.vz, o);
+        const float4 B = ld4((gcfl *)g.Bz, o);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (z + k >= g.pmlc && z + k < g.ntz - g.pmlc) {
+                float d = 0.f;
+#pragma unroll
+                for (int ic = 0; ic < LC; ic++)
+                    d = __builtin_fmaf(g.rc[ic] * (a[4 * R + k + ic + 1] - a[4 * R + k - ic]), g.idz, d);
+                el(v, k) = __builtin_fmaf(-elc(B, k), d, elc(v, k));
+            }
+        }
+        st4((gfl *)I.vz, o, v);
+    }
 }
+
+#define CELL_INDEX                                            \
+    const int iz = blockIdx.x * 64 + threadIdx.x;             \
+    const int ix = blockIdx.y * 4 + threadIdx.y;              \
+    if (ix >= g.ntx || iz >= g.ntz) return;                   \
+    const KInst &I = insts[blockIdx.z];                       \
+    const size_t ip = static_cast<size_t>(ix) * g.ldz + iz;   \
+    const size_t sx = g.ldz;
 
 // ---- exact adjoint of one forward step (Born adjoint) ----
 // A_p^T, first half: ex/ez = adjoint of dvx/dvz on the p domain, psi-hat update.
@@ -441,10 +674,22 @@ static const dim3 kCellBlock(64, 4, 1);
     default: return hipErrorInvalidValue;              \
     }
 
+struct QGeom {
+    int nzb, nxb, nblocks;
+};
+static QGeom qgeom(const KGrid &g, int ninst)
+{
+    QGeom q;
+    q.nzb = (g.ntz + 63) / 64;
+    q.nxb = (g.ntx + 15) / 16;
+    q.nblocks = (ninst * q.nzb * q.nxb + 7) / 8 * 8;
+    return q;
+}
+
 hipError_t launch_v(const KGrid &g, const KInst *insts, int ninst, bool neg, int comp, hipStream_t s)
 {
-    const dim3 gr = cell_grid(g, ninst);
-#define LV(NEGV, C) hipLaunchKernelGGL((k_v<L, NEGV, C>), gr, kCellBlock, 0, s, g, insts)
+    const QGeom q = qgeom(g, ninst);
+#define LV(NEGV, C) hipLaunchKernelGGL((k_v<L, NEGV, C>), dim3(q.nblocks), dim3(256), 0, s, g, insts, ninst, q.nzb, q.nxb)
     if (!neg) {
         if (comp == 1) { DISPATCH_LC(g.Lc, LV(false, 1)); }
         else if (comp == 2) { DISPATCH_LC(g.Lc, LV(false, 2)); }
@@ -460,25 +705,25 @@ hipError_t launch_v(const KGrid &g, const KInst *insts, int ninst, bool neg, int
 
 hipError_t launch_p(const KGrid &g, const KInst *insts, int ninst, bool neg, int it, hipStream_t s)
 {
-    const dim3 gr = cell_grid(g, ninst);
-    if (!neg) { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_p<L, false>), gr, kCellBlock, 0, s, g, insts, it)); }
-    else { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_p<L, true>), gr, kCellBlock, 0, s, g, insts, it)); }
+    const QGeom q = qgeom(g, ninst);
+    if (!neg) { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_p<L, false>), dim3(q.nblocks), dim3(256), 0, s, g, insts, ninst, q.nzb, q.nxb, it)); }
+    else { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_p<L, true>), dim3(q.nblocks), dim3(256), 0, s, g, insts, ninst, q.nzb, q.nxb, it)); }
     return hipGetLastError();
 }
 
 hipError_t launch_src_p(const KGrid &g, const KInst *insts, int ninst, int it, hipStream_t s)
 {
-    const dim3 gr = cell_grid(g, ninst);
-    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_p<L>), gr, kCellBlock, 0, s, g, insts, it));
+    const QGeom q = qgeom(g, ninst);
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_p<L>), dim3(q.nblocks), dim3(256), 0, s, g, insts, ninst, q.nzb, q.nxb, it));
     return hipGetLastError();
 }
 
 hipError_t launch_src_v(const KGrid &g, const KInst *insts, int ninst, int comp, hipStream_t s)
 {
-    const dim3 gr = cell_grid(g, ninst);
-    if (comp == 1) { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_v<L, 1>), gr, kCellBlock, 0, s, g, insts)); }
-    else if (comp == 2) { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_v<L, 2>), gr, kCellBlock, 0, s, g, insts)); }
-    else { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_v<L, 3>), gr, kCellBlock, 0, s, g, insts)); }
+    const QGeom q = qgeom(g, ninst);
+    if (comp == 1) { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_v<L, 1>), dim3(q.nblocks), dim3(256), 0, s, g, insts, ninst, q.nzb, q.nxb)); }
+    else if (comp == 2) { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_v<L, 2>), dim3(q.nblocks), dim3(256), 0, s, g, insts, ninst, q.nzb, q.nxb)); }
+    else { DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_src_v<L, 3>), dim3(q.nblocks), dim3(256), 0, s, g, insts, ninst, q.nzb, q.nxb)); }
     return hipGetLastError();
 }
 

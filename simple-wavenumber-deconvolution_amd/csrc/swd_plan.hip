@@ -5,9 +5,9 @@
 // rho*vp^2, each rounded exactly as the reference rounds them per cell), the
 // CPML vectors, and a growing workspace from which every call carves its
 // wavefields, strips, gathers and scratch.  Operators run a batch of shots;
-// on SWD_PATH_FUSED the batch shares each launch (one wavefield per
-// blockIdx.z), on SWD_PATH_PLAIN shots run one after another with one kernel
-// per sub-step, which is the single-shot baseline.
+// on SWD_PATH_BATCHED every wavefield of the batch shares each launch, on
+// SWD_PATH_PLAIN shots run one after another with one kernel per sub-step,
+// which is the single-shot baseline.
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -22,7 +22,7 @@
 #include "swd_internal.h"
 #include "swd_kernels.h"
 #include "swd_launch.h"
-#include "swd_fused.h"
+#include "swd_loops.h"
 
 // ------------------------------------------------------------------ errors
 static thread_local char g_last_error[1024] = "";
@@ -79,7 +79,7 @@ struct Plan {
     swd_grid g{};
     int pmlc = 0, ntx = 0, ntz = 0, ldz = 0, nframe = 0;
     size_t nfield = 0;   // ntx*ldz
-    int path = SWD_PATH_FUSED;
+    int path = SWD_PATH_BATCHED;
     bool have_coefs = false, have_model = false;
     float rc[SWD_MAX_LC] = {};
     float *d_coef = nullptr;    // 4*ntx + 4*ntz
@@ -319,7 +319,7 @@ extern "C" int swd_plan_set_model(void *plan, const float *vp, const float *rho)
 
 extern "C" int swd_plan_set_path(void *plan, int path)
 {
-    if (!plan || (path != SWD_PATH_PLAIN && path != SWD_PATH_FUSED && path != SWD_PATH_BATCHED)) return swd_fail(SWD_E_ARG, "swd_plan_set_path: bad path");
+    if (!plan || (path != SWD_PATH_PLAIN && path != SWD_PATH_BATCHED)) return swd_fail(SWD_E_ARG, "swd_plan_set_path: bad path");
     static_cast<Plan *>(plan)->path = path;
     return SWD_OK;
 }
@@ -350,16 +350,14 @@ int upload_insts(Plan &P, const std::vector<KInst> &h, size_t o_inst)
     return SWD_OK;
 }
 
-// Arrays per wavefield: p, vx, vz, psx, psz, pvx, pvz (primary set) and
-// p2, vx2, vz2, psx2, psz2 (second set of the double-buffered fused step).
-constexpr int kFieldArrays = 12;
+// Arrays per wavefield: p, vx, vz, psx, psz (CPML memory of the velocity
+// update), pvx, pvz (CPML memory of the pressure update).
+constexpr int kFieldArrays = 7;
 
 // Forward time loop of nshots shots with per_shot wavefields each.  Plain
-// path: shot after shot, one kernel per sub-step.  Fused path: every shot in
-// each launch, one launch per step.  *final_set tells which buffer set holds
-// the state after the last step (0 = primary, 1 = second).
-int forward_loop(Plan &P, const KInst *d_inst, const std::vector<KInst> &h_inst, int per_shot, int nshots,
-                 size_t o_tab, int *final_set)
+// path: shot after shot, one kernel per sub-step.  Batched path: every
+// wavefield of the batch in each launch, a velocity and a pressure kernel.
+int forward_loop(Plan &P, const KInst *d_inst, int per_shot, int nshots)
 {
     const KGrid &g = P.kg;
     long long launches = 0;
@@ -374,18 +372,8 @@ int forward_loop(Plan &P, const KInst *d_inst, const std::vector<KInst> &h_inst,
                 launches += 3;
             }
         }
-        *final_set = 0;
-    } else if (P.path == SWD_PATH_BATCHED) {
-        for (int it = 0; it < g.nt; it++) {
-            HIP_TRY(launch_v(g, d_inst, per_shot * nshots, false, 3, P.stream));
-            HIP_TRY(launch_p(g, d_inst, per_shot * nshots, false, it, P.stream));
-            launches += 2;
-        }
-        *final_set = 0;
     } else {
-        SWD_TRY(fused_build_table(h_inst.data(), per_shot * nshots, per_shot, P.ws + o_tab, P.stream));
-        SWD_TRY(fused_forward_steps(g, P.ws + o_tab, nshots, per_shot, 0, g.nt, P.stream, &launches));
-        *final_set = g.nt & 1;
+        SWD_TRY(batched_forward_steps(g, d_inst, per_shot * nshots, 0, g.nt, P.stream, &launches));
     }
     P.stats.fwd_ms = stats_end(P, 0);
     P.stats.fwd_launches = launches;
@@ -399,8 +387,6 @@ void bind_fields(KInst &I, float *f, size_t nfield)
 {
     I.p = f; I.vx = f + nfield; I.vz = f + 2 * nfield;
     I.psx = f + 3 * nfield; I.psz = f + 4 * nfield; I.pvx = f + 5 * nfield; I.pvz = f + 6 * nfield;
-    I.p2 = f + 7 * nfield; I.vx2 = f + 8 * nfield; I.vz2 = f + 9 * nfield;
-    I.psx2 = f + 10 * nfield; I.psz2 = f + 11 * nfield;
 }
 
 }  // namespace
@@ -419,7 +405,6 @@ int forward_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet
     const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
     const size_t o_inst = c.add<KInst>(nshots);
     const size_t o_wav = c.add<float>(nt);
-    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
     const size_t o_f = c.add<float>(kFieldArrays * P.nfield * nshots);
     SWD_TRY(ensure_ws(P, c.off));
     SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
@@ -439,8 +424,7 @@ int forward_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet
         }
     }
     SWD_TRY(upload_insts(P, h, o_inst));
-    int fs = 0;
-    SWD_TRY(forward_loop(P, at<KInst>(P, o_inst), h, 1, nshots, o_tab, &fs));
+    SWD_TRY(forward_loop(P, at<KInst>(P, o_inst), 1, nshots));
     HIP_TRY(launch_copy_rows(d_gather, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
     HIP_TRY(hipStreamSynchronize(P.stream));
     return SWD_OK;
@@ -531,7 +515,6 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     const size_t o_coef = c.add<float>(nfftz);
     const size_t o_gat = c.add<float>(3 * std::max<size_t>(ng, 1));
     const size_t o_cap = c.add<float2>(static_cast<size_t>(nt) * ncap * nfftz);
-    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
     const size_t o_f = c.add<float>(kFieldArrays * P->nfield * 2 * nshots);
     SWD_TRY(ensure_ws(*P, c.off));
     SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
@@ -569,8 +552,7 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
             }
         }
     SWD_TRY(upload_insts(*P, h, o_inst));
-    int fs = 0;
-    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), h, 2, nshots, o_tab, &fs));
+    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), 2, nshots));
     HIP_TRY(launch_copy_rows(d_gather, at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
     if (!cols.empty()) {
         stats_begin(*P, 1);
@@ -603,7 +585,6 @@ extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const 
     const size_t o_wav = c.add<float>(nt);
     const size_t o_strips = c.add<float>(static_cast<size_t>(nt) * P->nframe * nshots);
     const size_t o_img = c.add<float>(4 * nf * nshots);        // image, illum_s, illum_r, dpdt
-    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
     const size_t o_f = c.add<float>(kFieldArrays * nf * 2 * nshots);   // source-side then receiver-side fields
     SWD_TRY(ensure_ws(*P, c.off));
     SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
@@ -636,17 +617,7 @@ extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const 
     }
     SWD_TRY(upload_insts(*P, h, o_inst));
     const KInst *dS = at<KInst>(*P, o_inst), *dR = dS + nshots;
-    int fs = 0;
-    SWD_TRY(forward_loop(*P, dS, h, 1, nshots, o_tab, &fs));
-    if (fs == 1) {
-        // the final forward state (RK:403-415) is in the second buffer set
-        for (int k = 0; k < nshots; k++) {
-            std::swap(h[k].p, h[k].p2);
-            std::swap(h[k].vx, h[k].vx2);
-            std::swap(h[k].vz, h[k].vz2);
-        }
-        SWD_TRY(upload_insts(*P, h, o_inst));
-    }
+    SWD_TRY(forward_loop(*P, dS, 1, nshots));
     if (gather_syn)
         HIP_TRY(launch_copy_rows(gather_syn, at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
     // backward: it = nt-2 .. 0 (RK:1073)
@@ -665,7 +636,7 @@ extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const 
                 launches += 6;
             }
     } else {
-        SWD_TRY(fused_backward_loop(P->kg, dS, dR, h.data(), h.data() + nshots, nshots, P->stream, &launches));
+        SWD_TRY(batched_backward_loop(P->kg, dS, dR, nshots, P->stream, &launches));
     }
     P->stats.bwd_ms = stats_end(*P, 1);
     P->stats.bwd_launches = launches;
@@ -722,7 +693,6 @@ extern "C" int swd_born(void *plan, int nshots, const swd_shot *shots, const flo
     const size_t o_wav = c.add<float>(nt);
     const size_t o_m = c.add<float>(nint * nshots);
     const size_t o_d = c.add<float>(std::max<size_t>(ng, 1));
-    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
     const size_t o_f = c.add<float>(kFieldArrays * nf * 2 * nshots);
     SWD_TRY(ensure_ws(*P, c.off));
     SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
@@ -748,8 +718,7 @@ extern "C" int swd_born(void *plan, int nshots, const swd_shot *shots, const flo
         }
     }
     SWD_TRY(upload_insts(*P, h, o_inst));
-    int fs = 0;
-    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), h, 2, nshots, o_tab, &fs));
+    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), 2, nshots));
     HIP_TRY(launch_copy_rows(at<float>(*P, o_d), at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
     HIP_TRY(hipMemcpyAsync(d, at<float>(*P, o_d), sizeof(float) * ng, hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
@@ -778,7 +747,6 @@ extern "C" int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, c
     const size_t o_grad = c.add<float>(nint * nshots);
     const size_t o_s0 = c.add<float>(nint * seg * nshots);
     const size_t o_snap = c.add<float>(7 * nf * static_cast<size_t>(nseg) * nshots);
-    const size_t o_tab = c.add<char>(fused_table_bytes(nshots));
     const size_t per_a = kFieldArrays + 2;   // adjoint fields + ex, ez scratch
     const size_t o_f = c.add<float>((kFieldArrays + per_a) * nf * nshots);
     SWD_TRY(ensure_ws(*P, c.off));
@@ -813,48 +781,31 @@ extern "C" int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, c
     }
     const KInst *dB = at<KInst>(*P, o_inst), *dA = dB + nshots;
     const KGrid &g = P->kg;
-    const bool fused = P->path == SWD_PATH_FUSED;
     long long fl = 0, bl = 0;
-    // the plain path runs the background of all shots in each launch here too
-    // background state before step it: buffer set it&1 on the fused path (p..psz),
-    // pvx/pvz in place; a snapshot holds p, vx, vz, psx, psz, pvx, pvz
-    auto set_of = [&](int k, int it) -> float * { return (fused && (it & 1)) ? h[k].p2 : h[k].p; };
+    // a snapshot holds the background state before the first step of a segment
     auto snap = [&](int k, int s) -> float * { return at<float>(*P, o_snap) + 7 * nf * (static_cast<size_t>(nseg) * k + s); };
     auto steps = [&](int it0, int it1) -> int {
-        if (fused) return fused_forward_steps(g, P->ws + o_tab, nshots, 1, it0, it1, P->stream, &fl);
-        for (int it = it0; it < it1; it++) {
-            HIP_TRY(launch_v(g, dB, nshots, false, 3, P->stream));
-            HIP_TRY(launch_p(g, dB, nshots, false, it, P->stream));
-            fl += 2;
-        }
-        return SWD_OK;
+        return batched_forward_steps(g, dB, nshots, it0, it1, P->stream, &fl);
     };
     // checkpoint pass
     SWD_TRY(upload_insts(*P, h, o_inst));
-    if (fused) SWD_TRY(fused_build_table(h.data(), nshots, 1, P->ws + o_tab, P->stream));
     stats_begin(*P, 0);
     for (int s = 0; s < nseg; s++) {
         const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
-        for (int k = 0; k < nshots; k++) {
-            HIP_TRY(hipMemcpyAsync(snap(k, s), set_of(k, t0), sizeof(float) * 5 * nf, hipMemcpyDeviceToDevice, P->stream));
-            HIP_TRY(hipMemcpyAsync(snap(k, s) + 5 * nf, h[k].pvx, sizeof(float) * 2 * nf, hipMemcpyDeviceToDevice,
-                                   P->stream));
-        }
+        for (int k = 0; k < nshots; k++)
+            HIP_TRY(hipMemcpyAsync(snap(k, s), h[k].p, sizeof(float) * 7 * nf, hipMemcpyDeviceToDevice, P->stream));
         SWD_TRY(steps(t0, t1));
     }
     // reverse segments: recompute the background sums, then the adjoint steps
     for (int s = nseg - 1; s >= 0; s--) {
         const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
         for (int k = 0; k < nshots; k++) {
-            HIP_TRY(hipMemcpyAsync(set_of(k, t0), snap(k, s), sizeof(float) * 5 * nf, hipMemcpyDeviceToDevice, P->stream));
-            HIP_TRY(hipMemcpyAsync(h[k].pvx, snap(k, s) + 5 * nf, sizeof(float) * 2 * nf, hipMemcpyDeviceToDevice,
-                                   P->stream));
+            HIP_TRY(hipMemcpyAsync(h[k].p, snap(k, s), sizeof(float) * 7 * nf, hipMemcpyDeviceToDevice, P->stream));
             h[k].s0_out = at<float>(*P, o_s0) + nint * seg * k;
             h[k].s0_t0 = t0;
             h[nshots + k].s0_t0 = t0;
         }
         SWD_TRY(upload_insts(*P, h, o_inst));
-        if (fused) SWD_TRY(fused_build_table(h.data(), nshots, 1, P->ws + o_tab, P->stream));
         SWD_TRY(steps(t0, t1));
         for (int it = t1 - 1; it >= t0; it--) {
             HIP_TRY(launch_adjoint_step(g, dA, nshots, it, it == nt - 1, P->stream));

@@ -14,8 +14,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libswd.so")
 
 SWD_PATH_PLAIN = 0
-SWD_PATH_FUSED = 1
-SWD_PATH_BATCHED = 2
+SWD_PATH_BATCHED = 1
 ERRORS = {-1: "SWD_E_ARG", -2: "SWD_E_DEVICE", -3: "SWD_E_NOMEM", -4: "SWD_E_IO", -5: "SWD_E_STATE",
           -6: "SWD_E_RANGE"}
 
@@ -261,7 +260,7 @@ class ShotSet:
 class Plan:
     """A device plan: grid, coefficients and model resident on one GPU."""
 
-    def __init__(self, grid, coefs, vp, rho, device=0, path=SWD_PATH_FUSED):
+    def __init__(self, grid, coefs, vp, rho, device=0, path=SWD_PATH_BATCHED):
         self.L = load()
         self.grid = grid
         h = C.c_void_p()

@@ -16,11 +16,11 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def plan_of(case, path=swd.SWD_PATH_FUSED):
+def plan_of(case, path=swd.SWD_PATH_BATCHED):
     return swd.Plan(case.grid, case.coefs, case.vp, case.rho, path=path)
 
 
-@pytest.mark.parametrize("path", [swd.SWD_PATH_FUSED, swd.SWD_PATH_PLAIN])
+@pytest.mark.parametrize("path", [swd.SWD_PATH_BATCHED, swd.SWD_PATH_PLAIN])
 def test_forward_small(path):
     c = configs.small()
     print(f"case small forward path={path}", flush=True)
@@ -66,7 +66,7 @@ def test_forward_c1():
     assert e32 <= TOL and e64 <= TOL
 
 
-@pytest.mark.parametrize("path", [swd.SWD_PATH_FUSED, swd.SWD_PATH_PLAIN])
+@pytest.mark.parametrize("path", [swd.SWD_PATH_BATCHED, swd.SWD_PATH_PLAIN])
 def test_rtm_small(path):
     c = configs.small(nt=300)
     print(f"case small rtm path={path}", flush=True)
@@ -118,3 +118,49 @@ def test_born_and_adjoint_small():
     rhs = float(np.dot(m.astype(np.float64), g.astype(np.float64)))
     print(f"dot test <Lm,d>={lhs:.9e} <m,L^T d>={rhs:.9e} rel {abs(lhs - rhs) / abs(lhs):.2e}", flush=True)
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+@pytest.mark.parametrize("path", [swd.SWD_PATH_BATCHED, swd.SWD_PATH_PLAIN])
+def test_rtm_batch_duplicates_strip_source(path):
+    """Several shots, receivers with duplicate nodes, sources on the top frame row
+    (shot depth 0, as the reference's parameter.txt), every path against the oracle."""
+    c = configs.small(nx=150, nz=100, nt=260, nshots=3, dup=True)
+    shots = []
+    for k, (s_ix, s_iz, rx, rz) in enumerate(c.shots):
+        shots.append((s_ix, c.pmlc, rx, rz))
+    c.shots = shots
+    print(f"case rtm batch dup path={path}", flush=True)
+    Ps = [oracle_problem(c, k) for k in range(3)]
+    outs = [P.forward(strips=True, final=True) for P in Ps]
+    adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
+    img, ils, ilr, gs = plan_of(c, path).rtm(c.shots, c.wavelet, adj, want_gather=True)
+    row = 0
+    for k, P in enumerate(Ps):
+        d, st, fs = outs[k]
+        n = d.shape[0]
+        o = P.rtm_backward(fs, st, adj[row:row + n])
+        assert rel_l2(gs[row:row + n], d) <= TOL
+        for name, a, b in (("image", img[k], o[0]), ("illum_src", ils[k], o[1]), ("illum_rcv", ilr[k], o[2])):
+            e32, _ = report(f"rtm shot {k} {name}", a, b)
+            assert e32 <= TOL
+        row += n
+
+
+def test_rtm_c2_grid_two_shots():
+    """The c2 grid (590 x 933 padded, 401 receivers on 201 nodes) for 160 steps, 2 shots."""
+    c = configs.c2(nt=160, shot_ix=(300, 520))
+    print("case c2 grid rtm nt=160", flush=True)
+    Ps = [oracle_problem(c, k) for k in range(2)]
+    outs = [P.forward(strips=True, final=True) for P in Ps]
+    adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
+    for path in (swd.SWD_PATH_BATCHED, swd.SWD_PATH_PLAIN):
+        img, ils, ilr, gs = plan_of(c, path).rtm(c.shots, c.wavelet, adj, want_gather=True)
+        row = 0
+        for k, P in enumerate(Ps):
+            d, st, fs = outs[k]
+            n = d.shape[0]
+            o = P.rtm_backward(fs, st, adj[row:row + n])
+            e_g, _ = report(f"c2 path {path} shot {k} gather", gs[row:row + n], d)
+            e_i, _ = report(f"c2 path {path} shot {k} image", img[k], o[0])
+            assert e_g <= TOL and e_i <= TOL
+            row += n
