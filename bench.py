@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-nt", type=int, default=600)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also copy 256 MiB device-to-device once (reference for HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU shot sharding is "
@@ -115,6 +117,13 @@ def main():
         plan.rtm_dev(shots, case.wavelet, d_adj.data_ptr(), d_img[0].data_ptr(), d_img[1].data_ptr(),
                      d_img[2].data_ptr())
 
+    if a.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for i in range(a.warmup):
         step()
         log(f"bench: warmup {i + 1}/{a.warmup} done")

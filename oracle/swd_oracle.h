@@ -105,7 +105,7 @@ int or_afor_f64(const or_grid *g, const float *vp, const float *rho, const or_co
                 const float *kzcoef, int nfftz, double *gather, double *up, double *down);
 
 /* Born forward d = L m and its exact discrete adjoint m = L^T d (the linearised
- * CPML scheme with K -> K(1+m) on the interior; see DESIGN notes in the source). */
+ * CPML scheme with K -> K(1+m) on the interior, linearised in m at the background model). */
 int or_born_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
                 const or_shot *s, const float *wavelet, const float *m, float *d);
 int or_born_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,

@@ -1,6 +1,6 @@
 """GPU parity of the HIP path against the CPU oracle (fp32 host loops, fp64 twin).
 
-Bar (BASELINE north star): <= 1e-5 relative L2 against the fp32 restatement of the
+Bar: <= 1e-5 relative L2 against the fp32 restatement of the
 reference's per-cell equations; the fp64 error is printed for every result.
 Tolerances are written in each test.
 """
