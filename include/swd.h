@@ -91,6 +91,9 @@ typedef struct swd_stats {
     long long bwd_launches;
     long long fwd_cells;    /* cell-updates of the last forward loop (per wavefield, padded grid) */
     long long bwd_cells;
+    long long ws_bytes;     /* device workspace held by the plan */
+    long long ckpt_bytes;   /* last Born adjoint: snapshot + one segment of background sums */
+    long long full_bytes;   /* last Born adjoint: background sums of all nt steps (full storage) */
 } swd_stats;
 
 /* ---------------- errors and devices ---------------- */
@@ -164,6 +167,13 @@ int swd_born(void *plan, int nshots, const swd_shot *shots, const float *wavelet
              const float *m, float *d);
 int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
                      const float *d, float *m_grad, int nsnap);
+
+/* Same as swd_born / swd_born_adjoint with m, d and m_grad in device memory of
+ * the plan's device (wavelet stays a host array). */
+int swd_born_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+                 const float *m, float *d);
+int swd_born_adjoint_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+                         const float *d, float *m_grad, int nsnap);
 
 #ifdef __cplusplus
 }

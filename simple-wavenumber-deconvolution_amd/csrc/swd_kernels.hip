@@ -14,6 +14,8 @@
 // CPML memory variables are read and written only where the damping
 // coefficient a is non-zero: elsewhere they stay exactly zero (psi starts at 0
 // and psi' = b*0 + 0*d = 0), so skipping them leaves every result unchanged.
+#include <algorithm>
+
 #include "swd_kernels.h"
 #include "swd_launch.h"
 
@@ -581,6 +583,17 @@ __global__ void k_sum_rows(float *__restrict__ dst, const float *__restrict__ sr
         }
 }
 
+// Sum of per-shot arrays in shot order: dst[i] = src[0][i] + src[1][i] + ...
+__global__ void k_sum_shots(float *__restrict__ dst, const float *__restrict__ src, int nshots, size_t n)
+{
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        float acc = src[i];
+        for (int k = 1; k < nshots; k++) acc += src[static_cast<size_t>(k) * n + i];
+        dst[i] = acc;
+    }
+}
+
 // ---- kz up/down separation on receiver columns (FK:705, 762-777, 52-138) ----
 // One workgroup per (time step, receiver column): the captured column
 // p + i*htp, zero padded to nfftz, is transformed along z in LDS (radix-2
@@ -750,6 +763,14 @@ hipError_t launch_sum_rows(float *dst, const float *src, const int2 *pairs, int 
 {
     if (npairs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sum_rows, dim3((nt + 255) / 256), dim3(256), 0, s, dst, src, pairs, npairs, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_sum_shots, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, dst, src, nshots, n);
     return hipGetLastError();
 }
 

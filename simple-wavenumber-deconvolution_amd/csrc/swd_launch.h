@@ -1,4 +1,4 @@
-// swd_launch.h -- host-side launchers of the kernels in swd_kernels.hip and swd_fused.hip.
+// swd_launch.h -- host-side launchers of the kernels in swd_kernels.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "swd_kernels.h"
@@ -11,5 +11,6 @@ hipError_t launch_src_v(const KGrid &g, const KInst *insts, int ninst, int comp,
 hipError_t launch_adjoint_step(const KGrid &g, const KInst *insts, int ninst, int it, bool first, hipStream_t s);
 hipError_t launch_copy_rows(float *gather, const int2 *pairs, int npairs, int nt, hipStream_t s);
 hipError_t launch_sum_rows(float *dst, const float *src, const int2 *pairs, int npairs, int nt, hipStream_t s);
+hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, hipStream_t s);
 hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2 *tw, const float *coef,
                         const KCapCol *cols, const int2 *traces, float *up, float *down, int nt, hipStream_t s);

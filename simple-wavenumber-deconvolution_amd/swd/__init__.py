@@ -63,7 +63,8 @@ class Params(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("fwd_ms", C.c_double), ("bwd_ms", C.c_double), ("fft_ms", C.c_double),
                 ("fwd_launches", C.c_longlong), ("bwd_launches", C.c_longlong),
-                ("fwd_cells", C.c_longlong), ("bwd_cells", C.c_longlong)]
+                ("fwd_cells", C.c_longlong), ("bwd_cells", C.c_longlong),
+                ("ws_bytes", C.c_longlong), ("ckpt_bytes", C.c_longlong), ("full_bytes", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -76,7 +77,8 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
            "swd_plan_set_model", "swd_plan_set_path", "swd_plan_get_stats", "swd_forward", "swd_afor",
-           "swd_rtm", "swd_rtm_dev", "swd_born", "swd_born_adjoint")
+           "swd_rtm", "swd_rtm_dev", "swd_born", "swd_born_adjoint", "swd_born_dev",
+           "swd_born_adjoint_dev")
 
 
 def load():
@@ -263,6 +265,7 @@ class Plan:
     def __init__(self, grid, coefs, vp, rho, device=0, path=SWD_PATH_BATCHED):
         self.L = load()
         self.grid = grid
+        self.device = int(device)
         h = C.c_void_p()
         check(self.L.swd_plan_create(C.byref(grid), device, C.byref(h)))
         self.h = h
@@ -339,3 +342,15 @@ class Plan:
         g = np.zeros(self.grid.nx * self.grid.nz, np.float32)
         check(self.L.swd_born_adjoint(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(d)), _p(g), int(nsnap)))
         return g
+
+    def born_dev(self, shots, wavelet, m_ptr, d_ptr):
+        """Device-pointer Born forward: m [nshots][nx][nz] -> d [sum nrec][nt]."""
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        check(self.L.swd_born_dev(self.h, S.n, S.arr, _p(f32(wavelet)), C.c_void_p(int(m_ptr)),
+                                  C.c_void_p(int(d_ptr))))
+
+    def born_adjoint_dev(self, shots, wavelet, d_ptr, g_ptr, nsnap=0):
+        """Device-pointer Born adjoint: d [sum nrec][nt] -> m_grad [nx][nz] (summed over shots)."""
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        check(self.L.swd_born_adjoint_dev(self.h, S.n, S.arr, _p(f32(wavelet)), C.c_void_p(int(d_ptr)),
+                                          C.c_void_p(int(g_ptr)), int(nsnap)))

@@ -13,6 +13,9 @@ Workloads (interior nz x nx):
   c2  502 x 845, 10 m, dt 0.8 ms, nt 4000, pml 40, Lc 4: layered 1500-4500 m/s with
       8 random dipping reflectors (seed 2); migration model = 15x15 box average;
       20 Hz Ricker at the surface; 401 receivers at ix 420 spaced 5 m (nearest node).
+  c3  the c1 grid and model with nt 4000 for the Born operator; reflectivity
+      m = dK/K a band-limited random field (10-30 Hz equivalent at 1500 m/s,
+      seed 3), data-space probe seed 4, finite-difference directions seeds 5-9.
 """
 import dataclasses
 
@@ -87,6 +90,40 @@ def c1(nt=1000):
     rz = np.arange(10, 191, 2)
     return make_case("c1", 300, 200, 20, 4, 10.0, 10.0, 1e-3, nt, vp, 25.0,
                      [(150, 2, np.full(rz.size, 100), rz)])
+
+
+def bandlimited(nx, nz, dx, dz, fmin, fmax, vmin, seed):
+    """Seeded random field [nx][nz] whose spatial spectrum is confined to the
+    wavenumbers of fmin..fmax Hz at vmin (|k| in [fmin/vmin, fmax/vmin]
+    cycles/m), scaled to max|m| = 1."""
+    rng = np.random.default_rng(seed)
+    W = np.fft.rfft2(rng.standard_normal((nx, nz)))
+    kx = np.fft.fftfreq(nx, dx)[:, None]
+    kz = np.fft.rfftfreq(nz, dz)[None, :]
+    k = np.hypot(kx, kz)
+    m = np.fft.irfft2(W * ((k >= fmin / vmin) & (k <= fmax / vmin)), s=(nx, nz))
+    return (m / np.abs(m).max()).astype(np.float32)
+
+
+def c3(nt=4000):
+    """Born workload: c1 grid/model/geometry, nt steps, and the seeded fields."""
+    c = c1(nt=nt)
+    c.name = "c3"
+    g = c.grid
+    c.born_m = bandlimited(g.nx, g.nz, g.dx, g.dz, 10.0, 30.0, 1500.0, 3)
+    c.born_dirs = [bandlimited(g.nx, g.nz, g.dx, g.dz, 10.0, 30.0, 1500.0, s) for s in range(5, 10)]
+    ntr = c.shots[0][2].size
+    c.born_probe = np.random.default_rng(4).standard_normal((ntr, nt)).astype(np.float32)
+    return c
+
+
+def perturbed_vp(case, m):
+    """Velocity with K -> K (1 + m) on the interior (vp scaled by sqrt(1 + m), in
+    double, rounded once to float); the CPML frame is unchanged."""
+    vp = case.vp.astype(np.float64).copy()
+    p = case.pmlc
+    vp[p:p + case.grid.nx, p:p + case.grid.nz] *= np.sqrt(1.0 + np.asarray(m, np.float64).reshape(case.grid.nx, case.grid.nz))
+    return vp.astype(np.float32)
 
 
 def layered_c2(nx=845, nz=502, seed=2):
