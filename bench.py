@@ -84,6 +84,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-nt", type=int, default=600)
+    ap.add_argument("--tile", type=int, default=0, help="fused tile width (0 = tuned on the first steps)")
+    ap.add_argument("--path", choices=("fused", "batched"), default="fused")
+    ap.add_argument("--groups", type=int, nargs=2, default=(0, 0), metavar=("FWD", "BWD"),
+                    help="shots per fused RTM forward/backward loop (0 = sized to the cache)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy 256 MiB device-to-device once (reference for HBM byte counters)")
     a = ap.parse_args()
@@ -98,7 +102,9 @@ def main():
     log(f"bench: c2 RTM, batch {a.batch} shots, nt {a.nt}")
     case = build_case(a.batch, a.nt)
     g = case.grid
-    plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0)
+    path = swd.SWD_PATH_FUSED if a.path == "fused" else swd.SWD_PATH_BATCHED
+    plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0, path=path, tile=a.tile)
+    plan.set_groups(*a.groups)
     shots = swd.ShotSet(case.shots)
     # observed data on the true model, up-going integration as the adjoint source (RC:603-637)
     plan_true = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0)
@@ -173,7 +179,7 @@ def main():
         plain = {"shots_per_s": 1.0 / tp, "gcell_per_s": rtm_cells(g) / tp / 1e9}
         rel = float(np.linalg.norm(im1[0].cpu().numpy() - img[0]) / max(1e-30, np.linalg.norm(img[0])))
         plain["image_rel_l2_vs_batched"] = rel
-        plan.set_path(swd.SWD_PATH_BATCHED)
+        plan.set_path(path)
         log(f"bench: plain path one shot {tp:.2f}s, image rel diff vs batched {rel:.2e}")
 
     cpu = None

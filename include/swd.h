@@ -81,7 +81,8 @@ typedef struct swd_params {
 
 /* Execution path of a plan. */
 #define SWD_PATH_PLAIN 0   /* one kernel per sub-step (vx, vz, p), one shot per launch */
-#define SWD_PATH_BATCHED 1 /* velocity kernel + pressure kernel per step, whole batch per launch (default) */
+#define SWD_PATH_BATCHED 1 /* velocity kernel + pressure kernel per step, whole batch per launch */
+#define SWD_PATH_FUSED 2   /* one fused kernel per step for the whole batch (default); Born uses BATCHED */
 
 typedef struct swd_stats {
     double fwd_ms;          /* event time of the last forward time loop (all shots of the call) */
@@ -129,6 +130,15 @@ int swd_plan_destroy(void *plan);                                /* variables_fr
 int swd_plan_set_coefs(void *plan, const swd_coefs *c);          /* H2D of rc and CPML vectors (RK:278-297) */
 int swd_plan_set_model(void *plan, const float *vp, const float *rho); /* H2D of vp, rho (RK:281-282) */
 int swd_plan_set_path(void *plan, int path);
+/* Tile width (columns of a 64-row workgroup tile) of the fused path: 16, 32
+ * or 64, or 0 (default) to time the three on the first steps of a plan's
+ * first fused loop for a batch size and keep the fastest.  Results do not
+ * depend on it. */
+int swd_plan_set_tile(void *plan, int tile_x);
+/* Shots per fused RTM loop: the forward and the backward run their batch in
+ * groups of these sizes (0, the default: sized so a group's working set stays
+ * in the last-level cache).  Results do not depend on them. */
+int swd_plan_set_groups(void *plan, int fwd_shots, int bwd_shots);
 int swd_plan_get_stats(void *plan, swd_stats *st);
 
 /* ---------------- operators ----------------

@@ -12,6 +12,7 @@ struct KGrid {
     const float *ax, *axh, *bx, *bxh;   // [ntx]
     const float *az, *azh, *bz, *bzh;   // [ntz]
     const float *Bx, *Bz, *K, *K0;      // [ntx][ldz]: dt*buoyancy (x, z half points), dt*rho*vp^2, rho*vp^2
+    int fx0, fx1, fz0, fz1;             // interior box free of CPML damping (all a == 0), inclusive
 };
 
 // One propagated wavefield ("instance") of a batch and what the kernels do with it.
@@ -39,6 +40,31 @@ struct KInst {
     float *grad;                 // Born adjoint: gradient accumulator [nx][nz]
     float *ex, *ez;              // Born adjoint scratch [ntx][ldz]
     const float *dsum;           // Born adjoint: data summed per receiver node [ntr][nt]
+};
+
+// One wavefield of the fused step kernel (velocity and pressure of a time
+// step in one launch).  State arrays are double-buffered by step parity: a
+// launch reads slot (it+1)&1 of p and writes slot it&1, so no workgroup
+// overwrites what a neighbour still reads as its halo.  The source
+// reconstruction runs its velocity half one step behind, so its velocity
+// slots are swapped (see k_fstep).
+enum KStepMode { KSTEP_FWD = 0, KSTEP_SRC = 1, KSTEP_RCV = 2 };
+struct KStep {
+    int mode;                    // KStepMode
+    int s_ix, s_iz;
+    float *p[2], *vx[2], *vz[2];
+    float *psx[2], *psz[2];      // CPML memory of dp/dx, dp/dz (FWD, RCV)
+    float *pvx, *pvz;            // CPML memory of dvx/dx, dvz/dz, own cells only (FWD, RCV)
+    const float *wavelet;        // FWD: added at it; SRC: subtracted at it+1
+    const int *colmap, *rowmap;  // receiver lookup (FWD records, RCV replaces)
+    float *gather;               // FWD
+    const float *adj;            // RCV
+    float *strips;               // FWD saves, SRC restores
+    float2 *capture;             // FWD (AFor)
+    const int *colcap;
+    int capcomp, ncap, nfftz;
+    float *dpdt[2];              // SRC writes slot it&1; RCV reads slot (it+1)&1
+    float *image, *illum_s, *illum_r;   // RCV
 };
 
 // Column descriptor of the kz separation: one receiver column of one shot.

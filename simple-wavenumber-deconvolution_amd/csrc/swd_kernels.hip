@@ -16,79 +16,17 @@
 // and psi' = b*0 + 0*d = 0), so skipping them leaves every result unchanged.
 #include <algorithm>
 
+#include "swd_device.h"
 #include "swd_kernels.h"
 #include "swd_launch.h"
 
 namespace {
-
-__device__ __forceinline__ int frame_index(const KGrid &g, int ix, int iz)
-{
-    const int jx = ix - g.pmlc, jz = iz - g.pmlc, Lc = g.Lc, nx = g.nx, nz = g.nz;
-    if (jx < 0 || jx >= nx || jz < 0 || jz >= nz) return -1;
-    if (jx < Lc) return jx * nz + jz;
-    if (jx >= nx - Lc) return (jx - (nx - 2 * Lc)) * nz + jz;
-    if (jz < Lc) return 2 * Lc * nz + (jx - Lc) * 2 * Lc + jz;
-    if (jz >= nz - Lc) return 2 * Lc * nz + (jx - Lc) * 2 * Lc + Lc + (jz - (nz - Lc));
-    return -1;
-}
-
-__device__ __forceinline__ bool interior(const KGrid &g, int ix, int iz)
-{
-    return ix >= g.pmlc && ix < g.ntx - g.pmlc && iz >= g.pmlc && iz < g.ntz - g.pmlc;
-}
-
-// Forward difference from the integer grid to the half grid: the derivative
-// the velocity update uses (RK:55-58, RK:104-107).
-template <int LC>
-__device__ __forceinline__ float dplus(const float *__restrict__ f, size_t ip, size_t s, const float *rc, float inv)
-{
-    float d = 0.f;
-#pragma unroll
-    for (int ic = 0; ic < LC; ic++) d = __builtin_fmaf(rc[ic] * (f[ip + (ic + 1) * s] - f[ip - ic * s]), inv, d);
-    return d;
-}
-
-// Backward difference from the half grid to the integer grid: the derivative
-// the pressure update uses (RK:162-166).
-template <int LC>
-__device__ __forceinline__ float dminus(const float *__restrict__ f, size_t ip, size_t s, const float *rc, float inv)
-{
-    float d = 0.f;
-#pragma unroll
-    for (int ic = 0; ic < LC; ic++) d = __builtin_fmaf(rc[ic] * (f[ip + ic * s] - f[ip - (ic + 1) * s]), inv, d);
-    return d;
-}
 
 // ---- quad kernels: one thread updates 4 consecutive z cells of one column ----
 // A workgroup covers 64 rows x 16 columns (a wavefront = 4 columns x 64 rows,
 // each column read as one 256-byte segment with float4 accesses).  Workgroups
 // are ordered instance-fastest and handed to the XCDs in contiguous ranges, so
 // the shots of one tile share the model and the halo columns in one L2.
-typedef __attribute__((address_space(1))) float gfl;
-typedef __attribute__((address_space(1))) const float gcfl;
-
-__device__ __forceinline__ int xcd_block()
-{
-    const int b = blockIdx.x, per = gridDim.x >> 3;
-    return (b & 7) * per + (b >> 3);
-}
-
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(1))) const v4f gcv4;
-typedef __attribute__((address_space(1))) v4f gv4;
-__device__ __forceinline__ float4 ld4(gcfl *f, size_t o)
-{
-    const v4f v = *(gcv4 *)(f + o);
-    return make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ void st4(gfl *f, size_t o, float4 v)
-{
-    v4f w = {v.x, v.y, v.z, v.w};
-    *(gv4 *)(f + o) = w;
-}
-__device__ __forceinline__ float &el(float4 &v, int k) { return reinterpret_cast<float *>(&v)[k]; }
-__device__ __forceinline__ float elc(const float4 &v, int k) { return reinterpret_cast<const float *>(&v)[k]; }
-
 #define QUAD_INDEX                                                         \
     const int L = xcd_block();                                             \
     if (L >= ninst * nzb * nxb) return;                                    \
@@ -101,19 +39,6 @@ __device__ __forceinline__ float elc(const float4 &v, int k) { return reinterpre
     const size_t ldz = g.ldz;                                              \
     const size_t o = static_cast<size_t>(x) * ldz + z;
 
-// rows z-4R .. z+4R+3 of one column (R quads of halo on each side), zero outside
-template <int R>
-__device__ __forceinline__ void column_window(gcfl *f, size_t o, int z, int ldz, float (&a)[4 * (2 * R + 1)])
-{
-#pragma unroll
-    for (int q = -R; q <= R; q++) {
-        const int zq = z + 4 * q;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (zq >= 0 && zq < ldz) v = ld4(f, o + 4 * q);
-#pragma unroll
-        for (int k = 0; k < 4; k++) a[4 * (q + R) + k] = elc(v, k);
-    }
-}
 
 // Velocity update with CPML (RK:24-118).  NEG: receiver back-propagation,
 // called by the reference with (-dx, -dz, +dt) and widened domains
@@ -301,7 +226,7 @@ __global__ __launch_bounds__(256) void k_p(KGrid g, const KInst *__restrict__ in
             }
         }
     }
-    if (!NEG && I.wavelet && x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4) el(pn, I.s_iz - z) += I.wavelet[it];   // RK:175-178
+    if (!NEG && I.wavelet && x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4) add_at(pn, I.s_iz - z, I.wavelet[it]);   // RK:175-178
     if (I.colmap) {
         const int cm = I.colmap[x];
         if (cm >= 0) {
@@ -391,7 +316,7 @@ __global__ __launch_bounds__(256) void k_src_p(KGrid g, const KInst *__restrict_
         }
         st4((gfl *)I.dpdt, o, dp);
     }
-    if (x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4) el(pn, I.s_iz - z) -= I.wavelet[it + 1];
+    if (x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4) add_at(pn, I.s_iz - z, -I.wavelet[it + 1]);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int f = frame_index(g, x, z + k);
@@ -673,19 +598,6 @@ __global__ __launch_bounds__(256) void k_kzsep(const float2 *__restrict__ captur
 // ---------------------------------------------------------------- launchers
 static inline dim3 cell_grid(const KGrid &g, int ninst) { return dim3((g.ntz + 63) / 64, (g.ntx + 3) / 4, ninst); }
 static const dim3 kCellBlock(64, 4, 1);
-
-#define DISPATCH_LC(LC, CALL)                          \
-    switch (LC) {                                      \
-    case 1: { constexpr int L = 1; CALL; } break;      \
-    case 2: { constexpr int L = 2; CALL; } break;      \
-    case 3: { constexpr int L = 3; CALL; } break;      \
-    case 4: { constexpr int L = 4; CALL; } break;      \
-    case 5: { constexpr int L = 5; CALL; } break;      \
-    case 6: { constexpr int L = 6; CALL; } break;      \
-    case 7: { constexpr int L = 7; CALL; } break;      \
-    case 8: { constexpr int L = 8; CALL; } break;      \
-    default: return hipErrorInvalidValue;              \
-    }
 
 struct QGeom {
     int nzb, nxb, nblocks;

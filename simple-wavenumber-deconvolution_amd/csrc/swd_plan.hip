@@ -79,7 +79,7 @@ struct Plan {
     swd_grid g{};
     int pmlc = 0, ntx = 0, ntz = 0, ldz = 0, nframe = 0;
     size_t nfield = 0;   // ntx*ldz
-    int path = SWD_PATH_BATCHED;
+    int path = SWD_PATH_FUSED;
     bool have_coefs = false, have_model = false;
     float rc[SWD_MAX_LC] = {};
     float *d_coef = nullptr;    // 4*ntx + 4*ntz
@@ -87,8 +87,13 @@ struct Plan {
     char *ws = nullptr;
     size_t ws_cap = 0;
     hipEvent_t ev[4] = {};
+    hipEvent_t tev[4] = {};     // autotuner
     swd_stats stats{};
     KGrid kg{};
+    int tile_req = 0;           // fused tile width asked for (0 = tune)
+    int group_fwd = 0, group_bwd = 0;     // RTM shots per fused forward / backward loop (0 = sized to the cache)
+    int tile_fwd = 0, tile_bwd = 0;       // tuned widths and the batch they were tuned for
+    int tile_fwd_n = 0, tile_bwd_n = 0;
 };
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -235,6 +240,7 @@ extern "C" int swd_plan_create(const swd_grid *g, int device, void **out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreate(&P->stream);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&P->ev[i]);
+    for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&P->tev[i]);
     if (e == hipSuccess) e = hipMalloc(&P->d_coef, sizeof(float) * 4 * (P->ntx + P->ntz));
     if (e == hipSuccess) e = hipMalloc(&P->d_model, sizeof(float) * 4 * P->nfield);
     if (e == hipSuccess) e = hipMemset(P->d_model, 0, sizeof(float) * 4 * P->nfield);
@@ -247,6 +253,8 @@ extern "C" int swd_plan_create(const swd_grid *g, int device, void **out)
     k.Lc = g->Lc; k.pmlc = P->pmlc; k.nx = g->nx; k.nz = g->nz; k.nt = g->nt; k.nframe = P->nframe;
     k.idx = static_cast<float>(1.0 / g->dx);   // float one_over_dx = 1.0/dx (RK:47)
     k.idz = static_cast<float>(1.0 / g->dz);
+    k.fx0 = k.fz0 = 1;   // no fast tiles until the coefficients are known
+    k.fx1 = k.fz1 = 0;
     float *c = P->d_coef;
     k.ax = c; k.axh = c + P->ntx; k.bx = c + 2 * P->ntx; k.bxh = c + 3 * P->ntx;
     float *cz = c + 4 * P->ntx;
@@ -266,6 +274,8 @@ extern "C" int swd_plan_destroy(void *plan)
     if (P->d_coef) (void)hipFree(P->d_coef);
     if (P->d_model) (void)hipFree(P->d_model);
     for (auto &e : P->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : P->tev)
         if (e) (void)hipEventDestroy(e);
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -288,6 +298,21 @@ extern "C" int swd_plan_set_coefs(void *plan, const swd_coefs *c)
     }
     HIP_TRY(hipMemcpy(P->d_coef, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
     for (int i = 0; i < P->g.Lc; i++) P->kg.rc[i] = c->rc[i];
+    // interior box where every damping coefficient is zero (the fused kernel's fast tiles)
+    auto box = [&](const float *a, const float *ah, int n, int &lo, int &hi) {
+        lo = P->pmlc;
+        hi = n - P->pmlc - 1;
+        while (lo <= hi && (a[lo] != 0.f || ah[lo] != 0.f)) lo++;
+        while (hi >= lo && (a[hi] != 0.f || ah[hi] != 0.f)) hi--;
+        for (int i = lo; i <= hi; i++)
+            if (a[i] != 0.f || ah[i] != 0.f) {
+                lo = 1;
+                hi = 0;
+                break;
+            }
+    };
+    box(c->a_x, c->a_x_half, P->ntx, P->kg.fx0, P->kg.fx1);
+    box(c->a_z, c->a_z_half, P->ntz, P->kg.fz0, P->kg.fz1);
     P->have_coefs = true;
     return SWD_OK;
 }
@@ -319,8 +344,28 @@ extern "C" int swd_plan_set_model(void *plan, const float *vp, const float *rho)
 
 extern "C" int swd_plan_set_path(void *plan, int path)
 {
-    if (!plan || (path != SWD_PATH_PLAIN && path != SWD_PATH_BATCHED)) return swd_fail(SWD_E_ARG, "swd_plan_set_path: bad path");
+    if (!plan || (path != SWD_PATH_PLAIN && path != SWD_PATH_BATCHED && path != SWD_PATH_FUSED))
+        return swd_fail(SWD_E_ARG, "swd_plan_set_path: bad path %d", path);
     static_cast<Plan *>(plan)->path = path;
+    return SWD_OK;
+}
+
+extern "C" int swd_plan_set_groups(void *plan, int fwd_shots, int bwd_shots)
+{
+    if (!plan || fwd_shots < 0 || bwd_shots < 0) return swd_fail(SWD_E_ARG, "swd_plan_set_groups: bad arguments");
+    Plan *P = static_cast<Plan *>(plan);
+    P->group_fwd = fwd_shots;
+    P->group_bwd = bwd_shots;
+    return SWD_OK;
+}
+
+extern "C" int swd_plan_set_tile(void *plan, int tile_x)
+{
+    if (!plan || (tile_x != 0 && tile_x != 16 && tile_x != 32 && tile_x != 64))
+        return swd_fail(SWD_E_ARG, "swd_plan_set_tile: tile width %d not in {0, 16, 32, 64}", tile_x);
+    Plan *P = static_cast<Plan *>(plan);
+    P->tile_req = tile_x;
+    P->tile_fwd = P->tile_bwd = P->tile_fwd_n = P->tile_bwd_n = 0;
     return SWD_OK;
 }
 
@@ -389,16 +434,140 @@ void bind_fields(KInst &I, float *f, size_t nfield)
     I.psx = f + 3 * nfield; I.psz = f + 4 * nfield; I.pvx = f + 5 * nfield; I.pvz = f + 6 * nfield;
 }
 
+// Arrays per wavefield of the fused path: p, vx, vz, psx, psz in two slots,
+// then pvx, pvz.
+constexpr int kStepArrays = 12;
+
+void bind_step(KStep &S, float *f, size_t nf)
+{
+    for (int k = 0; k < 2; k++) {
+        S.p[k] = f + k * nf;
+        S.vx[k] = f + (2 + k) * nf;
+        S.vz[k] = f + (4 + k) * nf;
+        S.psx[k] = f + (6 + k) * nf;
+        S.psz[k] = f + (8 + k) * nf;
+    }
+    S.pvx = f + 10 * nf;
+    S.pvz = f + 11 * nf;
+}
+
+int upload_steps(Plan &P, const std::vector<KStep> &h, size_t o)
+{
+    HIP_TRY(hipMemcpyAsync(P.ws + o, h.data(), sizeof(KStep) * h.size(), hipMemcpyHostToDevice, P.stream));
+    return SWD_OK;
+}
+
+// Runs the fused launches its[0], its[1], ... (time steps in execution order)
+// of n instances.  The tile width is the plan's requested one, else the one
+// tuned earlier for this batch size, else it is tuned here: after two warm-up
+// launches, six launches with each candidate width are timed and the fastest
+// width runs the rest (the results do not depend on the width).
+int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int first_it, int &tile, int &tile_n,
+                long long *launches)
+{
+    int tx = P.tile_req;
+    if (tx == 0 && tile_n == n) tx = tile;
+    size_t k = 0;
+    auto launch = [&](int w) -> int {
+        const int it = its[k++];
+        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, P.stream));
+        (*launches)++;
+        return SWD_OK;
+    };
+    if (tx == 0) {
+        constexpr int kWarm = 2, kTry = 6;
+        const int cand[3] = {16, 32, 64};
+        if (its.size() >= static_cast<size_t>(kWarm + 3 * kTry)) {
+            for (int j = 0; j < kWarm; j++) SWD_TRY(launch(32));
+            HIP_TRY(hipEventRecord(P.tev[0], P.stream));
+            for (int c = 0; c < 3; c++) {
+                for (int j = 0; j < kTry; j++) SWD_TRY(launch(cand[c]));
+                HIP_TRY(hipEventRecord(P.tev[c + 1], P.stream));
+            }
+            HIP_TRY(hipEventSynchronize(P.tev[3]));
+            float best = 0.f;
+            for (int c = 0; c < 3; c++) {
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, P.tev[c], P.tev[c + 1]));
+                if (c == 0 || ms < best) {
+                    best = ms;
+                    tx = cand[c];
+                }
+            }
+            tile = tx;
+            tile_n = n;
+        } else {
+            tx = 32;
+        }
+    }
+    while (k < its.size()) SWD_TRY(launch(tx));
+    return SWD_OK;
+}
+
+std::vector<int> steps_up(int nt)
+{
+    std::vector<int> v(nt);
+    for (int it = 0; it < nt; it++) v[it] = it;
+    return v;
+}
+
+// Forward time loop of the fused path.
+int fused_forward_loop(Plan &P, const KStep *d, int n)
+{
+    long long launches = 0;
+    stats_begin(P, 0);
+    SWD_TRY(fused_steps(P, d, n, steps_up(P.g.nt), -1, P.tile_fwd, P.tile_fwd_n, &launches));
+    P.stats.fwd_ms = stats_end(P, 0);
+    P.stats.fwd_launches = launches;
+    P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * P.g.nt * n;
+    return SWD_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ operators
 namespace {
 
 // Forward modelling of a batch; gathers stay on the device at d_gather.
+int forward_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, const float *wavelet, float *d_gather)
+{
+    const int nt = P.g.nt;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KStep>(nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_f = c.add<float>(kStepArrays * P.nfield * nshots);
+    SWD_TRY(ensure_ws(P, c.off));
+    SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
+    zero_fields(P, at<float>(P, o_f), kStepArrays * P.nfield * nshots);
+    std::vector<KStep> h(nshots);
+    for (int k = 0; k < nshots; k++) {
+        KStep &S = h[k];
+        std::memset(&S, 0, sizeof S);
+        S.mode = KSTEP_FWD;
+        bind_step(S, at<float>(P, o_f) + kStepArrays * P.nfield * k, P.nfield);
+        S.wavelet = at<float>(P, o_wav);
+        S.s_ix = shots[k].s_ix; S.s_iz = shots[k].s_iz;
+        if (shots[k].nrec > 0) {
+            S.colmap = at<int>(P, o_ints) + b.col_off[k];
+            S.rowmap = at<int>(P, o_ints) + b.row_off[k];
+            S.gather = d_gather + static_cast<size_t>(b.toff[k]) * nt;
+        }
+    }
+    SWD_TRY(upload_steps(P, h, o_inst));
+    SWD_TRY(fused_forward_loop(P, at<KStep>(P, o_inst), nshots));
+    HIP_TRY(launch_copy_rows(d_gather, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
+    HIP_TRY(hipStreamSynchronize(P.stream));
+    return SWD_OK;
+}
+
 int forward_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet, float *d_gather)
 {
     Batch b;
     SWD_TRY(build_batch(P, nshots, shots, b));
+    if (P.path == SWD_PATH_FUSED) return forward_dev_fused(P, nshots, shots, b, wavelet, d_gather);
     const int nt = P.g.nt;
     Carver c;
     const size_t o_ints = c.add<int>(b.ints.size());
@@ -506,7 +675,9 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     Carver c;
     const size_t o_ints = c.add<int>(b.ints.size());
     const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
-    const size_t o_inst = c.add<KInst>(2 * nshots);
+    const bool fused = P->path == SWD_PATH_FUSED;
+    const int per_wf = fused ? kStepArrays : kFieldArrays;
+    const size_t o_inst = fused ? c.add<KStep>(2 * nshots) : c.add<KInst>(2 * nshots);
     const size_t o_wav = c.add<float>(2 * nt);
     const size_t o_colcap = c.add<int>(colcap.size());
     const size_t o_cols = c.add<KCapCol>(ncap);
@@ -515,7 +686,7 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     const size_t o_coef = c.add<float>(nfftz);
     const size_t o_gat = c.add<float>(3 * std::max<size_t>(ng, 1));
     const size_t o_cap = c.add<float2>(static_cast<size_t>(nt) * ncap * nfftz);
-    const size_t o_f = c.add<float>(kFieldArrays * P->nfield * 2 * nshots);
+    const size_t o_f = c.add<float>(per_wf * P->nfield * 2 * nshots);
     SWD_TRY(ensure_ws(*P, c.off));
     SWD_TRY(upload_batch(*P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P->ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P->stream));
@@ -528,10 +699,36 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     HIP_TRY(hipMemcpyAsync(P->ws + o_tw, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice, P->stream));
     HIP_TRY(hipMemcpyAsync(P->ws + o_coef, kzcoef, sizeof(float) * nfftz, hipMemcpyHostToDevice, P->stream));
     HIP_TRY(hipMemsetAsync(P->ws + o_cap, 0, sizeof(float2) * static_cast<size_t>(nt) * ncap * nfftz, P->stream));
-    zero_fields(*P, at<float>(*P, o_f), kFieldArrays * P->nfield * 2 * nshots);
+    zero_fields(*P, at<float>(*P, o_f), per_wf * P->nfield * 2 * nshots);
     float *d_gather = at<float>(*P, o_gat), *d_up = d_gather + ng, *d_down = d_up + ng;
-    std::vector<KInst> h(2 * nshots);
-    for (int k = 0; k < nshots; k++)
+    if (fused) {
+        std::vector<KStep> hs(2 * nshots);
+        for (int k = 0; k < nshots; k++)
+            for (int w = 0; w < 2; w++) {
+                KStep &S = hs[2 * k + w];
+                std::memset(&S, 0, sizeof S);
+                S.mode = KSTEP_FWD;
+                bind_step(S, at<float>(*P, o_f) + kStepArrays * P->nfield * (2 * k + w), P->nfield);
+                S.wavelet = at<float>(*P, o_wav) + w * nt;   // p: rick, htp: hrick (FK:707, 747)
+                S.s_ix = shots[k].s_ix; S.s_iz = shots[k].s_iz;
+                if (w == 0 && shots[k].nrec > 0) {
+                    S.colmap = at<int>(*P, o_ints) + b.col_off[k];
+                    S.rowmap = at<int>(*P, o_ints) + b.row_off[k];
+                    S.gather = d_gather + static_cast<size_t>(b.toff[k]) * nt;
+                }
+                if (shots[k].nrec > 0) {
+                    S.capture = at<float2>(*P, o_cap) + cap0[k] * static_cast<size_t>(nfftz);
+                    S.colcap = at<int>(*P, o_colcap) + static_cast<size_t>(k) * P->ntx;
+                    S.capcomp = w;
+                    S.ncap = ncap;
+                    S.nfftz = nfftz;
+                }
+            }
+        SWD_TRY(upload_steps(*P, hs, o_inst));
+        SWD_TRY(fused_forward_loop(*P, at<KStep>(*P, o_inst), 2 * nshots));
+    }
+    std::vector<KInst> h(fused ? 0 : 2 * nshots);
+    for (int k = 0; !fused && k < nshots; k++)
         for (int w = 0; w < 2; w++) {
             KInst &I = h[2 * k + w];
             std::memset(&I, 0, sizeof I);
@@ -551,8 +748,10 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
                 I.nfftz = nfftz;
             }
         }
-    SWD_TRY(upload_insts(*P, h, o_inst));
-    SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), 2, nshots));
+    if (!fused) {
+        SWD_TRY(upload_insts(*P, h, o_inst));
+        SWD_TRY(forward_loop(*P, at<KInst>(*P, o_inst), 2, nshots));
+    }
     HIP_TRY(launch_copy_rows(d_gather, at<int2>(*P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P->stream));
     if (!cols.empty()) {
         stats_begin(*P, 1);
@@ -567,6 +766,144 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     return SWD_OK;
 }
 
+namespace {
+
+// Shots per group of the fused RTM loops.  The per-shot working set of a
+// step (double-buffered state, CPML memories in the frame, and on the
+// backward the reconstruction, images and dp/dt) is kept within a budget of
+// the 256 MB last-level cache, leaving room for the model, the halos and the
+// streamed strips; groups are then evened out.
+constexpr double kCacheBudget = 160.0 * (1 << 20);
+
+int even_groups(int n, int g)
+{
+    g = std::max(1, std::min(n, g));
+    const int ng = (n + g - 1) / g;
+    return (n + ng - 1) / ng;
+}
+
+void rtm_groups(const Plan &P, int nshots, int &gf, int &gb)
+{
+    const double N = static_cast<double>(P.ntx) * P.ntz, Nint = static_cast<double>(P.g.nx) * P.g.nz;
+    const double Ncpml = N - Nint;
+    const double fwd = 4.0 * (6 * N + 6 * Ncpml);
+    const double bwd = 4.0 * (6 * Nint + 6 * N + 6 * Ncpml + 5 * Nint);
+    gf = P.group_fwd > 0 ? P.group_fwd : static_cast<int>(kCacheBudget / fwd);
+    gb = P.group_bwd > 0 ? P.group_bwd : static_cast<int>(kCacheBudget / bwd);
+    gf = even_groups(nshots, gf);
+    gb = even_groups(nshots, gb);
+}
+
+// RTM of a batch on the fused path: one launch per forward step, one launch
+// per backward step for the reconstruction and receiver wavefields of every
+// shot, imaging one launch behind (k_fstep).
+int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, const float *wavelet, const float *adj,
+                  float *image, float *illum_src, float *illum_rcv, float *gather_syn)
+{
+    const int nt = P.g.nt;
+    const size_t nf = P.nfield;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_fwd = c.add<KStep>(nshots);
+    const size_t o_bwd = c.add<KStep>(2 * nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_strips = c.add<float>(static_cast<size_t>(nt) * P.nframe * nshots);
+    const size_t o_img = c.add<float>(5 * nf * nshots);   // image, illum_s, illum_r, dpdt[2]
+    const size_t o_f = c.add<float>(kStepArrays * nf * 2 * nshots);   // source side, then receiver side
+    SWD_TRY(ensure_ws(P, c.off));
+    SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
+    zero_fields(P, at<float>(P, o_f), kStepArrays * nf * 2 * nshots);
+    zero_fields(P, at<float>(P, o_img), 5 * nf * nshots);
+    float *fields = at<float>(P, o_f), *imgs = at<float>(P, o_img);
+    std::vector<KStep> hf(nshots), hb(2 * nshots);
+    for (int k = 0; k < nshots; k++) {
+        float *fs = fields + kStepArrays * nf * k, *fr = fields + kStepArrays * nf * (nshots + k);
+        float *im = imgs + 5 * nf * k;
+        KStep &F = hf[k], &S = hb[2 * k], &R = hb[2 * k + 1];
+        std::memset(&F, 0, sizeof F);
+        std::memset(&S, 0, sizeof S);
+        std::memset(&R, 0, sizeof R);
+        F.mode = KSTEP_FWD;
+        S.mode = KSTEP_SRC;
+        R.mode = KSTEP_RCV;
+        bind_step(F, fs, nf);
+        bind_step(S, fs, nf);
+        bind_step(R, fr, nf);
+        F.wavelet = S.wavelet = at<float>(P, o_wav);
+        F.s_ix = S.s_ix = R.s_ix = shots[k].s_ix;
+        F.s_iz = S.s_iz = R.s_iz = shots[k].s_iz;
+        F.strips = S.strips = at<float>(P, o_strips) + static_cast<size_t>(nt) * P.nframe * k;
+        if (shots[k].nrec > 0) {
+            F.colmap = R.colmap = at<int>(P, o_ints) + b.col_off[k];
+            F.rowmap = R.rowmap = at<int>(P, o_ints) + b.row_off[k];
+            if (gather_syn) F.gather = gather_syn + static_cast<size_t>(b.toff[k]) * nt;
+            R.adj = adj + static_cast<size_t>(b.toff[k]) * nt;
+        }
+        S.dpdt[0] = R.dpdt[0] = im + 3 * nf;
+        S.dpdt[1] = R.dpdt[1] = im + 4 * nf;
+        R.image = im;
+        R.illum_s = im + nf;
+        R.illum_r = im + 2 * nf;
+    }
+    SWD_TRY(upload_steps(P, hf, o_fwd));
+    SWD_TRY(upload_steps(P, hb, o_bwd));
+    // Shots run in groups whose per-step working set stays in the last-level
+    // cache; every shot keeps its strips and final state, so the groups of the
+    // backward need not match those of the forward.
+    int gf = 0, gb = 0;
+    rtm_groups(P, nshots, gf, gb);
+    {
+        long long launches = 0;
+        stats_begin(P, 0);
+        for (int k0 = 0; k0 < nshots; k0 += gf) {
+            const int n = std::min(gf, nshots - k0);
+            SWD_TRY(fused_steps(P, at<KStep>(P, o_fwd) + k0, n, steps_up(nt), -1, P.tile_fwd, P.tile_fwd_n, &launches));
+        }
+        P.stats.fwd_ms = stats_end(P, 0);
+        P.stats.fwd_launches = launches;
+        P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * nt * nshots;
+    }
+    if (gather_syn)
+        HIP_TRY(launch_copy_rows(gather_syn, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
+    // outside the interior the reconstruction keeps the final forward state: both slots hold it
+    const int fin = (nt - 1) & 1, oth = (nt - 2) & 1;
+    for (int k = 0; k < nshots; k++) {
+        const KStep &F = hf[k];
+        HIP_TRY(hipMemcpyAsync(F.p[oth], F.p[fin], sizeof(float) * nf, hipMemcpyDeviceToDevice, P.stream));
+        HIP_TRY(hipMemcpyAsync(F.vx[oth], F.vx[fin], sizeof(float) * nf, hipMemcpyDeviceToDevice, P.stream));
+        HIP_TRY(hipMemcpyAsync(F.vz[oth], F.vz[fin], sizeof(float) * nf, hipMemcpyDeviceToDevice, P.stream));
+    }
+    // backward: it = nt-2 .. 0 (RK:1073), then the imaging of step 0
+    std::vector<int> its;
+    for (int it = nt - 2; it >= 0; it--) its.push_back(it);
+    long long launches = 0;
+    stats_begin(P, 1);
+    for (int k0 = 0; k0 < nshots; k0 += gb) {
+        const int n = std::min(gb, nshots - k0);
+        const KStep *d = at<KStep>(P, o_bwd) + 2 * k0;
+        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tile_bwd, P.tile_bwd_n, &launches));
+        HIP_TRY(launch_fimage(P.kg, d, 2 * n, P.stream));
+    }
+    P.stats.bwd_ms = stats_end(P, 1);
+    P.stats.bwd_launches = launches;
+    P.stats.bwd_cells = static_cast<long long>(nt - 1) * nshots *
+                        (static_cast<long long>(P.ntx) * P.ntz + static_cast<long long>(P.g.nx) * P.g.nz);
+    const size_t ntp = static_cast<size_t>(P.ntx) * P.ntz;
+    for (int k = 0; k < nshots; k++) {
+        float *outs[3] = {image + ntp * k, illum_src + ntp * k, illum_rcv + ntp * k};
+        for (int q = 0; q < 3; q++)
+            HIP_TRY(hipMemcpy2DAsync(outs[q], sizeof(float) * P.ntz, imgs + 5 * nf * k + q * nf, sizeof(float) * P.ldz,
+                                     sizeof(float) * P.ntz, P.ntx, hipMemcpyDeviceToDevice, P.stream));
+    }
+    HIP_TRY(hipStreamSynchronize(P.stream));
+    P.stats.ws_bytes = static_cast<long long>(P.ws_cap);
+    return SWD_OK;
+}
+
+}  // namespace
+
 extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
                            float *image, float *illum_src, float *illum_rcv, float *gather_syn)
 {
@@ -576,6 +913,8 @@ extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const 
         return swd_fail(SWD_E_ARG, "swd_rtm: null argument");
     Batch b;
     SWD_TRY(build_batch(*P, nshots, shots, b));
+    if (P->path == SWD_PATH_FUSED)
+        return rtm_dev_fused(*P, nshots, shots, b, wavelet, adj, image, illum_src, illum_rcv, gather_syn);
     const int nt = P->g.nt;
     const size_t nf = P->nfield;
     Carver c;

@@ -33,6 +33,7 @@ std::vector<Gathers> run_all(const drv::Setup &s, const drv::Options &o, const s
     check(swd_plan_set_coefs(plan, &c), "coefficients");
     check(swd_plan_set_model(plan, vp.data(), rho.data()), "model");
     check(swd_plan_set_path(plan, o.path), "path");
+    check(swd_plan_set_tile(plan, o.tile), "tile");
     const int nt = s.p.itmax;
     std::vector<Gathers> out(s.shots.size());
     for (size_t b0 = 0; b0 < s.shots.size(); b0 += o.batch) {

@@ -46,6 +46,7 @@ int main(int argc, char **argv)
     check(swd_plan_set_coefs(plan, &c), "coefficients");
     check(swd_plan_set_model(plan, vp.data(), rho.data()), "model");
     check(swd_plan_set_path(plan, o.path), "path");
+    check(swd_plan_set_tile(plan, o.tile), "tile");
     std::printf("====================\n %s RTM BEGIN\n====================\n", p.scatter_flag ? "PSF" : "NORMAL");
 
     std::vector<float> tmp1(s.np, 0.f), tmp3(s.np);

@@ -11,7 +11,8 @@ namespace drv {
 struct Options {
     int batch = 16;          // shots per GPU batch
     int device = 0;
-    int path = SWD_PATH_BATCHED;
+    int path = SWD_PATH_FUSED;
+    int tile = 0;            // fused tile width (0 = tuned)
     std::string workdir;     // run as if started in this directory
 };
 Options parse_options(int argc, char **argv, const char *prog);

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libswd.so")
 
 SWD_PATH_PLAIN = 0
 SWD_PATH_BATCHED = 1
+SWD_PATH_FUSED = 2
 ERRORS = {-1: "SWD_E_ARG", -2: "SWD_E_DEVICE", -3: "SWD_E_NOMEM", -4: "SWD_E_IO", -5: "SWD_E_STATE",
           -6: "SWD_E_RANGE"}
 
@@ -76,7 +77,7 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_cpml", "swd_model_in_pml", "swd_read_model", "swd_smooth_model", "swd_nfft", "swd_kz_taper",
            "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
-           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_get_stats", "swd_forward", "swd_afor",
+           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_groups", "swd_plan_get_stats", "swd_forward", "swd_afor",
            "swd_rtm", "swd_rtm_dev", "swd_born", "swd_born_adjoint", "swd_born_dev",
            "swd_born_adjoint_dev")
 
@@ -262,7 +263,7 @@ class ShotSet:
 class Plan:
     """A device plan: grid, coefficients and model resident on one GPU."""
 
-    def __init__(self, grid, coefs, vp, rho, device=0, path=SWD_PATH_BATCHED):
+    def __init__(self, grid, coefs, vp, rho, device=0, path=SWD_PATH_FUSED, tile=0):
         self.L = load()
         self.grid = grid
         self.device = int(device)
@@ -275,6 +276,7 @@ class Plan:
         check(self.L.swd_plan_set_coefs(self.h, C.byref(c)))
         self.set_model(vp, rho)
         self.set_path(path)
+        self.set_tile(tile)
 
     def set_model(self, vp, rho):
         vp, rho = f32(vp).reshape(-1), f32(rho).reshape(-1)
@@ -283,6 +285,14 @@ class Plan:
 
     def set_path(self, path):
         check(self.L.swd_plan_set_path(self.h, int(path)))
+
+    def set_groups(self, fwd_shots=0, bwd_shots=0):
+        """Shots per fused RTM forward / backward loop (0 = sized to the cache)."""
+        check(self.L.swd_plan_set_groups(self.h, int(fwd_shots), int(bwd_shots)))
+
+    def set_tile(self, tile):
+        """Fused-path tile width (16, 32, 64) or 0 to tune it on the first steps."""
+        check(self.L.swd_plan_set_tile(self.h, int(tile)))
 
     def close(self):
         if self.h:

@@ -14,13 +14,14 @@ from helpers import oracle_problem, rel_l2, report
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
+PATHS = [swd.SWD_PATH_FUSED, swd.SWD_PATH_BATCHED, swd.SWD_PATH_PLAIN]
 
 
-def plan_of(case, path=swd.SWD_PATH_BATCHED):
-    return swd.Plan(case.grid, case.coefs, case.vp, case.rho, path=path)
+def plan_of(case, path=swd.SWD_PATH_FUSED, tile=0):
+    return swd.Plan(case.grid, case.coefs, case.vp, case.rho, path=path, tile=tile)
 
 
-@pytest.mark.parametrize("path", [swd.SWD_PATH_BATCHED, swd.SWD_PATH_PLAIN])
+@pytest.mark.parametrize("path", PATHS)
 def test_forward_small(path):
     c = configs.small()
     print(f"case small forward path={path}", flush=True)
@@ -66,7 +67,7 @@ def test_forward_c1():
     assert e32 <= TOL and e64 <= TOL
 
 
-@pytest.mark.parametrize("path", [swd.SWD_PATH_BATCHED, swd.SWD_PATH_PLAIN])
+@pytest.mark.parametrize("path", PATHS)
 def test_rtm_small(path):
     c = configs.small(nt=300)
     print(f"case small rtm path={path}", flush=True)
@@ -84,13 +85,14 @@ def test_rtm_small(path):
         assert e32 <= TOL
 
 
-def test_afor_small():
+@pytest.mark.parametrize("path", [swd.SWD_PATH_FUSED, swd.SWD_PATH_BATCHED])
+def test_afor_small(path):
     c = configs.small(nt=300)
-    print("case small afor", flush=True)
+    print(f"case small afor path={path}", flush=True)
     nfftz = swd.nfft(c.grid.ntz)
     kz, _ = swd.kz_taper(nfftz, c.grid.dz)
     hw = swd.hilbert1d(c.wavelet)
-    g, up, dn = plan_of(c).afor(c.shots, c.wavelet, hw, kz)
+    g, up, dn = plan_of(c, path).afor(c.shots, c.wavelet, hw, kz)
     P = oracle_problem(c)
     og, ou, od = P.afor(hw, kz)
     og64, ou64, od64 = P.afor(hw, kz, np.float64)
@@ -120,7 +122,7 @@ def test_born_and_adjoint_small():
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
 
 
-@pytest.mark.parametrize("path", [swd.SWD_PATH_BATCHED, swd.SWD_PATH_PLAIN])
+@pytest.mark.parametrize("path", PATHS)
 def test_rtm_batch_duplicates_strip_source(path):
     """Several shots, receivers with duplicate nodes, sources on the top frame row
     (shot depth 0, as the reference's parameter.txt), every path against the oracle."""
@@ -153,7 +155,7 @@ def test_rtm_c2_grid_two_shots():
     Ps = [oracle_problem(c, k) for k in range(2)]
     outs = [P.forward(strips=True, final=True) for P in Ps]
     adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
-    for path in (swd.SWD_PATH_BATCHED, swd.SWD_PATH_PLAIN):
+    for path in PATHS:
         img, ils, ilr, gs = plan_of(c, path).rtm(c.shots, c.wavelet, adj, want_gather=True)
         row = 0
         for k, P in enumerate(Ps):
@@ -164,3 +166,58 @@ def test_rtm_c2_grid_two_shots():
             e_i, _ = report(f"c2 path {path} shot {k} image", img[k], o[0])
             assert e_g <= TOL and e_i <= TOL
             row += n
+
+
+@pytest.mark.parametrize("Lc", [1, 3, 4, 8])
+@pytest.mark.parametrize("tile", [16, 32, 64])
+def test_fused_tiles_forward_and_rtm(Lc, tile):
+    """Every fused tile width and halo depth (R = 1 for Lc <= 4, R = 2 above)
+    against the oracle, with a second shot whose source lies in the absorbing
+    frame outside the interior (its reconstruction keeps subtracting there)."""
+    c = configs.small(nx=100, nz=70, pml=8, Lc=Lc, nt=220, nshots=2)
+    s_ix, s_iz, rx, rz = c.shots[1]
+    c.shots[1] = (3, 5, rx, rz)
+    print(f"case fused tile={tile} Lc={Lc}", flush=True)
+    Ps = [oracle_problem(c, k) for k in range(2)]
+    outs = [P.forward(strips=True, final=True) for P in Ps]
+    adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
+    plan = plan_of(c, tile=tile)
+    g = plan.forward(c.shots, c.wavelet)
+    img, ils, ilr, gs = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
+    row = 0
+    for k, P in enumerate(Ps):
+        d, st, fs = outs[k]
+        n = d.shape[0]
+        o = P.rtm_backward(fs, st, adj[row:row + n])
+        e_f, _ = report(f"tile {tile} Lc {Lc} shot {k} forward", g[row:row + n], d)
+        assert e_f <= TOL and rel_l2(gs[row:row + n], d) <= TOL
+        for name, a, b in (("image", img[k], o[0]), ("illum_src", ils[k], o[1]), ("illum_rcv", ilr[k], o[2])):
+            e32, _ = report(f"tile {tile} Lc {Lc} shot {k} {name}", a, b)
+            assert e32 <= TOL
+        row += n
+
+
+def test_fused_autotuned_tile_matches_fixed():
+    """The tuned tile width (chosen on the first steps of the loop) gives the
+    same bits as a fixed width."""
+    c = configs.small(nt=120, nshots=2)
+    print("case fused autotune", flush=True)
+    a = plan_of(c, tile=0).forward(c.shots, c.wavelet)
+    b = plan_of(c, tile=32).forward(c.shots, c.wavelet)
+    assert np.array_equal(a, b)
+
+
+def test_fused_rtm_shot_groups_match():
+    """RTM in shot groups (forward and backward grouped differently, uneven
+    last group) gives the same images as one group."""
+    c = configs.small(nt=150, nshots=5, dup=True)
+    print("case fused rtm groups", flush=True)
+    outs = [oracle_problem(c, k).forward() for k in range(5)]
+    adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
+    plan = plan_of(c)
+    ref = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
+    for gf, gb in ((1, 1), (2, 3), (5, 2)):
+        plan.set_groups(gf, gb)
+        got = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), (gf, gb)
