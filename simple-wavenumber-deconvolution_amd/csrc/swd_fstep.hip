@@ -21,16 +21,17 @@
 //  SRC  source-wavefield reconstruction (RK:434-620), interior only, run
 //       velocity-then-pressure: the launch of step it first takes the
 //       velocities from it+2 to it+1 with the pressure of it+1 (RK:488-573),
-//       then the pressure from it+1 to it (RK:434-485) with the wavelet
-//       subtraction and the frame restore; dp/dt of step it goes to
-//       dpdt[it&1].  The first backward launch (it = nt-2) keeps the
-//       forward's final velocities.  Cells outside the interior are never
-//       written and must hold the forward's final state in both slots.
+//       then the pressure from it+1 to it (RK:434-485) with dp/dt, the
+//       wavelet subtraction and the frame restore.  The first backward launch
+//       (it = nt-2) keeps the forward's final velocities.  Cells outside the
+//       interior are never written and must hold the forward's final state in
+//       both slots.
 //  RCV  receiver back-propagation with (-dx, -dz) and widened domains
-//       (RK:623-782) and hard replacement by the adjoint source; it also
-//       applies the imaging condition (RK:785-817) of step it+1, whose dp/dt
-//       and receiver pressure were completed by the previous launch.  Step 0
-//       is imaged by k_fimage after the last launch.
+//       (RK:623-782) and hard replacement by the adjoint source.
+// A backward launch pairs the SRC and RCV wavefields of a shot in one
+// workgroup per tile: the reconstruction's dp/dt stays in registers and the
+// imaging condition of step it (RK:785-817) is applied right after the
+// receiver pressure of step it, so dp/dt never goes to memory.
 #include <algorithm>
 
 #include "swd_device.h"
@@ -51,6 +52,7 @@ struct FGeo {
     static constexpr int NPQ = 16 + 2 * RP;      // staged pressure quads -RP .. 15+RP
     static constexpr int NVX = TX + 2 * LC - 1;  // vx columns X0-Lc .. X1+Lc-2
     static constexpr int VZQ = 16 + 2 * R;       // vz quads -R .. 15+R
+    static constexpr int NOWN = (TX * 16 + 255) / 256;   // own quads per thread
 };
 
 // One tile of one wavefield.  FULL: the tile and every halo lie inside the
@@ -60,7 +62,8 @@ struct FGeo {
 // may differ, through the omitted "+ 0").
 template <int LC, int TX, bool FULL>
 __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X0, int Z0, int it, int first,
-                                           float4 *__restrict__ sP, float4 *__restrict__ sVX, float4 *__restrict__ sVZ)
+                                           float4 *__restrict__ sP, float4 *__restrict__ sVX, float4 *__restrict__ sVZ,
+                                           float4 (&dpa)[FGeo<LC, TX>::NOWN])
 {
     using G = FGeo<LC, TX>;
     constexpr int R = G::R, RP = G::RP, OWN = G::OWN, NPQ = G::NPQ, VZQ = G::VZQ;
@@ -104,17 +107,19 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
             bzv[r] = ld4((gcfl *)g.Bz, o);
         }
     }
-    float4 imv[I3], isv[I3], irv[I3], dpv[I3];
+    float4 imv[I3], isv[I3], irv[I3];
+    int cmv[I3];   // receiver map of each own column
 #pragma unroll
     for (int r = 0; r < I3; r++) {
         const int i = tid + 256 * r, q = i & 15, c = i >> 4;
         const int x = X0 + c, z = Z0 + 4 * q;
-        kv[r] = imv[r] = isv[r] = irv[r] = dpv[r] = zero4();
+        kv[r] = imv[r] = isv[r] = irv[r] = zero4();
+        cmv[r] = -1;
         if (i < N3 && (FULL || (x < ntx && z < ntz))) {
+            if (I.colmap) cmv[r] = ldgi(I.colmap, x);
             const unsigned o = static_cast<unsigned>(x) * ldz + z;
             kv[r] = ld4((gcfl *)g.K, o);
-            if (neg && I.image && !first && (FULL || (x >= pmlc && x < ntx - pmlc))) {
-                dpv[r] = ld4((gcfl *)I.dpdt[po], o);
+            if (neg && I.image && (FULL || (x >= pmlc && x < ntx - pmlc))) {
                 imv[r] = ld4((gcfl *)I.image, o);
                 isv[r] = ld4((gcfl *)I.illum_s, o);
                 irv[r] = ld4((gcfl *)I.illum_r, o);
@@ -343,7 +348,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                         el(dp, k) = elc(K0, k) * S;
                     }
                 }
-                st4((gfl *)I.dpdt[pn], o, dp);
+                dpa[r] = dp;
             }
             if (hassrc) add_at(pv, I.s_iz - z, -ldg(I.wavelet, it + 1));   // RK:478-481
             if (!FULL) {
@@ -410,8 +415,8 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         }
 #undef FSTEP_DERIVS
         if (!neg && I.wavelet && x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4) add_at(pv, I.s_iz - z, ldg(I.wavelet, it));   // RK:175-178
-        if (I.colmap) {
-            const int cm = ldgi(I.colmap, x);
+        {
+            const int cm = cmv[r];
             if (cm >= 0) {
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -443,17 +448,17 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                         if (FULL || z + k < ntz) dst[2 * k + I.capcomp] = elc(pv, k);   // FK:324-327, 529-532
                 }
             }
-        } else if (I.image && colint && !first) {
-            // imaging condition of step it+1: its dp/dt and receiver pressure (RK:785-817)
-            const float4 dp = dpv[r];
+        } else if (I.image && colint) {
+            // imaging condition of step it: the reconstruction's dp/dt and the receiver pressure (RK:785-817)
+            const float4 dp = dpa[r];
             float4 im = imv[r], is = isv[r], ir = irv[r];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int zk = z + k;
                 if (FULL || (zk >= pmlc && zk < ntz - pmlc)) {
-                    el(im, k) = __builtin_fmaf(elc(dp, k), elc(pold, k), elc(im, k));
+                    el(im, k) = __builtin_fmaf(elc(dp, k), elc(pv, k), elc(im, k));
                     el(is, k) = __builtin_fmaf(elc(dp, k), elc(dp, k), elc(is, k));
-                    el(ir, k) = __builtin_fmaf(elc(pold, k), elc(pold, k), elc(ir, k));
+                    el(ir, k) = __builtin_fmaf(elc(pv, k), elc(pv, k), elc(ir, k));
                 }
             }
             st4((gfl *)I.image, o, im);
@@ -463,7 +468,9 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     }
 }
 
-template <int LC, int TX>
+// PAIR = false: one FWD wavefield per instance.  PAIR = true: instances
+// 2k (SRC) and 2k+1 (RCV) of one shot share each workgroup.
+template <int LC, int TX, bool PAIR>
 __global__ __launch_bounds__(256) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb, int nxb,
                                                int it, int first)
 {
@@ -477,73 +484,63 @@ __global__ __launch_bounds__(256) void k_fstep(KGrid g, const KStep *__restrict_
     const int inst = L % ninst;
     const int zb = (L / ninst) % nzb, xb = L / (ninst * nzb);
     const int X0 = xb * TX, Z0 = zb * 64;
-    const KStep &I = insts[inst];
-    if (I.mode == KSTEP_SRC) {
-        // a reconstruction tile writes only interior cells and the source cell
-        const bool any = X0 + TX > g.pmlc && X0 < g.ntx - g.pmlc && Z0 + 64 > g.pmlc && Z0 < g.ntz - g.pmlc;
-        const bool src = I.s_ix >= X0 && I.s_ix < X0 + TX && I.s_iz >= Z0 && I.s_iz < Z0 + 64;
-        if (!any && !src) return;
-    }
     const bool full = X0 - G::OWN >= g.fx0 && X0 + TX + 2 * LC - 2 <= g.fx1 && Z0 - 4 * G::RP >= g.fz0 &&
                       Z0 + 64 + 4 * G::RP - 1 <= g.fz1;
-    if (full) fstep_tile<LC, TX, true>(g, I, X0, Z0, it, first, sP, sVX, sVZ);
-    else fstep_tile<LC, TX, false>(g, I, X0, Z0, it, first, sP, sVX, sVZ);
-}
-
-// Imaging condition of step 0 (after the last backward launch) for the RCV
-// instances: dp/dt in dpdt[0], receiver pressure in p[0].
-__global__ __launch_bounds__(256) void k_fimage(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb, int nxb)
-{
-    const int L = xcd_block();
-    if (L >= ninst * nzb * nxb) return;
-    const int inst = L % ninst;
-    const int zb = (L / ninst) % nzb, xb = L / (ninst * nzb);
-    const int x = xb * 16 + (threadIdx.x >> 4);
-    const int z = zb * 64 + (threadIdx.x & 15) * 4;
-    const KStep &I = insts[inst];
-    if (I.mode != KSTEP_RCV || !I.image) return;
-    if (x < g.pmlc || x >= g.ntx - g.pmlc || z >= g.ntz) return;
-    const size_t o = static_cast<size_t>(x) * g.ldz + z;
-    const float4 dp = ld4((gcfl *)I.dpdt[0], o), pr = ld4((gcfl *)I.p[0], o);
-    float4 im = ld4((gcfl *)I.image, o), is = ld4((gcfl *)I.illum_s, o), ir = ld4((gcfl *)I.illum_r, o);
+    float4 dpa[G::NOWN];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int zk = z + k;
-        if (zk >= g.pmlc && zk < g.ntz - g.pmlc) {
-            el(im, k) = __builtin_fmaf(elc(dp, k), elc(pr, k), elc(im, k));
-            el(is, k) = __builtin_fmaf(elc(dp, k), elc(dp, k), elc(is, k));
-            el(ir, k) = __builtin_fmaf(elc(pr, k), elc(pr, k), elc(ir, k));
-        }
+    for (int r = 0; r < G::NOWN; r++) dpa[r] = zero4();
+    if (!PAIR) {
+        const KStep &I = insts[inst];
+        if (full) fstep_tile<LC, TX, true>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa);
+        else fstep_tile<LC, TX, false>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa);
+        return;
     }
-    st4((gfl *)I.image, o, im);
-    st4((gfl *)I.illum_s, o, is);
-    st4((gfl *)I.illum_r, o, ir);
+    const KStep &S = insts[2 * inst], &Rv = insts[2 * inst + 1];
+    // the reconstruction writes only interior cells and the source cell
+    const bool any = X0 + TX > g.pmlc && X0 < g.ntx - g.pmlc && Z0 + 64 > g.pmlc && Z0 < g.ntz - g.pmlc;
+    const bool src = S.s_ix >= X0 && S.s_ix < X0 + TX && S.s_iz >= Z0 && S.s_iz < Z0 + 64;
+    if (any || src) {
+        if (full) fstep_tile<LC, TX, true>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa);
+        else fstep_tile<LC, TX, false>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa);
+        __syncthreads();   // LDS is staged again for the receiver wavefield
+    }
+    if (full) fstep_tile<LC, TX, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa);
+    else fstep_tile<LC, TX, false>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa);
 }
 
 }  // namespace
 
 hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, hipStream_t s)
 {
-    const int nzb = (g.ntz + 63) / 64;
-    const int nxb = (g.ntx + tx - 1) / tx;
-    const int nblocks = (ninst * nzb * nxb + 7) / 8 * 8;
-    const int f = first ? 1 : 0;
-#define LF(TXV) \
-    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_fstep<L, TXV>), dim3(nblocks), dim3(256), 0, s, g, insts, ninst, nzb, nxb, it, f))
-    switch (tx) {
-    case 16: LF(16); break;
-    case 32: LF(32); break;
-    case 64: LF(64); break;
-    default: return hipErrorInvalidValue;
-    }
-#undef LF
-    return hipGetLastError();
+    return launch_fstep(g, insts, ninst, it, first, tx, false, s);
 }
 
-hipError_t launch_fimage(const KGrid &g, const KStep *insts, int ninst, hipStream_t s)
+hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair,
+                        hipStream_t s)
 {
-    const int nzb = (g.ntz + 63) / 64, nxb = (g.ntx + 15) / 16;
-    const int nblocks = (ninst * nzb * nxb + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_fimage, dim3(nblocks), dim3(256), 0, s, g, insts, ninst, nzb, nxb);
+    const int n = pair ? ninst / 2 : ninst;
+    const int nzb = (g.ntz + 63) / 64;
+    const int nxb = (g.ntx + tx - 1) / tx;
+    const int nblocks = (n * nzb * nxb + 7) / 8 * 8;
+    const int f = first ? 1 : 0;
+#define LF(TXV, PV)                                                                                                   \
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_fstep<L, TXV, PV>), dim3(nblocks), dim3(256), 0, s, g, insts, n, nzb, nxb, \
+                                         it, f))
+    if (pair) {
+        switch (tx) {
+        case 16: LF(16, true); break;
+        case 32: LF(32, true); break;
+        case 64: LF(64, true); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (tx) {
+        case 16: LF(16, false); break;
+        case 32: LF(32, false); break;
+        case 64: LF(64, false); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+#undef LF
     return hipGetLastError();
 }

@@ -47,7 +47,7 @@ struct KInst {
 // launch reads slot (it+1)&1 of p and writes slot it&1, so no workgroup
 // overwrites what a neighbour still reads as its halo.  The source
 // reconstruction runs its velocity half one step behind, so its velocity
-// slots are swapped (see k_fstep).
+// slots are swapped (see swd_fstep.hip).
 enum KStepMode { KSTEP_FWD = 0, KSTEP_SRC = 1, KSTEP_RCV = 2 };
 struct KStep {
     int mode;                    // KStepMode
@@ -63,8 +63,7 @@ struct KStep {
     float2 *capture;             // FWD (AFor)
     const int *colcap;
     int capcomp, ncap, nfftz;
-    float *dpdt[2];              // SRC writes slot it&1; RCV reads slot (it+1)&1
-    float *image, *illum_s, *illum_r;   // RCV
+    float *image, *illum_s, *illum_r;   // RCV (paired with the SRC of the same shot)
 };
 
 // Column descriptor of the kz separation: one receiver column of one shot.

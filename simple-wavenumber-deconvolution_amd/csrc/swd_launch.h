@@ -12,8 +12,10 @@ hipError_t launch_adjoint_step(const KGrid &g, const KInst *insts, int ninst, in
 hipError_t launch_copy_rows(float *gather, const int2 *pairs, int npairs, int nt, hipStream_t s);
 hipError_t launch_sum_rows(float *dst, const float *src, const int2 *pairs, int npairs, int nt, hipStream_t s);
 hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, hipStream_t s);
-// fused step (swd_fstep.hip): tx = tile width in columns (16, 32 or 64)
+// fused step (swd_fstep.hip): tx = tile width in columns (16, 32 or 64);
+// pair: instances are (SRC, RCV) pairs of the RTM backward, ninst counts both
 hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, hipStream_t s);
-hipError_t launch_fimage(const KGrid &g, const KStep *insts, int ninst, hipStream_t s);
+hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair,
+                        hipStream_t s);
 hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2 *tw, const float *coef,
                         const KCapCol *cols, const int2 *traces, float *up, float *down, int nt, hipStream_t s);

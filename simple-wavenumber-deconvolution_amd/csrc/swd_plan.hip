@@ -463,14 +463,14 @@ int upload_steps(Plan &P, const std::vector<KStep> &h, size_t o)
 // launches, six launches with each candidate width are timed and the fastest
 // width runs the rest (the results do not depend on the width).
 int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int first_it, int &tile, int &tile_n,
-                long long *launches)
+                long long *launches, bool pair = false)
 {
     int tx = P.tile_req;
     if (tx == 0 && tile_n == n) tx = tile;
     size_t k = 0;
     auto launch = [&](int w) -> int {
         const int it = its[k++];
-        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, P.stream));
+        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, pair, P.stream));
         (*launches)++;
         return SWD_OK;
     };
@@ -773,7 +773,7 @@ namespace {
 // backward the reconstruction, images and dp/dt) is kept within a budget of
 // the 256 MB last-level cache, leaving room for the model, the halos and the
 // streamed strips; groups are then evened out.
-constexpr double kCacheBudget = 160.0 * (1 << 20);
+constexpr double kCacheBudget = 192.0 * (1 << 20);
 
 int even_groups(int n, int g)
 {
@@ -787,7 +787,7 @@ void rtm_groups(const Plan &P, int nshots, int &gf, int &gb)
     const double N = static_cast<double>(P.ntx) * P.ntz, Nint = static_cast<double>(P.g.nx) * P.g.nz;
     const double Ncpml = N - Nint;
     const double fwd = 4.0 * (6 * N + 6 * Ncpml);
-    const double bwd = 4.0 * (6 * Nint + 6 * N + 6 * Ncpml + 5 * Nint);
+    const double bwd = 4.0 * (9 * Nint + 6 * N + 6 * Ncpml);
     gf = P.group_fwd > 0 ? P.group_fwd : static_cast<int>(kCacheBudget / fwd);
     gb = P.group_bwd > 0 ? P.group_bwd : static_cast<int>(kCacheBudget / bwd);
     gf = even_groups(nshots, gf);
@@ -795,8 +795,8 @@ void rtm_groups(const Plan &P, int nshots, int &gf, int &gb)
 }
 
 // RTM of a batch on the fused path: one launch per forward step, one launch
-// per backward step for the reconstruction and receiver wavefields of every
-// shot, imaging one launch behind (k_fstep).
+// per backward step for the paired reconstruction and receiver wavefields of
+// every shot of a group, imaging in the same launch (k_fstep).
 int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, const float *wavelet, const float *adj,
                   float *image, float *illum_src, float *illum_rcv, float *gather_syn)
 {
@@ -809,18 +809,18 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     const size_t o_bwd = c.add<KStep>(2 * nshots);
     const size_t o_wav = c.add<float>(nt);
     const size_t o_strips = c.add<float>(static_cast<size_t>(nt) * P.nframe * nshots);
-    const size_t o_img = c.add<float>(5 * nf * nshots);   // image, illum_s, illum_r, dpdt[2]
+    const size_t o_img = c.add<float>(3 * nf * nshots);   // image, illum_s, illum_r
     const size_t o_f = c.add<float>(kStepArrays * nf * 2 * nshots);   // source side, then receiver side
     SWD_TRY(ensure_ws(P, c.off));
     SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
     zero_fields(P, at<float>(P, o_f), kStepArrays * nf * 2 * nshots);
-    zero_fields(P, at<float>(P, o_img), 5 * nf * nshots);
+    zero_fields(P, at<float>(P, o_img), 3 * nf * nshots);
     float *fields = at<float>(P, o_f), *imgs = at<float>(P, o_img);
     std::vector<KStep> hf(nshots), hb(2 * nshots);
     for (int k = 0; k < nshots; k++) {
         float *fs = fields + kStepArrays * nf * k, *fr = fields + kStepArrays * nf * (nshots + k);
-        float *im = imgs + 5 * nf * k;
+        float *im = imgs + 3 * nf * k;
         KStep &F = hf[k], &S = hb[2 * k], &R = hb[2 * k + 1];
         std::memset(&F, 0, sizeof F);
         std::memset(&S, 0, sizeof S);
@@ -841,8 +841,6 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
             if (gather_syn) F.gather = gather_syn + static_cast<size_t>(b.toff[k]) * nt;
             R.adj = adj + static_cast<size_t>(b.toff[k]) * nt;
         }
-        S.dpdt[0] = R.dpdt[0] = im + 3 * nf;
-        S.dpdt[1] = R.dpdt[1] = im + 4 * nf;
         R.image = im;
         R.illum_s = im + nf;
         R.illum_r = im + 2 * nf;
@@ -883,8 +881,7 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     for (int k0 = 0; k0 < nshots; k0 += gb) {
         const int n = std::min(gb, nshots - k0);
         const KStep *d = at<KStep>(P, o_bwd) + 2 * k0;
-        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tile_bwd, P.tile_bwd_n, &launches));
-        HIP_TRY(launch_fimage(P.kg, d, 2 * n, P.stream));
+        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tile_bwd, P.tile_bwd_n, &launches, true));
     }
     P.stats.bwd_ms = stats_end(P, 1);
     P.stats.bwd_launches = launches;
@@ -894,7 +891,7 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     for (int k = 0; k < nshots; k++) {
         float *outs[3] = {image + ntp * k, illum_src + ntp * k, illum_rcv + ntp * k};
         for (int q = 0; q < 3; q++)
-            HIP_TRY(hipMemcpy2DAsync(outs[q], sizeof(float) * P.ntz, imgs + 5 * nf * k + q * nf, sizeof(float) * P.ldz,
+            HIP_TRY(hipMemcpy2DAsync(outs[q], sizeof(float) * P.ntz, imgs + 3 * nf * k + q * nf, sizeof(float) * P.ldz,
                                      sizeof(float) * P.ntz, P.ntx, hipMemcpyDeviceToDevice, P.stream));
     }
     HIP_TRY(hipStreamSynchronize(P.stream));
