@@ -42,6 +42,43 @@ namespace {
 
 __device__ __forceinline__ float4 zero4() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 
+// Quads s[0 .. 2R] of an LDS column as 4(2R+1) consecutive rows.  The first
+// row is not used by the stencils; the empty asm keeps each quad a whole
+// 16-byte read (a narrowed read starts 4 bytes off and splits into
+// bank-conflicting dword pairs).
+template <int R>
+__device__ __forceinline__ void lds_window(const float4 *s, float (&a)[4 * (2 * R + 1)])
+{
+    v4f t[2 * R + 1];
+#pragma unroll
+    for (int j = 0; j <= 2 * R; j++) t[j] = *reinterpret_cast<const v4f *>(s + j);
+#pragma unroll
+    for (int j = 0; j <= 2 * R; j++) asm volatile("" ::"v"(t[j]));
+#pragma unroll
+    for (int j = 0; j <= 2 * R; j++) {
+        a[4 * j] = t[j].x;
+        a[4 * j + 1] = t[j].y;
+        a[4 * j + 2] = t[j].z;
+        a[4 * j + 3] = t[j].w;
+    }
+}
+
+// Velocity-z item i of a tile: the 16 own quads of each column first (16
+// consecutive lanes per column, so LDS accesses of 8-lane groups stay in one
+// column), then the 2R halo quads of each column.
+template <int TX, int R>
+__device__ __forceinline__ void vz_item(int i, int &c, int &qq)
+{
+    if (i < TX * 16) {
+        c = i >> 4;
+        qq = i & 15;
+    } else {
+        const int j = i - TX * 16, h = j % (2 * R);
+        c = j / (2 * R);
+        qq = h < R ? h - R : 16 + h - R;
+    }
+}
+
 // Tile geometry: 64 rows (16 quads) x TX columns, Lc-deep stencils.
 template <int LC, int TX>
 struct FGeo {
@@ -98,7 +135,9 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     }
 #pragma unroll
     for (int r = 0; r < I2; r++) {
-        const int i = tid + 256 * r, c = i / VZQ, qq = i - c * VZQ - R;
+        const int i = tid + 256 * r;
+        int c, qq;
+        vz_item<TX, R>(i, c, qq);
         const int x = X0 + c, z = Z0 + 4 * qq;
         vzo[r] = bzv[r] = zero4();
         if (i < N2 && (FULL || (x < ntx && z >= 0 && z < ntz))) {
@@ -170,8 +209,10 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                         for (int k = 0; k < 4; k++) el(d, k) = __builtin_fmaf(g.rc[ic] * (elc(a, k) - elc(b, k)), g.idx, el(d, k));
                     }
 #pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        if (FULL || (z + k >= pmlc && z + k < ntz - pmlc)) el(v, k) = __builtin_fmaf(-elc(B, k), elc(d, k), elc(v, k));
+                    for (int k = 0; k < 4; k++) {
+                        const float nv = __builtin_fmaf(-elc(B, k), elc(d, k), elc(v, k));
+                        el(v, k) = (FULL || (z + k >= pmlc && z + k < ntz - pmlc)) ? nv : elc(v, k);
+                    }
                     if (own) st4((gfl *)I.vx[vn], o, v);
                 }
             } else {
@@ -195,14 +236,10 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                         for (int k = 0; k < 4; k++) {
                             const int zk = z + k;
                             const bool in = neg ? zk < ntz : (zk >= LC && zk <= ntz - LC);
-                            if (in) {
-                                float ps = 0.f;
-                                if (ah != 0.f) {
-                                    ps = __builtin_fmaf(bh, elc(pso, k), ah * elc(d, k));
-                                    el(psn, k) = ps;
-                                }
-                                el(v, k) = __builtin_fmaf(elc(B, k), elc(d, k) + ps, elc(v, k));
-                            }
+                            const float ps = ah != 0.f ? __builtin_fmaf(bh, elc(pso, k), ah * elc(d, k)) : 0.f;
+                            el(psn, k) = in ? ps : elc(pso, k);
+                            const float nv = __builtin_fmaf(elc(B, k), elc(d, k) + ps, elc(v, k));
+                            el(v, k) = in ? nv : elc(v, k);
                         }
                         if (own && ah != 0.f) st4((gfl *)I.psx[vn], o, psn);
                     }
@@ -218,7 +255,8 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     for (int r = 0; r < I2; r++) {
         const int i = tid + 256 * r;
         if (i >= N2) break;
-        const int c = i / VZQ, qq = i - c * VZQ - R;
+        int c, qq;
+        vz_item<TX, R>(i, c, qq);
         const int x = X0 + c, z = Z0 + 4 * qq;
         float4 v = vzo[r];
         if (FULL || (x < ntx && z >= 0 && z < ntz)) {
@@ -226,24 +264,18 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
             const bool own = qq >= 0 && qq < 16;
             const float4 B = bzv[r];
             float a[4 * (2 * R + 1)];   // p rows z-4R .. z+4R+3
-#pragma unroll
-            for (int j = 0; j <= 2 * R; j++) {
-                const float4 t = sP[(c + OWN) * NPQ + qq + RP - R + j];
-#pragma unroll
-                for (int k = 0; k < 4; k++) a[4 * j + k] = elc(t, k);
-            }
+            lds_window<R>(&sP[(c + OWN) * NPQ + qq + RP - R], a);
             if (srcm) {
                 const bool upd = vupd && (FULL || (x >= pmlc && x < ntx - pmlc && z + 3 >= pmlc && z < ntz - pmlc));
                 if (upd) {
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        if (FULL || (z + k >= pmlc && z + k < ntz - pmlc)) {
-                            float d = 0.f;
+                        float d = 0.f;
 #pragma unroll
-                            for (int ic = 0; ic < LC; ic++)
-                                d = __builtin_fmaf(g.rc[ic] * (a[4 * R + k + ic + 1] - a[4 * R + k - ic]), g.idz, d);
-                            el(v, k) = __builtin_fmaf(-elc(B, k), d, elc(v, k));
-                        }
+                        for (int ic = 0; ic < LC; ic++)
+                            d = __builtin_fmaf(g.rc[ic] * (a[4 * R + k + ic + 1] - a[4 * R + k - ic]), g.idz, d);
+                        const float nv = __builtin_fmaf(-elc(B, k), d, elc(v, k));
+                        el(v, k) = (FULL || (z + k >= pmlc && z + k < ntz - pmlc)) ? nv : elc(v, k);
                     }
                     if (own) st4((gfl *)I.vz[vn], o, v);
                 }
@@ -274,18 +306,15 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
                             const int zk = z + k;
-                            if (zk >= LC - 1 && zk <= ntz - LC - 1) {
-                                float d = 0.f;
+                            const bool in = zk >= LC - 1 && zk <= ntz - LC - 1;
+                            float d = 0.f;
 #pragma unroll
-                                for (int ic = 0; ic < LC; ic++)
-                                    d = __builtin_fmaf(g.rc[ic] * (a[4 * R + k + ic + 1] - a[4 * R + k - ic]), idz, d);
-                                float ps = 0.f;
-                                if (ahz[k] != 0.f) {
-                                    ps = __builtin_fmaf(bhz[k], elc(pso, k), ahz[k] * d);
-                                    el(psn, k) = ps;
-                                }
-                                el(v, k) = __builtin_fmaf(elc(B, k), d + ps, elc(v, k));
-                            }
+                            for (int ic = 0; ic < LC; ic++)
+                                d = __builtin_fmaf(g.rc[ic] * (a[4 * R + k + ic + 1] - a[4 * R + k - ic]), idz, d);
+                            const float ps = ahz[k] != 0.f ? __builtin_fmaf(bhz[k], elc(pso, k), ahz[k] * d) : 0.f;
+                            el(psn, k) = (in && ahz[k] != 0.f) ? ps : elc(pso, k);
+                            const float nv = __builtin_fmaf(elc(B, k), d + ps, elc(v, k));
+                            el(v, k) = in ? nv : elc(v, k);
                         }
                         if (own && anyps) st4((gfl *)I.psz[vn], o, psn);
                     }
@@ -293,7 +322,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                 if (own) st4((gfl *)I.vz[vn], o, v);
             }
         }
-        sVZ[i] = v;
+        sVZ[c * VZQ + qq + R] = v;
     }
     __syncthreads();
 
@@ -338,15 +367,15 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const int zk = z + k;
-                    if (FULL || (zk >= pmlc && zk < ntz - pmlc)) {
-                        float dvz = 0.f;
+                    const bool in = FULL || (zk >= pmlc && zk < ntz - pmlc);
+                    float dvz = 0.f;
 #pragma unroll
-                        for (int ic = 0; ic < LC; ic++)
-                            dvz = __builtin_fmaf(g.rc[ic] * (w[4 * R + k + ic] - w[4 * R + k - ic - 1]), g.idz, dvz);
-                        const float S = elc(dvx, k) + dvz;
-                        el(pv, k) = __builtin_fmaf(-elc(K, k), S, elc(pv, k));
-                        el(dp, k) = elc(K0, k) * S;
-                    }
+                    for (int ic = 0; ic < LC; ic++)
+                        dvz = __builtin_fmaf(g.rc[ic] * (w[4 * R + k + ic] - w[4 * R + k - ic - 1]), g.idz, dvz);
+                    const float S = elc(dvx, k) + dvz;
+                    const float np = __builtin_fmaf(-elc(K, k), S, elc(pv, k));
+                    el(pv, k) = in ? np : elc(pv, k);
+                    el(dp, k) = in ? elc(K0, k) * S : 0.f;
                 }
                 dpa[r] = dp;
             }
@@ -391,23 +420,18 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const int zk = z + k;
-                    if (zk >= LC && zk <= ntz - LC) {
-                        float dvz = 0.f;
+                    const bool in = zk >= LC && zk <= ntz - LC;
+                    float dvz = 0.f;
 #pragma unroll
-                        for (int ic = 0; ic < LC; ic++)
-                            dvz = __builtin_fmaf(g.rc[ic] * (w[4 * R + k + ic] - w[4 * R + k - ic - 1]), idz, dvz);
-                        float qx = 0.f, qz = 0.f;
-                        if (ax != 0.f) {
-                            qx = __builtin_fmaf(bx, elc(qxo, k), ax * elc(dvx, k));
-                            el(qxn, k) = qx;
-                        }
-                        if (azr[k] != 0.f) {
-                            qz = __builtin_fmaf(bzr[k], elc(qzo, k), azr[k] * dvz);
-                            el(qzn, k) = qz;
-                        }
-                        const float S = ((elc(dvx, k) + qx) + dvz) + qz;
-                        el(pv, k) = __builtin_fmaf(elc(K, k), S, elc(pold, k));
-                    }
+                    for (int ic = 0; ic < LC; ic++)
+                        dvz = __builtin_fmaf(g.rc[ic] * (w[4 * R + k + ic] - w[4 * R + k - ic - 1]), idz, dvz);
+                    const float qx = ax != 0.f ? __builtin_fmaf(bx, elc(qxo, k), ax * elc(dvx, k)) : 0.f;
+                    const float qz = azr[k] != 0.f ? __builtin_fmaf(bzr[k], elc(qzo, k), azr[k] * dvz) : 0.f;
+                    el(qxn, k) = (in && ax != 0.f) ? qx : elc(qxo, k);
+                    el(qzn, k) = (in && azr[k] != 0.f) ? qz : elc(qzo, k);
+                    const float S = ((elc(dvx, k) + qx) + dvz) + qz;
+                    const float np = __builtin_fmaf(elc(K, k), S, elc(pold, k));
+                    el(pv, k) = in ? np : elc(pold, k);
                 }
                 if (ax != 0.f) st4((gfl *)I.pvx, o, qxn);
                 if (anyz) st4((gfl *)I.pvz, o, qzn);
@@ -455,11 +479,13 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int zk = z + k;
-                if (FULL || (zk >= pmlc && zk < ntz - pmlc)) {
-                    el(im, k) = __builtin_fmaf(elc(dp, k), elc(pv, k), elc(im, k));
-                    el(is, k) = __builtin_fmaf(elc(dp, k), elc(dp, k), elc(is, k));
-                    el(ir, k) = __builtin_fmaf(elc(pv, k), elc(pv, k), elc(ir, k));
-                }
+                const bool in = FULL || (zk >= pmlc && zk < ntz - pmlc);
+                const float a = __builtin_fmaf(elc(dp, k), elc(pv, k), elc(im, k));
+                const float b = __builtin_fmaf(elc(dp, k), elc(dp, k), elc(is, k));
+                const float c2 = __builtin_fmaf(elc(pv, k), elc(pv, k), elc(ir, k));
+                el(im, k) = in ? a : elc(im, k);
+                el(is, k) = in ? b : elc(is, k);
+                el(ir, k) = in ? c2 : elc(ir, k);
             }
             st4((gfl *)I.image, o, im);
             st4((gfl *)I.illum_s, o, is);
