@@ -10,7 +10,9 @@ the adjoint sources already in HBM.
 Prints one JSON line: Gcell-updates/s (cell-update = one leapfrog step of one
 wavefield at one padded node; RTM per shot = nt*N + (nt-1)*(N + N_int)),
 shots/s, the plain one-shot-per-launch path's shots/s, the roofline of the
-dominant kernel and a CPU baseline (the oracle's fp32 host loops with OpenMP,
+dominant kernel (the fused backward step: algorithmic bytes per launch over the
+HIP-event launch time; HBM traffic per launch from the committed rocprofv3
+counter summary) and a CPU baseline (the oracle's fp32 host loops with OpenMP,
 on a bounded sample of the same workload).
 """
 import argparse
@@ -151,7 +153,7 @@ def main():
     img = d_img[0].cpu().numpy()
     finite = bool(np.isfinite(img).all())
 
-    # roofline of the dominant kernel (the backward step) from event timing on the plan's stream
+    # roofline of the dominant kernel (the fused backward step) from HIP-event timing on the plan's stream
     nrec = len(case.shots[0][2])
     nframe = 2 * g.Lc * (g.nx + g.nz - 2 * g.Lc)
     bf, bb = rtm_bytes_per_step(g, nrec, nframe)
@@ -191,6 +193,13 @@ def main():
                          f"(forward with strips + backward + imaging), {dt:.1f}s"}
         log(f"bench: cpu baseline {rate:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
 
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "rtm_fused_c2_traffic.json")
+    if a.path == "fused" and os.path.exists(tfile) and (g.nx, g.nz, g.pml, g.Lc) == (845, 502, 40, 4):
+        with open(tfile) as f:
+            t = json.load(f)
+        traffic = t["hbm_bytes_per_shot_step"]["backward"] * a.batch / bwd_launches_per_step
+
     out = {
         "metric": "Gcell-updates/s and batched shots/s vs plain GPU path at <=1e-5 rel L2",
         "value": gcells,
@@ -213,9 +222,12 @@ def main():
                    "fwd_ms_per_step": fwd_ms / a.steps, "bwd_ms_per_step": bwd_ms / a.steps,
                    "fwd_achieved_GBs": fwd_achieved, "image_finite": finite},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "RTM backward step", "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_ms": bwd_launch_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_fstep (fused RTM backward step: reconstruction + receiver + imaging)"
+                               if a.path == "fused" else "RTM backward step (4 sub-step kernels)",
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": bwd_launch_ms,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per shot-step, "
+                                       "profiles/rtm_fused_c2_16shots.md" if traffic else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
