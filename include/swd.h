@@ -139,6 +139,10 @@ int swd_plan_set_tile(void *plan, int tile_x);
  * groups of these sizes (0, the default: sized so a group's working set stays
  * in the last-level cache).  Results do not depend on them. */
 int swd_plan_set_groups(void *plan, int fwd_shots, int bwd_shots);
+/* Replay fused time loops from captured hipGraphs (on by default): the
+ * launches of a loop are recorded once per configuration and replayed on
+ * later calls with the same batch layout. */
+int swd_plan_set_graphs(void *plan, int on);
 int swd_plan_get_stats(void *plan, swd_stats *st);
 
 /* ---------------- operators ----------------
@@ -184,6 +188,15 @@ int swd_born_dev(void *plan, int nshots, const swd_shot *shots, const float *wav
                  const float *m, float *d);
 int swd_born_adjoint_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
                          const float *d, float *m_grad, int nsnap);
+
+/* Forward modelling of several plans' batches of uneven grid sizes in one
+ * fused launch per time step (no reference counterpart: the reference runs one
+ * grid per call, FK:576-863).  All plans must be on one device and share Lc and
+ * nt; plan i runs nshots[i] shots shots[i] with wavelet wavelets[i][nt] and
+ * gathers[i] receives its [sum nrec][nt].  The launches run on plans[0]'s
+ * stream, from a captured hipGraph when plans[0] has graphs on. */
+int swd_multi_forward(int nplans, void *const *plans, const int *nshots, const swd_shot *const *shots,
+                      const float *const *wavelets, float *const *gathers);
 
 #ifdef __cplusplus
 }

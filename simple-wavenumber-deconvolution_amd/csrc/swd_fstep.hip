@@ -534,7 +534,44 @@ __global__ __launch_bounds__(256) void k_fstep(KGrid g, const KStep *__restrict_
     else fstep_tile<LC, TX, false>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa);
 }
 
+// Multi-grid launch: instances of different grid sizes (same Lc) in one
+// launch.  btab maps each logical workgroup to (instance, tile of that
+// instance's grid, numbered z-fastest); each instance reads its own grid.
+template <int LC>
+__global__ __launch_bounds__(256) void k_fstep_multi(const KStep *__restrict__ insts, const int2 *__restrict__ btab,
+                                                     int nblk, int it)
+{
+    constexpr int TX = 16;
+    using G = FGeo<LC, TX>;
+    __shared__ float4 sP[G::NPC * G::NPQ];
+    __shared__ float4 sVX[G::NVX * 16];
+    __shared__ float4 sVZ[TX * G::VZQ];
+
+    const int L = xcd_block();
+    if (L >= nblk) return;
+    const int2 bt = btab[L];
+    const KStep &I = insts[bt.x];
+    const KGrid &g = *I.grid;
+    if (it >= g.nt) return;
+    const int nzb = (g.ntz + 63) / 64;
+    const int zb = bt.y % nzb, xb = bt.y / nzb;
+    const int X0 = xb * TX, Z0 = zb * 64;
+    const bool full = X0 - G::OWN >= g.fx0 && X0 + TX + 2 * LC - 2 <= g.fx1 && Z0 - 4 * G::RP >= g.fz0 &&
+                      Z0 + 64 + 4 * G::RP - 1 <= g.fz1;
+    float4 dpa[G::NOWN];
+    dpa[0] = zero4();
+    if (full) fstep_tile<LC, TX, true>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa);
+    else fstep_tile<LC, TX, false>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa);
+}
+
 }  // namespace
+
+hipError_t launch_fstep_multi(int Lc, const KStep *insts, const int2 *btab, int nblk, int it, hipStream_t s)
+{
+    const int nblocks = (nblk + 7) / 8 * 8;
+    DISPATCH_LC(Lc, hipLaunchKernelGGL((k_fstep_multi<L>), dim3(nblocks), dim3(256), 0, s, insts, btab, nblk, it));
+    return hipGetLastError();
+}
 
 hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, hipStream_t s)
 {

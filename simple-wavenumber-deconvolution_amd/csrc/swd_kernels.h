@@ -52,6 +52,7 @@ enum KStepMode { KSTEP_FWD = 0, KSTEP_SRC = 1, KSTEP_RCV = 2 };
 struct KStep {
     int mode;                    // KStepMode
     int s_ix, s_iz;
+    const KGrid *grid;           // own grid (multi-grid launches only)
     float *p[2], *vx[2], *vz[2];
     float *psx[2], *psz[2];      // CPML memory of dp/dx, dp/dz (FWD, RCV)
     float *pvx, *pvz;            // CPML memory of dvx/dx, dvz/dz, own cells only (FWD, RCV)

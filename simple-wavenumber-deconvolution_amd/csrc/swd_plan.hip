@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -94,6 +95,10 @@ struct Plan {
     int group_fwd = 0, group_bwd = 0;     // RTM shots per fused forward / backward loop (0 = sized to the cache)
     int tile_fwd = 0, tile_bwd = 0;       // tuned widths and the batch they were tuned for
     int tile_fwd_n = 0, tile_bwd_n = 0;
+    bool graphs = true;         // replay fused time loops from captured hipGraphs
+    std::map<std::vector<long long>, hipGraphExec_t> graph_cache;
+    char *aux = nullptr;        // multi-grid launch tables
+    size_t aux_cap = 0;
 };
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -110,9 +115,16 @@ struct Carver {
     }
 };
 
+void drop_graphs(Plan &P)
+{
+    for (auto &kv : P.graph_cache) (void)hipGraphExecDestroy(kv.second);
+    P.graph_cache.clear();
+}
+
 int ensure_ws(Plan &P, size_t bytes)
 {
     if (bytes <= P.ws_cap) return SWD_OK;
+    drop_graphs(P);   // captured launches refer to the old workspace
     if (P.ws) {
         HIP_TRY(hipStreamSynchronize(P.stream));
         HIP_TRY(hipFree(P.ws));
@@ -127,6 +139,52 @@ int ensure_ws(Plan &P, size_t bytes)
 
 template <class T>
 T *at(Plan &P, size_t off) { return reinterpret_cast<T *>(P.ws + off); }
+
+int ensure_aux(Plan &P, size_t bytes)
+{
+    if (bytes <= P.aux_cap) return SWD_OK;
+    drop_graphs(P);
+    if (P.aux) {
+        HIP_TRY(hipStreamSynchronize(P.stream));
+        HIP_TRY(hipFree(P.aux));
+        P.aux = nullptr;
+        P.aux_cap = 0;
+    }
+    const size_t cap = align_up(bytes + bytes / 4, 1 << 16);
+    HIP_TRY(hipMalloc(&P.aux, cap));
+    P.aux_cap = cap;
+    return SWD_OK;
+}
+
+// Runs a launch sequence through a captured hipGraph: the first call with a
+// key records the sequence (record() issues the launches on P.stream under
+// capture) and instantiates it; later calls with the same key (same kernels,
+// arguments and workspace addresses) replay it.  Only kernel launches are
+// recorded.
+template <class F>
+int graph_run(Plan &P, const std::vector<long long> &key, F record)
+{
+    auto it = P.graph_cache.find(key);
+    if (it == P.graph_cache.end()) {
+        if (P.graph_cache.size() >= 16) drop_graphs(P);
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+        const int rc = record();
+        const hipError_t e = hipStreamEndCapture(P.stream, &graph);
+        if (rc != SWD_OK) {
+            if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIP_TRY(e);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(ei);
+        it = P.graph_cache.emplace(key, exec).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, P.stream));
+    return SWD_OK;
+}
 
 int check_plan(void *plan, Plan *&P)
 {
@@ -270,7 +328,10 @@ extern "C" int swd_plan_destroy(void *plan)
     Plan *P = static_cast<Plan *>(plan);
     (void)hipSetDevice(P->dev);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
+    for (auto &kv : P->graph_cache) (void)hipGraphExecDestroy(kv.second);
+    P->graph_cache.clear();
     if (P->ws) (void)hipFree(P->ws);
+    if (P->aux) (void)hipFree(P->aux);
     if (P->d_coef) (void)hipFree(P->d_coef);
     if (P->d_model) (void)hipFree(P->d_model);
     for (auto &e : P->ev)
@@ -298,6 +359,7 @@ extern "C" int swd_plan_set_coefs(void *plan, const swd_coefs *c)
     }
     HIP_TRY(hipMemcpy(P->d_coef, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
     for (int i = 0; i < P->g.Lc; i++) P->kg.rc[i] = c->rc[i];
+    drop_graphs(*P);   // captured launches carry the grid constants
     // interior box where every damping coefficient is zero (the fused kernel's fast tiles)
     auto box = [&](const float *a, const float *ah, int n, int &lo, int &hi) {
         lo = P->pmlc;
@@ -359,6 +421,15 @@ extern "C" int swd_plan_set_groups(void *plan, int fwd_shots, int bwd_shots)
     return SWD_OK;
 }
 
+extern "C" int swd_plan_set_graphs(void *plan, int on)
+{
+    if (!plan) return swd_fail(SWD_E_ARG, "swd_plan_set_graphs: null plan");
+    Plan *P = static_cast<Plan *>(plan);
+    P->graphs = on != 0;
+    if (!P->graphs) drop_graphs(*P);
+    return SWD_OK;
+}
+
 extern "C" int swd_plan_set_tile(void *plan, int tile_x)
 {
     if (!plan || (tile_x != 0 && tile_x != 16 && tile_x != 32 && tile_x != 64))
@@ -380,6 +451,14 @@ extern "C" int swd_plan_get_stats(void *plan, swd_stats *st)
 namespace {
 
 // Upload of the batch's integer maps and instance table into the workspace.
+int upload_batch_on(Plan &P, const Batch &b, size_t o_ints, size_t o_pairs, hipStream_t st)
+{
+    HIP_TRY(hipMemcpyAsync(P.ws + o_ints, b.ints.data(), sizeof(int) * b.ints.size(), hipMemcpyHostToDevice, st));
+    if (!b.dup_pairs.empty())
+        HIP_TRY(hipMemcpyAsync(P.ws + o_pairs, b.dup_pairs.data(), sizeof(int2) * b.dup_pairs.size(), hipMemcpyHostToDevice, st));
+    return SWD_OK;
+}
+
 int upload_batch(Plan &P, const Batch &b, size_t o_ints, size_t o_pairs)
 {
     HIP_TRY(hipMemcpyAsync(P.ws + o_ints, b.ints.data(), sizeof(int) * b.ints.size(), hipMemcpyHostToDevice, P.stream));
@@ -499,6 +578,18 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         } else {
             tx = 32;
         }
+    }
+    if (k < its.size() && P.graphs) {
+        const std::vector<long long> key = {1, reinterpret_cast<long long>(d), n, its[k], static_cast<long long>(its.size() - k),
+                                            first_it, tx, pair ? 1 : 0, reinterpret_cast<long long>(P.ws)};
+        const size_t k0 = k;
+        SWD_TRY(graph_run(P, key, [&]() -> int {
+            for (size_t j = k0; j < its.size(); j++)
+                HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, tx, pair, P.stream));
+            return SWD_OK;
+        }));
+        *launches += static_cast<long long>(its.size() - k0);
+        return SWD_OK;
     }
     while (k < its.size()) SWD_TRY(launch(tx));
     return SWD_OK;
@@ -1211,5 +1302,114 @@ extern "C" int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, c
     HIP_TRY(hipMemcpy(d_d, d, sizeof(float) * ng, hipMemcpyHostToDevice));
     SWD_TRY(born_adjoint_dev(*P, nshots, shots, wavelet, d_d, d_g, nsnap));
     HIP_TRY(hipMemcpy(m_grad, d_g, sizeof(float) * nint, hipMemcpyDeviceToHost));
+    return SWD_OK;
+}
+
+// ------------------------------------------------------------------ multi-grid forward
+extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nshots, const swd_shot *const *shots,
+                                 const float *const *wavelets, float *const *gathers)
+{
+    if (nplans < 1 || !plans || !nshots || !shots || !wavelets || !gathers)
+        return swd_fail(SWD_E_ARG, "swd_multi_forward: null argument");
+    std::vector<Plan *> Ps(nplans);
+    for (int i = 0; i < nplans; i++) {
+        for (int j = 0; j < i; j++)
+            if (plans[j] == plans[i]) return swd_fail(SWD_E_ARG, "swd_multi_forward: plan %d is listed twice", i);
+        SWD_TRY(check_plan(plans[i], Ps[i]));
+        if (nshots[i] < 1 || !shots[i] || !wavelets[i] || !gathers[i])
+            return swd_fail(SWD_E_ARG, "swd_multi_forward: plan %d has no shots or buffers", i);
+        if (Ps[i]->dev != Ps[0]->dev || Ps[i]->g.Lc != Ps[0]->g.Lc || Ps[i]->g.nt != Ps[0]->g.nt)
+            return swd_fail(SWD_E_ARG, "swd_multi_forward: plan %d differs from plan 0 in device, Lc or nt", i);
+    }
+    Plan &P0 = *Ps[0];
+    HIP_TRY(hipSetDevice(P0.dev));
+    const int nt = P0.g.nt;
+    // per plan: its batch, fields and gathers in its own workspace
+    std::vector<Batch> bs(nplans);
+    std::vector<size_t> o_ints(nplans), o_pairs(nplans), o_wav(nplans), o_f(nplans), o_g(nplans);
+    for (int i = 0; i < nplans; i++) {
+        Plan &P = *Ps[i];
+        SWD_TRY(build_batch(P, nshots[i], shots[i], bs[i]));
+        Carver c;
+        o_ints[i] = c.add<int>(bs[i].ints.size());
+        o_pairs[i] = c.add<int2>(std::max<size_t>(1, bs[i].dup_pairs.size()));
+        o_wav[i] = c.add<float>(nt);
+        o_g[i] = c.add<float>(std::max<size_t>(1, static_cast<size_t>(bs[i].ntr) * nt));
+        o_f[i] = c.add<float>(kStepArrays * P.nfield * nshots[i]);
+        SWD_TRY(ensure_ws(P, c.off));
+    }
+    // instance, grid and block tables (tiles of 64 rows x 16 columns)
+    std::vector<KGrid> grids(nplans);
+    std::vector<KStep> h;
+    std::vector<int2> btab;
+    std::vector<int> inst0(nplans);
+    for (int i = 0; i < nplans; i++) {
+        grids[i] = Ps[i]->kg;
+        inst0[i] = static_cast<int>(h.size());
+        for (int k = 0; k < nshots[i]; k++) h.emplace_back();
+    }
+    Carver ca;
+    const size_t oa_grid = ca.add<KGrid>(nplans);
+    const size_t oa_inst = ca.add<KStep>(h.size());
+    for (int i = 0; i < nplans; i++) {
+        const int nzb = (Ps[i]->ntz + 63) / 64, nxb = (Ps[i]->ntx + 15) / 16;
+        for (int t = 0; t < nzb * nxb; t++)   // shots of one tile next to each other (shared model in L2)
+            for (int k = 0; k < nshots[i]; k++) btab.push_back(make_int2(inst0[i] + k, t));
+    }
+    const size_t oa_btab = ca.add<int2>(btab.size());
+    SWD_TRY(ensure_aux(P0, ca.off));
+    const KGrid *d_grids = reinterpret_cast<const KGrid *>(P0.aux + oa_grid);
+    for (int i = 0; i < nplans; i++) {
+        Plan &P = *Ps[i];
+        SWD_TRY(upload_batch_on(P, bs[i], o_ints[i], o_pairs[i], P0.stream));
+        HIP_TRY(hipMemcpyAsync(P.ws + o_wav[i], wavelets[i], sizeof(float) * nt, hipMemcpyHostToDevice, P0.stream));
+        HIP_TRY(hipMemsetAsync(P.ws + o_f[i], 0, sizeof(float) * kStepArrays * P.nfield * nshots[i], P0.stream));
+        for (int k = 0; k < nshots[i]; k++) {
+            KStep &S = h[inst0[i] + k];
+            std::memset(&S, 0, sizeof S);
+            S.mode = KSTEP_FWD;
+            S.grid = d_grids + i;
+            bind_step(S, at<float>(P, o_f[i]) + kStepArrays * P.nfield * k, P.nfield);
+            S.wavelet = at<float>(P, o_wav[i]);
+            S.s_ix = shots[i][k].s_ix;
+            S.s_iz = shots[i][k].s_iz;
+            if (shots[i][k].nrec > 0) {
+                S.colmap = at<int>(P, o_ints[i]) + bs[i].col_off[k];
+                S.rowmap = at<int>(P, o_ints[i]) + bs[i].row_off[k];
+                S.gather = at<float>(P, o_g[i]) + static_cast<size_t>(bs[i].toff[k]) * nt;
+            }
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(P0.aux + oa_grid, grids.data(), sizeof(KGrid) * nplans, hipMemcpyHostToDevice, P0.stream));
+    HIP_TRY(hipMemcpyAsync(P0.aux + oa_inst, h.data(), sizeof(KStep) * h.size(), hipMemcpyHostToDevice, P0.stream));
+    HIP_TRY(hipMemcpyAsync(P0.aux + oa_btab, btab.data(), sizeof(int2) * btab.size(), hipMemcpyHostToDevice, P0.stream));
+    const KStep *d_inst = reinterpret_cast<const KStep *>(P0.aux + oa_inst);
+    const int2 *d_btab = reinterpret_cast<const int2 *>(P0.aux + oa_btab);
+    const int nblk = static_cast<int>(btab.size());
+    stats_begin(P0, 0);
+    auto record = [&]() -> int {
+        for (int it = 0; it < nt; it++) HIP_TRY(launch_fstep_multi(P0.g.Lc, d_inst, d_btab, nblk, it, P0.stream));
+        return SWD_OK;
+    };
+    if (P0.graphs) {
+        std::vector<long long> key = {2, reinterpret_cast<long long>(P0.aux), static_cast<long long>(h.size()), nblk, nt};
+        for (int i = 0; i < nplans; i++) key.push_back(reinterpret_cast<long long>(Ps[i]->ws));
+        SWD_TRY(graph_run(P0, key, record));
+    } else {
+        SWD_TRY(record());
+    }
+    P0.stats.fwd_ms = stats_end(P0, 0);
+    P0.stats.fwd_launches = nt;
+    long long cells = 0;
+    for (int i = 0; i < nplans; i++) cells += static_cast<long long>(Ps[i]->ntx) * Ps[i]->ntz * nt * nshots[i];
+    P0.stats.fwd_cells = cells;
+    for (int i = 0; i < nplans; i++) {
+        Plan &P = *Ps[i];
+        HIP_TRY(launch_copy_rows(at<float>(P, o_g[i]), at<int2>(P, o_pairs[i]), static_cast<int>(bs[i].dup_pairs.size()), nt,
+                                 P0.stream));
+        HIP_TRY(hipMemcpyAsync(gathers[i], at<float>(P, o_g[i]), sizeof(float) * bs[i].ntr * nt, hipMemcpyDeviceToHost,
+                               P0.stream));
+    }
+    HIP_TRY(hipStreamSynchronize(P0.stream));
     return SWD_OK;
 }

@@ -77,9 +77,9 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_cpml", "swd_model_in_pml", "swd_read_model", "swd_smooth_model", "swd_nfft", "swd_kz_taper",
            "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
-           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_groups", "swd_plan_get_stats", "swd_forward", "swd_afor",
+           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_groups", "swd_plan_set_graphs", "swd_plan_get_stats", "swd_forward", "swd_afor",
            "swd_rtm", "swd_rtm_dev", "swd_born", "swd_born_adjoint", "swd_born_dev",
-           "swd_born_adjoint_dev")
+           "swd_born_adjoint_dev", "swd_multi_forward")
 
 
 def load():
@@ -290,6 +290,10 @@ class Plan:
         """Shots per fused RTM forward / backward loop (0 = sized to the cache)."""
         check(self.L.swd_plan_set_groups(self.h, int(fwd_shots), int(bwd_shots)))
 
+    def set_graphs(self, on):
+        """Replay fused time loops from captured hipGraphs (default on)."""
+        check(self.L.swd_plan_set_graphs(self.h, 1 if on else 0))
+
     def set_tile(self, tile):
         """Fused-path tile width (16, 32, 64) or 0 to tune it on the first steps."""
         check(self.L.swd_plan_set_tile(self.h, int(tile)))
@@ -364,3 +368,20 @@ class Plan:
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
         check(self.L.swd_born_adjoint_dev(self.h, S.n, S.arr, _p(f32(wavelet)), C.c_void_p(int(d_ptr)),
                                           C.c_void_p(int(g_ptr)), int(nsnap)))
+
+
+def multi_forward(plans, shotsets, wavelets):
+    """Forward modelling of several plans (uneven grids, same Lc and nt) in
+    one fused launch per time step; returns one gather per plan."""
+    n = len(plans)
+    L = load()
+    sets = [S if isinstance(S, ShotSet) else ShotSet(S) for S in shotsets]
+    wav = [f32(w) for w in wavelets]
+    outs = [np.zeros((S.ntr, plans[i].grid.nt), np.float32) for i, S in enumerate(sets)]
+    arr_p = (C.c_void_p * n)(*[p.h.value for p in plans])
+    arr_n = (C.c_int * n)(*[S.n for S in sets])
+    arr_s = (C.c_void_p * n)(*[C.cast(S.arr, C.c_void_p).value for S in sets])
+    arr_w = (C.c_void_p * n)(*[w.ctypes.data for w in wav])
+    arr_g = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
+    check(L.swd_multi_forward(n, arr_p, arr_n, arr_s, arr_w, arr_g))
+    return outs

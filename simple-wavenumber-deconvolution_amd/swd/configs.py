@@ -13,6 +13,10 @@ Workloads (interior nz x nx):
   c2  502 x 845, 10 m, dt 0.8 ms, nt 4000, pml 40, Lc 4: layered 1500-4500 m/s with
       8 random dipping reflectors (seed 2); migration model = 15x15 box average;
       20 Hz Ricker at the surface; 401 receivers at ix 420 spaced 5 m (nearest node).
+  c4  point-scatterer (PSF) modelling: density scatterers on a 40 m lattice over
+      the c1 migration model (15 x 15 box average of the c1 model), 64 surface
+      shots every 12 m; plus 32 single-shot instances of seeded log-uniform
+      sizes from 128 x 128 to 502 x 845 (seed 10), all with Lc 4 and the same nt.
   c3  the c1 grid and model with nt 4000 for the Born operator; reflectivity
       m = dK/K a band-limited random field (10-30 Hz equivalent at 1500 m/s,
       seed 3), data-space probe seed 4, finite-difference directions seeds 5-9.
@@ -21,7 +25,7 @@ import dataclasses
 
 import numpy as np
 
-from . import Grid, cal_xishu, coefs_for, ricker
+from . import Grid, cal_xishu, coefs_for, ricker, scatterers
 
 
 @dataclasses.dataclass
@@ -185,3 +189,29 @@ def small(nx=120, nz=80, pml=10, Lc=4, nt=400, seed=7, f0=25.0, nshots=1, dup=Fa
     c.rho = c.rho * (1.0 + 0.1 * (np.arange(c.rho.shape[1])[None, :] % 7 == 0))   # density contrasts too
     c.rho = c.rho.astype(np.float32)
     return c
+
+
+def with_scatterers(case, swin):
+    """Density point scatterers (+1e-3 vp) on a lattice of swin cells (FC:697-705)."""
+    g = case.grid
+    case.rho = scatterers(case.rho.copy(), case.vp, g.ntx, g.ntz, g.pmlc, swin).reshape(case.rho.shape)
+    return case
+
+
+def c4(nt=1000, nshots=64, n_uneven=32, seed=10):
+    """[64-shot c1-grid PSF case] + 32 uneven single-shot PSF cases."""
+    vm = box_smooth(layered_c1(), 15)
+    rz = np.arange(10, 191, 2)
+    shots = [(60 + int(round(1.2 * k)), 0, np.full(rz.size, 100), rz) for k in range(nshots)]
+    base = with_scatterers(make_case("c4", 300, 200, 20, 4, 10.0, 10.0, 1e-3, nt, vm, 20.0, shots), 4)
+    rng = np.random.default_rng(seed)
+    cases = [base]
+    for k in range(n_uneven):
+        nz = int(round(np.exp(rng.uniform(np.log(128), np.log(502)))))
+        nx = int(round(np.exp(rng.uniform(np.log(128), np.log(845)))))
+        v = (1500.0 + 1500.0 * (np.arange(nz) / nz))[None, :] * np.ones((nx, 1))
+        rzk = np.arange(10, nz - 10, 2)
+        sh = [(nx // 2, 0, np.full(rzk.size, nx // 3), rzk)]
+        cases.append(with_scatterers(make_case(f"c4u{k}", nx, nz, 20, 4, 10.0, 10.0, 1e-3, nt, v.astype(np.float32),
+                                               20.0, sh), 4))
+    return cases
