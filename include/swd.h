@@ -95,6 +95,7 @@ typedef struct swd_stats {
     long long ws_bytes;     /* device workspace held by the plan */
     long long ckpt_bytes;   /* last Born adjoint: snapshot + one segment of background sums */
     long long full_bytes;   /* last Born adjoint: background sums of all nt steps (full storage) */
+    long long waves;        /* shot waves of the last call (see swd_plan_set_memory_limit) */
 } swd_stats;
 
 /* ---------------- errors and devices ---------------- */
@@ -143,6 +144,10 @@ int swd_plan_set_groups(void *plan, int fwd_shots, int bwd_shots);
  * launches of a loop are recorded once per configuration and replayed on
  * later calls with the same batch layout. */
 int swd_plan_set_graphs(void *plan, int on);
+/* Device memory a call may use for its workspace (0, the default: 85% of the
+ * free device memory).  swd_rtm*, swd_born* and swd_forward run larger
+ * batches in consecutive waves of shots that fit; results are the same. */
+int swd_plan_set_memory_limit(void *plan, size_t bytes);
 int swd_plan_get_stats(void *plan, swd_stats *st);
 
 /* ---------------- operators ----------------

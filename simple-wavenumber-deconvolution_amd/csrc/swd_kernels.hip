@@ -509,11 +509,13 @@ __global__ void k_sum_rows(float *__restrict__ dst, const float *__restrict__ sr
 }
 
 // Sum of per-shot arrays in shot order: dst[i] = src[0][i] + src[1][i] + ...
-__global__ void k_sum_shots(float *__restrict__ dst, const float *__restrict__ src, int nshots, size_t n)
+// With accumulate, dst already holds the sum of earlier shots and the shots
+// of src are added to it in order.
+__global__ void k_sum_shots(float *__restrict__ dst, const float *__restrict__ src, int nshots, size_t n, int accumulate)
 {
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        float acc = src[i];
+        float acc = accumulate ? dst[i] + src[i] : src[i];
         for (int k = 1; k < nshots; k++) acc += src[static_cast<size_t>(k) * n + i];
         dst[i] = acc;
     }
@@ -678,11 +680,12 @@ hipError_t launch_sum_rows(float *dst, const float *src, const int2 *pairs, int 
     return hipGetLastError();
 }
 
-hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, hipStream_t s)
+hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, bool accumulate, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_sum_shots, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, dst, src, nshots, n);
+    hipLaunchKernelGGL(k_sum_shots, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, dst, src, nshots, n,
+                       accumulate ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -11,7 +11,7 @@ hipError_t launch_src_v(const KGrid &g, const KInst *insts, int ninst, int comp,
 hipError_t launch_adjoint_step(const KGrid &g, const KInst *insts, int ninst, int it, bool first, hipStream_t s);
 hipError_t launch_copy_rows(float *gather, const int2 *pairs, int npairs, int nt, hipStream_t s);
 hipError_t launch_sum_rows(float *dst, const float *src, const int2 *pairs, int npairs, int nt, hipStream_t s);
-hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, hipStream_t s);
+hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, bool accumulate, hipStream_t s);
 // fused step (swd_fstep.hip): tx = tile width in columns (16, 32 or 64);
 // pair: instances are (SRC, RCV) pairs of the RTM backward, ninst counts both
 hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, hipStream_t s);

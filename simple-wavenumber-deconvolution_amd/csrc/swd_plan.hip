@@ -99,6 +99,7 @@ struct Plan {
     std::map<std::vector<long long>, hipGraphExec_t> graph_cache;
     char *aux = nullptr;        // multi-grid launch tables
     size_t aux_cap = 0;
+    size_t mem_limit = 0;       // device bytes a call may use (0 = the free device memory)
 };
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -713,6 +714,40 @@ struct DevBuf {
 
 }  // namespace
 
+// ------------------------------------------------------------------ shot waves
+namespace {
+
+// Shots per wave of a call whose workspace needs fixed + n * per_shot bytes:
+// as many as fit the plan's memory limit (by default 85% of the free device
+// memory plus the plan's current workspace, which the call may reallocate).
+int wave_shots(Plan &P, int nshots, double per_shot, double fixed, int &wave)
+{
+    double budget;
+    if (P.mem_limit > 0) {
+        budget = static_cast<double>(P.mem_limit);
+    } else {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(hipMemGetInfo(&fr, &tot));
+        budget = 0.85 * (static_cast<double>(fr) + static_cast<double>(P.ws_cap));
+    }
+    const double room = budget - fixed;
+    if (room < per_shot)
+        return swd_fail(SWD_E_NOMEM, "one shot needs %.0f MB of device memory, the budget is %.0f MB",
+                        (per_shot + fixed) / 1e6, budget / 1e6);
+    wave = static_cast<int>(std::min<double>(nshots, std::floor(room / per_shot)));
+    P.stats.waves = (nshots + wave - 1) / wave;
+    return SWD_OK;
+}
+
+int traces_before(const swd_shot *shots, int k)
+{
+    int n = 0;
+    for (int i = 0; i < k; i++) n += shots[i].nrec > 0 ? shots[i].nrec : 0;
+    return n;
+}
+
+}  // namespace
+
 extern "C" int swd_forward(void *plan, int nshots, const swd_shot *shots, const float *wavelet, float *gather)
 {
     Plan *P;
@@ -721,7 +756,11 @@ extern "C" int swd_forward(void *plan, int nshots, const swd_shot *shots, const 
     const size_t n = static_cast<size_t>(total_traces(nshots, shots)) * P->g.nt;
     DevBuf d;
     SWD_TRY(d.alloc(sizeof(float) * n));
-    SWD_TRY(forward_dev(*P, nshots, shots, wavelet, d.as<float>()));
+    int wave = nshots;
+    SWD_TRY(wave_shots(*P, nshots, 4.0 * (P->path == SWD_PATH_FUSED ? 12.0 : 7.0) * P->nfield, 4.0 * P->g.nt * 4, wave));
+    for (int s0 = 0; s0 < nshots; s0 += wave)
+        SWD_TRY(forward_dev(*P, std::min(wave, nshots - s0), shots + s0, wavelet,
+                            d.as<float>() + static_cast<size_t>(traces_before(shots, s0)) * P->g.nt));
     HIP_TRY(hipMemcpy(gather, d.p, sizeof(float) * n, hipMemcpyDeviceToHost));
     return SWD_OK;
 }
@@ -992,13 +1031,10 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
 
 }  // namespace
 
-extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
-                           float *image, float *illum_src, float *illum_rcv, float *gather_syn)
+namespace {
+int rtm_dev_impl(Plan *P, int nshots, const swd_shot *shots, const float *wavelet, const float *adj, float *image,
+                 float *illum_src, float *illum_rcv, float *gather_syn)
 {
-    Plan *P;
-    SWD_TRY(check_plan(plan, P));
-    if (!wavelet || !adj || !image || !illum_src || !illum_rcv || !shots || nshots < 1)
-        return swd_fail(SWD_E_ARG, "swd_rtm: null argument");
     Batch b;
     SWD_TRY(build_batch(*P, nshots, shots, b));
     if (P->path == SWD_PATH_FUSED)
@@ -1081,6 +1117,36 @@ extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const 
     return SWD_OK;
 }
 
+}  // namespace
+
+extern "C" int swd_plan_set_memory_limit(void *plan, size_t bytes)
+{
+    if (!plan) return swd_fail(SWD_E_ARG, "swd_plan_set_memory_limit: null plan");
+    static_cast<Plan *>(plan)->mem_limit = bytes;
+    return SWD_OK;
+}
+
+extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
+                           float *image, float *illum_src, float *illum_rcv, float *gather_syn)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !adj || !image || !illum_src || !illum_rcv || !shots || nshots < 1)
+        return swd_fail(SWD_E_ARG, "swd_rtm: null argument");
+    const double nt = P->g.nt, nf = static_cast<double>(P->nfield);
+    const double per = 4.0 * (nt * P->nframe + (P->path == SWD_PATH_FUSED ? 27.0 : 18.0) * nf);
+    int wave = nshots;
+    SWD_TRY(wave_shots(*P, nshots, per, 4.0 * nt * 4, wave));
+    const size_t ntp = static_cast<size_t>(P->ntx) * P->ntz;
+    for (int s0 = 0; s0 < nshots; s0 += wave) {
+        const int n = std::min(wave, nshots - s0);
+        const size_t tr = static_cast<size_t>(traces_before(shots, s0)) * P->g.nt;
+        SWD_TRY(rtm_dev_impl(P, n, shots + s0, wavelet, adj + tr, image + ntp * s0, illum_src + ntp * s0,
+                             illum_rcv + ntp * s0, gather_syn ? gather_syn + tr : nullptr));
+    }
+    return SWD_OK;
+}
+
 extern "C" int swd_rtm(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
                        float *image, float *illum_src, float *illum_rcv, float *gather_syn)
 {
@@ -1157,7 +1223,7 @@ int born_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet, c
 // reverse, which stores the background sums of one segment at a time instead
 // of all nt steps.
 int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet, const float *d, float *m_grad,
-                     int nsnap)
+                     int nsnap, bool accumulate)
 {
     Batch b;
     SWD_TRY(build_batch(P, nshots, shots, b));
@@ -1243,7 +1309,7 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
     P.stats.fwd_ms = stats_end(P, 0);
     P.stats.fwd_launches = fl;
     P.stats.bwd_launches = bl;
-    HIP_TRY(launch_sum_shots(m_grad, at<float>(P, o_grad), nshots, nint, P.stream));
+    HIP_TRY(launch_sum_shots(m_grad, at<float>(P, o_grad), nshots, nint, accumulate, P.stream));
     HIP_TRY(hipStreamSynchronize(P.stream));
     P.stats.ws_bytes = static_cast<long long>(P.ws_cap);
     P.stats.ckpt_bytes = static_cast<long long>(sizeof(float)) * nshots *
@@ -1260,7 +1326,14 @@ extern "C" int swd_born_dev(void *plan, int nshots, const swd_shot *shots, const
     Plan *P;
     SWD_TRY(check_plan(plan, P));
     if (!wavelet || !m || !d || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_born_dev: null argument");
-    return born_dev(*P, nshots, shots, wavelet, m, d);
+    const size_t nint = static_cast<size_t>(P->g.nx) * P->g.nz;
+    int wave = nshots;
+    SWD_TRY(wave_shots(*P, nshots, 4.0 * 14.0 * P->nfield, 4.0 * P->g.nt * 4, wave));
+    for (int s0 = 0; s0 < nshots; s0 += wave) {
+        const int n = std::min(wave, nshots - s0);
+        SWD_TRY(born_dev(*P, n, shots + s0, wavelet, m + nint * s0, d + static_cast<size_t>(traces_before(shots, s0)) * P->g.nt));
+    }
+    return SWD_OK;
 }
 
 extern "C" int swd_born_adjoint_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
@@ -1269,7 +1342,21 @@ extern "C" int swd_born_adjoint_dev(void *plan, int nshots, const swd_shot *shot
     Plan *P;
     SWD_TRY(check_plan(plan, P));
     if (!wavelet || !m_grad || !d || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_born_adjoint_dev: null argument");
-    return born_adjoint_dev(*P, nshots, shots, wavelet, d, m_grad, nsnap);
+    const int nt = P->g.nt;
+    const double nf = static_cast<double>(P->nfield), nint = static_cast<double>(P->g.nx) * P->g.nz;
+    const int seg = nsnap > 0 ? (nt + nsnap - 1) / nsnap : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(nt))));
+    const int nseg = (nt + seg - 1) / seg;
+    int ntr_max = 0;
+    for (int k = 0; k < nshots; k++) ntr_max = std::max(ntr_max, shots[k].nrec);
+    const double per = 4.0 * (7.0 * nf * nseg + nint * seg + 16.0 * nf + nint + static_cast<double>(ntr_max) * nt);
+    int wave = nshots;
+    SWD_TRY(wave_shots(*P, nshots, per, 4.0 * nt * 4, wave));
+    for (int s0 = 0; s0 < nshots; s0 += wave) {
+        const int n = std::min(wave, nshots - s0);
+        SWD_TRY(born_adjoint_dev(*P, n, shots + s0, wavelet, d + static_cast<size_t>(traces_before(shots, s0)) * nt, m_grad,
+                                 nsnap, s0 > 0));
+    }
+    return SWD_OK;
 }
 
 extern "C" int swd_born(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *m, float *d)
@@ -1283,7 +1370,7 @@ extern "C" int swd_born(void *plan, int nshots, const swd_shot *shots, const flo
     SWD_TRY(buf.alloc(sizeof(float) * (nint * nshots + ng)));
     float *d_m = buf.as<float>(), *d_d = d_m + nint * nshots;
     HIP_TRY(hipMemcpy(d_m, m, sizeof(float) * nint * nshots, hipMemcpyHostToDevice));
-    SWD_TRY(born_dev(*P, nshots, shots, wavelet, d_m, d_d));
+    SWD_TRY(swd_born_dev(plan, nshots, shots, wavelet, d_m, d_d));
     HIP_TRY(hipMemcpy(d, d_d, sizeof(float) * ng, hipMemcpyDeviceToHost));
     return SWD_OK;
 }
@@ -1300,7 +1387,7 @@ extern "C" int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, c
     SWD_TRY(buf.alloc(sizeof(float) * (nint + std::max<size_t>(ng, 1))));
     float *d_g = buf.as<float>(), *d_d = d_g + nint;
     HIP_TRY(hipMemcpy(d_d, d, sizeof(float) * ng, hipMemcpyHostToDevice));
-    SWD_TRY(born_adjoint_dev(*P, nshots, shots, wavelet, d_d, d_g, nsnap));
+    SWD_TRY(swd_born_adjoint_dev(plan, nshots, shots, wavelet, d_d, d_g, nsnap));
     HIP_TRY(hipMemcpy(m_grad, d_g, sizeof(float) * nint, hipMemcpyDeviceToHost));
     return SWD_OK;
 }

@@ -65,7 +65,8 @@ class Stats(C.Structure):
     _fields_ = [("fwd_ms", C.c_double), ("bwd_ms", C.c_double), ("fft_ms", C.c_double),
                 ("fwd_launches", C.c_longlong), ("bwd_launches", C.c_longlong),
                 ("fwd_cells", C.c_longlong), ("bwd_cells", C.c_longlong),
-                ("ws_bytes", C.c_longlong), ("ckpt_bytes", C.c_longlong), ("full_bytes", C.c_longlong)]
+                ("ws_bytes", C.c_longlong), ("ckpt_bytes", C.c_longlong), ("full_bytes", C.c_longlong),
+                ("waves", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -77,7 +78,8 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_cpml", "swd_model_in_pml", "swd_read_model", "swd_smooth_model", "swd_nfft", "swd_kz_taper",
            "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
-           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_groups", "swd_plan_set_graphs", "swd_plan_get_stats", "swd_forward", "swd_afor",
+           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_groups", "swd_plan_set_graphs",
+           "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor",
            "swd_rtm", "swd_rtm_dev", "swd_born", "swd_born_adjoint", "swd_born_dev",
            "swd_born_adjoint_dev", "swd_multi_forward")
 
@@ -289,6 +291,12 @@ class Plan:
     def set_groups(self, fwd_shots=0, bwd_shots=0):
         """Shots per fused RTM forward / backward loop (0 = sized to the cache)."""
         check(self.L.swd_plan_set_groups(self.h, int(fwd_shots), int(bwd_shots)))
+
+    def set_memory_limit(self, nbytes):
+        """Device bytes a call may use (0 = 85% of the free memory); larger batches run in waves."""
+        L = self.L
+        L.swd_plan_set_memory_limit.argtypes = [C.c_void_p, C.c_size_t]
+        check(L.swd_plan_set_memory_limit(self.h, int(nbytes)))
 
     def set_graphs(self, on):
         """Replay fused time loops from captured hipGraphs (default on)."""

@@ -17,6 +17,10 @@ Workloads (interior nz x nx):
       the c1 migration model (15 x 15 box average of the c1 model), 64 surface
       shots every 12 m; plus 32 single-shot instances of seeded log-uniform
       sizes from 128 x 128 to 502 x 845 (seed 10), all with Lc 4 and the same nt.
+  c5  1001 x 1690, 5 m, dt 0.4 ms, nt 8000, pml 60, Lc 4: the c2 kind of model
+      (layered 1500-4500 m/s with 8 dipping reflectors, seed 5) on the finer
+      grid, 15 x 15 box-smoothed migration model, 128 seeded surface shots
+      (seed 6), 801 receivers at ix 840 spaced 2.5 m (nearest node).
   c3  the c1 grid and model with nt 4000 for the Born operator; reflectivity
       m = dK/K a band-limited random field (10-30 Hz equivalent at 1500 m/s,
       seed 3), data-space probe seed 4, finite-difference directions seeds 5-9.
@@ -215,3 +219,33 @@ def c4(nt=1000, nshots=64, n_uneven=32, seed=10):
         cases.append(with_scatterers(make_case(f"c4u{k}", nx, nz, 20, 4, 10.0, 10.0, 1e-3, nt, v.astype(np.float32),
                                                20.0, sh), 4))
     return cases
+
+
+def layered(nx, nz, dz, seed, nrefl=8):
+    """Smooth 1500 -> 4500 m/s background with nrefl seeded dipping reflectors."""
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz)[None, :] * dz
+    x = np.arange(nx)[:, None] * dz
+    depth = nz * dz
+    v = 1500.0 + 3000.0 * (z / depth) ** 0.8 + 0.0 * x
+    for _ in range(nrefl):
+        z0 = rng.uniform(0.1 * depth, 0.94 * depth)
+        dip = np.tan(np.deg2rad(rng.uniform(-15, 15)))
+        dv = rng.uniform(-250, 350)
+        v = v + dv * (z > z0 + dip * (x - nx * dz / 2))
+    return np.clip(v, 1500, 4500).astype(np.float32)
+
+
+def c5(nt=8000, nshots=128, shot_index=None):
+    """c5 (see the module docstring); shot_index selects some of the 128 shots."""
+    nx, nz, dz = 1690, 1001, 5.0
+    vt = layered(nx, nz, dz, 5)
+    vm = box_smooth(vt, 15)
+    rz = np.floor(np.arange(801) * 2.5 / dz + 0.5).astype(np.int32)
+    rz = rz[rz < nz]
+    sx = np.sort(np.random.default_rng(6).integers(60, nx - 60, size=nshots))
+    idx = range(nshots) if shot_index is None else shot_index
+    shots = [(int(sx[k]), 0, np.full(rz.size, 840), rz) for k in idx]
+    c = make_case("c5", nx, nz, 60, 4, dz, dz, 4e-4, nt, vm, 20.0, shots)
+    c.vp_true = pad_model(vt, c.pmlc)
+    return c
