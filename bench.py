@@ -56,7 +56,7 @@ def build_case(nshots, nt):
     return configs.c2(nt=nt, shot_ix=xs)
 
 
-def cpu_baseline(case, nt_sample, threads):
+def cpu_baseline(case, nt_sample, threads, nshots=2):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
     from helpers import oracle_problem
@@ -65,14 +65,15 @@ def cpu_baseline(case, nt_sample, threads):
     O.set_threads(threads)
     g = case.grid
     sub = swd.configs.Case(case.name, swd.Grid(g.nx, g.nz, g.pml, g.Lc, g.dx, g.dz, g.dt, nt_sample),
-                           case.vp, case.rho, case.coefs, case.wavelet[:nt_sample], case.shots[:1], case.f0)
-    P = oracle_problem(sub)
+                           case.vp, case.rho, case.coefs, case.wavelet[:nt_sample], case.shots[:nshots], case.f0)
     t0 = time.perf_counter()
-    d, st, fs = P.forward(strips=True, final=True)
-    adj = O.adjoint_source(d, g.dt)
-    P.rtm_backward(fs, st, adj)
+    for k in range(nshots):
+        P = oracle_problem(sub, k)
+        d, st, fs = P.forward(strips=True, final=True)
+        adj = O.adjoint_source(d, g.dt)
+        P.rtm_backward(fs, st, adj)
     dt = time.perf_counter() - t0
-    cells = rtm_cells(sub.grid)
+    cells = rtm_cells(sub.grid) * nshots
     return cells / dt / 1e9, dt
 
 
@@ -85,7 +86,7 @@ def main():
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-nt", type=int, default=600)
+    ap.add_argument("--cpu-nt", type=int, default=4000)
     ap.add_argument("--tile", type=int, default=0, help="fused tile width (0 = tuned on the first steps)")
     ap.add_argument("--path", choices=("fused", "batched"), default="fused")
     ap.add_argument("--groups", type=int, nargs=2, default=(0, 0), metavar=("FWD", "BWD"),
@@ -189,7 +190,7 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         rate, dt = cpu_baseline(case, a.cpu_nt, threads)
         cpu = {"value": rate, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-               "sample": f"oracle fp32 host loops (OpenMP) on one c2 shot with nt={a.cpu_nt} "
+               "sample": f"oracle fp32 host loops (OpenMP) on 2 c2 shots with nt={a.cpu_nt} "
                          f"(forward with strips + backward + imaging), {dt:.1f}s"}
         log(f"bench: cpu baseline {rate:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
 
