@@ -79,6 +79,10 @@ __device__ __forceinline__ void vz_item(int i, int &c, int &qq)
     }
 }
 
+// Threads per workgroup: one own quad per thread up to 32 columns.
+template <int TX>
+constexpr int fstep_threads() { return TX >= 32 ? 512 : 256; }
+
 // Tile geometry: 64 rows (16 quads) x TX columns, Lc-deep stencils.
 template <int LC, int TX>
 struct FGeo {
@@ -89,7 +93,8 @@ struct FGeo {
     static constexpr int NPQ = 16 + 2 * RP;      // staged pressure quads -RP .. 15+RP
     static constexpr int NVX = TX + 2 * LC - 1;  // vx columns X0-Lc .. X1+Lc-2
     static constexpr int VZQ = 16 + 2 * R;       // vz quads -R .. 15+R
-    static constexpr int NOWN = (TX * 16 + 255) / 256;   // own quads per thread
+    static constexpr int NT = fstep_threads<TX>();       // threads per workgroup
+    static constexpr int NOWN = (TX * 16 + NT - 1) / NT;  // own quads per thread
 };
 
 // One tile of one wavefield.  FULL: the tile and every halo lie inside the
@@ -105,7 +110,8 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     using G = FGeo<LC, TX>;
     constexpr int R = G::R, RP = G::RP, OWN = G::OWN, NPQ = G::NPQ, VZQ = G::VZQ;
     constexpr int N0 = G::NPC * NPQ, N1 = G::NVX * 16, N2 = TX * VZQ, N3 = TX * 16;
-    constexpr int I0 = (N0 + 255) / 256, I1 = (N1 + 255) / 256, I2 = (N2 + 255) / 256, I3 = (N3 + 255) / 256;
+    constexpr int NT = G::NT;
+    constexpr int I0 = (N0 + NT - 1) / NT, I1 = (N1 + NT - 1) / NT, I2 = (N2 + NT - 1) / NT, I3 = (N3 + NT - 1) / NT;
     const int tid = threadIdx.x;
     const int mode = I.mode;
     const int pmlc = g.pmlc, ntx = g.ntx, ntz = g.ntz;
@@ -124,7 +130,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     float4 vxo[I1], bxv[I1], vzo[I2], bzv[I2], kv[I3];
 #pragma unroll
     for (int r = 0; r < I1; r++) {
-        const int i = tid + 256 * r, q = i & 15, cx = i >> 4;
+        const int i = tid + NT * r, q = i & 15, cx = i >> 4;
         const int x = X0 - LC + cx, z = Z0 + 4 * q;
         vxo[r] = bxv[r] = zero4();
         if (i < N1 && (FULL || (x >= 0 && x < ntx && z < ntz))) {
@@ -135,7 +141,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     }
 #pragma unroll
     for (int r = 0; r < I2; r++) {
-        const int i = tid + 256 * r;
+        const int i = tid + NT * r;
         int c, qq;
         vz_item<TX, R>(i, c, qq);
         const int x = X0 + c, z = Z0 + 4 * qq;
@@ -150,7 +156,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     int cmv[I3];   // receiver map of each own column
 #pragma unroll
     for (int r = 0; r < I3; r++) {
-        const int i = tid + 256 * r, q = i & 15, c = i >> 4;
+        const int i = tid + NT * r, q = i & 15, c = i >> 4;
         const int x = X0 + c, z = Z0 + 4 * q;
         kv[r] = imv[r] = isv[r] = irv[r] = zero4();
         cmv[r] = -1;
@@ -171,7 +177,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         float4 st[I0];
 #pragma unroll
         for (int r = 0; r < I0; r++) {
-            const int i = tid + 256 * r, col = i / NPQ, qq = i - col * NPQ - RP;
+            const int i = tid + NT * r, col = i / NPQ, qq = i - col * NPQ - RP;
             const int x = X0 - OWN + col, z = Z0 + 4 * qq;
             const bool need = i < N0 && ((qq >= 0 && qq < 16) || (col >= OWN && col < OWN + TX));
             st[r] = zero4();
@@ -179,7 +185,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         }
 #pragma unroll
         for (int r = 0; r < I0; r++) {
-            const int i = tid + 256 * r;
+            const int i = tid + NT * r;
             if (i < N0) sP[i] = st[r];
         }
     }
@@ -188,7 +194,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     // ---- velocity x on columns X0-Lc .. X1+Lc-2 of the tile's rows
 #pragma unroll
     for (int r = 0; r < I1; r++) {
-        const int i = tid + 256 * r;
+        const int i = tid + NT * r;
         if (i >= N1) break;
         const int q = i & 15, cx = i >> 4;
         const int x = X0 - LC + cx, z = Z0 + 4 * q;
@@ -253,7 +259,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     // ---- velocity z on quads -R .. 15+R of the tile's columns
 #pragma unroll
     for (int r = 0; r < I2; r++) {
-        const int i = tid + 256 * r;
+        const int i = tid + NT * r;
         if (i >= N2) break;
         int c, qq;
         vz_item<TX, R>(i, c, qq);
@@ -329,7 +335,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     // ---- pressure of the tile's own cells from the velocities in LDS
 #pragma unroll
     for (int r = 0; r < I3; r++) {
-        const int i = tid + 256 * r;
+        const int i = tid + NT * r;
         if (i >= N3) break;
         const int q = i & 15, c = i >> 4;
         const int x = X0 + c, z = Z0 + 4 * q;
@@ -497,7 +503,8 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 // PAIR = false: one FWD wavefield per instance.  PAIR = true: instances
 // 2k (SRC) and 2k+1 (RCV) of one shot share each workgroup.
 template <int LC, int TX, bool PAIR>
-__global__ __launch_bounds__(256) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb, int nxb,
+__global__ __launch_bounds__(fstep_threads<TX>()) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb,
+                                                              int nxb,
                                                int it, int first)
 {
     using G = FGeo<LC, TX>;
@@ -587,7 +594,8 @@ hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, b
     const int nblocks = (n * nzb * nxb + 7) / 8 * 8;
     const int f = first ? 1 : 0;
 #define LF(TXV, PV)                                                                                                   \
-    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_fstep<L, TXV, PV>), dim3(nblocks), dim3(256), 0, s, g, insts, n, nzb, nxb, \
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_fstep<L, TXV, PV>), dim3(nblocks), dim3(FGeo<L, TXV>::NT), 0, s, g, insts, n, \
+                                         nzb, nxb, \
                                          it, f))
     if (pair) {
         switch (tx) {
