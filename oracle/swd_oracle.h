@@ -115,6 +115,29 @@ int or_born_adjoint_f32(const or_grid *g, const float *vp, const float *rho, con
 int or_born_adjoint_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
                         const or_shot *s, const float *wavelet, const float *d, double *mgrad);
 
+/* Wavenumber-domain PSF deconvolution (kkdecon_acoustic/kk_decon_interp_psf_minewindow.cpp,
+ * DC), one rank.  image, psf, dec, coef are [nx][nz] (z fastest). */
+typedef struct or_decon_params {
+    int nx, nz, psfwinx, psfwinz, filterwinx, filterwinz, snapwinx, snapwinz;
+    float epsilon;
+} or_decon_params;
+typedef struct or_decon_geom {
+    int halfpsfwx, halfpsfwz, halfltwx, halfltwz, nxwpsf, nzwpsf, fullwx, fullwz, midpx, midpz;
+    int outwinx, outwinz, halfoutwx, halfoutwz, stepwinx, stepwinz, stepnumx, stepnumz, totalnum, nxp, nzp;
+} or_decon_geom;
+int or_decon_geometry(const or_decon_params *p, or_decon_geom *g);
+void or_decon_coefr(const or_decon_params *p, const or_decon_geom *g, float *coefr);
+void or_decon_expand(const float *a, int nx, int nz, float *af, int nxp, int nzp);
+void or_twiddles(int n, double *c, double *s);
+void or_decon_spectrum(const double *x, int mx, int mz, int offx, int offz, int n, double *re, double *im);
+void or_decon_inverse(const double *dr, const double *di, int n, int r0, int nr, int c0, int nc, double *out);
+void or_decon_cell(const or_decon_params *p, const or_decon_geom *g, int ix, int iz, int *cx, int *cz);
+void or_decon_psf(const or_decon_params *p, const or_decon_geom *g, const float *psff, int cx, int cz, double *sr,
+                  double *si, float *tden);
+void or_decon_window(const or_decon_params *p, const or_decon_geom *g, const float *imgf, const float *psff,
+                     int step, double *out);
+int or_decon(const or_decon_params *p, const float *image, const float *psf, float *dec, float *coef);
+
 int or_frame_size(const or_grid *g);
 void or_set_threads(int n);
 

@@ -24,6 +24,14 @@ Workloads (interior nz x nx):
   c3  the c1 grid and model with nt 4000 for the Born operator; reflectivity
       m = dK/K a band-limited random field (10-30 Hz equivalent at 1500 m/s,
       seed 3), data-space probe seed 4, finite-difference directions seeds 5-9.
+
+PSF deconvolution workload (``decon_case``), [nx][nz] float32 images:
+  d1  845 x 702 (the reference's deconvolution grid), psfwin 31, filterwin 71,
+      snapwin 11, epsilon 5e-4: a PSF image with a depth-varying 2D wavelet
+      (Ricker along z, Gaussian along x, peak frequency falling and width
+      growing with depth) at every lattice node (15 + 31 i, 15 + 31 j), and a
+      migrated image = 8 dipping reflectors + 300 point diffractors (seed 11)
+      blurred by the mid-depth wavelet.
 """
 import dataclasses
 
@@ -249,3 +257,45 @@ def c5(nt=8000, nshots=128, shot_index=None):
     c = make_case("c5", nx, nz, 60, 4, dz, dz, 4e-4, nt, vm, 20.0, shots)
     c.vp_true = pad_model(vt, c.pmlc)
     return c
+
+
+DECON_PARAMS = dict(psfwin=31, filterwin=71, snapwin=11, epsilon=5e-4)
+
+
+def _psf_wavelet(w, depth_frac, rng_tilt=0.0):
+    """w x w wavelet centred in the patch: Ricker along z, Gaussian along x."""
+    h = w // 2
+    ax, az = np.meshgrid(np.arange(-h, h + 1), np.arange(-h, h + 1), indexing="ij")
+    az = az + rng_tilt * ax
+    fz = 0.16 - 0.07 * depth_frac              # cycles per cell along z
+    sx = 2.0 + 4.0 * depth_frac                # Gaussian width along x (cells)
+    a = (np.pi * fz * az) ** 2
+    return ((1.0 - 2.0 * a) * np.exp(-a) * np.exp(-0.5 * (ax / sx) ** 2)).astype(np.float32)
+
+
+def decon_case(nx=845, nz=702, seed=11, psfwin=31):
+    """(image, psf) [nx][nz] float32 of the d1 kind on an nx x nz grid."""
+    rng = np.random.default_rng(seed)
+    psf = np.zeros((nx, nz), np.float32)
+    h = psfwin // 2
+    for cx in range(h, nx - h, psfwin):
+        for cz in range(h, nz - h, psfwin):
+            psf[cx - h:cx + h + 1, cz - h:cz + h + 1] = _psf_wavelet(psfwin, cz / nz, 0.1 * (cx / nx - 0.5))
+    refl = np.zeros((nx, nz), np.float32)
+    xs = np.arange(nx)
+    for _ in range(8):
+        z0 = rng.uniform(0.1, 0.9) * nz
+        dip = rng.uniform(-0.15, 0.15)
+        iz = np.clip(np.round(z0 + dip * (xs - nx / 2)).astype(int), 0, nz - 1)
+        refl[xs, iz] += rng.uniform(-1.0, 1.0)
+    px = rng.integers(0, nx, 300)
+    pz = rng.integers(0, nz, 300)
+    np.add.at(refl, (px, pz), rng.uniform(-2.0, 2.0, 300).astype(np.float32))
+    k = _psf_wavelet(psfwin, 0.5)
+    img = np.zeros((nx + psfwin - 1, nz + psfwin - 1), np.float64)
+    for i in range(psfwin):
+        for j in range(psfwin):
+            if k[i, j] != 0.0:
+                img[i:i + nx, j:j + nz] += k[i, j] * refl
+    image = (1e6 * img[h:h + nx, h:h + nz]).astype(np.float32)
+    return np.ascontiguousarray(image), np.ascontiguousarray(1e5 * psf)

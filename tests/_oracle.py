@@ -264,3 +264,133 @@ def edge_zero(mig, nx, nz, Lc):
 
 def set_threads(n):
     lib().or_set_threads(n)
+
+
+# ---------------------------------------------------------------------------
+# Wavenumber-domain PSF deconvolution (kkdecon_acoustic/kk_decon_interp_psf_minewindow.cpp)
+
+class DeconParams(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nx", "nz", "psfwinx", "psfwinz", "filterwinx", "filterwinz",
+                                       "snapwinx", "snapwinz")] + [("epsilon", C.c_float)]
+
+
+GEOM_FIELDS = ("halfpsfwx", "halfpsfwz", "halfltwx", "halfltwz", "nxwpsf", "nzwpsf", "fullwx", "fullwz",
+               "midpx", "midpz", "outwinx", "outwinz", "halfoutwx", "halfoutwz", "stepwinx", "stepwinz",
+               "stepnumx", "stepnumz", "totalnum", "nxp", "nzp")
+
+
+class DeconGeom(C.Structure):
+    _fields_ = [(n, C.c_int) for n in GEOM_FIELDS]
+
+
+def _decon_lib():
+    L = lib()
+    if not getattr(L, "_decon_bound", False):
+        P, G = C.POINTER(DeconParams), C.POINTER(DeconGeom)
+        L.or_decon_geometry.argtypes = [P, G]
+        L.or_decon_geometry.restype = C.c_int
+        L.or_decon_coefr.argtypes = [P, G, f32p]
+        L.or_decon_expand.argtypes = [f32p, C.c_int, C.c_int, f32p, C.c_int, C.c_int]
+        L.or_twiddles.argtypes = [C.c_int, f64p, f64p]
+        L.or_decon_spectrum.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
+        L.or_decon_inverse.argtypes = [f64p, f64p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f64p]
+        L.or_decon_cell.argtypes = [P, G, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_decon_psf.argtypes = [P, G, f32p, C.c_int, C.c_int, f64p, f64p, f32p]
+        L.or_decon_window.argtypes = [P, G, f32p, f32p, C.c_int, f64p]
+        L.or_decon.argtypes = [P, f32p, f32p, f32p, f32p]
+        L.or_decon.restype = C.c_int
+        L._decon_bound = True
+    return L
+
+
+def decon_params(nx, nz, psfwin=31, filterwin=71, snapwin=11, epsilon=5e-4):
+    return DeconParams(nx, nz, psfwin, psfwin, filterwin, filterwin, snapwin, snapwin, epsilon)
+
+
+def decon_geometry(p):
+    g = DeconGeom()
+    if _decon_lib().or_decon_geometry(C.byref(p), C.byref(g)) != 0:
+        raise ValueError("invalid deconvolution windows")
+    return g
+
+
+def decon_coefr(p):
+    g = decon_geometry(p)
+    c = np.zeros((g.outwinx, g.outwinz), np.float32)
+    _decon_lib().or_decon_coefr(C.byref(p), C.byref(g), c)
+    return c
+
+
+def decon_expand(a, nxp, nzp):
+    a = np.ascontiguousarray(a, np.float32)
+    out = np.zeros((nxp, nzp), np.float32)
+    _decon_lib().or_decon_expand(a, a.shape[0], a.shape[1], out, nxp, nzp)
+    return out
+
+
+def twiddles(n):
+    c = np.zeros(n)
+    sn = np.zeros(n)
+    _decon_lib().or_twiddles(n, c, sn)
+    return c, sn
+
+
+def decon_spectrum(x, offx, offz, n):
+    """Half spectrum [n][n/2+1] of the n x n buffer holding x at (offx, offz)."""
+    x = np.ascontiguousarray(x, np.float64)
+    nh = n // 2 + 1
+    re = np.zeros((n, nh))
+    im = np.zeros((n, nh))
+    _decon_lib().or_decon_spectrum(x, x.shape[0], x.shape[1], offx, offz, n, re, im)
+    return re + 1j * im
+
+
+def decon_inverse(d, r0, nr, c0, nc):
+    """Real part of the inverse DFT of the Hermitian spectrum with half d, on a block."""
+    n = d.shape[0]
+    dr = np.ascontiguousarray(d.real)
+    di = np.ascontiguousarray(d.imag)
+    out = np.zeros((nr, nc))
+    _decon_lib().or_decon_inverse(dr, di, n, r0, nr, c0, nc, out)
+    return out
+
+
+def decon_cell(p, ix, iz):
+    g = decon_geometry(p)
+    cx, cz = C.c_int(), C.c_int()
+    _decon_lib().or_decon_cell(C.byref(p), C.byref(g), ix, iz, C.byref(cx), C.byref(cz))
+    return cx.value, cz.value
+
+
+def decon_psf(p, psf, cx, cz):
+    """(S, tden): normalised PSF half spectrum of lattice cell (cx, cz) and the Wiener denominators."""
+    g = decon_geometry(p)
+    psff = decon_expand(psf, g.nxp, g.nzp)
+    n, nh = g.fullwx, g.fullwx // 2 + 1
+    sr = np.zeros((n, nh))
+    si = np.zeros((n, nh))
+    t = np.zeros((n, nh), np.float32)
+    _decon_lib().or_decon_psf(C.byref(p), C.byref(g), psff, cx, cz, sr, si, t)
+    return sr + 1j * si, t
+
+
+def decon_window(p, image, psf, step):
+    """Deconvolved real part of window `step` at its outwin x outwin output points."""
+    g = decon_geometry(p)
+    imgf = decon_expand(image, g.nxp, g.nzp)
+    psff = decon_expand(psf, g.nxp, g.nzp)
+    out = np.zeros((g.outwinx, g.outwinz))
+    _decon_lib().or_decon_window(C.byref(p), C.byref(g), imgf, psff, int(step), out)
+    return out
+
+
+def decon(p, image, psf):
+    """(mig_dec, mig_coef), both [nx][nz] float32."""
+    image = np.ascontiguousarray(image, np.float32)
+    psf = np.ascontiguousarray(psf, np.float32)
+    assert image.shape == (p.nx, p.nz) and psf.shape == (p.nx, p.nz)
+    dec = np.zeros((p.nx, p.nz), np.float32)
+    coef = np.zeros((p.nx, p.nz), np.float32)
+    if _decon_lib().or_decon(C.byref(p), image, psf, dec, coef) != 0:
+        raise ValueError("invalid deconvolution windows")
+    return dec, coef
