@@ -6,7 +6,8 @@
  * host drivers and its device code.  Each entry point names the reference
  * function it replaces (RC = acoustic_HLVSPRTMC.cpp, RK = acoustic_HLVSPRTMLc.cu,
  * FC = forward/acoustic_hilbert_fftkz_HLForC.cpp,
- * FK = forward/acoustic_hilbert_fftkz_HLForLc.cu, H = headrtm.h).
+ * FK = forward/acoustic_hilbert_fftkz_HLForLc.cu, H = headrtm.h,
+ * DC = kkdecon_acoustic/kk_decon_interp_psf_minewindow.cpp).
  *
  * Conventions
  *  - Every call returns an int status: 0 = ok, < 0 = error (see SWD_E_*); the
@@ -202,6 +203,54 @@ int swd_born_adjoint_dev(void *plan, int nshots, const swd_shot *shots, const fl
  * stream, from a captured hipGraph when plans[0] has graphs on. */
 int swd_multi_forward(int nplans, void *const *plans, const int *nshots, const swd_shot *const *shots,
                       const float *const *wavelets, float *const *gathers);
+
+/* ---------------- wavenumber-domain PSF deconvolution ----------------
+ * The windowed Wiener deconvolution of a migrated image by the nearest PSF of
+ * a point-scatterer lattice (decon2d_mine, DC:17-785), one process.  Images
+ * are [nx][nz] float32 (z contiguous), as the raw .bin files (DC:255-261). */
+
+/* kkdecon parameter.txt values used by the deconvolution (DC:1527-1586). */
+typedef struct swd_decon_params {
+    int nx, nz;
+    int psfwinx, psfwinz;        /* PSF lattice spacing / patch (odd) */
+    int filterwinx, filterwinz;  /* image window (odd) */
+    int snapwinx, snapwinz;      /* taper length of the output window */
+    float epsilon;               /* Wiener regularisation, times max |S|^2 */
+} swd_decon_params;
+
+/* The file names and flags of the same parameter.txt (DC:1587-1626). */
+typedef struct swd_decon_files {
+    char vp[256], vp0[256], rho[256], image[256], psf[256], decon[256], inv_psf[256];
+    int shift_flag;
+    float reg_eps;
+} swd_decon_files;
+
+/* Derived sizes (DC:84-129): n = filterwin + psfwin - 1 (square windows
+ * only), the output window, the window step and counts, the padded image. */
+typedef struct swd_decon_geom {
+    int n, outwinx, outwinz, stepwinx, stepwinz, stepnumx, stepnumz, windows, nxp, nzp, cells;
+} swd_decon_geom;
+
+typedef struct swd_decon_stats {
+    double sums_ms, cells_ms, windows_ms, gather_ms, total_ms; /* event times of the last call */
+    int windows, cells, n;
+    long long ws_bytes;                                        /* device memory held by the plan */
+} swd_decon_stats;
+
+int swd_decon_read_parameters(const char *path, swd_decon_params *p, swd_decon_files *files); /* input_parameters (DC:1527-1626) */
+int swd_decon_geometry(const swd_decon_params *p, swd_decon_geom *g);  /* DC:84-129 */
+int swd_decon_taper(const swd_decon_params *p, float *coefr);         /* coefr[outwinx][outwinz] (DC:355-382) */
+/* A deconvolution plan holds the twiddle tables, taper and workspace of one
+ * parameter set on one device and stream (the FFTW plans and buffers of DC:449-493). */
+int swd_decon_plan_create(const swd_decon_params *p, int device, void **plan);
+int swd_decon_plan_destroy(void *plan);
+/* Deconvolution of image by the PSF image (DC:543-682): dec receives
+ * migration_dudecon (DC:724-731), coef the summed taper weights, migration_coef
+ * (DC:702-709; may be NULL).  Host arrays [nx][nz]. */
+int swd_decon(void *plan, const float *image, const float *psf, float *dec, float *coef);
+/* Same with device arrays on the plan's device (coef required). */
+int swd_decon_dev(void *plan, const float *image, const float *psf, float *dec, float *coef);
+int swd_decon_get_stats(void *plan, swd_decon_stats *st);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,28 @@ extern "C" int swd_read_parameters(const char *path, swd_params *o)
     return SWD_OK;
 }
 
+extern "C" int swd_decon_read_parameters(const char *path, swd_decon_params *o, swd_decon_files *f)
+{
+    if (!path || !o) return swd_fail(SWD_E_ARG, "swd_decon_read_parameters: null argument");
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return swd_fail(SWD_E_IO, "Cannot open the parameters file %s", path);
+    swd_decon_files tmp;
+    std::memset(&tmp, 0, sizeof tmp);
+    bool ok = read_pair(fp, "%d", &o->nx) && read_pair(fp, "%d", &o->nz) && read_pair(fp, "%d", &o->psfwinx) &&
+              read_pair(fp, "%d", &o->psfwinz) && read_pair(fp, "%d", &o->filterwinx) &&
+              read_pair(fp, "%d", &o->filterwinz) && read_pair(fp, "%d", &o->snapwinx) &&
+              read_pair(fp, "%d", &o->snapwinz) && read_pair(fp, "%f", &o->epsilon);
+    if (ok)   // file names and flags; a file that stops after epsilon leaves them empty
+        read_pair(fp, "%255s", tmp.vp) && read_pair(fp, "%255s", tmp.vp0) && read_pair(fp, "%255s", tmp.rho) &&
+            read_pair(fp, "%255s", tmp.image) && read_pair(fp, "%255s", tmp.psf) &&
+            read_pair(fp, "%255s", tmp.decon) && read_pair(fp, "%255s", tmp.inv_psf) &&
+            read_pair(fp, "%d", &tmp.shift_flag) && read_pair(fp, "%f", &tmp.reg_eps);
+    std::fclose(fp);
+    if (!ok) return swd_fail(SWD_E_IO, "parameter file %s is short or malformed", path);
+    if (f) *f = tmp;
+    return SWD_OK;
+}
+
 extern "C" int swd_cal_xishu(int Lx, float *rx)
 {
     if (Lx < 1 || Lx > SWD_MAX_LC || !rx) return swd_fail(SWD_E_ARG, "swd_cal_xishu: Lc must be 1..%d", SWD_MAX_LC);

@@ -81,7 +81,9 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_groups", "swd_plan_set_graphs",
            "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor",
            "swd_rtm", "swd_rtm_dev", "swd_born", "swd_born_adjoint", "swd_born_dev",
-           "swd_born_adjoint_dev", "swd_multi_forward")
+           "swd_born_adjoint_dev", "swd_multi_forward", "swd_decon_read_parameters", "swd_decon_geometry",
+           "swd_decon_taper", "swd_decon_plan_create", "swd_decon_plan_destroy", "swd_decon", "swd_decon_dev",
+           "swd_decon_get_stats")
 
 
 def load():
@@ -393,3 +395,100 @@ def multi_forward(plans, shotsets, wavelets):
     arr_g = (C.c_void_p * n)(*[o.ctypes.data for o in outs])
     check(L.swd_multi_forward(n, arr_p, arr_n, arr_s, arr_w, arr_g))
     return outs
+
+
+# ---------------------------------------------------------------------------
+# Wavenumber-domain PSF deconvolution (kkdecon_acoustic/kk_decon_interp_psf_minewindow.cpp)
+
+class DeconParams(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nx", "nz", "psfwinx", "psfwinz", "filterwinx", "filterwinz",
+                                       "snapwinx", "snapwinz")] + [("epsilon", C.c_float)]
+
+
+class DeconFiles(C.Structure):
+    _fields_ = [(n, C.c_char * 256) for n in ("vp", "vp0", "rho", "image", "psf", "decon", "inv_psf")] + \
+        [("shift_flag", C.c_int), ("reg_eps", C.c_float)]
+
+
+class DeconGeom(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("n", "outwinx", "outwinz", "stepwinx", "stepwinz", "stepnumx", "stepnumz",
+                                       "windows", "nxp", "nzp", "cells")]
+
+
+class DeconStats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("sums_ms", "cells_ms", "windows_ms", "gather_ms", "total_ms")] + \
+        [(n, C.c_int) for n in ("windows", "cells", "n")] + [("ws_bytes", C.c_longlong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def decon_params(nx, nz, psfwin=31, filterwin=71, snapwin=11, epsilon=5e-4):
+    """Parameters of the reference's kkdecon parameter.txt (square windows by default)."""
+    return DeconParams(int(nx), int(nz), psfwin, psfwin, filterwin, filterwin, snapwin, snapwin, float(epsilon))
+
+
+def decon_read_parameters(path):
+    """(DeconParams, dict of file names and flags) from a kkdecon parameter.txt."""
+    p, f = DeconParams(), DeconFiles()
+    check(load().swd_decon_read_parameters(path.encode(), C.byref(p), C.byref(f)))
+    files = {k: getattr(f, k).decode() for k in ("vp", "vp0", "rho", "image", "psf", "decon", "inv_psf")}
+    files.update(shift_flag=f.shift_flag, reg_eps=f.reg_eps)
+    return p, files
+
+
+def decon_geometry(p):
+    g = DeconGeom()
+    check(load().swd_decon_geometry(C.byref(p), C.byref(g)))
+    return {k: getattr(g, k) for k, _ in g._fields_}
+
+
+def decon_taper(p):
+    g = decon_geometry(p)
+    c = np.zeros((g["outwinx"], g["outwinz"]), np.float32)
+    check(load().swd_decon_taper(C.byref(p), _p(c)))
+    return c
+
+
+class Decon:
+    """A deconvolution plan: twiddle tables, taper and workspace of one
+    parameter set on one GPU."""
+
+    def __init__(self, params, device=0):
+        self.L = load()
+        self.params = params
+        self.device = int(device)
+        h = C.c_void_p()
+        check(self.L.swd_decon_plan_create(C.byref(params), self.device, C.byref(h)))
+        self.h = h
+
+    def run(self, image, psf):
+        """(dec, coef) [nx][nz] float32 of host arrays image, psf [nx][nz]."""
+        p = self.params
+        image, psf = f32(image), f32(psf)
+        assert image.shape == (p.nx, p.nz) and psf.shape == (p.nx, p.nz)
+        dec = np.zeros((p.nx, p.nz), np.float32)
+        coef = np.zeros((p.nx, p.nz), np.float32)
+        check(self.L.swd_decon(self.h, _p(image), _p(psf), _p(dec), _p(coef)))
+        return dec, coef
+
+    def run_dev(self, image_ptr, psf_ptr, dec_ptr, coef_ptr):
+        """Same on device arrays of the plan's GPU (raw pointers)."""
+        check(self.L.swd_decon_dev(self.h, C.c_void_p(image_ptr), C.c_void_p(psf_ptr), C.c_void_p(dec_ptr),
+                                   C.c_void_p(coef_ptr)))
+
+    def stats(self):
+        s = DeconStats()
+        check(self.L.swd_decon_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if self.h:
+            self.L.swd_decon_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
