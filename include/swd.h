@@ -232,7 +232,9 @@ typedef struct swd_decon_geom {
 } swd_decon_geom;
 
 typedef struct swd_decon_stats {
-    double sums_ms, cells_ms, windows_ms, gather_ms, total_ms; /* event times of the last call */
+    double sums_ms;                                  /* 0: the window sums run inside the window kernel */
+    double cells_ms, windows_ms, gather_ms;          /* event times of the last call's kernels */
+    double total_ms;
     int windows, cells, n;
     long long ws_bytes;                                        /* device memory held by the plan */
 } swd_decon_stats;

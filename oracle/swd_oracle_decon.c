@@ -18,9 +18,11 @@
  *  - twiddles tw[t] = (cos(2 pi t / n), sin(2 pi t / n)) from libm;
  *  - forward, real input x[r][c] supported on rows offx.. and columns offz..:
  *    along z first, half spectrum k1 = 0..n/2 (y = sum_c x e^{-i..}, fma
- *    chains in c order), then along x (m = sum_r e^{-i..} y, fma chains in r
- *    order); the k1 > n/2 half is the conjugate of the computed half, so the
- *    fft_max maximum runs over the computed half;
+ *    chains in c order), then along x over pairs of wavenumbers (p, n - p),
+ *    whose twiddles are conjugate: P = sum_r cos(.) y, Q = sum_r sin(.) y
+ *    (fma chains in r order), M[p] = P - iQ, M[n-p] = P + iQ; the k1 > n/2
+ *    half is the conjugate of the computed half, so the fft_max maximum runs
+ *    over the computed half;
  *  - the image normalisation by pp_max is folded into the Wiener divisor:
  *    d = (M S*) / (pp_max * tmpfloat), pp_max * tmpfloat being exact in double
  *    (a product of two floats), instead of dividing the window before the
@@ -130,19 +132,24 @@ void or_decon_spectrum(const double *x, int mx, int mz, int offx, int offz, int 
             yr[r * nh + k1] = ar;
             yi[r * nh + k1] = ai;
         }
-    for (int k0 = 0; k0 < n; k0++)
+    /* pairs (p, n - p): P = sum c y, Q = sum s y, M[p] = P - iQ, M[n-p] = P + iQ */
+    for (int p = 0; p < nh; p++)
         for (int k1 = 0; k1 < nh; k1++) {
-            double ar = 0.0, ai = 0.0;
+            double pr = 0.0, pi = 0.0, qr = 0.0, qi = 0.0;
             for (int r = 0; r < mx; r++) {
-                const int t = (int)(((long)k0 * (r + offx)) % n);
+                const int t = (int)(((long)p * (r + offx)) % n);
                 const double a = yr[r * nh + k1], b = yi[r * nh + k1];
-                ar = fma(a, c[t], ar);
-                ar = fma(b, s[t], ar);
-                ai = fma(b, c[t], ai);
-                ai = fma(-a, s[t], ai);
+                pr = fma(a, c[t], pr);
+                pi = fma(b, c[t], pi);
+                qr = fma(a, s[t], qr);
+                qi = fma(b, s[t], qi);
             }
-            re[k0 * nh + k1] = ar;
-            im[k0 * nh + k1] = ai;
+            re[p * nh + k1] = pr + qi;
+            im[p * nh + k1] = pi - qr;
+            if (p > 0 && 2 * p != n) {
+                re[(n - p) * nh + k1] = pr - qi;
+                im[(n - p) * nh + k1] = pi + qr;
+            }
         }
     free(c);
     free(s);

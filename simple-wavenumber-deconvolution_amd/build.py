@@ -1,9 +1,9 @@
-"""Builds libswd.so (HIP kernels for gfx950 + C-ABI) and the AFor / AVSPRTM drivers in-tree.
+"""Builds libswd.so (HIP kernels for gfx950 + C-ABI) and the AFor / AVSPRTM / Decon drivers in-tree.
 
     python simple-wavenumber-deconvolution_amd/build.py [--jobs N]
 
 Outputs: simple-wavenumber-deconvolution_amd/lib/libswd.so,
-         simple-wavenumber-deconvolution_amd/bin/AFor, .../bin/AVSPRTM
+         simple-wavenumber-deconvolution_amd/bin/AFor, .../bin/AVSPRTM, .../bin/Decon
 """
 import argparse
 import os
@@ -33,7 +33,8 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-ffp-contract=of
 
 HIP_SOURCES = ["swd_plan.hip", "swd_kernels.hip", "swd_loops.hip", "swd_fstep.hip", "swd_decon.hip"]
 CXX_SOURCES = ["swd_setup.cpp"]
-DRIVERS = {"AFor": ["afor.cpp", "driver_common.cpp"], "AVSPRTM": ["avsprtm.cpp", "driver_common.cpp"]}
+DRIVERS = {"AFor": ["afor.cpp", "driver_common.cpp"], "AVSPRTM": ["avsprtm.cpp", "driver_common.cpp"],
+           "Decon": ["decon.cpp", "driver_common.cpp"]}
 
 
 def run(cmd):

@@ -6,16 +6,15 @@
 // Per window of side n (fullw = filterwin + psfwin - 1) the reference runs two
 // forward 2D FFTs, a Wiener division and one inverse 2D FFT in double, then
 // overlap-adds a tapered outwin x outwin block in window order.  Here:
-//   k_decon_sums     one thread per window: the float sum of magnitudes of the
-//                    image window (pp_max, DC:555-574), a serial chain kept in
-//                    the reference's order;
 //   k_decon_cells    one workgroup per PSF lattice cell (windows share cells,
 //                    2484 windows use 374 cells at the reference size): the
 //                    normalised PSF's half spectrum and the Wiener denominators
 //                    tmpfloat = float(|S|^2 + eps * max|S|^2) (DC:583-648);
-//   k_decon_windows  one workgroup per window: half spectrum of the image
-//                    window, Wiener division, inverse only at the output block
-//                    (DC:653-673);
+//   k_decon_windows  one workgroup per window: the float sum of magnitudes of
+//                    the window (pp_max, DC:555-574, a serial chain in the
+//                    reference's order on one wavefront), half spectrum of the
+//                    image window on the others, Wiener division, inverse only
+//                    at the output block (DC:653-673);
 //   k_decon_gather   one thread per output cell: the tapered overlap-add of
 //                    the windows covering it, in window order (DC:653-673).
 // The transforms are direct DFTs as small complex matrix products: n = 101 is
@@ -118,56 +117,74 @@ __device__ __forceinline__ void rows_pass(const XT *X, int mx, int mz, int offz,
     }
 }
 
-// Columns pass: M[k0][k1] = sum_r e^{-2 pi i k0 (r + offx) / n} Y[r][k1] for
-// the P wavenumbers k0 = w P + kk of wave w.  A[r][k0] (rows of NW P entries,
-// zero-padded) is the launch-constant twiddle table, read wave-uniformly
-// through the scalar cache; the four waves a SIMD holds cover each other's
-// scalar-load latency.
-template <int NW, int P>
+// Columns pass over pairs of x wavenumbers (p, n - p), whose twiddles are
+// conjugate: with (c, s) = (cos, sin)(2 pi p (r + offx) / n),
+//   P = sum_r c Y[r][k1],  Q = sum_r s Y[r][k1],
+//   M[p] = P - i Q,  M[n - p] = P + i Q,
+// half the FMAs of the two transforms.  Wave w owns the PP pairs p = w PP + j;
+// slot 2j holds k0 = p, slot 2j + 1 holds k0 = n - p (see slot_k0).
+// A[r][p] (rows of NW PP entries, zero-padded) is the launch-constant twiddle
+// table, read wave-uniformly through the scalar cache; the waves a SIMD holds
+// cover each other's scalar-load latency.
+__device__ __forceinline__ int slot_k0(int n, int nh, int p, int b)
+{
+    if (p >= nh) return -1;
+    if (b == 0) return p;
+    return (p == 0 || 2 * p == n) ? -1 : n - p;
+}
+
+template <int NW, int PP>
 __device__ __forceinline__ void cols_pass(const double2 *Y, int mx, int nh, const double2 *__restrict__ A,
-                                          double (&mr)[P], double (&mi)[P])
+                                          double (&mr)[2 * PP], double (&mi)[2 * PP])
 {
     const int lane = threadIdx.x & 63, w = uniform(threadIdx.x >> 6);
     const int k1 = min(lane, nh - 1);
-    constexpr int LD = NW * P;
-    const double2 *Aw = A + w * P;
+    constexpr int LD = NW * PP;
+    const double2 *Aw = A + w * PP;
+    double pr[PP], pi[PP], qr[PP], qi[PP];
 #pragma unroll
-    for (int kk = 0; kk < P; kk++) mr[kk] = mi[kk] = 0.0;
+    for (int j = 0; j < PP; j++) pr[j] = pi[j] = qr[j] = qi[j] = 0.0;
     for (int r = 0; r < mx; r++) {
         const double2 y = Y[r * nh + k1];
 #pragma unroll
-        for (int kk = 0; kk < P; kk++) {
-            const double2 a = Aw[r * LD + kk];
-            mr[kk] = fma(y.x, a.x, mr[kk]);
-            mr[kk] = fma(y.y, a.y, mr[kk]);
-            mi[kk] = fma(y.y, a.x, mi[kk]);
-            mi[kk] = fma(-y.x, a.y, mi[kk]);
+        for (int j = 0; j < PP; j++) {
+            const double2 a = Aw[r * LD + j];
+            pr[j] = fma(y.x, a.x, pr[j]);
+            pi[j] = fma(y.y, a.x, pi[j]);
+            qr[j] = fma(y.x, a.y, qr[j]);
+            qi[j] = fma(y.y, a.y, qi[j]);
         }
+    }
+#pragma unroll
+    for (int j = 0; j < PP; j++) {
+        mr[2 * j] = pr[j] + qi[j];
+        mi[2 * j] = pi[j] - qr[j];
+        mr[2 * j + 1] = pr[j] - qi[j];
+        mi[2 * j + 1] = pi[j] + qr[j];
     }
 }
 
-// Inverse along x at the output rows: Q[i][k1] = sum_k0 e^{+2 pi i k0 (r0x + i) / n} D[k0][k1]
-// for the PQ rows i = w PQ + ii of wave w, U wavenumbers k0 a step; Tq[k0][i]
-// (rows of NW PQ entries) is read through the scalar cache.  D and Tq are
-// zero past row n, up to a whole number of steps.
+// Inverse along x at the output rows: Q[i][k1] += sum_k0 e^{+2 pi i k0 (r0x + i) / n} D[k0][k1]
+// over the rows k0 = k0base + kl (kl < rows) of one half of D, for the PQ
+// output rows i = w PQ + ii of wave w, U wavenumbers a step; Tq[k0][i] (rows
+// of NW PQ entries) is read through the scalar cache.  rows is a multiple of
+// U; D and Tq are zero past row n.
 template <int NW, int PQ>
-__device__ __forceinline__ void inv_rows_pass(const double2 *D, int n, int nh, const double2 *__restrict__ Tq,
-                                              double (&qr)[PQ], double (&qi)[PQ])
+__device__ __forceinline__ void inv_rows_pass(const double2 *D, int rows, int k0base, int nh,
+                                              const double2 *__restrict__ Tq, double (&qr)[PQ], double (&qi)[PQ])
 {
     constexpr int U = PQ >= 8 ? 1 : 8 / PQ;
     constexpr int LD = NW * PQ;
     const int lane = threadIdx.x & 63, w = uniform(threadIdx.x >> 6);
     const int k1 = min(lane, nh - 1);
-    const double2 *Tw = Tq + w * PQ;
+    const double2 *Tw = Tq + (size_t)k0base * LD + w * PQ;
+    for (int kl = 0; kl < rows; kl += U) {
 #pragma unroll
-    for (int ii = 0; ii < PQ; ii++) qr[ii] = qi[ii] = 0.0;
-    for (int k0 = 0; k0 < n; k0 += U) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {   // rows k0 >= n of D and Tq are zero: they add nothing
-            const double2 d = D[(k0 + u) * nh + k1];
+        for (int u = 0; u < U; u++) {   // rows past n of D and Tq are zero: they add nothing
+            const double2 d = D[(kl + u) * nh + k1];
 #pragma unroll
             for (int ii = 0; ii < PQ; ii++) {
-                const double2 e = Tw[(k0 + u) * LD + ii];
+                const double2 e = Tw[(kl + u) * LD + ii];
                 qr[ii] = fma(d.x, e.x, qr[ii]);
                 qr[ii] = fma(-d.y, e.y, qr[ii]);
                 qi[ii] = fma(d.y, e.x, qi[ii]);
@@ -177,36 +194,50 @@ __device__ __forceinline__ void inv_rows_pass(const double2 *D, int n, int nh, c
     }
 }
 
-// Rows of D and of the Tq table: n rounded up to whole steps of inv_rows_pass.
-__host__ __device__ constexpr int inv_rows_pad(int n, int PQ)
+// D is held in LDS in two halves of k0: the first ceil(n/2) wavenumbers, then
+// the rest, each padded with zero rows to a whole number of inv_rows_pass steps.
+__host__ __device__ constexpr int inv_step(int PQ) { return PQ >= 8 ? 1 : 8 / PQ; }
+__host__ __device__ constexpr int half_rows(int n) { return (n + 1) / 2; }
+__host__ __device__ constexpr int half_rows_pad(int n, int PQ)
 {
-    return (n + (PQ >= 8 ? 1 : 8 / PQ) - 1) / (PQ >= 8 ? 1 : 8 / PQ) * (PQ >= 8 ? 1 : 8 / PQ);
+    return (half_rows(n) + inv_step(PQ) - 1) / inv_step(PQ) * inv_step(PQ);
+}
+// Rows of the Tq table: the second half's steps start at k0 = half_rows(n).
+__host__ __device__ constexpr int tq_rows(int n, int PQ) { return half_rows(n) + half_rows_pad(n, PQ); }
+
+// One lane sums |a[0..count)| (float, in index order) from LDS; a is padded
+// with zeros to a multiple of 32 entries (adding +0 changes no sum).  The
+// reads of the next 32 entries are issued before the adds of the current ones.
+__host__ __device__ constexpr int sum_pad(int count) { return (count + 31) / 32 * 32; }
+
+__device__ __forceinline__ float serial_abs_sum(const float *a, int count)
+{
+    const float4 *a4 = reinterpret_cast<const float4 *>(a);
+    const int nb = sum_pad(count) / 32;
+    float acc = 0.0f;
+    float4 cur[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) cur[j] = a4[j];
+    for (int b = 0; b < nb; b++) {
+        const int bn = min(b + 1, nb - 1);
+        float4 nxt[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) nxt[j] = a4[bn * 8 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            acc = acc + fabsf(cur[j].x);
+            acc = acc + fabsf(cur[j].y);
+            acc = acc + fabsf(cur[j].z);
+            acc = acc + fabsf(cur[j].w);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) cur[j] = nxt[j];
+    }
+    return acc;
 }
 
 // ---------------------------------------------------------------- kernels
-__global__ __launch_bounds__(256) void k_decon_sums(DGeo G, const float *__restrict__ img, float *__restrict__ pp)
-{
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= G.totalnum) return;
-    const int x0 = G.stepwinx * (s / G.stepnumz) - G.halfltwx, z0 = G.stepwinz * (s % G.stepnumz) - G.halfltwz;
-    // float(double(acc) + |v|) of DC:568 is the float sum acc + |v| (binary64
-    // holds more than twice binary32's precision, so the double rounding is exact)
-    float acc = 0.0f;
-    for (int i = 0; i < G.mxi; i++) {
-        const int x = x0 + i;
-        if (x < 0 || x >= G.nxp) continue;
-        const float *row = img + (size_t)min(x, G.nx - 1) * G.nz;
-        for (int j = 0; j < G.mzi; j++) {
-            const int z = z0 + j;
-            if (z < 0 || z >= G.nzp) continue;
-            acc = acc + fabsf(row[min(z, G.nz - 1)]);
-        }
-    }
-    if (static_cast<double>(acc) <= 1.0e-15) acc = 1.0e-10f;
-    pp[s] = acc;
-}
-
-template <int NW, int R, int P>
+template <int NW, int R, int PP>
 __global__ __launch_bounds__(NW * 64) void k_decon_cells(DGeo G, const float *__restrict__ psf,
                                                          const double2 *__restrict__ tw_g,
                                                          const double2 *__restrict__ Apsf, double2 *__restrict__ S,
@@ -216,39 +247,38 @@ __global__ __launch_bounds__(NW * 64) void k_decon_cells(DGeo G, const float *__
     extern __shared__ __align__(16) unsigned char lds[];
     const int n = G.n, nh = G.nh, mx = G.mxp, mz = G.mzp;
     double2 *tw = reinterpret_cast<double2 *>(lds);
-    double2 *Y = tw + n;                                                   // mx (+1) rows
-    double *X = reinterpret_cast<double *>(Y + (mx + 1) * nh);             // NW R rows
-    float *scal = reinterpret_cast<float *>(X + NW * R * mz);              // [0] psf_max, [1] max |S|^2 (bits)
+    double2 *Y = tw + n;                                                   // mx rows
+    double *X = reinterpret_cast<double *>(Y + mx * nh);                   // NW R rows
+    float *raw = reinterpret_cast<float *>(X + NW * R * mz);               // the patch, padded to 32
+    float *scal = raw + sum_pad(mx * mz);                                  // [0] psf_max, [1] max |S|^2 (bits)
     const int cell = blockIdx.x;
     const int cx = 3 + cell / G.ncz, cz = 3 + cell % G.ncz;
     const int x0 = cx * G.psfwinx, z0 = cz * G.psfwinz;   // ixp - halfpsfwx, izp - halfpsfwz (DC:597-612)
     for (int t = threadIdx.x; t < n; t += NT) tw[t] = tw_g[t];
-    for (int e = threadIdx.x; e < (mx + 1) * nh; e += NT) Y[e] = make_double2(0.0, 0.0);
+    for (int e = threadIdx.x; e < sum_pad(mx * mz); e += NT)
+        raw[e] = e < mx * mz ? expanded(psf, G, x0 + e / mz, z0 + e % mz) : 0.0f;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        float acc = 0.0f;   // psf_max (DC:617-619), float sum in the reference's order
-        for (int i = 0; i < mx; i++)
-            for (int j = 0; j < mz; j++) acc = acc + fabsf(expanded(psf, G, x0 + i, z0 + j));
-        scal[0] = acc;
+        scal[0] = serial_abs_sum(raw, mx * mz);   // psf_max (DC:617-619), in the reference's order
         reinterpret_cast<unsigned *>(scal)[1] = 0u;
     }
     __syncthreads();
     const double pm = static_cast<double>(scal[0]);
-    for (int e = threadIdx.x; e < NW * R * mz; e += NT) {
-        const int i = e / mz;
-        X[e] = i < mx ? static_cast<double>(expanded(psf, G, x0 + i, z0 + e % mz)) / pm : 0.0;
-    }
+    for (int e = threadIdx.x; e < NW * R * mz; e += NT)
+        X[e] = e < mx * mz ? static_cast<double>(raw[e]) / pm : 0.0;
     __syncthreads();
     rows_pass<NW, R>(X, mx, mz, 0, n, nh, tw, Y);
     __syncthreads();
-    double mr[P], mi[P];
-    cols_pass<NW, P>(Y, mx, nh, Apsf, mr, mi);
+    double mr[2 * PP], mi[2 * PP];
+    cols_pass<NW, PP>(Y, mx, nh, Apsf, mr, mi);
     const int lane = threadIdx.x & 63, w = uniform(threadIdx.x >> 6);
     // fft_max: max over the bins of float(|S|^2) (DC:625-633); non-negative
     // floats order as their bit patterns
     float qmax = 0.0f;
 #pragma unroll
-    for (int kk = 0; kk < P; kk++)
-        if (w * P + kk < n && lane < nh) qmax = fmaxf(qmax, static_cast<float>(mr[kk] * mr[kk] + mi[kk] * mi[kk]));
+    for (int q = 0; q < 2 * PP; q++)
+        if (slot_k0(n, nh, w * PP + q / 2, q & 1) >= 0 && lane < nh)
+            qmax = fmaxf(qmax, static_cast<float>(mr[q] * mr[q] + mi[q] * mi[q]));
     atomicMax(reinterpret_cast<unsigned *>(scal) + 1, __float_as_uint(qmax));
     __syncthreads();
     float fft_max = __uint_as_float(reinterpret_cast<unsigned *>(scal)[1]) * G.epsilon;
@@ -257,78 +287,131 @@ __global__ __launch_bounds__(NW * 64) void k_decon_cells(DGeo G, const float *__
         double2 *Sc = S + (size_t)cell * n * nh;
         float *Tc = T + (size_t)cell * n * nh;
 #pragma unroll
-        for (int kk = 0; kk < P; kk++) {
-            const int k0 = w * P + kk;
-            if (k0 < n) {
-                Sc[k0 * nh + lane] = make_double2(mr[kk], mi[kk]);
-                Tc[k0 * nh + lane] = static_cast<float>((mr[kk] * mr[kk] + mi[kk] * mi[kk]) + static_cast<double>(fft_max));
+        for (int q = 0; q < 2 * PP; q++) {
+            const int k0 = slot_k0(n, nh, w * PP + q / 2, q & 1);
+            if (k0 >= 0) {
+                Sc[k0 * nh + lane] = make_double2(mr[q], mi[q]);
+                Tc[k0 * nh + lane] = static_cast<float>((mr[q] * mr[q] + mi[q] * mi[q]) + static_cast<double>(fft_max));
             }
         }
     }
 }
 
-template <int NW, int R, int P, int PQ>
-__global__ __launch_bounds__(NW * 64) void k_decon_windows(DGeo G, const float *__restrict__ img,
-                                                           const float *__restrict__ pp,
-                                                           const double2 *__restrict__ tw_g,
-                                                           const double2 *__restrict__ Aimg,
-                                                           const double2 *__restrict__ Tinv,
-                                                           const double2 *__restrict__ S,
-                                                           const float *__restrict__ T, double *__restrict__ wout)
+// LDS of a window workgroup: the twiddle table and pp_max, then one region
+// that holds the image window X (floats) followed by Y, later one half of D,
+// later Q.  Rows of X past mx are read (into Y's space) but their results are
+// dropped.
+__host__ __device__ constexpr size_t win_x_bytes(int mx, int mz) { return ((size_t)mx * mz * 4 + 15) / 16 * 16; }
+__host__ __device__ constexpr size_t win_head_bytes(int n) { return sizeof(double2) * n + 16; }
+
+// Adds |X[i][j]| for rows i0 <= i < i1, in row-major order, to acc (one lane).
+__device__ __forceinline__ float window_abs_sum(const float *X, int i0, int i1, int mz, float acc)
 {
-    constexpr int NT = NW * 64;
+    for (int i = i0; i < i1; i++) {
+        const float *r = X + i * mz;
+        int j = 0;
+        for (; j + 16 <= mz; j += 16) {
+            float v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) v[u] = r[j + u];
+#pragma unroll
+            for (int u = 0; u < 16; u++) acc = acc + fabsf(v[u]);
+        }
+        for (; j < mz; j++) acc = acc + fabsf(r[j]);
+    }
+    return acc;
+}
+
+// NW wavefronts: NW - 1 run the transforms; the last one forms pp_max, the
+// float sum of |window| (DC:555-574) as a serial chain in the reference's
+// order (rows outer, columns inner), in two halves that run beside the rows
+// and the columns passes.  float(double(acc) + |v|) of DC:568 is the float sum
+// acc + |v| (binary64 holds more than twice binary32's precision, so the
+// double rounding is exact).
+template <int NW, int R, int PP, int PQ, int WPE>
+__global__ __launch_bounds__(NW * 64, WPE) void k_decon_windows(DGeo G, const float *__restrict__ img,
+                                                              const double2 *__restrict__ tw_g,
+                                                              const double2 *__restrict__ Aimg,
+                                                              const double2 *__restrict__ Tinv,
+                                                              const double2 *__restrict__ S,
+                                                              const float *__restrict__ T, double *__restrict__ wout)
+{
+    constexpr int NT = NW * 64, NWC = NW - 1;
     extern __shared__ __align__(16) unsigned char lds[];
     const int n = G.n, nh = G.nh, mx = G.mxi, mz = G.mzi;
     double2 *tw = reinterpret_cast<double2 *>(lds);
-    double2 *Rg = tw + n;                                            // Y, then D, then Q
-    float *X = reinterpret_cast<float *>(Rg + (mx + 1) * nh);        // image window beside Y, NW R rows
+    float *ppl = reinterpret_cast<float *>(tw + n);
+    unsigned char *reg = lds + win_head_bytes(n);
+    float *X = reinterpret_cast<float *>(reg);
+    double2 *Y = reinterpret_cast<double2 *>(reg + win_x_bytes(mx, mz));
+    double2 *Rg = reinterpret_cast<double2 *>(reg);                  // D halves, then Q
     const int s = blockIdx.x;
+    const int lane = threadIdx.x & 63, w = uniform(threadIdx.x >> 6);
     const int x0 = G.stepwinx * (s / G.stepnumz) - G.halfltwx, z0 = G.stepwinz * (s % G.stepnumz) - G.halfltwz;
     for (int t = threadIdx.x; t < n; t += NT) tw[t] = tw_g[t];
-    for (int e = threadIdx.x; e < NW * R * mz; e += NT) {
-        const int i = e / mz;
-        X[e] = i < mx ? expanded(img, G, x0 + i, z0 + e % mz) : 0.0f;
+    for (int e = threadIdx.x; e < mx * mz; e += NT) X[e] = expanded(img, G, x0 + e / mz, z0 + e % mz);
+    __syncthreads();
+    // The chain issues one add at a time: at the highest wave priority its
+    // instructions go first on its SIMD instead of waiting their turn behind
+    // the transform waves.
+    float acc = 0.0f;   // pp_max, on lane 0 of wave NWC
+    if (w < NWC) rows_pass<NWC, R>(X, mx, mz, G.offz, n, nh, tw, Y);
+    else if (lane == 0) {
+        __builtin_amdgcn_s_setprio(3);
+        acc = window_abs_sum(X, 0, mx / 2, mz, acc);
     }
-    for (int e = mx * nh + threadIdx.x; e < (mx + 1) * nh; e += NT) Rg[e] = make_double2(0.0, 0.0);
     __syncthreads();
-    rows_pass<NW, R>(X, mx, mz, G.offz, n, nh, tw, Rg);
+    double mr[2 * PP], mi[2 * PP];
+    if (w < NWC) cols_pass<NWC, PP>(Y, mx, nh, Aimg, mr, mi);
+    else if (lane == 0) {
+        acc = window_abs_sum(X, mx / 2, mx, mz, acc);
+        if (static_cast<double>(acc) <= 1.0e-15) acc = 1.0e-10f;
+        *ppl = acc;
+        __builtin_amdgcn_s_setprio(0);
+    }
     __syncthreads();
-    double mr[P], mi[P];
-    cols_pass<NW, P>(Rg, mx, nh, Aimg, mr, mi);
     // Wiener division (DC:642-651) with the image normalisation in the divisor
-    const int lane = threadIdx.x & 63, w = uniform(threadIdx.x >> 6);
     const int k1 = min(lane, nh - 1);
-    const int cell = uniform(cell_of(G, s));
-    const double ppd = static_cast<double>(pp[s]);
-    const double2 *Sc = S + (size_t)cell * n * nh;
-    const float *Tc = T + (size_t)cell * n * nh;
+    if (w < NWC) {
+        const int cell = uniform(cell_of(G, s));
+        const double ppd = static_cast<double>(*ppl);
+        const double2 *Sc = S + (size_t)cell * n * nh;
+        const float *Tc = T + (size_t)cell * n * nh;
 #pragma unroll
-    for (int kk = 0; kk < P; kk++) {
-        const int k0 = w * P + kk;
-        if (k0 < n) {
-            const double2 sv = Sc[k0 * nh + k1];
-            const double den = ppd * static_cast<double>(Tc[k0 * nh + k1]);
-            const double dr = (mr[kk] * sv.x + mi[kk] * sv.y) / den;
-            const double di = (-mr[kk] * sv.y + mi[kk] * sv.x) / den;
-            mr[kk] = dr;
-            mi[kk] = di;
+        for (int q = 0; q < 2 * PP; q++) {
+            const int k0 = slot_k0(n, nh, w * PP + q / 2, q & 1);
+            if (k0 >= 0) {
+                const double2 sv = Sc[k0 * nh + k1];
+                const double den = ppd * static_cast<double>(Tc[k0 * nh + k1]);
+                const double dr = (mr[q] * sv.x + mi[q] * sv.y) / den;
+                const double di = (-mr[q] * sv.y + mi[q] * sv.x) / den;
+                mr[q] = dr;
+                mi[q] = di;
+            }
         }
     }
-    __syncthreads();   // every wave is past its reads of Y
-    if (lane < nh) {
-#pragma unroll
-        for (int kk = 0; kk < P; kk++) {
-            const int k0 = w * P + kk;
-            if (k0 < n) Rg[k0 * nh + lane] = make_double2(mr[kk], mi[kk]);
-        }
-    }
-    for (int e = n * nh + threadIdx.x; e < inv_rows_pad(n, PQ) * nh; e += NT) Rg[e] = make_double2(0.0, 0.0);
-    __syncthreads();
+    // inverse along x at the output rows, over the two halves of D in k0 order
     const int outx = G.outx, outz = G.outz;
+    const int H = half_rows(n), HP = half_rows_pad(n, PQ);
     double qr[PQ], qi[PQ];
-    inv_rows_pass<NW, PQ>(Rg, n, nh, Tinv, qr, qi);
+#pragma unroll
+    for (int ii = 0; ii < PQ; ii++) qr[ii] = qi[ii] = 0.0;
+    for (int h = 0; h < 2; h++) {
+        const int lo = h * H, cnt = h ? n - H : H;
+        __syncthreads();   // every wave is past its reads of Y / of the previous half
+        if (w < NWC && lane < nh) {
+#pragma unroll
+            for (int q = 0; q < 2 * PP; q++) {
+                const int k0 = slot_k0(n, nh, w * PP + q / 2, q & 1);
+                if (k0 >= lo && k0 < lo + cnt) Rg[(k0 - lo) * nh + lane] = make_double2(mr[q], mi[q]);
+            }
+        }
+        for (int e = cnt * nh + threadIdx.x; e < HP * nh; e += NT) Rg[e] = make_double2(0.0, 0.0);
+        __syncthreads();
+        if (w < NWC) inv_rows_pass<NWC, PQ>(Rg, HP, lo, nh, Tinv, qr, qi);
+    }
     __syncthreads();   // every wave is past its reads of D
-    if (lane < nh) {
+    if (w < NWC && lane < nh) {
 #pragma unroll
         for (int ii = 0; ii < PQ; ii++) {
             const int i = w * PQ + ii;
@@ -342,17 +425,17 @@ __global__ __launch_bounds__(NW * 64) void k_decon_windows(DGeo G, const float *
         const int i = o / outz, j = o % outz;
         const int step = (G.c0z + j) % n;
         int t = 0;
-        double acc = 0.0;
+        double acc2 = 0.0;
         for (int kk = 0; kk < nh; kk++) {
             const double wgt = (kk == 0 || 2 * kk == n) ? 1.0 : 2.0;
             const double2 q = Rg[i * nh + kk];
             const double2 e = tw[t];
-            acc = fma(wgt * q.x, e.x, acc);
-            acc = fma(-(wgt * q.y), e.y, acc);
+            acc2 = fma(wgt * q.x, e.x, acc2);
+            acc2 = fma(-(wgt * q.y), e.y, acc2);
             t += step;
             if (t >= n) t -= n;
         }
-        ws[o] = acc;
+        ws[o] = acc2;
     }
 }
 
@@ -387,11 +470,17 @@ __global__ __launch_bounds__(256) void k_decon_gather(DGeo G, const double *__re
 // Launch configurations: rows / wavenumbers per wavefront of each stage, with
 // kNW wavefronts per workgroup.  The first fits the reference's windows
 // (filterwin 71, psfwin 31, n 101, outwin 29), the second any window up to kMaxN.
+// The window kernel's transforms run on kNW - 1 wavefronts (the last forms
+// pp_max); the cell kernel's on all kNW.
 struct KCfg {
-    int R, P, PQ, Rc, Pc;
+    int R, PP, PQ;   // window kernel: image rows, wavenumber pairs, output rows per wavefront
+    int Rc;          // cell kernel: PSF rows per wavefront
+    int wpe;         // wavefronts per SIMD the window kernel is compiled for (2 workgroups a CU when 8)
 };
 constexpr int kNW = 16;
-constexpr KCfg kCfgs[2] = {{5, 7, 2, 2, 7}, {8, 8, 8, 8, 8}};
+constexpr int kNWC = kNW - 1;
+constexpr int kPPc = 4;   // cell kernel: wavenumber pairs per wavefront, kNW kPPc >= n/2 + 1 for n <= kMaxN
+constexpr KCfg kCfgs[2] = {{5, 4, 2, 2, 8}, {9, 5, 9, 8, 4}};
 
 // ---------------------------------------------------------------- host
 int decon_geometry(const swd_decon_params &p, DGeo &G)
@@ -455,12 +544,12 @@ struct DeconPlan {
     int device = 0;
     int cfg = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[5] = {};
+    hipEvent_t ev[4] = {};
     // launch-constant tables
     double2 *tw = nullptr, *Aimg = nullptr, *Apsf = nullptr, *Tinv = nullptr;
     float *coefr = nullptr;
     // workspace
-    float *pp = nullptr, *T = nullptr;
+    float *T = nullptr;
     double2 *S = nullptr;
     double *wout = nullptr;
     // host-call staging
@@ -481,10 +570,10 @@ int dev_alloc(DeconPlan &P, T **ptr, size_t count)
 void free_plan(DeconPlan *P)
 {
     (void)hipSetDevice(P->device);
-    for (void *q : {(void *)P->tw, (void *)P->Aimg, (void *)P->Apsf, (void *)P->Tinv, (void *)P->coefr, (void *)P->pp,
+    for (void *q : {(void *)P->tw, (void *)P->Aimg, (void *)P->Apsf, (void *)P->Tinv, (void *)P->coefr,
                     (void *)P->T, (void *)P->S, (void *)P->wout, (void *)P->io})
         if (q) (void)hipFree(q);
-    for (hipEvent_t &e : P->ev)
+    for (hipEvent_t e : P->ev)
         if (e) (void)hipEventDestroy(e);
     if (P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
@@ -492,16 +581,17 @@ void free_plan(DeconPlan *P)
 
 bool cfg_fits(const DGeo &G, const KCfg &c)
 {
-    return G.mxi <= kNW * c.R && G.n <= kNW * c.P && G.outx <= kNW * c.PQ && G.mxp <= kNW * c.Rc && G.n <= kNW * c.Pc;
+    return G.mxi <= kNWC * c.R && G.outx <= kNWC * c.PQ && G.nh <= kNWC * c.PP && G.mxp <= kNW * c.Rc &&
+           G.nh <= kNW * kPPc;
 }
 
 template <int C>
 int set_lds_attrs(const DeconPlan &P)
 {
     constexpr KCfg c = kCfgs[C];
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_decon_cells<kNW, c.Rc, c.Pc>),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_decon_cells<kNW, c.Rc, kPPc>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(P.lds_cells)));
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_decon_windows<kNW, c.R, c.P, c.PQ>),
+    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_decon_windows<kNW, c.R, c.PP, c.PQ, c.wpe>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(P.lds_windows)));
     return SWD_OK;
 }
@@ -510,10 +600,13 @@ int set_lds_limits(DeconPlan &P)
 {
     const DGeo &G = P.G;
     const KCfg &c = kCfgs[P.cfg];
-    P.lds_cells = sizeof(double2) * (G.n + (G.mxp + 1) * G.nh) + sizeof(double) * kNW * c.Rc * G.mzp + 2 * sizeof(float);
-    const size_t region = std::max({sizeof(double2) * (G.mxi + 1) * G.nh + sizeof(float) * kNW * c.R * G.mzi,
-                                    sizeof(double2) * inv_rows_pad(G.n, c.PQ) * G.nh, sizeof(double2) * G.outx * G.nh});
-    P.lds_windows = sizeof(double2) * G.n + region;
+    P.lds_cells = sizeof(double2) * (G.n + G.mxp * G.nh) + sizeof(double) * kNW * c.Rc * G.mzp +
+                  sizeof(float) * (sum_pad(G.mxp * G.mzp) + 2);
+    const size_t region = std::max({win_x_bytes(G.mxi, G.mzi) + sizeof(double2) * G.mxi * G.nh,
+                                    sizeof(float) * kNWC * c.R * G.mzi,
+                                    sizeof(double2) * half_rows_pad(G.n, c.PQ) * G.nh,
+                                    sizeof(double2) * G.outx * G.nh});
+    P.lds_windows = win_head_bytes(G.n) + region;
     int max_lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, P.device));
     if (P.lds_windows > static_cast<size_t>(max_lds) || P.lds_cells > static_cast<size_t>(max_lds))
@@ -522,8 +615,8 @@ int set_lds_limits(DeconPlan &P)
     return P.cfg == 0 ? set_lds_attrs<0>(P) : set_lds_attrs<1>(P);
 }
 
-// Twiddle tables, zero-padded to the launch configuration: Aimg [mxi+1][kNW P],
-// Apsf [mxp+1][kNW Pc], Tinv [inv_rows_pad][kNW PQ].
+// Twiddle tables, zero-padded to the launch configuration: Aimg [mxi][kNWC PP]
+// and Apsf [mxp][kNW kPPc] over wavenumber pairs, Tinv [tq_rows][kNWC PQ].
 int upload_tables(DeconPlan &P)
 {
     const DGeo &G = P.G;
@@ -537,11 +630,11 @@ int upload_tables(DeconPlan &P)
             for (int k = 0; k < cols; k++) A[(size_t)r * ld + k] = tw[index(r, k)];
         return A;
     };
-    const auto aimg = table(G.mxi + 1, G.mxi, kNW * c.P, n,
-                            [&](int r, int k0) { return (static_cast<long>(k0) * (r + G.offx)) % n; });
-    const auto apsf = table(G.mxp + 1, G.mxp, kNW * c.Pc, n, [&](int r, int k0) { return (static_cast<long>(k0) * r) % n; });
-    const int qrows = inv_rows_pad(n, c.PQ);
-    const auto tinv = table(qrows, n, kNW * c.PQ, G.outx,
+    const auto aimg = table(G.mxi, G.mxi, kNWC * c.PP, G.nh,
+                            [&](int r, int p) { return (static_cast<long>(p) * (r + G.offx)) % n; });
+    const auto apsf = table(G.mxp, G.mxp, kNW * kPPc, G.nh, [&](int r, int p) { return (static_cast<long>(p) * r) % n; });
+    const int qrows = tq_rows(n, c.PQ);
+    const auto tinv = table(qrows, n, kNWC * c.PQ, G.outx,
                             [&](int k0, int i) { return (static_cast<long>(k0) * (G.r0x + i)) % n; });
     std::vector<float> coefr((size_t)G.outx * G.outz);
     swd_decon_taper(&P.p, coefr.data());
@@ -559,14 +652,19 @@ int upload_tables(DeconPlan &P)
 }
 
 template <int C>
-void launch_transforms(DeconPlan &P, const float *img, const float *psf)
+void launch_cells(DeconPlan &P, const float *psf)
 {
     constexpr KCfg c = kCfgs[C];
-    const DGeo &G = P.G;
-    k_decon_cells<kNW, c.Rc, c.Pc><<<G.ncx * G.ncz, kNW * 64, P.lds_cells, P.stream>>>(G, psf, P.tw, P.Apsf, P.S, P.T);
-    (void)hipEventRecord(P.ev[2], P.stream);
-    k_decon_windows<kNW, c.R, c.P, c.PQ><<<G.totalnum, kNW * 64, P.lds_windows, P.stream>>>(
-        G, img, P.pp, P.tw, P.Aimg, P.Tinv, P.S, P.T, P.wout);
+    k_decon_cells<kNW, c.Rc, kPPc><<<P.G.ncx * P.G.ncz, kNW * 64, P.lds_cells, P.stream>>>(P.G, psf, P.tw, P.Apsf, P.S,
+                                                                                        P.T);
+}
+
+template <int C>
+void launch_windows(DeconPlan &P, const float *img)
+{
+    constexpr KCfg c = kCfgs[C];
+    k_decon_windows<kNW, c.R, c.PP, c.PQ, c.wpe><<<P.G.totalnum, kNW * 64, P.lds_windows, P.stream>>>(
+        P.G, img, P.tw, P.Aimg, P.Tinv, P.S, P.T, P.wout);
 }
 
 int run_dev(DeconPlan &P, const float *img, const float *psf, float *dec, float *coef)
@@ -574,24 +672,24 @@ int run_dev(DeconPlan &P, const float *img, const float *psf, float *dec, float 
     const DGeo &G = P.G;
     HIP_TRY(hipSetDevice(P.device));
     HIP_TRY(hipEventRecord(P.ev[0], P.stream));
-    k_decon_sums<<<(G.totalnum + 255) / 256, 256, 0, P.stream>>>(G, img, P.pp);
+    if (P.cfg == 0) launch_cells<0>(P, psf);
+    else launch_cells<1>(P, psf);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(P.ev[1], P.stream));
-    if (P.cfg == 0) launch_transforms<0>(P, img, psf);
-    else launch_transforms<1>(P, img, psf);
+    if (P.cfg == 0) launch_windows<0>(P, img);
+    else launch_windows<1>(P, img);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(P.ev[3], P.stream));
+    HIP_TRY(hipEventRecord(P.ev[2], P.stream));
     k_decon_gather<<<(G.nx * G.nz + 255) / 256, 256, 0, P.stream>>>(G, P.wout, P.coefr, dec, coef);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(P.ev[4], P.stream));
+    HIP_TRY(hipEventRecord(P.ev[3], P.stream));
     HIP_TRY(hipStreamSynchronize(P.stream));
-    float ms[4];
-    for (int k = 0; k < 4; k++) HIP_TRY(hipEventElapsedTime(&ms[k], P.ev[k], P.ev[k + 1]));
-    P.st.sums_ms = ms[0];
-    P.st.cells_ms = ms[1];
-    P.st.windows_ms = ms[2];
-    P.st.gather_ms = ms[3];
-    P.st.total_ms = static_cast<double>(ms[0]) + ms[1] + ms[2] + ms[3];
+    float ms[3];
+    for (int k = 0; k < 3; k++) HIP_TRY(hipEventElapsedTime(&ms[k], P.ev[k], P.ev[k + 1]));
+    P.st.cells_ms = ms[0];
+    P.st.windows_ms = ms[1];
+    P.st.gather_ms = ms[2];
+    P.st.total_ms = static_cast<double>(ms[0]) + ms[1] + ms[2];
     return SWD_OK;
 }
 
@@ -664,7 +762,7 @@ extern "C" int swd_decon_plan_create(const swd_decon_params *p, int device, void
         if (hipEventCreate(&e) != hipSuccess) return fail(swd_fail(SWD_E_DEVICE, "hipEventCreate failed"));
     if ((r = set_lds_limits(*P)) || (r = upload_tables(*P))) return fail(r);
     const size_t bins = (size_t)G.n * G.nh, cells = (size_t)G.ncx * G.ncz;
-    if ((r = dev_alloc(*P, &P->pp, G.totalnum)) || (r = dev_alloc(*P, &P->S, cells * bins)) ||
+    if ((r = dev_alloc(*P, &P->S, cells * bins)) ||
         (r = dev_alloc(*P, &P->T, cells * bins)) ||
         (r = dev_alloc(*P, &P->wout, (size_t)G.totalnum * G.outx * G.outz)))
         return fail(r);

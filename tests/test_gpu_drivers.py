@@ -137,3 +137,45 @@ def test_afor_and_avsprtm_psf(tmp_path):
     run("AVSPRTM", tmp)
     psf = read(tmp, "migration_psf.bin")
     assert np.isfinite(psf).all() and np.abs(psf).max() > 0
+
+
+def test_decon_driver(tmp_path):
+    """Decon in the kkdecon_acoustic layout on the reference-data crop
+    (tests/golden): the deconvolved image and the taper weights equal the
+    oracle's bit for bit, and the model by-products follow DC:199-252."""
+    from test_decon_oracle import crop_inputs
+
+    img, psf = crop_inputs()
+    nx, nz = img.shape
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "input"))
+    os.makedirs(os.path.join(d, "output"))
+    rng = np.random.default_rng(21)
+    v_true = (2000.0 + 500.0 * rng.random((nx, nz))).astype(np.float32)
+    v_mig = (2000.0 + 500.0 * rng.random((nx, nz))).astype(np.float32)
+    files = {"vp": "./input/acc_vp.bin", "vp0": "./input/ini_vp.bin", "rho": "./input/ini_rho.bin",
+             "image": "./input/migration_dupp.bin", "psf": "./input/migration_dupsf.bin",
+             "decon": "./output/migration_dudecon.bin", "inv": "./output/inv_psf.bin"}
+    v_true.tofile(os.path.join(d, files["vp"]))
+    v_mig.tofile(os.path.join(d, files["vp0"]))
+    img.tofile(os.path.join(d, files["image"]))
+    psf.tofile(os.path.join(d, files["psf"]))
+    vals = [nx, nz, 31, 31, 71, 71, 11, 11, "5.0e-4", files["vp"], files["vp0"], files["rho"], files["image"],
+            files["psf"], files["decon"], files["inv"], 0, "2.0e-2"]
+    with open(os.path.join(d, "parameter.txt"), "w") as f:
+        for k, v in enumerate(vals):
+            f.write(f"--entry {k}\n  {v}\n")
+    print(f"case Decon driver on the {nx} x {nz} reference-data crop", flush=True)
+    run("Decon", d)
+    out = lambda name: np.fromfile(os.path.join(d, "output", name), np.float32).reshape(nx, nz)
+    odec, ocoef = O.decon(O.decon_params(nx, nz), img, psf)
+    np.testing.assert_array_equal(out("migration_dudecon.bin"), odec)
+    np.testing.assert_array_equal(out("migration_coef.bin"), ocoef)
+    np.testing.assert_array_equal(out("migration_pp.bin"), img)
+    np.testing.assert_array_equal(out("acc_ref.bin"), (v_true - v_mig) / v_true)
+    ref = np.zeros_like(v_true)
+    ref[:, 1:] = (v_true[:, 1:] - v_true[:, :-1]) / (v_true[:, 1:] + v_true[:, :-1])
+    np.testing.assert_array_equal(out("true_vertical_ref.bin"), ref)
+    assert os.path.exists(os.path.join(d, "output", "end.flag"))
+    txt = open(os.path.join(d, "output", "migration_coef.txt"), "rb").read().split(b"\r\t\n")
+    assert len(txt) == nz + 1 and len(txt[0].split()) == nx
