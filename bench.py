@@ -178,6 +178,8 @@ def main():
                     help="shots per fused RTM forward/backward loop (0 = sized to the cache)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy 256 MiB device-to-device once (reference for HBM byte counters)")
+    ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
+                    help="RTM shot groups in flight at once (swd_plan_set_streams)")
     ap.add_argument("--workload", choices=("rtm", "decon"), default="rtm",
                     help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution")
     a = ap.parse_args()
@@ -199,6 +201,7 @@ def main():
     path = swd.SWD_PATH_FUSED if a.path == "fused" else swd.SWD_PATH_BATCHED
     plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0, path=path, tile=a.tile)
     plan.set_groups(*a.groups)
+    plan.set_streams(a.streams)
     shots = swd.ShotSet(case.shots)
     # observed data on the true model, up-going integration as the adjoint source (RC:603-637)
     plan_true = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0)

@@ -141,6 +141,12 @@ int swd_plan_set_tile(void *plan, int tile_x);
  * groups of these sizes (0, the default: sized so a group's working set stays
  * in the last-level cache).  Results do not depend on them. */
 int swd_plan_set_groups(void *plan, int fwd_shots, int bwd_shots);
+/* Streams the shot groups of swd_rtm run on at once, 1 or 2 (default): with
+ * 2, consecutive groups alternate between the plan's stream and a second one
+ * (default priority), so one group's launch gaps and tails overlap the other's
+ * work, and groups are sized to half the cache budget.  Results do not depend
+ * on it. */
+int swd_plan_set_streams(void *plan, int streams);
 /* Replay fused time loops from captured hipGraphs (on by default): the
  * launches of a loop are recorded once per configuration and replayed on
  * later calls with the same batch layout. */

@@ -93,8 +93,10 @@ struct Plan {
     KGrid kg{};
     int tile_req = 0;           // fused tile width asked for (0 = tune)
     int group_fwd = 0, group_bwd = 0;     // RTM shots per fused forward / backward loop (0 = sized to the cache)
-    int tile_fwd = 0, tile_bwd = 0;       // tuned widths and the batch they were tuned for
-    int tile_fwd_n = 0, tile_bwd_n = 0;
+    std::map<int, int> tuned_fwd, tuned_bwd;   // tuned tile width by instances per launch
+    int streams = 2;            // RTM shot groups run on this many streams at once (1 or 2)
+    hipStream_t side = nullptr; // the second stream (default priority)
+    hipEvent_t fork = nullptr, join = nullptr;
     bool graphs = true;         // replay fused time loops from captured hipGraphs
     std::map<std::vector<long long>, hipGraphExec_t> graph_cache;
     char *aux = nullptr;        // multi-grid launch tables
@@ -163,15 +165,15 @@ int ensure_aux(Plan &P, size_t bytes)
 // arguments and workspace addresses) replay it.  Only kernel launches are
 // recorded.
 template <class F>
-int graph_run(Plan &P, const std::vector<long long> &key, F record)
+int graph_run(Plan &P, const std::vector<long long> &key, F record, hipStream_t st)
 {
     auto it = P.graph_cache.find(key);
     if (it == P.graph_cache.end()) {
         if (P.graph_cache.size() >= 16) drop_graphs(P);
         hipGraph_t graph = nullptr;
-        HIP_TRY(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         const int rc = record();
-        const hipError_t e = hipStreamEndCapture(P.stream, &graph);
+        const hipError_t e = hipStreamEndCapture(st, &graph);
         if (rc != SWD_OK) {
             if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
             return rc;
@@ -183,8 +185,14 @@ int graph_run(Plan &P, const std::vector<long long> &key, F record)
         HIP_TRY(ei);
         it = P.graph_cache.emplace(key, exec).first;
     }
-    HIP_TRY(hipGraphLaunch(it->second, P.stream));
+    HIP_TRY(hipGraphLaunch(it->second, st));
     return SWD_OK;
+}
+
+template <class F>
+int graph_run(Plan &P, const std::vector<long long> &key, F record)
+{
+    return graph_run(P, key, record, P.stream);
 }
 
 int check_plan(void *plan, Plan *&P)
@@ -300,6 +308,9 @@ extern "C" int swd_plan_create(const swd_grid *g, int device, void **out)
     if (e == hipSuccess) e = hipStreamCreate(&P->stream);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&P->ev[i]);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&P->tev[i]);
+    if (e == hipSuccess) e = hipStreamCreate(&P->side);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&P->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&P->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&P->d_coef, sizeof(float) * 4 * (P->ntx + P->ntz));
     if (e == hipSuccess) e = hipMalloc(&P->d_model, sizeof(float) * 4 * P->nfield);
     if (e == hipSuccess) e = hipMemset(P->d_model, 0, sizeof(float) * 4 * P->nfield);
@@ -329,6 +340,7 @@ extern "C" int swd_plan_destroy(void *plan)
     Plan *P = static_cast<Plan *>(plan);
     (void)hipSetDevice(P->dev);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
+    if (P->side) (void)hipStreamSynchronize(P->side);
     for (auto &kv : P->graph_cache) (void)hipGraphExecDestroy(kv.second);
     P->graph_cache.clear();
     if (P->ws) (void)hipFree(P->ws);
@@ -339,7 +351,10 @@ extern "C" int swd_plan_destroy(void *plan)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : P->tev)
         if (e) (void)hipEventDestroy(e);
+    if (P->fork) (void)hipEventDestroy(P->fork);
+    if (P->join) (void)hipEventDestroy(P->join);
     if (P->stream) (void)hipStreamDestroy(P->stream);
+    if (P->side) (void)hipStreamDestroy(P->side);
     delete P;
     return SWD_OK;
 }
@@ -431,13 +446,22 @@ extern "C" int swd_plan_set_graphs(void *plan, int on)
     return SWD_OK;
 }
 
+extern "C" int swd_plan_set_streams(void *plan, int streams)
+{
+    if (!plan || (streams != 1 && streams != 2))
+        return swd_fail(SWD_E_ARG, "swd_plan_set_streams: %d streams, not 1 or 2", streams);
+    static_cast<Plan *>(plan)->streams = streams;
+    return SWD_OK;
+}
+
 extern "C" int swd_plan_set_tile(void *plan, int tile_x)
 {
     if (!plan || (tile_x != 0 && tile_x != 16 && tile_x != 32 && tile_x != 64))
         return swd_fail(SWD_E_ARG, "swd_plan_set_tile: tile width %d not in {0, 16, 32, 64}", tile_x);
     Plan *P = static_cast<Plan *>(plan);
     P->tile_req = tile_x;
-    P->tile_fwd = P->tile_bwd = P->tile_fwd_n = P->tile_bwd_n = 0;
+    P->tuned_fwd.clear();
+    P->tuned_bwd.clear();
     return SWD_OK;
 }
 
@@ -538,19 +562,24 @@ int upload_steps(Plan &P, const std::vector<KStep> &h, size_t o)
 }
 
 // Runs the fused launches its[0], its[1], ... (time steps in execution order)
-// of n instances.  The tile width is the plan's requested one, else the one
-// tuned earlier for this batch size, else it is tuned here: after two warm-up
-// launches, six launches with each candidate width are timed and the fastest
-// width runs the rest (the results do not depend on the width).
-int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int first_it, int &tile, int &tile_n,
-                long long *launches, bool pair = false)
+// of n instances on stream st (the plan's by default).  The tile width is the
+// plan's requested one, else the one tuned earlier for n instances, else it is
+// tuned here: after two warm-up launches, six launches with each candidate
+// width are timed and the fastest width runs the rest (the results do not
+// depend on the width).
+int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int first_it, std::map<int, int> &tuned,
+                long long *launches, bool pair = false, hipStream_t st = nullptr)
 {
+    if (!st) st = P.stream;
     int tx = P.tile_req;
-    if (tx == 0 && tile_n == n) tx = tile;
+    if (tx == 0) {
+        const auto f = tuned.find(n);
+        if (f != tuned.end()) tx = f->second;
+    }
     size_t k = 0;
     auto launch = [&](int w) -> int {
         const int it = its[k++];
-        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, pair, P.stream));
+        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, pair, st));
         (*launches)++;
         return SWD_OK;
     };
@@ -559,10 +588,10 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         const int cand[3] = {16, 32, 64};
         if (its.size() >= static_cast<size_t>(kWarm + 3 * kTry)) {
             for (int j = 0; j < kWarm; j++) SWD_TRY(launch(32));
-            HIP_TRY(hipEventRecord(P.tev[0], P.stream));
+            HIP_TRY(hipEventRecord(P.tev[0], st));
             for (int c = 0; c < 3; c++) {
                 for (int j = 0; j < kTry; j++) SWD_TRY(launch(cand[c]));
-                HIP_TRY(hipEventRecord(P.tev[c + 1], P.stream));
+                HIP_TRY(hipEventRecord(P.tev[c + 1], st));
             }
             HIP_TRY(hipEventSynchronize(P.tev[3]));
             float best = 0.f;
@@ -574,8 +603,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
                     tx = cand[c];
                 }
             }
-            tile = tx;
-            tile_n = n;
+            tuned[n] = tx;
         } else {
             tx = 32;
         }
@@ -584,11 +612,14 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         const std::vector<long long> key = {1, reinterpret_cast<long long>(d), n, its[k], static_cast<long long>(its.size() - k),
                                             first_it, tx, pair ? 1 : 0, reinterpret_cast<long long>(P.ws)};
         const size_t k0 = k;
-        SWD_TRY(graph_run(P, key, [&]() -> int {
-            for (size_t j = k0; j < its.size(); j++)
-                HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, tx, pair, P.stream));
-            return SWD_OK;
-        }));
+        SWD_TRY(graph_run(
+            P, key,
+            [&]() -> int {
+                for (size_t j = k0; j < its.size(); j++)
+                    HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, tx, pair, st));
+                return SWD_OK;
+            },
+            st));
         *launches += static_cast<long long>(its.size() - k0);
         return SWD_OK;
     }
@@ -608,7 +639,7 @@ int fused_forward_loop(Plan &P, const KStep *d, int n)
 {
     long long launches = 0;
     stats_begin(P, 0);
-    SWD_TRY(fused_steps(P, d, n, steps_up(P.g.nt), -1, P.tile_fwd, P.tile_fwd_n, &launches));
+    SWD_TRY(fused_steps(P, d, n, steps_up(P.g.nt), -1, P.tuned_fwd, &launches));
     P.stats.fwd_ms = stats_end(P, 0);
     P.stats.fwd_launches = launches;
     P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * P.g.nt * n;
@@ -918,8 +949,9 @@ void rtm_groups(const Plan &P, int nshots, int &gf, int &gb)
     const double Ncpml = N - Nint;
     const double fwd = 4.0 * (6 * N + 6 * Ncpml);
     const double bwd = 4.0 * (9 * Nint + 6 * N + 6 * Ncpml);
-    gf = P.group_fwd > 0 ? P.group_fwd : static_cast<int>(kCacheBudget / fwd);
-    gb = P.group_bwd > 0 ? P.group_bwd : static_cast<int>(kCacheBudget / bwd);
+    const double budget = kCacheBudget / P.streams;   // groups in flight share the cache
+    gf = P.group_fwd > 0 ? P.group_fwd : static_cast<int>(budget / fwd);
+    gb = P.group_bwd > 0 ? P.group_bwd : static_cast<int>(budget / bwd);
     gf = even_groups(nshots, gf);
     gb = even_groups(nshots, gb);
 }
@@ -982,13 +1014,32 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     // backward need not match those of the forward.
     int gf = 0, gb = 0;
     rtm_groups(P, nshots, gf, gb);
+    // With two streams, consecutive groups alternate between them so that one
+    // group's launch gaps and tails overlap the other's work.
+    const bool two = P.streams == 2;
+    auto group_stream = [&](int g) { return (two && (g & 1)) ? P.side : P.stream; };
+    auto fork = [&]() -> int {
+        if (!two) return SWD_OK;
+        HIP_TRY(hipEventRecord(P.fork, P.stream));
+        HIP_TRY(hipStreamWaitEvent(P.side, P.fork, 0));
+        return SWD_OK;
+    };
+    auto join = [&]() -> int {
+        if (!two) return SWD_OK;
+        HIP_TRY(hipEventRecord(P.join, P.side));
+        HIP_TRY(hipStreamWaitEvent(P.stream, P.join, 0));
+        return SWD_OK;
+    };
     {
         long long launches = 0;
         stats_begin(P, 0);
-        for (int k0 = 0; k0 < nshots; k0 += gf) {
+        SWD_TRY(fork());
+        for (int k0 = 0, g = 0; k0 < nshots; k0 += gf, g++) {
             const int n = std::min(gf, nshots - k0);
-            SWD_TRY(fused_steps(P, at<KStep>(P, o_fwd) + k0, n, steps_up(nt), -1, P.tile_fwd, P.tile_fwd_n, &launches));
+            SWD_TRY(fused_steps(P, at<KStep>(P, o_fwd) + k0, n, steps_up(nt), -1, P.tuned_fwd, &launches, false,
+                                group_stream(g)));
         }
+        SWD_TRY(join());
         P.stats.fwd_ms = stats_end(P, 0);
         P.stats.fwd_launches = launches;
         P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * nt * nshots;
@@ -1008,11 +1059,13 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     for (int it = nt - 2; it >= 0; it--) its.push_back(it);
     long long launches = 0;
     stats_begin(P, 1);
-    for (int k0 = 0; k0 < nshots; k0 += gb) {
+    SWD_TRY(fork());
+    for (int k0 = 0, g = 0; k0 < nshots; k0 += gb, g++) {
         const int n = std::min(gb, nshots - k0);
         const KStep *d = at<KStep>(P, o_bwd) + 2 * k0;
-        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tile_bwd, P.tile_bwd_n, &launches, true));
+        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tuned_bwd, &launches, true, group_stream(g)));
     }
+    SWD_TRY(join());
     P.stats.bwd_ms = stats_end(P, 1);
     P.stats.bwd_launches = launches;
     P.stats.bwd_cells = static_cast<long long>(nt - 1) * nshots *

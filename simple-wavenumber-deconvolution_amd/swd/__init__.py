@@ -78,7 +78,7 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_cpml", "swd_model_in_pml", "swd_read_model", "swd_smooth_model", "swd_nfft", "swd_kz_taper",
            "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
-           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_groups", "swd_plan_set_graphs",
+           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
            "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor",
            "swd_rtm", "swd_rtm_dev", "swd_born", "swd_born_adjoint", "swd_born_dev",
            "swd_born_adjoint_dev", "swd_multi_forward", "swd_decon_read_parameters", "swd_decon_geometry",
@@ -303,6 +303,10 @@ class Plan:
     def set_graphs(self, on):
         """Replay fused time loops from captured hipGraphs (default on)."""
         check(self.L.swd_plan_set_graphs(self.h, 1 if on else 0))
+
+    def set_streams(self, streams):
+        """Shot groups of an RTM call in flight at once (1 or 2); results do not change."""
+        check(self.L.swd_plan_set_streams(self.h, int(streams)))
 
     def set_tile(self, tile):
         """Fused-path tile width (16, 32, 64) or 0 to tune it on the first steps."""

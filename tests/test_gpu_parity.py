@@ -209,15 +209,17 @@ def test_fused_autotuned_tile_matches_fixed():
 
 def test_fused_rtm_shot_groups_match():
     """RTM in shot groups (forward and backward grouped differently, uneven
-    last group) gives the same images as one group."""
+    last group), on one stream or with groups alternating over two, gives the
+    same images as one group."""
     c = configs.small(nt=150, nshots=5, dup=True)
     print("case fused rtm groups", flush=True)
     outs = [oracle_problem(c, k).forward() for k in range(5)]
     adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
     plan = plan_of(c)
     ref = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
-    for gf, gb in ((1, 1), (2, 3), (5, 2)):
+    for gf, gb, streams in ((1, 1, 1), (2, 3, 1), (5, 2, 1), (1, 1, 2), (2, 3, 2), (0, 0, 2)):
         plan.set_groups(gf, gb)
+        plan.set_streams(streams)
         got = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
         for a, b in zip(got, ref):
-            assert np.array_equal(a, b), (gf, gb)
+            assert np.array_equal(a, b), (gf, gb, streams)
