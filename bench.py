@@ -321,6 +321,10 @@ def main():
                      "kernel": "k_fstep (fused RTM backward step: reconstruction + receiver + imaging)"
                                if a.path == "fused" else "RTM backward step (4 sub-step kernels)",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": bwd_launch_ms,
+                     "timing": ("HIP events on the plan's stream around the backward phase (joining the second "
+                                "stream) / launches: with two streams consecutive groups' launches overlap, so this "
+                                "is the interval between launch completions" if a.streams == 2 else
+                                "HIP events on the plan's stream around the backward phase / launches"),
                      "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per shot-step, "
                                        "profiles/rtm_fused_c2_16shots.md" if traffic else None},
         "cpu_baseline": cpu,
