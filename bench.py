@@ -178,7 +178,7 @@ def main():
                     help="shots per fused RTM forward/backward loop (0 = sized to the cache)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy 256 MiB device-to-device once (reference for HBM byte counters)")
-    ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
+    ap.add_argument("--streams", type=int, default=2, choices=(1, 2, 3, 4),
                     help="RTM shot groups in flight at once (swd_plan_set_streams)")
     ap.add_argument("--workload", choices=("rtm", "decon"), default="rtm",
                     help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution")
@@ -321,9 +321,9 @@ def main():
                      "kernel": "k_fstep (fused RTM backward step: reconstruction + receiver + imaging)"
                                if a.path == "fused" else "RTM backward step (4 sub-step kernels)",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": bwd_launch_ms,
-                     "timing": ("HIP events on the plan's stream around the backward phase (joining the second "
-                                "stream) / launches: with two streams consecutive groups' launches overlap, so this "
-                                "is the interval between launch completions" if a.streams == 2 else
+                     "timing": ("HIP events on the plan's stream around the backward phase (joining the other "
+                                "streams) / launches: with several streams consecutive groups' launches overlap, so this "
+                                "is the interval between launch completions" if a.streams > 1 else
                                 "HIP events on the plan's stream around the backward phase / launches"),
                      "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per shot-step, "
                                        "profiles/rtm_fused_c2_16shots.md" if traffic else None},

@@ -22,8 +22,10 @@
  *    files are [nx][nz].  Gathers are trace-major [ntrace][nt].
  *  - Source/receiver coordinates are padded-grid indices, as Source.s_ix/s_iz
  *    and r_iz in the reference (H:9-13).
- *  - A plan is bound to one device and one stream; it is not thread-safe, but
- *    independent plans may be used from independent threads.
+ *  - A plan is bound to one device; its work runs on its own stream (and, for
+ *    alternate RTM shot groups, on its other streams between a fork and a
+ *    join).  It is not thread-safe, but independent plans may be used from
+ *    independent threads.
  */
 #ifndef SWD_H
 #define SWD_H
@@ -141,11 +143,11 @@ int swd_plan_set_tile(void *plan, int tile_x);
  * groups of these sizes (0, the default: sized so a group's working set stays
  * in the last-level cache).  Results do not depend on them. */
 int swd_plan_set_groups(void *plan, int fwd_shots, int bwd_shots);
-/* Streams the shot groups of swd_rtm run on at once, 1 or 2 (default): with
- * 2, consecutive groups alternate between the plan's stream and a second one
- * (default priority), so one group's launch gaps and tails overlap the other's
- * work, and groups are sized to half the cache budget.  Results do not depend
- * on it. */
+/* Streams the shot groups of swd_rtm run on at once, 1 to 4 (default 2):
+ * consecutive groups take the plan's stream and its other streams (default
+ * priority) in turn, so one group's launch gaps and tails overlap the others'
+ * work, and the cache budget of a group is divided by the count.  Results do
+ * not depend on it. */
 int swd_plan_set_streams(void *plan, int streams);
 /* Replay fused time loops from captured hipGraphs (on by default): the
  * launches of a loop are recorded once per configuration and replayed on

@@ -94,9 +94,9 @@ struct Plan {
     int tile_req = 0;           // fused tile width asked for (0 = tune)
     int group_fwd = 0, group_bwd = 0;     // RTM shots per fused forward / backward loop (0 = sized to the cache)
     std::map<int, int> tuned_fwd, tuned_bwd;   // tuned tile width by instances per launch
-    int streams = 2;            // RTM shot groups run on this many streams at once (1 or 2)
-    hipStream_t side = nullptr; // the second stream (default priority)
-    hipEvent_t fork = nullptr, join = nullptr;
+    int streams = 2;            // RTM shot groups run on this many streams at once (1 .. kMaxStreams)
+    hipStream_t side[3] = {};   // the other streams (default priority)
+    hipEvent_t fork = nullptr, join[3] = {};
     bool graphs = true;         // replay fused time loops from captured hipGraphs
     std::map<std::vector<long long>, hipGraphExec_t> graph_cache;
     char *aux = nullptr;        // multi-grid launch tables
@@ -308,9 +308,9 @@ extern "C" int swd_plan_create(const swd_grid *g, int device, void **out)
     if (e == hipSuccess) e = hipStreamCreate(&P->stream);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&P->ev[i]);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&P->tev[i]);
-    if (e == hipSuccess) e = hipStreamCreate(&P->side);
+    for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipStreamCreate(&P->side[i]);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&P->fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&P->join, hipEventDisableTiming);
+    for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&P->join[i], hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&P->d_coef, sizeof(float) * 4 * (P->ntx + P->ntz));
     if (e == hipSuccess) e = hipMalloc(&P->d_model, sizeof(float) * 4 * P->nfield);
     if (e == hipSuccess) e = hipMemset(P->d_model, 0, sizeof(float) * 4 * P->nfield);
@@ -340,7 +340,8 @@ extern "C" int swd_plan_destroy(void *plan)
     Plan *P = static_cast<Plan *>(plan);
     (void)hipSetDevice(P->dev);
     if (P->stream) (void)hipStreamSynchronize(P->stream);
-    if (P->side) (void)hipStreamSynchronize(P->side);
+    for (hipStream_t st : P->side)
+        if (st) (void)hipStreamSynchronize(st);
     for (auto &kv : P->graph_cache) (void)hipGraphExecDestroy(kv.second);
     P->graph_cache.clear();
     if (P->ws) (void)hipFree(P->ws);
@@ -352,9 +353,11 @@ extern "C" int swd_plan_destroy(void *plan)
     for (auto &e : P->tev)
         if (e) (void)hipEventDestroy(e);
     if (P->fork) (void)hipEventDestroy(P->fork);
-    if (P->join) (void)hipEventDestroy(P->join);
+    for (hipEvent_t e : P->join)
+        if (e) (void)hipEventDestroy(e);
     if (P->stream) (void)hipStreamDestroy(P->stream);
-    if (P->side) (void)hipStreamDestroy(P->side);
+    for (hipStream_t st : P->side)
+        if (st) (void)hipStreamDestroy(st);
     delete P;
     return SWD_OK;
 }
@@ -448,8 +451,8 @@ extern "C" int swd_plan_set_graphs(void *plan, int on)
 
 extern "C" int swd_plan_set_streams(void *plan, int streams)
 {
-    if (!plan || (streams != 1 && streams != 2))
-        return swd_fail(SWD_E_ARG, "swd_plan_set_streams: %d streams, not 1 or 2", streams);
+    if (!plan || streams < 1 || streams > 4)
+        return swd_fail(SWD_E_ARG, "swd_plan_set_streams: %d streams, not 1 .. 4", streams);
     static_cast<Plan *>(plan)->streams = streams;
     return SWD_OK;
 }
@@ -1014,20 +1017,21 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     // backward need not match those of the forward.
     int gf = 0, gb = 0;
     rtm_groups(P, nshots, gf, gb);
-    // With two streams, consecutive groups alternate between them so that one
-    // group's launch gaps and tails overlap the other's work.
-    const bool two = P.streams == 2;
-    auto group_stream = [&](int g) { return (two && (g & 1)) ? P.side : P.stream; };
+    // With several streams, consecutive groups take them in turn so that one
+    // group's launch gaps and tails overlap the others' work.
+    const int ns = P.streams;
+    auto group_stream = [&](int g) { return g % ns == 0 ? P.stream : P.side[g % ns - 1]; };
     auto fork = [&]() -> int {
-        if (!two) return SWD_OK;
+        if (ns == 1) return SWD_OK;
         HIP_TRY(hipEventRecord(P.fork, P.stream));
-        HIP_TRY(hipStreamWaitEvent(P.side, P.fork, 0));
+        for (int i = 0; i < ns - 1; i++) HIP_TRY(hipStreamWaitEvent(P.side[i], P.fork, 0));
         return SWD_OK;
     };
     auto join = [&]() -> int {
-        if (!two) return SWD_OK;
-        HIP_TRY(hipEventRecord(P.join, P.side));
-        HIP_TRY(hipStreamWaitEvent(P.stream, P.join, 0));
+        for (int i = 0; i < ns - 1; i++) {
+            HIP_TRY(hipEventRecord(P.join[i], P.side[i]));
+            HIP_TRY(hipStreamWaitEvent(P.stream, P.join[i], 0));
+        }
         return SWD_OK;
     };
     {
