@@ -262,10 +262,11 @@ def c5(nt=8000, nshots=128, shot_index=None):
 DECON_PARAMS = dict(psfwin=31, filterwin=71, snapwin=11, epsilon=5e-4)
 
 
-def _psf_wavelet(w, depth_frac, rng_tilt=0.0):
-    """w x w wavelet centred in the patch: Ricker along z, Gaussian along x."""
-    h = w // 2
-    ax, az = np.meshgrid(np.arange(-h, h + 1), np.arange(-h, h + 1), indexing="ij")
+def _psf_wavelet(w, depth_frac, rng_tilt=0.0, wz=None):
+    """w x wz wavelet centred in the patch: Ricker along z, Gaussian along x."""
+    wz = w if wz is None else wz
+    h, hz = w // 2, wz // 2
+    ax, az = np.meshgrid(np.arange(-h, h + 1), np.arange(-hz, hz + 1), indexing="ij")
     az = az + rng_tilt * ax
     fz = 0.16 - 0.07 * depth_frac              # cycles per cell along z
     sx = 2.0 + 4.0 * depth_frac                # Gaussian width along x (cells)
@@ -273,14 +274,16 @@ def _psf_wavelet(w, depth_frac, rng_tilt=0.0):
     return ((1.0 - 2.0 * a) * np.exp(-a) * np.exp(-0.5 * (ax / sx) ** 2)).astype(np.float32)
 
 
-def decon_case(nx=845, nz=702, seed=11, psfwin=31):
-    """(image, psf) [nx][nz] float32 of the d1 kind on an nx x nz grid."""
+def decon_case(nx=845, nz=702, seed=11, psfwin=31, psfwin_z=None):
+    """(image, psf) [nx][nz] float32 of the d1 kind on an nx x nz grid (PSF
+    lattice spacing psfwin along x and psfwin_z, default psfwin, along z)."""
     rng = np.random.default_rng(seed)
+    pwz = psfwin if psfwin_z is None else psfwin_z
     psf = np.zeros((nx, nz), np.float32)
-    h = psfwin // 2
+    h, hz = psfwin // 2, pwz // 2
     for cx in range(h, nx - h, psfwin):
-        for cz in range(h, nz - h, psfwin):
-            psf[cx - h:cx + h + 1, cz - h:cz + h + 1] = _psf_wavelet(psfwin, cz / nz, 0.1 * (cx / nx - 0.5))
+        for cz in range(hz, nz - hz, pwz):
+            psf[cx - h:cx + h + 1, cz - hz:cz + hz + 1] = _psf_wavelet(psfwin, cz / nz, 0.1 * (cx / nx - 0.5), pwz)
     refl = np.zeros((nx, nz), np.float32)
     xs = np.arange(nx)
     for _ in range(8):
@@ -291,11 +294,11 @@ def decon_case(nx=845, nz=702, seed=11, psfwin=31):
     px = rng.integers(0, nx, 300)
     pz = rng.integers(0, nz, 300)
     np.add.at(refl, (px, pz), rng.uniform(-2.0, 2.0, 300).astype(np.float32))
-    k = _psf_wavelet(psfwin, 0.5)
-    img = np.zeros((nx + psfwin - 1, nz + psfwin - 1), np.float64)
+    k = _psf_wavelet(psfwin, 0.5, 0.0, pwz)
+    img = np.zeros((nx + psfwin - 1, nz + pwz - 1), np.float64)
     for i in range(psfwin):
-        for j in range(psfwin):
+        for j in range(pwz):
             if k[i, j] != 0.0:
                 img[i:i + nx, j:j + nz] += k[i, j] * refl
-    image = (1e6 * img[h:h + nx, h:h + nz]).astype(np.float32)
+    image = (1e6 * img[h:h + nx, hz:hz + nz]).astype(np.float32)
     return np.ascontiguousarray(image), np.ascontiguousarray(1e5 * psf)

@@ -127,6 +127,26 @@ def test_window_matches_reference_procedure_on_real_data():
         assert err <= 1e-12
 
 
+def test_window_matches_reference_procedure_with_unequal_axes():
+    """Windows whose sizes differ along x and z (psf 31 x 21, filter 71 x 81,
+    snap 11 x 9: offsets, output blocks and steps all differ by axis)."""
+    from swd import configs
+
+    img, psf = configs.decon_case(nx=300, nz=280, seed=14, psfwin=31, psfwin_z=21)
+    p = O.DeconParams(300, 280, 31, 21, 71, 81, 11, 9, 5e-4)
+    g = O.decon_geometry(p)
+    imgf, psff = O.decon_expand(img, g.nxp, g.nzp), O.decon_expand(psf, g.nxp, g.nzp)
+    for step in (0, 40, g.totalnum // 2 + 3, g.totalnum - 2):
+        got = O.decon_window(p, img, psf, step)
+        ref = reference_order_window(p, g, imgf, psff, step)
+        assert got.shape == (g.outwinx, g.outwinz)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        print(f"unequal axes, window {step}: max rel diff {err:.2e}")
+        assert err <= 1e-12
+    _, coef = O.decon(p, img, psf)
+    assert np.abs(coef.astype(np.float64) - 1.0).max() <= 2.5e-7
+
+
 def test_output_taper_overlap_adds_to_one():
     img, psf = crop_inputs()
     _, coef = O.decon(O.decon_params(*img.shape), img, psf)

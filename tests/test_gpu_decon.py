@@ -66,6 +66,22 @@ def test_generic_launch_configuration_matches_oracle():
     _check("generic launch (n 125)", (31, 95, 13, 1e-3), img, psf)
 
 
+def test_unequal_axes_match_oracle():
+    """psf 31 x 21, filter 71 x 81, snap 11 x 9: every x/z offset, output block
+    and window step differs by axis."""
+    img, psf = configs.decon_case(nx=300, nz=280, seed=14, psfwin=31, psfwin_z=21)
+    nx, nz = img.shape
+    print("case unequal axes 300 x 280", flush=True)
+    plan = swd.Decon(swd.DeconParams(nx, nz, 31, 21, 71, 81, 11, 9, 5e-4), 0)
+    try:
+        dec, coef = plan.run(img, psf)
+    finally:
+        plan.close()
+    odec, ocoef = O.decon(O.DeconParams(nx, nz, 31, 21, 71, 81, 11, 9, 5e-4), img, psf)
+    np.testing.assert_array_equal(dec, odec)
+    np.testing.assert_array_equal(coef, ocoef)
+
+
 def test_device_pointers_and_repeat_calls():
     img, psf = crop_inputs()
     nx, nz = img.shape
