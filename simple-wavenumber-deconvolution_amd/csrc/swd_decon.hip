@@ -23,8 +23,9 @@
 // scalar cache, so each lane owns one z wavenumber k1 and the fp64 FMAs take
 // one vector operand from LDS and one scalar operand.  All arithmetic is in the
 // fixed order of the host oracle (fma chains in index order, the Hermitian half
-// spectrum along z, the image normalisation folded into the exact divisor
-// pp_max * tmpfloat), so results agree with it bit for bit.
+// spectrum along z, conjugate wavenumber pairs along x, the image
+// normalisation folded into the exact divisor pp_max * tmpfloat), so results
+// agree with it bit for bit.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -83,9 +84,10 @@ __device__ __forceinline__ int cell_of(const DGeo &G, int s)
 
 // ---------------------------------------------------------------- stages
 // Rows pass: Y[r][k1] = sum_c X[r][c] e^{-2 pi i k1 (c + offz) / n} for the
-// half spectrum k1 < nh; lane = k1, wave w owns rows w + NW rr (rr < R, X is
-// zero-padded to NW R rows); X is read as an LDS broadcast, the twiddle per
-// lane from the LDS table.
+// half spectrum k1 < nh; lane = k1, wave w owns rows w + NW rr (rr < R).  X
+// must be readable for NW R rows; rows past mx are computed from whatever the
+// LDS holds there and never stored.  X is read as an LDS broadcast, the
+// twiddle per lane from the LDS table.
 template <int NW, int R, class XT>
 __device__ __forceinline__ void rows_pass(const XT *X, int mx, int mz, int offz, int n, int nh, const double2 *tw,
                                           double2 *Y)

@@ -68,3 +68,17 @@ def test_plan_without_device():
     h = C.c_void_p()
     rc = swd.load().swd_decon_plan_create(C.byref(swd.decon_params(845, 702)), 0, C.byref(h))
     assert rc == -2 and "device" in swd.last_error()
+
+
+def test_null_arguments_fail_cleanly():
+    L = swd.load()
+    p = swd.decon_params(845, 702)
+    assert L.swd_decon(None, None, None, None, None) == -1
+    assert L.swd_decon_dev(None, None, None, None, None) == -1
+    assert L.swd_decon_plan_destroy(None) == -1
+    assert L.swd_decon_get_stats(None, None) == -1
+    assert L.swd_decon_geometry(C.byref(p), None) == -1
+    assert L.swd_decon_taper(None, None) == -1
+    assert L.swd_decon_read_parameters(None, None, None) == -1
+    h = C.c_void_p()
+    assert L.swd_decon_plan_create(None, 0, C.byref(h)) == -1
