@@ -20,10 +20,10 @@ P = dict(nx=80, nz=60, pml=10, Lc=4, dx=10.0, dz=10.0, itmax=300, dt=1e-3, f0=25
          shotdx=200.0, shotdep=0.0, recdix=400.0, offsetz=300.0, swin=11, window=3)
 
 
-def write_params(path, scatter_flag):
+def write_params(path, scatter_flag, **over):
     keys = ["nx", "nz", "pml", "Lc", "dx", "dz", "itmax", "dt", "f0", "ns", "sx0", "shotdx", "shotdep",
             "recdix", "offsetz", "scatter_flag", "swin", "window"]
-    vals = dict(P, scatter_flag=scatter_flag)
+    vals = dict(P, scatter_flag=scatter_flag, **over)
     with open(path, "w") as f:
         for k in keys:
             f.write(f"--{k}\n  {vals[k]}\n")
@@ -179,3 +179,35 @@ def test_decon_driver(tmp_path):
     assert os.path.exists(os.path.join(d, "output", "end.flag"))
     txt = open(os.path.join(d, "output", "migration_coef.txt"), "rb").read().split(b"\r\t\n")
     assert len(txt) == nz + 1 and len(txt[0].split()) == nx
+
+
+def test_pipeline_afor_avsprtm_decon(tmp_path):
+    """The reference's pipeline in its directory layout: AFor and AVSPRTM for
+    the observed data (migration_pp.bin), again with scatter_flag 1 on a
+    7-cell scatterer lattice (migration_psf.bin), then Decon in a
+    kkdecon_acoustic-style directory on those two images; its result equals
+    the oracle's deconvolution of the same images bit for bit."""
+    tmp = str(tmp_path)
+    setup_dir(tmp)
+    print("case pipeline AFor -> AVSPRTM (pp, psf) -> Decon", flush=True)
+    for flag in (0, 1):
+        write_params(os.path.join(tmp, "parameter.txt"), flag, swin=7)
+        run("AFor", os.path.join(tmp, "forward"))
+        run("AVSPRTM", tmp)
+    nx, nz = P["nx"], P["nz"]
+    pp = read(tmp, "migration_pp.bin", (nx, nz))
+    psf = read(tmp, "migration_psf.bin", (nx, nz))
+    assert np.abs(pp).max() > 0 and np.abs(psf).max() > 0
+    kk = os.path.join(tmp, "kkdecon")
+    os.makedirs(os.path.join(kk, "output"))
+    vals = [nx, nz, 7, 7, 15, 15, 3, 3, "5.0e-4", "../input/acc_vp.bin", "../output/ini_vp.bin",
+            "../input/acc_vp.bin", "../output/migration_pp.bin", "../output/migration_psf.bin",
+            "./output/migration_dudecon.bin", "./output/inv_psf.bin", 0, "2.0e-2"]
+    with open(os.path.join(kk, "parameter.txt"), "w") as f:
+        for k, v in enumerate(vals):
+            f.write(f"--entry {k}\n  {v}\n")
+    run("Decon", kk)
+    dec = np.fromfile(os.path.join(kk, "output", "migration_dudecon.bin"), np.float32).reshape(nx, nz)
+    odec, _ = O.decon(O.DeconParams(nx, nz, 7, 7, 15, 15, 3, 3, 5e-4), pp, psf)
+    np.testing.assert_array_equal(dec, odec)
+    assert np.isfinite(dec).all() and np.abs(dec).max() > 0
