@@ -77,6 +77,23 @@ def cpu_baseline(case, nt_sample, threads, nshots=2):
     return cells / dt / 1e9, dt
 
 
+def copy_bandwidth(torch, dev, nbytes=1 << 30, reps=5):
+    """Achievable HBM bandwidth on this device: a device-to-device copy of 1 GiB
+    (read + write bytes over the best of a few HIP-event-timed copies)."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    return 2 * nbytes / (best * 1e-3) / 1e9
+
+
 def decon_window_flops(g):
     """FMAs x 2 of one window in k_decon_windows: the rows pass over the half
     spectrum, the columns pass over conjugate wavenumber pairs, the Wiener
@@ -227,6 +244,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    copy_gbs = copy_bandwidth(torch, dev)
+    log(f"bench: device copy bandwidth {copy_gbs:.0f} GB/s")
     for i in range(a.warmup):
         step()
         log(f"bench: warmup {i + 1}/{a.warmup} done")
@@ -318,6 +337,7 @@ def main():
                    "fwd_achieved_GBs": fwd_achieved, "image_finite": finite},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
                      "kernel": "k_fstep (fused RTM backward step: reconstruction + receiver + imaging)"
                                if a.path == "fused" else "RTM backward step (4 sub-step kernels)",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": bwd_launch_ms,
