@@ -104,6 +104,22 @@ int or_afor_f64(const or_grid *g, const float *vp, const float *rho, const or_co
                 const or_shot *s, const float *wavelet, const float *hwavelet,
                 const float *kzcoef, int nfftz, double *gather, double *up, double *down);
 
+/* Up/down-separated RTM images of one shot: two reconstructed source fields
+ * (wavelet and hwavelet, each with its final state and strips) and two
+ * receiver fields (adj and hadj), split along kz per interior column at every
+ * backward step; du/ud/uu/dd are [ntp].  Not in the reference's live code
+ * (FK:140-165, headrtm.h:50-51 declare it unused); defined here. */
+int or_rtm_updown_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                      const or_shot *s, const float *wavelet, const float *hwavelet,
+                      const float *final_state, const float *strips, const float *hfinal_state, const float *hstrips,
+                      const float *adj, const float *hadj, const float *kzcoef, int nfftz,
+                      float *du, float *ud, float *uu, float *dd);
+int or_rtm_updown_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                      const or_shot *s, const float *wavelet, const float *hwavelet,
+                      const double *final_state, const double *strips, const double *hfinal_state,
+                      const double *hstrips, const float *adj, const float *hadj, const float *kzcoef, int nfftz,
+                      double *du, double *ud, double *uu, double *dd);
+
 /* Born forward d = L m and its exact discrete adjoint m = L^T d (the linearised
  * CPML scheme with K -> K(1+m) on the interior, linearised in m at the background model). */
 int or_born_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,

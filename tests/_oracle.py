@@ -71,7 +71,7 @@ def lib():
         L.or_set_threads.argtypes = [C.c_int]
         for name in ("or_forward_f32", "or_forward_f64", "or_rtm_backward_f32", "or_rtm_backward_f64",
                      "or_afor_f32", "or_afor_f64", "or_born_f32", "or_born_f64",
-                     "or_born_adjoint_f32", "or_born_adjoint_f64"):
+                     "or_born_adjoint_f32", "or_born_adjoint_f64", "or_rtm_updown_f32", "or_rtm_updown_f64"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -114,13 +114,14 @@ class Problem:
     def _args(self):
         return (C.byref(self.g), ptr(self.vp), ptr(self.rho), C.byref(self.c), C.byref(self.shot))
 
-    def forward(self, dtype=np.float32, strips=False, final=False):
+    def forward(self, dtype=np.float32, strips=False, final=False, wavelet=None):
         L = lib()
+        w = self.wavelet if wavelet is None else np.ascontiguousarray(wavelet, np.float32)
         gather = np.zeros((self.nrec, self.nt), dtype)
         st = np.zeros((self.nt, self.frame_size()), dtype) if strips else None
         fs = np.zeros((3, self.ntp), dtype) if final else None
         fn = L.or_forward_f32 if dtype == np.float32 else L.or_forward_f64
-        rc = fn(*self._args(), ptr(self.wavelet), ptr(gather), ptr(st), ptr(fs))
+        rc = fn(*self._args(), ptr(w), ptr(gather), ptr(st), ptr(fs))
         assert rc == 0, rc
         return gather, st, fs
 
@@ -132,6 +133,21 @@ class Problem:
         fs = np.ascontiguousarray(final_state, dtype)
         st = np.ascontiguousarray(strips, dtype)
         rc = fn(*self._args(), ptr(self.wavelet), ptr(fs), ptr(st), ptr(adj), *[ptr(o) for o in out])
+        assert rc == 0, rc
+        return out
+
+    def rtm_updown(self, hwavelet, fwd, hfwd, adj, hadj, kzcoef, dtype=np.float32):
+        """Up/down-separated images (du, ud, uu, dd), each [ntp].  fwd and hfwd
+        are the (final_state, strips) of the forward runs with the wavelet and
+        with hwavelet, in dtype."""
+        L = lib()
+        nfftz = kzcoef.size
+        out = [np.zeros(self.ntp, dtype) for _ in range(4)]
+        fn = L.or_rtm_updown_f32 if dtype == np.float32 else L.or_rtm_updown_f64
+        arrs = [np.ascontiguousarray(a, dtype) for a in (fwd[0], fwd[1], hfwd[0], hfwd[1])]
+        f32 = [np.ascontiguousarray(a, np.float32) for a in (hwavelet, adj, hadj, kzcoef)]
+        rc = fn(*self._args(), ptr(self.wavelet), ptr(f32[0]), *[ptr(a) for a in arrs], ptr(f32[1]), ptr(f32[2]),
+                ptr(f32[3]), nfftz, *[ptr(o) for o in out])
         assert rc == 0, rc
         return out
 
