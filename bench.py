@@ -179,6 +179,137 @@ def bench_decon(a):
     print(json.dumps(out), flush=True)
 
 
+def updown_kernel_bytes(g, nshots, nimg):
+    """Algorithmic HBM bytes of one k_updown launch: per interior column of a
+    shot the four field columns are read (4 x ntz floats) and nimg images are
+    read and written on the interior rows."""
+    return nshots * g.nx * (16 * g.ntz + 8 * nimg * g.nz)
+
+
+def bench_updown(a):
+    """Up/down-separated RTM of the c2 batch: one step = forward of p and its
+    time-Hilbert companion with strips, the paired backward of both, and the
+    kz split + cross-correlation of every interior column after every backward
+    step (du, ud images), with the adjoint sources and their Hilbert
+    transforms resident in HBM."""
+    import torch
+    import swd
+
+    log(f"bench: c2 up/down RTM, batch {a.batch} shots, nt {a.nt}")
+    case = build_case(a.batch, a.nt)
+    g = case.grid
+    plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0, tile=a.tile)
+    plan.set_groups(*a.groups)
+    plan.set_streams(a.streams)
+    shots = swd.ShotSet(case.shots)
+    plan_true = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0)
+    obs = plan_true.forward(shots, case.wavelet)
+    plan_true.close()
+    adj = swd.adjoint_source(obs, g.dt)
+    t0 = time.perf_counter()
+    hadj = swd.hilbert_traces(adj)
+    log(f"bench: Hilbert transform of {adj.shape[0]} adjoint traces on the host in {time.perf_counter() - t0:.1f}s")
+    hw = swd.hilbert1d(case.wavelet)
+    nfftz = swd.nfft(g.ntz)
+    kz, _ = swd.kz_taper(nfftz, g.dz)
+    dev = torch.device("cuda:0")
+    ntp = g.ntx * g.ntz
+    d_adj, d_hadj = torch.from_numpy(adj).to(dev), torch.from_numpy(hadj).to(dev)
+    d_img = torch.zeros((2, a.batch, ntp), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        plan.rtm_updown_dev(shots, case.wavelet, hw, d_adj.data_ptr(), d_hadj.data_ptr(), kz,
+                            {"du": d_img[0].data_ptr(), "ud": d_img[1].data_ptr()})
+
+    copy_gbs = copy_bandwidth(torch, dev)
+    for i in range(a.warmup):
+        step()
+        log(f"bench: warmup {i + 1}/{a.warmup} done")
+    torch.cuda.synchronize()
+    fwd_ms = bwd_ms = 0.0
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        step()
+        st = plan.stats()
+        fwd_ms += st["fwd_ms"]
+        bwd_ms += st["bwd_ms"]
+        log(f"bench: step {i + 1}/{a.steps} fwd {st['fwd_ms']:.1f} ms bwd {st['bwd_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    shots_per_s = a.batch * a.steps / elapsed
+    cells = 2 * rtm_cells(g) * a.batch
+    du = d_img[0].cpu().numpy()
+
+    # the plain RTM of the same batch, for the cost of the separation
+    rtm = None
+    if not a.no_plain:
+        im = torch.zeros((3, a.batch, ntp), dtype=torch.float32, device=dev)
+        plan.rtm_dev(shots, case.wavelet, d_adj.data_ptr(), im[0].data_ptr(), im[1].data_ptr(), im[2].data_ptr())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plan.rtm_dev(shots, case.wavelet, d_adj.data_ptr(), im[0].data_ptr(), im[1].data_ptr(), im[2].data_ptr())
+        torch.cuda.synchronize()
+        tr = time.perf_counter() - t1
+        rtm = {"shots_per_s": a.batch / tr, "updown_cost_vs_rtm": (a.batch / tr) / shots_per_s}
+        del im
+
+    # k_updown alone: one imaging step of n_k c2 shots on the plan's stream, HIP events around 40 launches
+    n_k = 4
+    rng = np.random.default_rng(1)
+    f = torch.from_numpy(rng.standard_normal((4, n_k, ntp)).astype(np.float32)).to(dev)
+    im = torch.zeros((2, n_k, ntp), dtype=torch.float32, device=dev)
+    ptrs = {"du": im[0].data_ptr(), "ud": im[1].data_ptr()}
+    args = (n_k, f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr(), f[3].data_ptr(), kz, ptrs)
+    torch.cuda.synchronize()
+    ps = torch.cuda.ExternalStream(plan.stream())
+    reps = 40
+    with torch.cuda.stream(ps):
+        for _ in range(3):
+            plan.updown_image_dev(*args)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            plan.updown_image_dev(*args)
+        e1.record()
+    e1.synchronize()
+    k_ms = e0.elapsed_time(e1) / reps
+    k_bytes = updown_kernel_bytes(g, n_k, 2)
+    k_flops = n_k * g.nx * 4 * 5 * nfftz * int(np.log2(nfftz))
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    del f, im
+
+    out = {
+        "metric": "up/down-separated RTM Gcell-updates/s (c2 batch, du and ud images)",
+        "value": cells * a.steps / elapsed / 1e9,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered model with dipping reflectors, c2 geometry)",
+        "config": {"workload": "c2 batched up/down-separated VSP RTM: 502x845 interior (590x933 padded), nt "
+                               f"{g.nt}, nfftz {nfftz}, p and Hilbert-companion wavefields on both sides, "
+                               "kz split of every interior column after every backward step",
+                   "shots_per_step": a.batch, "shots_per_s": shots_per_s, "plain_rtm": rtm,
+                   "fwd_ms_per_step": fwd_ms / a.steps, "bwd_ms_per_step": bwd_ms / a.steps,
+                   "du_finite": bool(np.isfinite(du).all())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "copy_GBs": copy_gbs,
+                     "kernel": f"k_updown (kz split + imaging of one backward step, {n_k} c2 shots, "
+                               f"{g.nx} columns each, nfftz {nfftz})",
+                     "bytes_per_launch": k_bytes, "avg_launch_ms": k_ms,
+                     "fft_gflops": k_flops / (k_ms * 1e-3) / 1e9,
+                     "timing": "HIP events on the plan's stream around 40 back-to-back launches"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,8 +328,9 @@ def main():
                     help="also copy 256 MiB device-to-device once (reference for HBM byte counters)")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2, 3, 4),
                     help="RTM shot groups in flight at once (swd_plan_set_streams)")
-    ap.add_argument("--workload", choices=("rtm", "decon"), default="rtm",
-                    help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution")
+    ap.add_argument("--workload", choices=("rtm", "decon", "updown"), default="rtm",
+                    help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution; "
+                         "updown: the up/down-separated RTM")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU shot sharding is "
@@ -207,6 +339,9 @@ def main():
 
     if a.workload == "decon":
         bench_decon(a)
+        return
+    if a.workload == "updown":
+        bench_updown(a)
         return
 
     import torch

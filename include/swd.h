@@ -158,6 +158,9 @@ int swd_plan_set_graphs(void *plan, int on);
  * batches in consecutive waves of shots that fit; results are the same. */
 int swd_plan_set_memory_limit(void *plan, size_t bytes);
 int swd_plan_get_stats(void *plan, swd_stats *st);
+/* The plan's HIP stream (a hipStream_t), for callers that order their own
+ * work or timing events with the plan's queued work. */
+int swd_plan_get_stream(void *plan, void **stream);
 
 /* ---------------- operators ----------------
  * nshots shots of one plan run as one batch; wavelets are shared [nt];
@@ -185,6 +188,44 @@ int swd_rtm(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
  * plan's device; gather_syn may be NULL). */
 int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
                 const float *adj, float *image, float *illum_src, float *illum_rcv, float *gather_syn);
+
+/* Up/down-separated RTM (the image_ppdu/ppud and d_image_ppdu/ppud fields of
+ * forward/headrtm.h:51,119, whose separation kernel FK:140-165 the reference
+ * never launches).
+ * The source side propagates the wavelet and hwavelet (its Hilbert transform
+ * along time), the receiver side adj and hadj (the adjoint source and its
+ * Hilbert transform along time, [sum nrec][nt] each).  At every backward step
+ * the analytic columns S = dp/dt + i dp_h/dt (source, zero off the interior)
+ * and R = p + i p_h (receiver) of each interior column are zero padded to
+ * nfftz, transformed along z, weighted by kzcoef and split as in swd_afor
+ * (bins 0..nfftz/2 "up", the rest "down"), transformed back and divided by
+ * nfftz; the real parts are cross-correlated on the interior:
+ *   du += Sd Ru, ud += Su Rd, uu += Su Ru, dd += Sd Rd.
+ * With kzcoef all ones du+ud+uu+dd is the plain image of swd_rtm.
+ * nfftz is a power of two in [max(ntz, 128), 4096].  Runs on the fused kernels for
+ * every path.  Outputs are [nshots][ntx*ntz]; du and ud are required, uu and
+ * dd optional together, and image/illum_src/illum_rcv (the plain RTM outputs
+ * of the same run) optional together. */
+typedef struct swd_updown_images {
+    float *du, *ud, *uu, *dd;
+    float *image, *illum_src, *illum_rcv;
+} swd_updown_images;
+int swd_rtm_updown(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *hwavelet,
+                   const float *adj, const float *hadj, const float *kzcoef, int nfftz, const swd_updown_images *out);
+/* Same with adj, hadj and the outputs in device memory of the plan's device. */
+int swd_rtm_updown_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *hwavelet,
+                       const float *adj, const float *hadj, const float *kzcoef, int nfftz,
+                       const swd_updown_images *out);
+
+/* One step of that imaging condition on caller-held wavefields, all device
+ * pointers [nshots][ntx*ntz]: src/hsrc are the two source fields (dp/dt of
+ * the wavelet's and of hwavelet's reconstruction, zero off the interior),
+ * rcv/hrcv the two receiver pressures; du, ud (and uu, dd) accumulate on the
+ * interior.  Queued on the plan's stream without waiting (no reference
+ * counterpart: the per-step building block of swd_rtm_updown for callers
+ * running their own time loop). */
+int swd_updown_image_dev(void *plan, int nshots, const float *src, const float *hsrc, const float *rcv,
+                         const float *hrcv, const float *kzcoef, int nfftz, const swd_updown_images *out);
 
 /* Born modelling d = L m, m = dK/K on the interior [nx][nz] (no reference
  * counterpart: linearisation of the RK:24-221 scheme), and its exact discrete

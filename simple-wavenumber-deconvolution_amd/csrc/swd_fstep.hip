@@ -384,6 +384,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                     el(dp, k) = in ? elc(K0, k) * S : 0.f;
                 }
                 dpa[r] = dp;
+                if (I.dpdt) st4((gfl *)I.dpdt, o, dp);   // for the up/down imaging that follows the step
             }
             if (hassrc) add_at(pv, I.s_iz - z, -ldg(I.wavelet, it + 1));   // RK:478-481
             if (!FULL) {

@@ -65,6 +65,20 @@ struct KStep {
     const int *colcap;
     int capcomp, ncap, nfftz;
     float *image, *illum_s, *illum_r;   // RCV (paired with the SRC of the same shot)
+    float *dpdt;                 // SRC: dp/dt of step it stored on the interior [ntx][ldz], or null
+};
+
+// One shot of the up/down imaging that follows a backward step: the two
+// reconstructions' dp/dt (the wavelet's and the Hilbert-transformed wavelet's)
+// and the two receiver pressures (adjoint source and its Hilbert transform)
+// are split along kz per interior column and cross-correlated.
+struct KUpDown {
+    const float *ds, *dh;        // dp/dt of the two reconstructions [ntx][ld], zero off the interior
+    const float *pr[2], *ph[2];  // receiver pressures, both slots [ntx][ld]
+    float *du, *ud, *uu, *dd;    // images [ntx][ld] (interior cells); uu and dd may be null
+    const float2 *tw;            // e^{-2 pi i k / nfftz}, k < nfftz/2
+    const float *coef;           // kz taper [nfftz]
+    int ld;                      // column stride (floats) of all the arrays above
 };
 
 // Column descriptor of the kz separation: one receiver column of one shot.

@@ -699,3 +699,4 @@ hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2
                        up, down, nt);
     return hipGetLastError();
 }
+

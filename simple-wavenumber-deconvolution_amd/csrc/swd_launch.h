@@ -21,3 +21,5 @@ hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, b
 hipError_t launch_fstep_multi(int Lc, const KStep *insts, const int2 *btab, int nblk, int it, hipStream_t s);
 hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2 *tw, const float *coef,
                         const KCapCol *cols, const int2 *traces, float *up, float *down, int nt, hipStream_t s);
+// up/down imaging of step it for nshot shots (KUpDown each) over the interior columns
+hipError_t launch_updown(const KGrid &g, const KUpDown *shots, int nshot, int nfftz, int it, hipStream_t s);

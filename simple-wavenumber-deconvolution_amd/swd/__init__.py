@@ -72,6 +72,12 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class UpDownImages(C.Structure):
+    """swd_updown_images: output pointers of swd_rtm_updown (NULL = not wanted)."""
+    NAMES = ("du", "ud", "uu", "dd", "image", "illum_src", "illum_rcv")
+    _fields_ = [(k, C.c_void_p) for k in NAMES]
+
+
 _lib = None
 
 EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd_cal_xishu", "swd_ricker",
@@ -80,7 +86,8 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
            "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
            "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor",
-           "swd_rtm", "swd_rtm_dev", "swd_born", "swd_born_adjoint", "swd_born_dev",
+           "swd_rtm", "swd_rtm_dev", "swd_rtm_updown", "swd_rtm_updown_dev", "swd_updown_image_dev",
+           "swd_plan_get_stream", "swd_born", "swd_born_adjoint", "swd_born_dev",
            "swd_born_adjoint_dev", "swd_multi_forward", "swd_decon_read_parameters", "swd_decon_geometry",
            "swd_decon_taper", "swd_decon_plan_create", "swd_decon_plan_destroy", "swd_decon", "swd_decon_dev",
            "swd_decon_get_stats")
@@ -92,6 +99,13 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libswd.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        # PyTorch ships its own libamdhip64.so.7 and links it by another name:
+        # loaded after ours it would be a second HIP runtime that finds no
+        # device.  Loaded first, libswd binds to it and both share one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int
@@ -199,6 +213,13 @@ def hilbert1d(rick):
     h = np.zeros_like(r)
     check(load().swd_hilbert1d(_p(r), _p(h), r.size))
     return h
+
+
+def hilbert_traces(d):
+    """Hilbert transform along time of every trace of d [..., nt] (swd_hilbert1d per trace)."""
+    d = f32(d)
+    rows = d.reshape(-1, d.shape[-1])
+    return np.stack([hilbert1d(r) for r in rows]).reshape(d.shape)
 
 
 def geometry(nx, nz, pml, Lc, dx, dz, sx0, shotdx, shotdep, offsetz, is_):
@@ -356,6 +377,47 @@ class Plan:
         vp = lambda x: None if x is None else C.c_void_p(int(x))
         check(self.L.swd_rtm_dev(self.h, S.n, S.arr, _p(f32(wavelet)), vp(adj_ptr), vp(image_ptr),
                                  vp(illum_s_ptr), vp(illum_r_ptr), vp(gather_ptr)))
+
+    def rtm_updown(self, shots, wavelet, hwavelet, adj, hadj, kzcoef, want_same_dir=False, want_image=False):
+        """Up/down-separated RTM images (swd_rtm_updown): returns a dict of
+        [nshots][ntx*ntz] arrays du, ud, and with want_same_dir uu, dd, with
+        want_image the plain image, illum_src and illum_rcv of the same run.
+        hwavelet and hadj are the Hilbert transforms along time of wavelet and adj."""
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        ntp = self.grid.ntx * self.grid.ntz
+        names = ["du", "ud"] + (["uu", "dd"] if want_same_dir else []) + (
+            ["image", "illum_src", "illum_rcv"] if want_image else [])
+        outs = {k: np.zeros((S.n, ntp), np.float32) for k in names}
+        imgs = UpDownImages(*[C.c_void_p(outs[k].ctypes.data) if k in outs else None for k in UpDownImages.NAMES])
+        kz = f32(kzcoef)
+        check(self.L.swd_rtm_updown(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(hwavelet)), _p(f32(adj)),
+                                    _p(f32(hadj)), _p(kz), kz.size, C.byref(imgs)))
+        return outs
+
+    def rtm_updown_dev(self, shots, wavelet, hwavelet, adj_ptr, hadj_ptr, kzcoef, out_ptrs):
+        """Device-pointer up/down RTM; out_ptrs maps output names (UpDownImages.NAMES)
+        to device pointers (integers, e.g. torch tensor data_ptr())."""
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        imgs = UpDownImages(*[C.c_void_p(int(out_ptrs[k])) if out_ptrs.get(k) else None for k in UpDownImages.NAMES])
+        kz = f32(kzcoef)
+        check(self.L.swd_rtm_updown_dev(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(hwavelet)),
+                                        C.c_void_p(int(adj_ptr)), C.c_void_p(int(hadj_ptr)), _p(kz), kz.size,
+                                        C.byref(imgs)))
+
+    def updown_image_dev(self, nshots, src_ptr, hsrc_ptr, rcv_ptr, hrcv_ptr, kzcoef, out_ptrs):
+        """One step of the up/down imaging condition on device fields [nshots][ntx*ntz]
+        (swd_updown_image_dev); queued on the plan's stream."""
+        imgs = UpDownImages(*[C.c_void_p(int(out_ptrs[k])) if out_ptrs.get(k) else None for k in UpDownImages.NAMES])
+        kz = f32(kzcoef)
+        vp = lambda x: C.c_void_p(int(x))
+        check(self.L.swd_updown_image_dev(self.h, int(nshots), vp(src_ptr), vp(hsrc_ptr), vp(rcv_ptr), vp(hrcv_ptr),
+                                          _p(kz), kz.size, C.byref(imgs)))
+
+    def stream(self):
+        """The plan's hipStream_t as an integer (e.g. for torch.cuda.ExternalStream)."""
+        s = C.c_void_p()
+        check(self.L.swd_plan_get_stream(self.h, C.byref(s)))
+        return s.value or 0
 
     def born(self, shots, wavelet, m):
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
