@@ -108,7 +108,7 @@ int or_afor_f64(const or_grid *g, const float *vp, const float *rho, const or_co
  * (wavelet and hwavelet, each with its final state and strips) and two
  * receiver fields (adj and hadj), split along kz per interior column at every
  * backward step; du/ud/uu/dd are [ntp].  Not in the reference's live code
- * (FK:140-165, headrtm.h:50-51 declare it unused); defined here. */
+ * (FK:140-165 and forward/headrtm.h:51,119 declare it unused); defined here. */
 int or_rtm_updown_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
                       const or_shot *s, const float *wavelet, const float *hwavelet,
                       const float *final_state, const float *strips, const float *hfinal_state, const float *hstrips,

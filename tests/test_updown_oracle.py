@@ -1,7 +1,7 @@
 """Pins the oracle of the up/down-separated RTM images (or_rtm_updown).
 
 The reference declares this imaging without running it (image_ppdu/ppud in
-forward/headrtm.h:50-51, the unused wavefield_seperation_kernel FK:140-165),
+forward/headrtm.h:51,119, the unused wavefield_seperation_kernel FK:140-165),
 so no reference output exists.  The restatement is pinned by what its
 definition fixes:
   - completeness: with a taper of ones, du + ud + uu + dd is the plain RTM
