@@ -5,7 +5,11 @@
 // ./input/acc_vp.bin and the up-going gathers AFor wrote to ./output/
 // (<n>source_seismogram_obsup.bin, or ..._scatterup.bin when scatter_flag is 1),
 // migrates them with the smoothed model and writes the per-shot images and the
-// summed, Laplacian-filtered, edge-zeroed migration_{pp|psf}.bin.
+// summed, Laplacian-filtered, edge-zeroed migration_{pp|psf}.bin.  With
+// --updown the same run also images the up/down-separated wavefields
+// (swd_rtm_updown) and writes migration_{pp|psf}du.bin and ..ud.bin
+// (down-going source with up-going receiver field, and the reverse),
+// normalised, filtered and summed like the plain image.
 #include <cmath>
 #include <algorithm>
 #include <cstdio>
@@ -50,6 +54,17 @@ int main(int argc, char **argv)
     std::printf("====================\n %s RTM BEGIN\n====================\n", p.scatter_flag ? "PSF" : "NORMAL");
 
     std::vector<float> tmp1(s.np, 0.f), tmp3(s.np);
+    // up/down imaging: the Hilbert-transformed wavelet and kz taper (FK:1257-1324, FC:354-377)
+    std::vector<float> hrick(nt), kz, accdu, accud;
+    int nfftz = 0;
+    if (o.updown) {
+        check(swd_hilbert1d(rick.data(), hrick.data(), nt), "hilbert");
+        nfftz = std::max(128, swd_nfft(s.ntz));
+        kz.resize(nfftz);
+        check(swd_kz_taper(kz.data(), nfftz, p.dz), "kz taper");
+        accdu.assign(s.np, 0.f);
+        accud.assign(s.np, 0.f);
+    }
     const double t0 = drv::wall_seconds();
     for (size_t b0 = 0; b0 < s.shots.size(); b0 += o.batch) {
         const size_t b1 = std::min(s.shots.size(), b0 + static_cast<size_t>(o.batch));
@@ -75,8 +90,26 @@ int main(int argc, char **argv)
                   "adjoint source");
         }
         std::vector<float> img(static_cast<size_t>(s.ntp) * nb), ils(img.size()), ilr(img.size()), syn(ntr * nt);
-        check(swd_rtm(plan, nb, sh.data(), rick.data(), adj.data(), img.data(), ils.data(), ilr.data(), syn.data()),
-              "RTM");
+        std::vector<float> du, ud;
+        if (o.updown) {
+            // adjoint sources' Hilbert transforms along time, trace by trace
+            std::vector<float> hadj(adj.size());
+            for (size_t tr = 0; tr < ntr; tr++)
+                check(swd_hilbert1d(adj.data() + tr * nt, hadj.data() + tr * nt, nt), "hilbert");
+            du.resize(img.size());
+            ud.resize(img.size());
+            const swd_updown_images out = {du.data(), ud.data(), nullptr, nullptr, img.data(), ils.data(), ilr.data()};
+            check(swd_rtm_updown(plan, nb, sh.data(), rick.data(), hrick.data(), adj.data(), hadj.data(), kz.data(),
+                                 nfftz, &out),
+                  "up/down RTM");
+            bool diag = false;   // the synthetic gathers are only written for every 20th shot
+            for (size_t k = b0; k < b1; k++) diag |= k % 20 == 0;
+            if (diag) check(swd_forward(plan, nb, sh.data(), rick.data(), syn.data()), "forward");
+        } else {
+            check(swd_rtm(plan, nb, sh.data(), rick.data(), adj.data(), img.data(), ils.data(), ilr.data(),
+                          syn.data()),
+                  "RTM");
+        }
         for (size_t k = b0; k < b1; k++) {
             const size_t n = s.shots[k].rec_ix.size();
             const std::string id = std::to_string(k + 1);
@@ -92,11 +125,27 @@ int main(int argc, char **argv)
             check(swd_image_laplacian(I, vp.data(), tmp3.data(), tmp1.data(), s.ntx, s.ntz, s.pmlc, p.dx, p.dz),
                   "laplacian");   // RC:722-735
             drv::write_floats("./output/fimage_" + std::string(kind) + id + ".bin", tmp3.data(), tmp3.size());
+            if (o.updown) {
+                float *D = du.data() + static_cast<size_t>(s.ntp) * (k - b0);
+                float *U = ud.data() + static_cast<size_t>(s.ntp) * (k - b0);
+                check(swd_image_normalize(D, S, s.ntp), "normalize");
+                check(swd_image_normalize(U, S, s.ntp), "normalize");
+                check(swd_image_laplacian(D, vp.data(), tmp3.data(), accdu.data(), s.ntx, s.ntz, s.pmlc, p.dx, p.dz),
+                      "laplacian");
+                check(swd_image_laplacian(U, vp.data(), tmp3.data(), accud.data(), s.ntx, s.ntz, s.pmlc, p.dx, p.dz),
+                      "laplacian");
+            }
         }
         std::printf("shots %zu..%zu migrated\n", b0 + 1, b1);
     }
     check(swd_edge_zero(tmp1.data(), p.nx, p.nz, p.Lc), "edge zero");   // RC:754-768
     drv::write_floats(std::string("./output/migration_") + kind + ".bin", tmp1.data(), tmp1.size());
+    if (o.updown) {
+        check(swd_edge_zero(accdu.data(), p.nx, p.nz, p.Lc), "edge zero");
+        check(swd_edge_zero(accud.data(), p.nx, p.nz, p.Lc), "edge zero");
+        drv::write_floats(std::string("./output/migration_") + kind + "du.bin", accdu.data(), accdu.size());
+        drv::write_floats(std::string("./output/migration_") + kind + "ud.bin", accud.data(), accud.size());
+    }
     swd_plan_destroy(plan);
     std::printf("====================\n      THE END\n====================\n");
     std::printf("The cost of the run time is %f seconds\n", drv::wall_seconds() - t0);

@@ -44,9 +44,11 @@ Options parse_options(int argc, char **argv, const char *prog)
         else if (a == "--plain") o.path = SWD_PATH_PLAIN;
         else if (a == "--batched") o.path = SWD_PATH_BATCHED;
         else if (a == "--tile") o.tile = std::atoi(next());
+        else if (a == "--updown") o.updown = true;
         else if (a == "-C") o.workdir = next();
         else if (a == "-h" || a == "--help") {
-            std::printf("usage: %s [-C dir] [--batch N] [--device D] [--plain | --batched] [--tile 0|16|32|64]\n", prog);
+            std::printf("usage: %s [-C dir] [--batch N] [--device D] [--plain | --batched] [--tile 0|16|32|64]"
+                        " [--updown]\n", prog);
             std::exit(0);
         } else die("%s: unknown option %s", prog, a.c_str());
     }

@@ -13,6 +13,7 @@ struct Options {
     int device = 0;
     int path = SWD_PATH_FUSED;
     int tile = 0;            // fused tile width (0 = tuned)
+    bool updown = false;     // AVSPRTM: also the up/down-separated images (swd_rtm_updown)
     std::string workdir;     // run as if started in this directory
 };
 Options parse_options(int argc, char **argv, const char *prog);

@@ -42,8 +42,8 @@ def setup_dir(tmp):
     return v
 
 
-def run(exe, cwd):
-    r = subprocess.run([os.path.join(BIN, exe)], cwd=cwd, capture_output=True, text=True, timeout=120)
+def run(exe, cwd, *args):
+    r = subprocess.run([os.path.join(BIN, exe), *args], cwd=cwd, capture_output=True, text=True, timeout=120)
     print(r.stdout[-600:], r.stderr[-600:], flush=True)
     assert r.returncode == 0, r.stderr
 
@@ -111,6 +111,46 @@ def test_afor_and_avsprtm_observed(tmp_path):
     e = rel_l2(read(tmp, "migration_pp.bin"), mig)
     print(f"migration_pp: relL2 {e:.2e}")
     assert e <= TOL
+
+
+def test_avsprtm_updown(tmp_path):
+    """AVSPRTM --updown: the plain migration as without the option, and the
+    up/down-separated migrations (du: down-going source with up-going receiver
+    field, ud: the reverse) through the same normalisation, filter, sum and
+    edge zeroing, against the oracle (or_rtm_updown) pipeline."""
+    tmp = str(tmp_path)
+    v = setup_dir(tmp)
+    write_params(os.path.join(tmp, "parameter.txt"), 0)
+    run("AFor", os.path.join(tmp, "forward"))
+    run("AVSPRTM", tmp, "--updown")
+    probs_m, vpm, _, (ntx, ntz, pmlc) = oracle_case(v, smooth=True)
+    nfftz = O.nfft(ntz)
+    kz, _ = O.kz_taper(nfftz, P["dz"])
+    hw = O.hilbert1d(probs_m[0].wavelet)
+    migs = {k: np.zeros(P["nx"] * P["nz"], np.float32) for k in ("pp", "ppdu", "ppud")}
+    for is_, pr in enumerate(probs_m):
+        n = pr.nrec
+        up = read(tmp, f"{is_ + 1}source_seismogram_obsup.bin", (n, P["itmax"]))
+        adj = O.adjoint_source(up, P["dt"])
+        hadj = np.stack([O.hilbert1d(r) for r in adj])
+        _, st, fs = pr.forward(strips=True, final=True)
+        _, hst, hfs = pr.forward(strips=True, final=True, wavelet=hw)
+        img, ils, _ = pr.rtm_backward(fs, st, adj)
+        du, ud, _, _ = pr.rtm_updown(hw, (fs, st), (hfs, hst), adj, hadj, kz)
+        for name, a in (("pp", img), ("ppdu", du), ("ppud", ud)):
+            a = O.image_normalize(a, ils)
+            O.image_laplacian(a, vpm, ntx, ntz, pmlc, P["dx"], P["dz"], acc=migs[name])
+    ref = {k: O.edge_zero(m, P["nx"], P["nz"], P["Lc"]) for k, m in migs.items()}
+    got = {k: read(tmp, f"migration_{k}.bin") for k in ref}
+    e = rel_l2(got["pp"], ref["pp"])
+    print(f"migration_pp: relL2 {e:.2e}")
+    assert e <= TOL
+    scale = np.linalg.norm(ref["ppdu"])
+    for k in ("ppdu", "ppud"):
+        e = np.linalg.norm(got[k].astype(np.float64) - ref[k]) / scale
+        print(f"migration_{k}: |got - oracle| / |du| {e:.2e}, relL2 {rel_l2(got[k], ref[k]):.2e}, "
+              f"norm ratio to du {np.linalg.norm(ref[k]) / scale:.3f}")
+        assert e <= TOL
 
 
 def test_afor_and_avsprtm_psf(tmp_path):
