@@ -234,7 +234,7 @@ __device__ __forceinline__ void first_pass(float2 (&v)[16], int t, float2 *L)
 template <int LOGN>
 __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const KUpDown *__restrict__ shots, int it)
 {
-    constexpr int N = 1 << LOGN, T = N / 16, NP = N + N / 16, H = N / 2;
+    constexpr int N = 1 << LOGN, T = N / 16, NP = N + N / 16;
     __shared__ float2 lds[2 * NP];
     const int col = g.pmlc + blockIdx.x;
     const KUpDown &U = shots[blockIdx.y];
@@ -260,19 +260,19 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const 
     const float sc = 0.5f / static_cast<float>(N);   // a power of two: exact
 #pragma unroll
     for (int r = 0; r < 16; r++) {
+        // k = t + r N/16 lies below N/2 exactly when r < 8
         const int k = t + r * T, m = (N - k) & (N - 1);
         const float2 xk = L[pad(k)], xm = L[pad(m)];
         const float ck = U.coef[k] * sc, cm = U.coef[m] * sc;
         float2 y;
-        if (k == 0 || k == H) {
-            y = make_float2(2.f * ck * xk.x, 0.f);   // self-paired DC and Nyquist bins are "up"
-        } else if (k < H) {
+        if (r < 8) {
             const float2 u = make_float2(ck * xk.x, ck * xk.y), d = make_float2(cm * xm.x, cm * xm.y);
             y = make_float2(u.x + d.y, u.y + d.x);   // c_k X_k + i conj(c_m X_m)
         } else {
             const float2 u = make_float2(cm * xm.x, cm * xm.y), d = make_float2(ck * xk.x, ck * xk.y);
             y = make_float2(u.x - d.y, d.x - u.y);   // conj(c_m X_m) + i c_k X_k
         }
+        if ((r == 0 || r == 8) && t == 0) y = make_float2(2.f * ck * xk.x, 0.f);   // DC and Nyquist are "up"
         v[r] = conjf2(y);
     }
     __syncthreads();   // the spectrum is read before the inverse's first pass overwrites it
