@@ -352,15 +352,19 @@ void or_edge_zero(float *mig, int nx, int nz, int Lc)
 #define REAL float
 #define SFX f32
 #define FMA fmaf
+#define CEILR ceilf
 #include "swd_oracle_prop.inc"
 #undef REAL
 #undef SFX
 #undef FMA
+#undef CEILR
 
 #define REAL double
 #define SFX f64
 #define FMA fma
+#define CEILR ceil
 #include "swd_oracle_prop.inc"
 #undef REAL
 #undef SFX
 #undef FMA
+#undef CEILR

@@ -120,6 +120,28 @@ int or_rtm_updown_f64(const or_grid *g, const float *vp, const float *rho, const
                       const double *hstrips, const float *adj, const float *hadj, const float *kzcoef, int nfftz,
                       double *du, double *ud, double *uu, double *dd);
 
+/* RTM with angle-domain common-image gathers (sum_image_GPU_kernel_imageCIG,
+ * RK:819-905, launched nowhere in the reference: RK:1149-1157 is commented out).
+ * Same as or_rtm_backward plus cig [ntp][thetan]: Poynting-vector opening
+ * half-angles binned in degrees, delt-step time averaging of the vector products. */
+int or_rtm_cig_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                   const or_shot *s, const float *wavelet, const float *final_state,
+                   const float *strips, const float *adj, int thetan, int delt,
+                   float *image, float *illum_s, float *illum_r, float *cig);
+int or_rtm_cig_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                   const or_shot *s, const float *wavelet, const double *final_state,
+                   const double *strips, const float *adj, int thetan, int delt,
+                   double *image, double *illum_s, double *illum_r, double *cig);
+
+/* The angle-gather update of one step on n independent cells (flat arrays of
+ * the six fields; rings [delt][n]; cig [n][thetan]): the per-cell part of
+ * or_rtm_cig, exposed for tests of the binning. */
+void or_cig_cells_f32(int n, const float *ps, const float *vxs, const float *vzs, const float *pr, const float *vxr,
+                      const float *vzr, int it, int nt, int delt, int thetan, float *ringF, float *ringG, float *cig);
+void or_cig_cells_f64(int n, const double *ps, const double *vxs, const double *vzs, const double *pr,
+                      const double *vxr, const double *vzr, int it, int nt, int delt, int thetan, double *ringF,
+                      double *ringG, double *cig);
+
 /* Born forward d = L m and its exact discrete adjoint m = L^T d (the linearised
  * CPML scheme with K -> K(1+m) on the interior, linearised in m at the background model). */
 int or_born_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,

@@ -71,7 +71,8 @@ def lib():
         L.or_set_threads.argtypes = [C.c_int]
         for name in ("or_forward_f32", "or_forward_f64", "or_rtm_backward_f32", "or_rtm_backward_f64",
                      "or_afor_f32", "or_afor_f64", "or_born_f32", "or_born_f64",
-                     "or_born_adjoint_f32", "or_born_adjoint_f64", "or_rtm_updown_f32", "or_rtm_updown_f64"):
+                     "or_born_adjoint_f32", "or_born_adjoint_f64", "or_rtm_updown_f32", "or_rtm_updown_f64",
+                     "or_rtm_cig_f32", "or_rtm_cig_f64"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -135,6 +136,20 @@ class Problem:
         rc = fn(*self._args(), ptr(self.wavelet), ptr(fs), ptr(st), ptr(adj), *[ptr(o) for o in out])
         assert rc == 0, rc
         return out
+
+    def rtm_cig(self, final_state, strips, adj, thetan=10, delt=1, dtype=np.float32):
+        """RTM with angle gathers: image, illum_s, illum_r [ntp] and cig [ntp][thetan]."""
+        L = lib()
+        adj = np.ascontiguousarray(adj, np.float32)
+        out = [np.zeros(self.ntp, dtype) for _ in range(3)]
+        cig = np.zeros((self.ntp, thetan), dtype)
+        fn = L.or_rtm_cig_f32 if dtype == np.float32 else L.or_rtm_cig_f64
+        fs = np.ascontiguousarray(final_state, dtype)
+        st = np.ascontiguousarray(strips, dtype)
+        rc = fn(*self._args(), ptr(self.wavelet), ptr(fs), ptr(st), ptr(adj), int(thetan), int(delt),
+                *[ptr(o) for o in out], ptr(cig))
+        assert rc == 0, rc
+        return (*out, cig)
 
     def rtm_updown(self, hwavelet, fwd, hfwd, adj, hadj, kzcoef, dtype=np.float32):
         """Up/down-separated images (du, ud, uu, dd), each [ntp].  fwd and hfwd
@@ -215,6 +230,18 @@ def model_in_pml(vp_padded, ntz, ntx, pmlc):
     v = np.ascontiguousarray(vp_padded, np.float32).reshape(-1).copy()
     lib().or_model_in_pml(v, ntz, ntx, pmlc)
     return v
+
+
+def cig_cells(ps, vxs, vzs, pr, vxr, vzr, it, nt, delt, thetan, ringF, ringG, cig, dtype=np.float32):
+    """One step of the angle-gather update on n cells (or_cig_cells); rings
+    [delt][n] and cig [n][thetan] are updated in place (arrays of dtype)."""
+    L = lib()
+    fn = L.or_cig_cells_f32 if dtype == np.float32 else L.or_cig_cells_f64
+    f = [np.ascontiguousarray(a, dtype) for a in (ps, vxs, vzs, pr, vxr, vzr)]
+    for a in (ringF, ringG, cig):
+        assert a.dtype == dtype and a.flags.c_contiguous
+    fn(int(f[0].size), *[ptr(a) for a in f], int(it), int(nt), int(delt), int(thetan), ptr(ringF), ptr(ringG),
+       ptr(cig))
 
 
 def nfft(n):
