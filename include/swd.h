@@ -189,6 +189,23 @@ int swd_rtm(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
 int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
                 const float *adj, float *image, float *illum_src, float *illum_rcv, float *gather_syn);
 
+/* RTM with angle-domain common-image gathers (sum_image_GPU_kernel_imageCIG,
+ * RK:819-905, whose launch RK:1149-1157 is commented out; thetan = 10 and
+ * delt = 1 in RC:235).  Same as swd_rtm, plus cig [nshots][ntx*ntz][thetan]
+ * (the reference's d_image_ppcig layout): after every backward step each
+ * interior cell bins half the angle between the source and receiver Poynting
+ * vectors (in degrees, averaged over the last delt steps) into bins
+ * ceil(angle + ii - 0.5), ii = -2..2, in [2, thetan), with weights
+ * exp(-ii^2/18) times p_s p_r.  The receiver vector is +p v of the
+ * time-reversed field, where RK:866-867 writes -p v (which would give 90
+ * degrees minus the reflection angle).  thetan in [1, 1024], delt in [1, 64].
+ * Runs on the fused kernels for every path. */
+int swd_rtm_cig(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
+                int thetan, int delt, float *image, float *illum_src, float *illum_rcv, float *cig);
+/* Same with adj, image, illum_* and cig in device memory of the plan's device. */
+int swd_rtm_cig_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
+                    int thetan, int delt, float *image, float *illum_src, float *illum_rcv, float *cig);
+
 /* Up/down-separated RTM (the image_ppdu/ppud and d_image_ppdu/ppud fields of
  * forward/headrtm.h:51,119, whose separation kernel FK:140-165 the reference
  * never launches).

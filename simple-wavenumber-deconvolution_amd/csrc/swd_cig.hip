@@ -1,0 +1,86 @@
+// swd_cig.hip -- angle-domain common-image gathers: the update that follows
+// each backward step (the reference's sum_image_GPU_kernel_imageCIG,
+// RK:819-905, whose launch at RK:1149-1157 is commented out).
+//
+// One thread per interior cell.  The gather wants the reconstruction's
+// velocities at the step's own level; the fused backward leaves them one step
+// behind (its next launch updates them), so the thread forms them here with
+// the same stencil and arithmetic as that update (RK:488-573):
+//   vx(it) = fma(-Bx, D+x p(it), vx(it+1)),  vz(it) = fma(-Bz, D+z p(it), vz(it+1)).
+// Then, per cell,
+//   source Poynting vector   (fx, fz)   = (-p_s vx_s, -p_s vz_s)
+//   receiver Poynting vector (fxr, fzr) = (p_r vx_r, p_r vz_r)
+// (the receiver field is stepped with (-dx, -dz), so +p v is the Poynting
+// vector of the time-reversed wave; the reference's -p v there would measure
+// 90 degrees minus the reflection angle),
+//   F = fma(fx, fxr, fz fzr),  G = |(fx, fz)| |(fxr, fzr)|,
+// averaged over the last delt steps through rings, tht = F/G,
+// ftheta = 0.5 acos(tht) 180 / PI in degrees (PI = 3.1415926), and the five
+// bins ceil(ftheta + ii - 0.5), ii = -2..2, inside [2, thetan) take
+// fma(p_s p_r, exp(-ii^2/18), bin).  sqrt and acos are evaluated in double and
+// rounded to float; an undefined angle (G = 0) adds nothing.
+#include "swd_device.h"
+#include "swd_kernels.h"
+#include "swd_launch.h"
+
+namespace {
+
+constexpr double kRefPi = 3.1415926;   // RK:10
+
+template <int LC>
+__global__ __launch_bounds__(256) void k_cig(KGrid g, const KCig *__restrict__ shots, int thetan, int delt, float w1,
+                                             float w2, int it)
+{
+    const int nint = g.nx * g.nz;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nint) return;
+    const KCig &K = shots[blockIdx.y];
+    const int ix = g.pmlc + c / g.nz, iz = g.pmlc + c % g.nz;
+    const size_t ld = static_cast<size_t>(g.ldz), o = static_cast<size_t>(ix) * ld + iz;
+    const int s = it & 1, s1 = (it + 1) & 1;
+    const float *P = K.ps[s];
+    const float ps = P[o];
+    const float vxs = __builtin_fmaf(-g.Bx[o], dplus<LC>(P, o, ld, g.rc, g.idx), K.vxs[s1][o]);
+    const float vzs = __builtin_fmaf(-g.Bz[o], dplus<LC>(P, o, 1, g.rc, g.idz), K.vzs[s1][o]);
+    const float pr = K.pr[s][o], vxr = K.vxr[s][o], vzr = K.vzr[s][o];
+    const float fx = -ps * vxs, fz = -ps * vzs;
+    const float fxr = pr * vxr, fzr = pr * vzr;
+    const float F = __builtin_fmaf(fx, fxr, fz * fzr);
+    const float G = static_cast<float>(sqrt(static_cast<double>(__builtin_fmaf(fx, fx, fz * fz)))) *
+                    static_cast<float>(sqrt(static_cast<double>(__builtin_fmaf(fxr, fxr, fzr * fzr))));
+    float pre = F, pst = G;
+    if (delt > 1) {
+        float *rF = K.ring, *rG = K.ring + static_cast<size_t>(delt) * nint;
+        rF[static_cast<size_t>(it % delt) * nint + c] = F;
+        rG[static_cast<size_t>(it % delt) * nint + c] = G;
+        if (g.nt - 2 - it + 1 < delt) return;
+        pre = 0.f;
+        pst = 0.f;
+        for (int k = 0; k < delt; k++) {
+            pre += rF[static_cast<size_t>(k) * nint + c];
+            pst += rG[static_cast<size_t>(k) * nint + c];
+        }
+    }
+    const float tht = pre / pst;
+    const float ac = static_cast<float>(acos(static_cast<double>(tht)));
+    const float ftheta = static_cast<float>(0.5 * static_cast<double>(ac) * 180.0 / kRefPi);
+    if (ftheta != ftheta) return;
+    const float pp = ps * pr;
+    float *bins = K.cig + static_cast<size_t>(c) * thetan;
+#pragma unroll
+    for (int ii = -2; ii <= 2; ii++) {
+        const int itheta = static_cast<int>(ceilf(static_cast<float>(static_cast<double>(ftheta + static_cast<float>(ii)) - 0.5)));
+        const float w = ii == 0 ? 1.f : (ii == 1 || ii == -1) ? w1 : w2;
+        if (itheta >= 2 && itheta < thetan) bins[itheta] = __builtin_fmaf(pp, w, bins[itheta]);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_cig(const KGrid &g, const KCig *shots, int nshot, int thetan, int delt, float w1, float w2, int it,
+                      hipStream_t s)
+{
+    const dim3 grid((g.nx * g.nz + 255) / 256, nshot);
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_cig<L>), grid, dim3(256), 0, s, g, shots, thetan, delt, w1, w2, it));
+    return hipGetLastError();
+}

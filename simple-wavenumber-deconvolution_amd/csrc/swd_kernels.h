@@ -68,6 +68,16 @@ struct KStep {
     float *dpdt;                 // SRC: dp/dt of step it stored on the interior [ntx][ldz], or null
 };
 
+// One shot of the angle-gather update that follows a backward step: the
+// reconstruction's and the receiver field's slots (both parities) and the
+// gather storage.
+struct KCig {
+    const float *ps[2], *vxs[2], *vzs[2];   // source reconstruction [ntx][ldz]: p at level L in slot L&1, v likewise
+    const float *pr[2], *vxr[2], *vzr[2];   // receiver field [ntx][ldz]
+    float *ring;                            // [2][delt][nx*nz] time-averaging rings (delt > 1), or null
+    float *cig;                             // [nx*nz][thetan], interior cells
+};
+
 // One shot of the up/down imaging that follows a backward step: the two
 // reconstructions' dp/dt (the wavelet's and the Hilbert-transformed wavelet's)
 // and the two receiver pressures (adjoint source and its Hilbert transform)

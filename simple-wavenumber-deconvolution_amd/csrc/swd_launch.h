@@ -23,3 +23,7 @@ hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2
                         const KCapCol *cols, const int2 *traces, float *up, float *down, int nt, hipStream_t s);
 // up/down imaging of step it for nshot shots (KUpDown each) over the interior columns
 hipError_t launch_updown(const KGrid &g, const KUpDown *shots, int nshot, int nfftz, int it, hipStream_t s);
+// angle-gather update of step it for nshot shots (KCig each) over the interior cells;
+// w1, w2: the bin weights exp(-1/18), exp(-4/18) rounded to float
+hipError_t launch_cig(const KGrid &g, const KCig *shots, int nshot, int thetan, int delt, float w1, float w2, int it,
+                      hipStream_t s);

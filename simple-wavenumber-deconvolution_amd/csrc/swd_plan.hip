@@ -1017,11 +1017,27 @@ int join_streams(Plan &P)
     return SWD_OK;
 }
 
+// Angle gathers asked of an RTM call: thetan one-degree bins, delt-step
+// averaging, output [nshots][ntx*ntz][thetan] in device memory.
+struct CigSpec {
+    int thetan, delt;
+    float *out;
+};
+
+// Bytes per shot the angle gathers add to an RTM call's workspace.
+double cig_bytes(const Plan &P, const CigSpec *cig)
+{
+    if (!cig) return 0.0;
+    const double nint = static_cast<double>(P.g.nx) * P.g.nz;
+    return 4.0 * nint * (cig->thetan + (cig->delt > 1 ? 2.0 * cig->delt : 0.0)) + sizeof(KCig);
+}
+
 // RTM of a batch on the fused path: one launch per forward step, one launch
 // per backward step for the paired reconstruction and receiver wavefields of
-// every shot of a group, imaging in the same launch (k_fstep).
+// every shot of a group, imaging in the same launch (k_fstep); with angle
+// gathers, k_cig after every backward launch (in the same captured graph).
 int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, const float *wavelet, const float *adj,
-                  float *image, float *illum_src, float *illum_rcv, float *gather_syn)
+                  float *image, float *illum_src, float *illum_rcv, float *gather_syn, const CigSpec *cig = nullptr)
 {
     const int nt = P.g.nt;
     const size_t nf = P.nfield;
@@ -1034,6 +1050,11 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     const size_t o_strips = c.add<float>(static_cast<size_t>(nt) * P.nframe * nshots);
     const size_t o_img = c.add<float>(3 * nf * nshots);   // image, illum_s, illum_r
     const size_t o_f = c.add<float>(kStepArrays * nf * 2 * nshots);   // source side, then receiver side
+    const size_t nint = static_cast<size_t>(P.g.nx) * P.g.nz;
+    const size_t n_ring = cig && cig->delt > 1 ? 2 * static_cast<size_t>(cig->delt) * nint : 0;
+    const size_t o_kcig = cig ? c.add<KCig>(nshots) : 0;
+    const size_t o_cig = cig ? c.add<float>(nint * cig->thetan * nshots) : 0;
+    const size_t o_ring = cig ? c.add<float>(std::max<size_t>(1, n_ring * nshots)) : 0;
     SWD_TRY(ensure_ws(P, c.off));
     SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
@@ -1070,6 +1091,26 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     }
     SWD_TRY(upload_steps(P, hf, o_fwd));
     SWD_TRY(upload_steps(P, hb, o_bwd));
+    if (cig) {
+        zero_fields(P, at<float>(P, o_cig), nint * cig->thetan * nshots);
+        if (n_ring) zero_fields(P, at<float>(P, o_ring), n_ring * nshots);
+        std::vector<KCig> hk(nshots);
+        for (int k = 0; k < nshots; k++) {
+            const KStep &S = hb[2 * k], &R = hb[2 * k + 1];
+            KCig &K = hk[k];
+            for (int q = 0; q < 2; q++) {
+                K.ps[q] = S.p[q];
+                K.vxs[q] = S.vx[q];
+                K.vzs[q] = S.vz[q];
+                K.pr[q] = R.p[q];
+                K.vxr[q] = R.vx[q];
+                K.vzr[q] = R.vz[q];
+            }
+            K.ring = n_ring ? at<float>(P, o_ring) + n_ring * k : nullptr;
+            K.cig = at<float>(P, o_cig) + nint * cig->thetan * k;
+        }
+        HIP_TRY(hipMemcpyAsync(P.ws + o_kcig, hk.data(), sizeof(KCig) * nshots, hipMemcpyHostToDevice, P.stream));
+    }
     // Shots run in groups whose per-step working set stays in the last-level
     // cache; every shot keeps its strips and final state, so the groups of the
     // backward need not match those of the forward.
@@ -1110,10 +1151,25 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     stats_begin(P, 1);
     SWD_TRY(fork());
     const std::vector<int> gbb = group_bounds(nshots, gb);
+    // bin weights exp(-ii^2/18) of the angle gathers, rounded to float
+    const float w1 = static_cast<float>(std::exp(-static_cast<double>(static_cast<float>(1.0 / 18.0))));
+    const float w2 = static_cast<float>(std::exp(-static_cast<double>(static_cast<float>(4.0 / 18.0))));
     for (size_t g = 0; g + 1 < gbb.size(); g++) {
         const int k0 = gbb[g], n = gbb[g + 1] - gbb[g];
         const KStep *d = at<KStep>(P, o_bwd) + 2 * k0;
-        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tuned_bwd, &launches, true, group_stream(static_cast<int>(g))));
+        StepHook hook;
+        if (cig) {
+            const KCig *dk = at<KCig>(P, o_kcig) + k0;
+            const int thetan = cig->thetan, delt = cig->delt;
+            hook.run = [&P, dk, n, thetan, delt, w1, w2, &launches](int it, hipStream_t st) -> int {
+                HIP_TRY(launch_cig(P.kg, dk, n, thetan, delt, w1, w2, it, st));
+                launches++;
+                return SWD_OK;
+            };
+            hook.key = {3, reinterpret_cast<long long>(dk), thetan, delt};
+        }
+        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tuned_bwd, &launches, true, group_stream(static_cast<int>(g)),
+                            cig ? &hook : nullptr));
     }
     SWD_TRY(join());
     P.stats.bwd_ms = stats_end(P, 1);
@@ -1126,6 +1182,16 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
         for (int q = 0; q < 3; q++)
             HIP_TRY(hipMemcpy2DAsync(outs[q], sizeof(float) * P.ntz, imgs + 3 * nf * k + q * nf, sizeof(float) * P.ldz,
                                      sizeof(float) * P.ntz, P.ntx, hipMemcpyDeviceToDevice, P.stream));
+    }
+    if (cig) {
+        // interior [nx][nz][thetan] -> [ntx][ntz][thetan], zero outside
+        const size_t th = static_cast<size_t>(cig->thetan), per = ntp * th;
+        HIP_TRY(hipMemsetAsync(cig->out, 0, sizeof(float) * per * nshots, P.stream));
+        for (int k = 0; k < nshots; k++)
+            HIP_TRY(hipMemcpy2DAsync(cig->out + per * k + (static_cast<size_t>(P.pmlc) * P.ntz + P.pmlc) * th,
+                                     sizeof(float) * P.ntz * th, at<float>(P, o_cig) + nint * th * k,
+                                     sizeof(float) * P.g.nz * th, sizeof(float) * P.g.nz * th, P.g.nx,
+                                     hipMemcpyDeviceToDevice, P.stream));
     }
     HIP_TRY(hipStreamSynchronize(P.stream));
     P.stats.ws_bytes = static_cast<long long>(P.ws_cap);
@@ -1268,6 +1334,57 @@ extern "C" int swd_rtm(void *plan, int nshots, const swd_shot *shots, const floa
     HIP_TRY(hipMemcpy(illum_src, d_img + ntp * nshots, sizeof(float) * ntp * nshots, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(illum_rcv, d_img + 2 * ntp * nshots, sizeof(float) * ntp * nshots, hipMemcpyDeviceToHost));
     if (gather_syn) HIP_TRY(hipMemcpy(gather_syn, d_gs, sizeof(float) * ng, hipMemcpyDeviceToHost));
+    return SWD_OK;
+}
+
+// ------------------------------------------------------------------ angle gathers
+extern "C" int swd_rtm_cig_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
+                               int thetan, int delt, float *image, float *illum_src, float *illum_rcv, float *cig)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !adj || !image || !illum_src || !illum_rcv || !cig || !shots || nshots < 1)
+        return swd_fail(SWD_E_ARG, "swd_rtm_cig: null argument");
+    if (thetan < 1 || thetan > 1024 || delt < 1 || delt > 64)
+        return swd_fail(SWD_E_ARG, "swd_rtm_cig: thetan=%d must be in [1, 1024] and delt=%d in [1, 64]", thetan, delt);
+    const double nt = P->g.nt, nf = static_cast<double>(P->nfield);
+    CigSpec spec{thetan, delt, nullptr};
+    const double per = 4.0 * (nt * P->nframe + 27.0 * nf) + cig_bytes(*P, &spec);
+    int wave = nshots;
+    SWD_TRY(wave_shots(*P, nshots, per, 4.0 * nt * 4, wave));
+    const size_t ntp = static_cast<size_t>(P->ntx) * P->ntz;
+    for (int s0 = 0; s0 < nshots; s0 += wave) {
+        const int n = std::min(wave, nshots - s0);
+        Batch b;
+        SWD_TRY(build_batch(*P, n, shots + s0, b));
+        const size_t tr = static_cast<size_t>(traces_before(shots, s0)) * P->g.nt;
+        spec.out = cig + ntp * thetan * s0;
+        SWD_TRY(rtm_dev_fused(*P, n, shots + s0, b, wavelet, adj + tr, image + ntp * s0, illum_src + ntp * s0,
+                              illum_rcv + ntp * s0, nullptr, &spec));
+    }
+    return SWD_OK;
+}
+
+extern "C" int swd_rtm_cig(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *adj,
+                           int thetan, int delt, float *image, float *illum_src, float *illum_rcv, float *cig)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!adj || !image || !illum_src || !illum_rcv || !cig || !shots || nshots < 1)
+        return swd_fail(SWD_E_ARG, "swd_rtm_cig: null argument");
+    if (thetan < 1 || thetan > 1024) return swd_fail(SWD_E_ARG, "swd_rtm_cig: thetan=%d must be in [1, 1024]", thetan);
+    const size_t ng = static_cast<size_t>(total_traces(nshots, shots)) * P->g.nt;
+    const size_t ntp = static_cast<size_t>(P->ntx) * P->ntz, nc = ntp * thetan * nshots;
+    DevBuf d;
+    SWD_TRY(d.alloc(sizeof(float) * (ng + 3 * ntp * nshots + nc)));
+    float *d_adj = d.as<float>(), *d_img = d_adj + ng, *d_cig = d_img + 3 * ntp * nshots;
+    HIP_TRY(hipMemcpy(d_adj, adj, sizeof(float) * ng, hipMemcpyHostToDevice));
+    SWD_TRY(swd_rtm_cig_dev(plan, nshots, shots, wavelet, d_adj, thetan, delt, d_img, d_img + ntp * nshots,
+                            d_img + 2 * ntp * nshots, d_cig));
+    HIP_TRY(hipMemcpy(image, d_img, sizeof(float) * ntp * nshots, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(illum_src, d_img + ntp * nshots, sizeof(float) * ntp * nshots, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(illum_rcv, d_img + 2 * ntp * nshots, sizeof(float) * ntp * nshots, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cig, d_cig, sizeof(float) * nc, hipMemcpyDeviceToHost));
     return SWD_OK;
 }
 

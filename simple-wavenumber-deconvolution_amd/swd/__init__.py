@@ -86,7 +86,7 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
            "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
            "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor",
-           "swd_rtm", "swd_rtm_dev", "swd_rtm_updown", "swd_rtm_updown_dev", "swd_updown_image_dev",
+           "swd_rtm", "swd_rtm_dev", "swd_rtm_cig", "swd_rtm_cig_dev", "swd_rtm_updown", "swd_rtm_updown_dev", "swd_updown_image_dev",
            "swd_plan_get_stream", "swd_born", "swd_born_adjoint", "swd_born_dev",
            "swd_born_adjoint_dev", "swd_multi_forward", "swd_decon_read_parameters", "swd_decon_geometry",
            "swd_decon_taper", "swd_decon_plan_create", "swd_decon_plan_destroy", "swd_decon", "swd_decon_dev",
@@ -377,6 +377,24 @@ class Plan:
         vp = lambda x: None if x is None else C.c_void_p(int(x))
         check(self.L.swd_rtm_dev(self.h, S.n, S.arr, _p(f32(wavelet)), vp(adj_ptr), vp(image_ptr),
                                  vp(illum_s_ptr), vp(illum_r_ptr), vp(gather_ptr)))
+
+    def rtm_cig(self, shots, wavelet, adj, thetan=10, delt=1):
+        """RTM with angle-domain common-image gathers (swd_rtm_cig): image,
+        illum_src, illum_rcv [nshots][ntx*ntz] and cig [nshots][ntx*ntz][thetan]."""
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        ntp = self.grid.ntx * self.grid.ntz
+        outs = [np.zeros((S.n, ntp), np.float32) for _ in range(3)]
+        cig = np.zeros((S.n, ntp, int(thetan)), np.float32)
+        check(self.L.swd_rtm_cig(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(adj)), int(thetan), int(delt),
+                                 *[_p(o) for o in outs], _p(cig)))
+        return (*outs, cig)
+
+    def rtm_cig_dev(self, shots, wavelet, adj_ptr, thetan, delt, image_ptr, illum_s_ptr, illum_r_ptr, cig_ptr):
+        """Device-pointer RTM with angle gathers (pointers are integers)."""
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        vp = lambda x: C.c_void_p(int(x))
+        check(self.L.swd_rtm_cig_dev(self.h, S.n, S.arr, _p(f32(wavelet)), vp(adj_ptr), int(thetan), int(delt),
+                                     vp(image_ptr), vp(illum_s_ptr), vp(illum_r_ptr), vp(cig_ptr)))
 
     def rtm_updown(self, shots, wavelet, hwavelet, adj, hadj, kzcoef, want_same_dir=False, want_image=False):
         """Up/down-separated RTM images (swd_rtm_updown): returns a dict of
