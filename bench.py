@@ -310,6 +310,100 @@ def bench_updown(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_cig(a):
+    """RTM of the c2 batch with angle-domain common-image gathers (thetan 10,
+    delt 1 as the reference's RC:235): one step = the fused RTM plus the
+    gather update after every backward step, adjoint sources resident in HBM.
+    The plain RTM of the same batch runs beside it for the cost of the gathers."""
+    import torch
+    import swd
+
+    thetan, delt = a.thetan, 1
+    log(f"bench: c2 RTM with angle gathers (thetan {thetan}), batch {a.batch} shots, nt {a.nt}")
+    case = build_case(a.batch, a.nt)
+    g = case.grid
+    plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0, tile=a.tile)
+    plan.set_groups(*a.groups)
+    plan.set_streams(a.streams)
+    shots = swd.ShotSet(case.shots)
+    plan_true = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0)
+    obs = plan_true.forward(shots, case.wavelet)
+    plan_true.close()
+    adj = swd.adjoint_source(obs, g.dt)
+    dev = torch.device("cuda:0")
+    ntp = g.ntx * g.ntz
+    d_adj = torch.from_numpy(adj).to(dev)
+    d_img = torch.zeros((3, a.batch, ntp), dtype=torch.float32, device=dev)
+    d_cig = torch.zeros((a.batch, ntp, thetan), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        plan.rtm_cig_dev(shots, case.wavelet, d_adj.data_ptr(), thetan, delt, d_img[0].data_ptr(),
+                         d_img[1].data_ptr(), d_img[2].data_ptr(), d_cig.data_ptr())
+
+    def plain():
+        plan.rtm_dev(shots, case.wavelet, d_adj.data_ptr(), d_img[0].data_ptr(), d_img[1].data_ptr(),
+                     d_img[2].data_ptr())
+
+    for i in range(a.warmup):
+        step()
+        plain()
+    torch.cuda.synchronize()
+    bwd = {"cig": 0.0, "plain": 0.0}
+    launches = 0
+    t_cig = t_plain = 0.0
+    for i in range(a.steps):
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        t_cig += time.perf_counter() - t0
+        st = plan.stats()
+        bwd["cig"] += st["bwd_ms"]
+        launches = st["bwd_launches"]
+        t0 = time.perf_counter()
+        plain()
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        bwd["plain"] += plan.stats()["bwd_ms"]
+        log(f"bench: step {i + 1}/{a.steps} backward with gathers {st['bwd_ms']:.1f} ms, plain "
+            f"{plan.stats()['bwd_ms']:.1f} ms")
+    cells = rtm_cells(g) * a.batch
+    n_cig = launches // 2   # one k_cig after each backward k_fstep launch
+    added_ms = (bwd["cig"] - bwd["plain"]) / a.steps / max(1, n_cig)
+    shots_per_launch = a.batch * (g.nt - 1) / max(1, n_cig)
+    nint = g.nx * g.nz
+    k_bytes = 32.0 * nint * shots_per_launch   # p, vx, vz of both fields, Bx, Bz per interior cell
+    achieved = k_bytes / (added_ms * 1e-3) / 1e9 if added_ms > 0 else None
+    out = {
+        "metric": "RTM with angle-domain CIGs, Gcell-updates/s (c2 batch)",
+        "value": cells * a.steps / t_cig / 1e9,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t_cig * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered model with dipping reflectors, c2 geometry)",
+        "config": {"workload": f"c2 batched VSP RTM with angle gathers (thetan {thetan}, delt {delt}): 502x845 "
+                               f"interior, nt {g.nt}",
+                   "shots_per_step": a.batch, "plain_rtm_gcell_per_s": cells * a.steps / t_plain / 1e9,
+                   "bwd_ms_with_gathers": bwd["cig"] / a.steps, "bwd_ms_plain": bwd["plain"] / a.steps,
+                   "gather_cost_fraction": (t_cig - t_plain) / t_plain,
+                   "cig_finite": bool(torch.isfinite(d_cig).all().item())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel": "k_cig (angle-gather update of one backward step, one launch per group)",
+                     "bytes_per_launch": k_bytes, "avg_launch_ms": added_ms,
+                     "timing": "backward-phase HIP events with gathers minus without, over the k_cig launches: "
+                               "the time the gathers add, with the two streams' overlap"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,9 +422,10 @@ def main():
                     help="also copy 256 MiB device-to-device once (reference for HBM byte counters)")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2, 3, 4),
                     help="RTM shot groups in flight at once (swd_plan_set_streams)")
-    ap.add_argument("--workload", choices=("rtm", "decon", "updown"), default="rtm",
+    ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig"), default="rtm",
                     help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution; "
-                         "updown: the up/down-separated RTM")
+                         "updown: the up/down-separated RTM; cig: RTM with angle gathers")
+    ap.add_argument("--thetan", type=int, default=10, help="angle bins of --workload cig (RC:235: 10)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU shot sharding is "
@@ -342,6 +437,9 @@ def main():
         return
     if a.workload == "updown":
         bench_updown(a)
+        return
+    if a.workload == "cig":
+        bench_cig(a)
         return
 
     import torch

@@ -2,7 +2,8 @@
 // each backward step (the reference's sum_image_GPU_kernel_imageCIG,
 // RK:819-905, whose launch at RK:1149-1157 is commented out).
 //
-// One thread per interior cell.  The gather wants the reconstruction's
+// One thread per interior cell (cells in z-fastest order, spread over the
+// XCDs in contiguous ranges).  The gather wants the reconstruction's
 // velocities at the step's own level; the fused backward leaves them one step
 // behind (its next launch updates them), so the thread forms them here with
 // the same stencil and arithmetic as that update (RK:488-573):
@@ -32,7 +33,9 @@ __global__ __launch_bounds__(256) void k_cig(KGrid g, const KCig *__restrict__ s
                                              float w2, int it)
 {
     const int nint = g.nx * g.nz;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    // consecutive cells on one XCD: the x stencil's neighbouring columns then
+    // come from that XCD's L2
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= nint) return;
     const KCig &K = shots[blockIdx.y];
     const int ix = g.pmlc + c / g.nz, iz = g.pmlc + c % g.nz;
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(256) void k_cig(KGrid g, const KCig *__restrict__ s
 hipError_t launch_cig(const KGrid &g, const KCig *shots, int nshot, int thetan, int delt, float w1, float w2, int it,
                       hipStream_t s)
 {
-    const dim3 grid((g.nx * g.nz + 255) / 256, nshot);
+    const dim3 grid(((g.nx * g.nz + 255) / 256 + 7) / 8 * 8, nshot);   // a multiple of 8 for xcd_block
     DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_cig<L>), grid, dim3(256), 0, s, g, shots, thetan, delt, w1, w2, it));
     return hipGetLastError();
 }

@@ -577,7 +577,8 @@ int upload_steps(Plan &P, const std::vector<KStep> &h, size_t o)
 // width are timed and the fastest width runs the rest (the results do not
 // depend on the width).
 // Work launched after every fused step (inside the same captured graph):
-// run(it, stream); key identifies its launches and arguments for the graph cache.
+// run(it, stream) queues one kernel launch; key identifies its arguments for
+// the graph cache.  The launch counts count it beside the fused step.
 struct StepHook {
     std::function<int(int, hipStream_t)> run;
     std::vector<long long> key;
@@ -597,7 +598,10 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         const int it = its[k++];
         HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, pair, st));
         (*launches)++;
-        if (hook) SWD_TRY(hook->run(it, st));
+        if (hook) {
+            SWD_TRY(hook->run(it, st));
+            (*launches)++;
+        }
         return SWD_OK;
     };
     if (tx == 0) {
@@ -640,7 +644,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
                 return SWD_OK;
             },
             st));
-        *launches += static_cast<long long>(its.size() - k0);
+        *launches += static_cast<long long>(its.size() - k0) * (hook ? 2 : 1);
         return SWD_OK;
     }
     while (k < its.size()) SWD_TRY(launch(tx));
@@ -1161,9 +1165,8 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
         if (cig) {
             const KCig *dk = at<KCig>(P, o_kcig) + k0;
             const int thetan = cig->thetan, delt = cig->delt;
-            hook.run = [&P, dk, n, thetan, delt, w1, w2, &launches](int it, hipStream_t st) -> int {
+            hook.run = [&P, dk, n, thetan, delt, w1, w2](int it, hipStream_t st) -> int {
                 HIP_TRY(launch_cig(P.kg, dk, n, thetan, delt, w1, w2, it, st));
-                launches++;
                 return SWD_OK;
             };
             hook.key = {3, reinterpret_cast<long long>(dk), thetan, delt};
@@ -1528,9 +1531,8 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
         const int k0 = gbb[g], n = gbb[g + 1] - gbb[g];
         const KUpDown *du = at<KUpDown>(P, o_ud) + k0;
         StepHook hook;
-        hook.run = [&P, du, n, nfftz, &launches](int it, hipStream_t st) -> int {
+        hook.run = [&P, du, n, nfftz](int it, hipStream_t st) -> int {
             HIP_TRY(launch_updown(P.kg, du, n, nfftz, it, st));
-            launches++;
             return SWD_OK;
         };
         hook.key = {2, reinterpret_cast<long long>(du), nfftz};

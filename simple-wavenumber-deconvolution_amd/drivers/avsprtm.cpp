@@ -9,7 +9,11 @@
 // --updown the same run also images the up/down-separated wavefields
 // (swd_rtm_updown) and writes migration_{pp|psf}du.bin and ..ud.bin
 // (down-going source with up-going receiver field, and the reverse),
-// normalised, filtered and summed like the plain image.
+// normalised, filtered and summed like the plain image.  With --cig [--thetan
+// N] [--delt N] it also computes the angle-domain common-image gathers
+// (swd_rtm_cig; thetan 10 and delt 1 by default, RC:235) and writes their
+// shot sum migration_{pp|psf}cig.bin, [nx][nz][thetan] (the reference's
+// commented-out migration_ppcig.bin, RC:784-786).
 #include <cmath>
 #include <algorithm>
 #include <cstdio>
@@ -65,6 +69,8 @@ int main(int argc, char **argv)
         accdu.assign(s.np, 0.f);
         accud.assign(s.np, 0.f);
     }
+    std::vector<float> acccig;
+    if (o.cig) acccig.assign(static_cast<size_t>(s.np) * o.thetan, 0.f);
     const double t0 = drv::wall_seconds();
     for (size_t b0 = 0; b0 < s.shots.size(); b0 += o.batch) {
         const size_t b1 = std::min(s.shots.size(), b0 + static_cast<size_t>(o.batch));
@@ -90,7 +96,14 @@ int main(int argc, char **argv)
                   "adjoint source");
         }
         std::vector<float> img(static_cast<size_t>(s.ntp) * nb), ils(img.size()), ilr(img.size()), syn(ntr * nt);
-        std::vector<float> du, ud;
+        std::vector<float> du, ud, cig;
+        bool need_syn = true;   // the plain RTM call records the synthetic gathers on its own
+        if (o.cig) {
+            cig.resize(static_cast<size_t>(s.ntp) * o.thetan * nb);
+            check(swd_rtm_cig(plan, nb, sh.data(), rick.data(), adj.data(), o.thetan, o.delt, img.data(), ils.data(),
+                              ilr.data(), cig.data()),
+                  "RTM with angle gathers");
+        }
         if (o.updown) {
             // adjoint sources' Hilbert transforms along time, trace by trace
             std::vector<float> hadj(adj.size());
@@ -98,17 +111,34 @@ int main(int argc, char **argv)
                 check(swd_hilbert1d(adj.data() + tr * nt, hadj.data() + tr * nt, nt), "hilbert");
             du.resize(img.size());
             ud.resize(img.size());
-            const swd_updown_images out = {du.data(), ud.data(), nullptr, nullptr, img.data(), ils.data(), ilr.data()};
+            // the plain image comes from the angle-gather run when there is one
+            const swd_updown_images out = {du.data(), ud.data(), nullptr, nullptr, o.cig ? nullptr : img.data(),
+                                           o.cig ? nullptr : ils.data(), o.cig ? nullptr : ilr.data()};
             check(swd_rtm_updown(plan, nb, sh.data(), rick.data(), hrick.data(), adj.data(), hadj.data(), kz.data(),
                                  nfftz, &out),
                   "up/down RTM");
-            bool diag = false;   // the synthetic gathers are only written for every 20th shot
-            for (size_t k = b0; k < b1; k++) diag |= k % 20 == 0;
-            if (diag) check(swd_forward(plan, nb, sh.data(), rick.data(), syn.data()), "forward");
-        } else {
+        }
+        if (!o.cig && !o.updown) {
             check(swd_rtm(plan, nb, sh.data(), rick.data(), adj.data(), img.data(), ils.data(), ilr.data(),
                           syn.data()),
                   "RTM");
+            need_syn = false;
+        }
+        if (need_syn) {
+            bool diag = false;   // the synthetic gathers are only written for every 20th shot
+            for (size_t k = b0; k < b1; k++) diag |= k % 20 == 0;
+            if (diag) check(swd_forward(plan, nb, sh.data(), rick.data(), syn.data()), "forward");
+        }
+        if (o.cig) {   // raw gathers of the interior, summed over shots: [nx][nz][thetan]
+            for (int k = 0; k < nb; k++)
+                for (int ix = 0; ix < p.nx; ix++)
+                    for (int iz = 0; iz < p.nz; iz++) {
+                        const float *src = cig.data() + (static_cast<size_t>(s.ntp) * k +
+                                                         static_cast<size_t>(ix + s.pmlc) * s.ntz + iz + s.pmlc) *
+                                                            o.thetan;
+                        float *dst = acccig.data() + (static_cast<size_t>(ix) * p.nz + iz) * o.thetan;
+                        for (int t = 0; t < o.thetan; t++) dst[t] += src[t];
+                    }
         }
         for (size_t k = b0; k < b1; k++) {
             const size_t n = s.shots[k].rec_ix.size();
@@ -140,6 +170,8 @@ int main(int argc, char **argv)
     }
     check(swd_edge_zero(tmp1.data(), p.nx, p.nz, p.Lc), "edge zero");   // RC:754-768
     drv::write_floats(std::string("./output/migration_") + kind + ".bin", tmp1.data(), tmp1.size());
+    if (o.cig)
+        drv::write_floats(std::string("./output/migration_") + kind + "cig.bin", acccig.data(), acccig.size());
     if (o.updown) {
         check(swd_edge_zero(accdu.data(), p.nx, p.nz, p.Lc), "edge zero");
         check(swd_edge_zero(accud.data(), p.nx, p.nz, p.Lc), "edge zero");

@@ -45,10 +45,13 @@ Options parse_options(int argc, char **argv, const char *prog)
         else if (a == "--batched") o.path = SWD_PATH_BATCHED;
         else if (a == "--tile") o.tile = std::atoi(next());
         else if (a == "--updown") o.updown = true;
+        else if (a == "--cig") o.cig = true;
+        else if (a == "--thetan") o.thetan = std::atoi(next());
+        else if (a == "--delt") o.delt = std::atoi(next());
         else if (a == "-C") o.workdir = next();
         else if (a == "-h" || a == "--help") {
             std::printf("usage: %s [-C dir] [--batch N] [--device D] [--plain | --batched] [--tile 0|16|32|64]"
-                        " [--updown]\n", prog);
+                        " [--updown] [--cig [--thetan N] [--delt N]]\n", prog);
             std::exit(0);
         } else die("%s: unknown option %s", prog, a.c_str());
     }

@@ -14,6 +14,8 @@ struct Options {
     int path = SWD_PATH_FUSED;
     int tile = 0;            // fused tile width (0 = tuned)
     bool updown = false;     // AVSPRTM: also the up/down-separated images (swd_rtm_updown)
+    bool cig = false;        // AVSPRTM: also the angle-domain common-image gathers (swd_rtm_cig)
+    int thetan = 10, delt = 1;   // their bins and time averaging (RC:235)
     std::string workdir;     // run as if started in this directory
 };
 Options parse_options(int argc, char **argv, const char *prog);

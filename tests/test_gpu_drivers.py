@@ -153,6 +153,36 @@ def test_avsprtm_updown(tmp_path):
         assert e <= TOL
 
 
+def test_avsprtm_cig(tmp_path):
+    """AVSPRTM --cig --thetan 30 --delt 2: the plain migration as without the
+    option, and migration_ppcig.bin, the shot sum of the interior angle
+    gathers [nx][nz][thetan], bit-identical to the oracle's (or_rtm_cig)."""
+    tmp = str(tmp_path)
+    v = setup_dir(tmp)
+    write_params(os.path.join(tmp, "parameter.txt"), 0)
+    run("AFor", os.path.join(tmp, "forward"))
+    run("AVSPRTM", tmp, "--cig", "--thetan", "30", "--delt", "2")
+    probs_m, vpm, _, (ntx, ntz, pmlc) = oracle_case(v, smooth=True)
+    nx, nz = P["nx"], P["nz"]
+    mig = np.zeros(nx * nz, np.float32)
+    cig = np.zeros((nx, nz, 30), np.float32)
+    for is_, pr in enumerate(probs_m):
+        n = pr.nrec
+        up = read(tmp, f"{is_ + 1}source_seismogram_obsup.bin", (n, P["itmax"]))
+        adj = O.adjoint_source(up, P["dt"])
+        _, st, fs = pr.forward(strips=True, final=True)
+        img, ils, _, c = pr.rtm_cig(fs, st, adj, 30, 2)
+        O.image_laplacian(O.image_normalize(img, ils), vpm, ntx, ntz, pmlc, P["dx"], P["dz"], acc=mig)
+        cig += c.reshape(ntx, ntz, 30)[pmlc:ntx - pmlc, pmlc:ntz - pmlc]
+    mig = O.edge_zero(mig, nx, nz, P["Lc"])
+    e = rel_l2(read(tmp, "migration_pp.bin"), mig)
+    print(f"migration_pp: relL2 {e:.2e}")
+    assert e <= TOL
+    got = read(tmp, "migration_ppcig.bin", (nx, nz, 30))
+    print(f"migration_ppcig: {np.count_nonzero(got != cig)} values differ, max |cig| {np.abs(cig).max():.3e}")
+    assert np.array_equal(got, cig) and np.abs(cig).max() > 0
+
+
 def test_afor_and_avsprtm_psf(tmp_path):
     tmp = str(tmp_path)
     v = setup_dir(tmp)
