@@ -43,6 +43,18 @@ def test_cig_small(thetan, delt):
     assert np.abs(cig).max() > 0
 
 
+@pytest.mark.parametrize("Lc", [1, 2, 5, 8])
+def test_cig_stencil_orders(Lc):
+    """The reconstruction velocities k_cig re-forms use the Lc-deep stencil."""
+    c = configs.small(nx=90, nz=60, pml=8, Lc=Lc, nt=200)
+    print(f"case cig Lc={Lc}", flush=True)
+    P = oracle_problem(c)
+    d, st, fs = P.forward(strips=True, final=True)
+    adj = O.adjoint_source(d, c.grid.dt)
+    _, _, _, cig = plan_of(c).rtm_cig(c.shots, c.wavelet, adj, 45, 1)
+    compare(f"Lc {Lc} cig", cig[0], P.rtm_cig(fs, st, adj, 45, 1)[3])
+
+
 def test_cig_batch_groups_streams_device():
     import torch
     c = configs.small(nx=110, nz=70, nt=220, nshots=3, dup=True)

@@ -128,6 +128,23 @@ def test_updown_batch_groups_streams_and_device_pointers():
         assert np.array_equal(v.cpu().numpy(), ref[k]), k
 
 
+@pytest.mark.parametrize("Lc", [2, 8])
+def test_updown_stencil_orders(Lc):
+    c = configs.small(nx=90, nz=60, pml=8, Lc=Lc, nt=200)
+    print(f"case updown Lc={Lc}", flush=True)
+    P = oracle_problem(c)
+    adj = O.adjoint_source(P.forward()[0], c.grid.dt)
+    hadj = swd.hilbert_traces(adj)
+    hw = swd.hilbert1d(c.wavelet)
+    nfftz = swd.nfft(c.grid.ntz)
+    kz, _ = swd.kz_taper(nfftz, c.grid.dz)
+    g = plan_of(c).rtm_updown(c.shots, c.wavelet, hw, adj, hadj, kz, want_same_dir=True)
+    o, _ = oracle_updown(P, adj, hadj, kz, hw)
+    for k, name in enumerate(NAMES):
+        e32, _ = report(f"Lc {Lc} updown {name}", g[name][0], o[k])
+        assert e32 <= TOL
+
+
 def test_updown_c2_grid():
     """The c2 grid (590 x 933 padded, nfftz 1024) for 120 steps, one shot."""
     c = configs.c2(nt=120, shot_ix=(420,))
