@@ -213,13 +213,14 @@ int swd_rtm_cig_dev(void *plan, int nshots, const swd_shot *shots, const float *
  * along time), the receiver side adj and hadj (the adjoint source and its
  * Hilbert transform along time, [sum nrec][nt] each).  At every backward step
  * the analytic columns S = dp/dt + i dp_h/dt (source, zero off the interior)
- * and R = p + i p_h (receiver) of each interior column are zero padded to
+ * and R = p + i p_h (receiver) of each interior column (the whole padded
+ * column when nfftz >= ntz, else its nz interior rows) are zero padded to
  * nfftz, transformed along z, weighted by kzcoef and split as in swd_afor
  * (bins 0..nfftz/2 "up", the rest "down"), transformed back and divided by
  * nfftz; the real parts are cross-correlated on the interior:
  *   du += Sd Ru, ud += Su Rd, uu += Su Ru, dd += Sd Rd.
  * With kzcoef all ones du+ud+uu+dd is the plain image of swd_rtm.
- * nfftz is a power of two in [max(ntz, 128), 4096].  Runs on the fused kernels for
+ * nfftz is a power of two in [max(nz, 128), 4096].  Runs on the fused kernels for
  * every path.  Outputs are [nshots][ntx*ntz]; du and ud are required, uu and
  * dd optional together, and image/illum_src/illum_rcv (the plain RTM outputs
  * of the same run) optional together. */

@@ -1569,8 +1569,9 @@ int check_updown_args(const Plan &P, const char *fn, int nshots, const swd_shot 
         return swd_fail(SWD_E_ARG, "%s: uu and dd are both given or both null", fn);
     if ((out->image == nullptr) != (out->illum_src == nullptr) || (out->image == nullptr) != (out->illum_rcv == nullptr))
         return swd_fail(SWD_E_ARG, "%s: image, illum_src and illum_rcv are all given or all null", fn);
-    if (nfftz < P.ntz || nfftz < 128 || (nfftz & (nfftz - 1)) || nfftz > 4096)
-        return swd_fail(SWD_E_ARG, "%s: nfftz=%d must be a power of two in [max(ntz=%d, 128), 4096]", fn, nfftz, P.ntz);
+    if (nfftz < P.g.nz || nfftz < 128 || (nfftz & (nfftz - 1)) || nfftz > 4096)
+        return swd_fail(SWD_E_ARG, "%s: nfftz=%d must be a power of two in [max(nz=%d, 128), 4096]", fn, nfftz,
+                        P.g.nz);
     return SWD_OK;
 }
 
@@ -1611,9 +1612,9 @@ extern "C" int swd_updown_image_dev(void *plan, int nshots, const float *src, co
     if (!out->du || !out->ud) return swd_fail(SWD_E_ARG, "swd_updown_image_dev: du and ud outputs are required");
     if ((out->uu == nullptr) != (out->dd == nullptr))
         return swd_fail(SWD_E_ARG, "swd_updown_image_dev: uu and dd are both given or both null");
-    if (nfftz < P->ntz || nfftz < 128 || (nfftz & (nfftz - 1)) || nfftz > 4096)
-        return swd_fail(SWD_E_ARG, "swd_updown_image_dev: nfftz=%d must be a power of two in [max(ntz=%d, 128), 4096]",
-                        nfftz, P->ntz);
+    if (nfftz < P->g.nz || nfftz < 128 || (nfftz & (nfftz - 1)) || nfftz > 4096)
+        return swd_fail(SWD_E_ARG, "swd_updown_image_dev: nfftz=%d must be a power of two in [max(nz=%d, 128), 4096]",
+                        nfftz, P->g.nz);
     Carver c;
     const size_t o_ud = c.add<KUpDown>(nshots);
     const size_t o_tw = c.add<float2>(nfftz / 2);

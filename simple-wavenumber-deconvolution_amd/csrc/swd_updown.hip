@@ -6,8 +6,9 @@
 // A workgroup takes one interior column of one shot.  Its first N/16 threads
 // transform the analytic source column S = ds + i dh (the reconstructions'
 // dp/dt, zero off the interior), the other N/16 the analytic receiver column
-// R = pr + i ph (the receiver pressures of step it); both are zero padded to
-// N = nfftz.  Only the real parts of the separated fields are needed, so a
+// R = pr + i ph (the receiver pressures of step it), over the whole padded
+// column when it fits in N = nfftz, else over the nz interior rows; both are
+// zero padded to N.  Only the real parts of the separated fields are needed, so a
 // field's up and down halves share one inverse transform: with X the
 // spectrum, c the taper, bins 0..N/2 "up" and the rest "down", m = -k mod N,
 //   Y[k] = (c X)_up[k] + conj((c X)_up[m])  +  i ((c X)_down[k] + conj((c X)_down[m])),
@@ -241,9 +242,11 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const 
     const int f = threadIdx.x / T, t = threadIdx.x - f * T;   // field 0: source, 1: receiver
     float2 *L = lds + f * NP;
     const size_t c0 = static_cast<size_t>(col) * U.ld;
+    // the whole padded column when it fits in N, else its interior rows
+    const int z0 = N >= g.ntz ? 0 : g.pmlc, nzw = z0 ? g.nz : g.ntz;
     const int slot = it & 1, ntz = g.ntz;
-    const float *re = (f ? U.pr[slot] : U.ds) + c0;
-    const float *im = (f ? U.ph[slot] : U.dh) + c0;
+    const float *re = (f ? U.pr[slot] : U.ds) + c0 + z0;
+    const float *im = (f ? U.ph[slot] : U.dh) + c0 + z0;
     const float2 *__restrict__ tw = U.tw;
 
     // forward transform of the column, zero padded from ntz to N
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const 
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int i = t + r * T;
-        v[r] = i < ntz ? make_float2(re[i], im[i]) : make_float2(0.f, 0.f);
+        v[r] = i < nzw ? make_float2(re[i], im[i]) : make_float2(0.f, 0.f);
     }
     first_pass<N>(v, t, L);
     passes<N, 16, false>(v, t, L, tw);
@@ -284,7 +287,7 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const 
     float *du = U.du + c0, *ud = U.ud + c0;
     float *uu = U.uu ? U.uu + c0 : nullptr, *dd = U.dd ? U.dd + c0 : nullptr;
     for (int iz = g.pmlc + static_cast<int>(threadIdx.x); iz < ntz - g.pmlc; iz += 2 * T) {
-        const float2 s = S[pad(iz)], r = Rv[pad(iz)];
+        const float2 s = S[pad(iz - z0)], r = Rv[pad(iz - z0)];
         du[iz] = __builtin_fmaf(s.y, r.x, du[iz]);
         ud[iz] = __builtin_fmaf(s.x, r.y, ud[iz]);
         if (uu) {

@@ -145,21 +145,21 @@ def test_updown_stencil_orders(Lc):
         assert e32 <= TOL
 
 
-def test_updown_c2_grid():
-    """The c2 grid (590 x 933 padded, nfftz 1024) for 120 steps, one shot."""
+@pytest.mark.parametrize("nfftz", [1024, 512])
+def test_updown_c2_grid(nfftz):
+    """The c2 grid (590 x 933 padded) for 120 steps, one shot: nfftz 1024 over
+    whole padded columns, 512 over the 502 interior rows ("nz padded to 512")."""
     c = configs.c2(nt=120, shot_ix=(420,))
-    print("case c2 updown nt=120", flush=True)
+    print(f"case c2 updown nt=120 nfftz={nfftz}", flush=True)
     P = oracle_problem(c)
     adj = O.adjoint_source(P.forward()[0], c.grid.dt)
     hadj = swd.hilbert_traces(adj)
     hw = swd.hilbert1d(c.wavelet)
-    nfftz = swd.nfft(c.grid.ntz)
-    assert nfftz == 1024
     kz, _ = swd.kz_taper(nfftz, c.grid.dz)
     g = plan_of(c).rtm_updown(c.shots, c.wavelet, hw, adj, hadj, kz, want_same_dir=True)
     o, _ = oracle_updown(P, adj, hadj, kz, hw)
     for k, name in enumerate(NAMES):
-        e32, _ = report(f"c2 updown {name}", g[name][0], o[k])
+        e32, _ = report(f"c2 nfftz {nfftz} updown {name}", g[name][0], o[k])
         assert e32 <= TOL
 
 
@@ -179,13 +179,16 @@ def numpy_updown_step(src, hsrc, rcv, hrcv, kz, nfftz, ntx, ntz, pmlc):
     of or_rtm_updown, written with numpy's FFT): returns du, ud, uu, dd."""
     n = nfftz
     up = np.arange(n) <= n // 2
+    z0, z1 = (0, ntz) if n >= ntz else (pmlc, ntz - pmlc)   # transformed rows
 
     def split(a, b):
         x = np.zeros((ntx, n), complex)
-        x[:, :ntz] = a.reshape(ntx, ntz) + 1j * b.reshape(ntx, ntz)
+        x[:, :z1 - z0] = (a.reshape(ntx, ntz) + 1j * b.reshape(ntx, ntz))[:, z0:z1]
         X = np.fft.fft(x, axis=1) * kz[None, :].astype(np.float64)
-        u = np.fft.ifft(np.where(up[None, :], X, 0), axis=1).real[:, :ntz]
-        d = np.fft.ifft(np.where(up[None, :], 0, X), axis=1).real[:, :ntz]
+        u = np.zeros((ntx, ntz))
+        d = np.zeros((ntx, ntz))
+        u[:, z0:z1] = np.fft.ifft(np.where(up[None, :], X, 0), axis=1).real[:, :z1 - z0]
+        d[:, z0:z1] = np.fft.ifft(np.where(up[None, :], 0, X), axis=1).real[:, :z1 - z0]
         return u, d
 
     su, sd = split(src, hsrc)
@@ -196,15 +199,18 @@ def numpy_updown_step(src, hsrc, rcv, hrcv, kz, nfftz, ntx, ntz, pmlc):
     return [np.where(mask, o, 0.0).reshape(-1) for o in out]
 
 
-@pytest.mark.parametrize("nz,pml", [(80, 10), (160, 20), (380, 20), (560, 20), (1000, 20), (2100, 20)])
-def test_updown_image_step_vs_numpy(nz, pml):
+@pytest.mark.parametrize("nz,pml,interior", [(80, 10, False), (160, 20, False), (380, 20, False), (560, 20, False),
+                                             (1000, 20, False), (2100, 20, False), (120, 10, True),
+                                             (500, 20, True), (1020, 20, True)])
+def test_updown_image_step_vs_numpy(nz, pml, interior):
     """swd_updown_image_dev on random fields against numpy's fp64 FFT; two
     shots, accumulation onto non-zero images, a non-trivial taper."""
     import torch
     c = configs.small(nx=70, nz=nz, pml=pml, nt=10)
     g = c.grid
     ntx, ntz, pmlc = g.ntx, g.ntz, g.pmlc
-    nfftz = swd.nfft(ntz)
+    nfftz = swd.nfft(g.nz) if interior else swd.nfft(ntz)
+    assert (nfftz < ntz) == interior
     print(f"case updown step ntz={ntz} nfftz={nfftz}", flush=True)
     kz, _ = swd.kz_taper(nfftz, g.dz)
     rng = np.random.default_rng(5)

@@ -5,7 +5,8 @@ forward/headrtm.h:51,119, the unused wavefield_seperation_kernel FK:140-165),
 so no reference output exists.  The restatement is pinned by what its
 definition fixes:
   - completeness: with a taper of ones, du + ud + uu + dd is the plain RTM
-    image of or_rtm_backward (the split of each analytic field is exact);
+    image of or_rtm_backward (the split of each analytic field is exact),
+    over whole padded columns (nfftz >= ntz) or their interior rows;
   - the split point: a taper that keeps only one half of the kz axis empties
     the images of the other half exactly;
   - direction: a flat reflector under a surface source and surface receivers
@@ -111,6 +112,24 @@ def test_flat_reflector_images_in_down_up(reflector, taper):
     assert abs(10 + int(np.argmax(prof)) - 39) <= 2   # the image peaks at the contrast
     assert rms["du"] > 100 * rms["ud"] and rms["du"] > 100 * rms["dd"] and rms["du"] > 50 * rms["uu"]
     assert abs(rms["du"] / rms["image"] - 1) < 0.01
+
+
+@pytest.mark.parametrize("nfftz", [128, 256])
+def test_interior_window(nfftz):
+    """nz = 120 with a 14-cell frame: ntz = 148.  nfftz 256 transforms whole
+    padded columns, nfftz 128 (< ntz, >= nz) the 120 interior rows.  Both are
+    complete and image the reflector in du."""
+    P, d = reflector_problem(nz=120)
+    assert P.ntz == 148
+    (du, ud, uu, dd), plain = updown(P, d, np.ones(nfftz, np.float32))
+    e = rel_l2(du + ud + uu + dd, plain)
+    rows, x = slice(32, 48), slice(20, 100)
+    rms = {n: float(np.sqrt(np.mean(interior(P, a)[x, rows] ** 2))) for n, a in
+           (("du", du), ("ud", ud), ("dd", dd), ("image", plain))}
+    print(f"nfftz {nfftz}: completeness {e:.2e}, reflector rows rms {rms}")
+    assert e < 1e-12
+    assert rms["du"] > 100 * rms["ud"] and rms["du"] > 50 * rms["dd"]
+    assert abs(rms["du"] / rms["image"] - 1) < 0.02
 
 
 def test_fp32_vs_fp64_with_kz_taper():
