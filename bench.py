@@ -179,11 +179,13 @@ def bench_decon(a):
     print(json.dumps(out), flush=True)
 
 
-def updown_kernel_bytes(g, nshots, nimg):
+def updown_kernel_bytes(g, nshots, nimg, nfftz):
     """Algorithmic HBM bytes of one k_updown launch: per interior column of a
-    shot the four field columns are read (4 x ntz floats) and nimg images are
-    read and written on the interior rows."""
-    return nshots * g.nx * (16 * g.ntz + 8 * nimg * g.nz)
+    shot the four field columns are read over the transformed rows (the whole
+    padded column when nfftz >= ntz, else the nz interior rows) and nimg images
+    are read and written on the interior rows."""
+    nzw = g.ntz if nfftz >= g.ntz else g.nz
+    return nshots * g.nx * (16 * nzw + 8 * nimg * g.nz)
 
 
 def bench_updown(a):
@@ -210,7 +212,8 @@ def bench_updown(a):
     hadj = swd.hilbert_traces(adj)
     log(f"bench: Hilbert transform of {adj.shape[0]} adjoint traces on the host in {time.perf_counter() - t0:.1f}s")
     hw = swd.hilbert1d(case.wavelet)
-    nfftz = swd.nfft(g.ntz)
+    # the c2 configuration's "nz padded to 512": the interior rows of each column
+    nfftz = a.nfftz or max(128, swd.nfft(g.nz))
     kz, _ = swd.kz_taper(nfftz, g.dz)
     dev = torch.device("cuda:0")
     ntp = g.ntx * g.ntz
@@ -274,7 +277,7 @@ def bench_updown(a):
         e1.record()
     e1.synchronize()
     k_ms = e0.elapsed_time(e1) / reps
-    k_bytes = updown_kernel_bytes(g, n_k, 2)
+    k_bytes = updown_kernel_bytes(g, n_k, 2, nfftz)
     k_flops = n_k * g.nx * 4 * 5 * nfftz * int(np.log2(nfftz))
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     del f, im
@@ -293,8 +296,9 @@ def bench_updown(a):
         "dtype": "f32",
         "data": "synthetic (seeded layered model with dipping reflectors, c2 geometry)",
         "config": {"workload": "c2 batched up/down-separated VSP RTM: 502x845 interior (590x933 padded), nt "
-                               f"{g.nt}, nfftz {nfftz}, p and Hilbert-companion wavefields on both sides, "
-                               "kz split of every interior column after every backward step",
+                               f"{g.nt}, nfftz {nfftz} ({'whole padded columns' if nfftz >= g.ntz else 'interior rows'}), "
+                               "p and Hilbert-companion wavefields on both sides, kz split of every interior column "
+                               "after every backward step",
                    "shots_per_step": a.batch, "shots_per_s": shots_per_s, "plain_rtm": rtm,
                    "fwd_ms_per_step": fwd_ms / a.steps, "bwd_ms_per_step": bwd_ms / a.steps,
                    "du_finite": bool(np.isfinite(du).all())},
@@ -426,6 +430,8 @@ def main():
                     help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution; "
                          "updown: the up/down-separated RTM; cig: RTM with angle gathers")
     ap.add_argument("--thetan", type=int, default=10, help="angle bins of --workload cig (RC:235: 10)")
+    ap.add_argument("--nfftz", type=int, default=0,
+                    help="kz transform length of --workload updown (0: nz padded to a power of two, at least 128)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU shot sharding is "

@@ -63,7 +63,7 @@ int main(int argc, char **argv)
     int nfftz = 0;
     if (o.updown) {
         check(swd_hilbert1d(rick.data(), hrick.data(), nt), "hilbert");
-        nfftz = std::max(128, swd_nfft(s.ntz));
+        nfftz = std::max(128, swd_nfft(p.nz));   // interior rows padded to a power of two (whole columns if they fit)
         kz.resize(nfftz);
         check(swd_kz_taper(kz.data(), nfftz, p.dz), "kz taper");
         accdu.assign(s.np, 0.f);

@@ -124,7 +124,7 @@ def test_avsprtm_updown(tmp_path):
     run("AFor", os.path.join(tmp, "forward"))
     run("AVSPRTM", tmp, "--updown")
     probs_m, vpm, _, (ntx, ntz, pmlc) = oracle_case(v, smooth=True)
-    nfftz = O.nfft(ntz)
+    nfftz = max(128, O.nfft(P["nz"]))
     kz, _ = O.kz_taper(nfftz, P["dz"])
     hw = O.hilbert1d(probs_m[0].wavelet)
     migs = {k: np.zeros(P["nx"] * P["nz"], np.float32) for k in ("pp", "ppdu", "ppud")}
