@@ -1,236 +1,26 @@
 // swd_updown.hip -- up/down imaging of one backward step: the kz split of
 // the AFor separation (FK:52-138, there on receiver columns only) applied to
 // whole columns of the backward's wavefields, and the cross-correlation of
-// the separated fields.
+// the separated fields, on the Stockham FFT of swd_fft.h.
 //
 // A workgroup takes one interior column of one shot.  Its first N/16 threads
 // transform the analytic source column S = ds + i dh (the reconstructions'
 // dp/dt, zero off the interior), the other N/16 the analytic receiver column
 // R = pr + i ph (the receiver pressures of step it), over the whole padded
 // column when it fits in N = nfftz, else over the nz interior rows; both are
-// zero padded to N.  Only the real parts of the separated fields are needed, so a
-// field's up and down halves share one inverse transform: with X the
-// spectrum, c the taper, bins 0..N/2 "up" and the rest "down", m = -k mod N,
-//   Y[k] = (c X)_up[k] + conj((c X)_up[m])  +  i ((c X)_down[k] + conj((c X)_down[m])),
-// scaled by 1/(2N), has the inverse transform Re(up) + i Re(down).  The
-// images of the interior rows then take
+// zero padded to N and split with the normalised inverse.  The images of the
+// interior rows then take
 //   du += Sd Ru,  ud += Su Rd,  uu += Su Ru,  dd += Sd Rd.
 //
-// Transforms: radix-16 Stockham autosort passes (a last pass of radix 2, 4
-// or 8 when log2 N is not a multiple of 4).  Each thread holds the 16 values
-// of its butterflies in registers; a pass reads them from LDS, waits, applies
-// the twiddles and an in-register DFT, and writes its outputs in place; the
-// last pass writes natural order.  An N-point transform thus makes
-// ceil(log2 N / 4) trips through LDS instead of log2 N.  Twiddles come from
-// the table e^{-2 pi i q/N} (q < N/2, evaluated in double and rounded): a
-// radix-R butterfly loads w, w^2, w^4, w^8 and forms the other powers by at
-// most three complex products.  The inverse transform is
-// conj(FFT(conj(Y))).
+// (The AFor separation itself, k_kzsep, keeps a radix-2 LDS transform that
+// follows the oracle's operation order and is bit-exact: its scattered-field
+// gathers are differences of two runs, which magnify any rounding change.)
 #include "swd_device.h"
+#include "swd_fft.h"
 #include "swd_kernels.h"
 #include "swd_launch.h"
 
 namespace {
-
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b)
-{
-    return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
-}
-__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
-__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
-
-// LDS index with one pad element per 16: the first pass writes 16 consecutive
-// values per thread, which would otherwise put every other thread on the same banks.
-__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
-
-// e^{-2 pi i q/N} for 0 <= q < N from the half table
-template <int N>
-__device__ __forceinline__ float2 twf(const float2 *__restrict__ tw, int q)
-{
-    const float2 w = tw[q & (N / 2 - 1)];
-    return q < N / 2 ? w : make_float2(-w.x, -w.y);
-}
-
-// In-register DFTs (forward, e^{-2 pi i jk/R}), natural order in and out, on
-// v[O .. O+R-1].
-template <int O>
-__device__ __forceinline__ void dft2(float2 (&v)[16])
-{
-    const float2 a = v[O], b = v[O + 1];
-    v[O] = cadd(a, b);
-    v[O + 1] = csub(a, b);
-}
-
-__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3)
-{
-    const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_mi(csub(a1, a3));
-    a0 = cadd(t0, t2);
-    a2 = csub(t0, t2);
-    a1 = cadd(t1, t3);
-    a3 = csub(t1, t3);
-}
-
-template <int O>
-__device__ __forceinline__ void dft4(float2 (&v)[16])
-{
-    dft4(v[O], v[O + 1], v[O + 2], v[O + 3]);
-}
-
-constexpr float kC8 = 0.92387953251128674f;   // cos(pi/8)
-constexpr float kS8 = 0.38268343236508977f;   // sin(pi/8)
-constexpr float kR2 = 0.70710678118654752f;   // sqrt(1/2)
-
-template <int O>
-__device__ __forceinline__ void dft8(float2 (&v)[16])
-{
-    // even and odd halves, then X[k] = E[k] + W8^k O[k], X[k+4] = E[k] - W8^k O[k]
-    float2 e0 = v[O], e1 = v[O + 2], e2 = v[O + 4], e3 = v[O + 6];
-    float2 o0 = v[O + 1], o1 = v[O + 3], o2 = v[O + 5], o3 = v[O + 7];
-    dft4(e0, e1, e2, e3);
-    dft4(o0, o1, o2, o3);
-    o1 = make_float2(kR2 * (o1.x + o1.y), kR2 * (o1.y - o1.x));    // * (1 - i)/sqrt2
-    o2 = mul_mi(o2);                                                // * -i
-    o3 = make_float2(kR2 * (o3.y - o3.x), -kR2 * (o3.x + o3.y));   // * (-1 - i)/sqrt2
-    v[O] = cadd(e0, o0);
-    v[O + 4] = csub(e0, o0);
-    v[O + 1] = cadd(e1, o1);
-    v[O + 5] = csub(e1, o1);
-    v[O + 2] = cadd(e2, o2);
-    v[O + 6] = csub(e2, o2);
-    v[O + 3] = cadd(e3, o3);
-    v[O + 7] = csub(e3, o3);
-}
-
-// 16 = 4 x 4: X[k1 + 4 k2] = sum_r2 W16^(r2 k1) W4^(r2 k2) sum_r1 x[4 r1 + r2] W4^(r1 k1)
-__device__ __forceinline__ void dft16(float2 (&v)[16])
-{
-#pragma unroll
-    for (int r2 = 0; r2 < 4; r2++) dft4(v[r2], v[4 + r2], v[8 + r2], v[12 + r2]);
-    // slot 4 k1 + r2 holds the inner DFT; twiddle W16^(r2 k1)
-    const float2 w1 = make_float2(kC8, -kS8), w2 = make_float2(kR2, -kR2), w3 = make_float2(kS8, -kC8);
-    const float2 w6 = make_float2(-kR2, -kR2), w9 = make_float2(-kC8, kS8);
-    v[5] = cmul(v[5], w1);
-    v[6] = cmul(v[6], w2);
-    v[7] = cmul(v[7], w3);
-    v[9] = cmul(v[9], w2);
-    v[10] = mul_mi(v[10]);
-    v[11] = cmul(v[11], w6);
-    v[13] = cmul(v[13], w3);
-    v[14] = cmul(v[14], w6);
-    v[15] = cmul(v[15], w9);
-#pragma unroll
-    for (int k1 = 0; k1 < 4; k1++) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
-    // slot 4 k1 + k2 holds X[k1 + 4 k2]: transpose to natural order
-    float2 t[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) t[i] = v[i];
-#pragma unroll
-    for (int k1 = 0; k1 < 4; k1++)
-#pragma unroll
-        for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = t[4 * k1 + k2];
-}
-
-template <int R, int O>
-__device__ __forceinline__ void dft_r(float2 (&v)[16])
-{
-    if constexpr (R == 2) dft2<O>(v);
-    else if constexpr (R == 4) dft4<O>(v);
-    else if constexpr (R == 8) dft8<O>(v);
-    else dft16(v);
-}
-
-// Twiddles of butterfly j of a pass (span NS, radix R): x[r] *= W^(r k),
-// W = e^{-2 pi i / (NS R)}, k = j mod NS.
-template <int N, int R, int NS, int O>
-__device__ __forceinline__ void twiddle(float2 (&v)[16], int j, const float2 *__restrict__ tw)
-{
-    const int q1 = (j & (NS - 1)) * (N / (NS * R));
-    const float2 w1 = twf<N>(tw, q1);
-    v[O + 1] = cmul(v[O + 1], w1);
-    if constexpr (R >= 4) {
-        const float2 w2 = twf<N>(tw, 2 * q1), w3 = cmul(w1, w2);
-        v[O + 2] = cmul(v[O + 2], w2);
-        v[O + 3] = cmul(v[O + 3], w3);
-        if constexpr (R >= 8) {
-            const float2 w4 = twf<N>(tw, 4 * q1);
-            v[O + 4] = cmul(v[O + 4], w4);
-            v[O + 5] = cmul(v[O + 5], cmul(w4, w1));
-            v[O + 6] = cmul(v[O + 6], cmul(w4, w2));
-            v[O + 7] = cmul(v[O + 7], cmul(w4, w3));
-            if constexpr (R >= 16) {
-                const float2 w8 = twf<N>(tw, 8 * q1);
-                v[8] = cmul(v[8], w8);
-                v[9] = cmul(v[9], cmul(w8, w1));
-                v[10] = cmul(v[10], cmul(w8, w2));
-                v[11] = cmul(v[11], cmul(w8, w3));
-                v[12] = cmul(v[12], cmul(w8, w4));
-                v[13] = cmul(v[13], cmul(w8, cmul(w4, w1)));
-                v[14] = cmul(v[14], cmul(w8, cmul(w4, w2)));
-                v[15] = cmul(v[15], cmul(w8, cmul(w4, w3)));
-            }
-        }
-    }
-}
-
-// One Stockham pass after the first (span NS > 1) of the thread's 16/R
-// butterflies j = t + (N/16) b: inputs x[j + r N/R], outputs at
-// (j / NS) NS R + (j mod NS) + r NS.  CONJ conjugates what is stored.
-template <int N, int R, int NS, bool CONJ>
-__device__ __forceinline__ void pass(float2 (&v)[16], int t, float2 *L, const float2 *__restrict__ tw)
-{
-    constexpr int T = N / 16, NB = 16 / R;
-#pragma unroll
-    for (int b = 0; b < NB; b++)
-#pragma unroll
-        for (int r = 0; r < R; r++) v[b * R + r] = L[pad(t + T * b + r * (N / R))];
-    __syncthreads();   // every input read before any output overwrites it
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-        const int j = t + T * b;
-        if constexpr (R == 16) {
-            twiddle<N, 16, NS, 0>(v, j, tw);
-            dft16(v);
-        } else {
-            // b is a compile-time constant after unrolling; dispatch on it
-            if (b == 0) { twiddle<N, R, NS, 0>(v, j, tw); dft_r<R, 0>(v); }
-            if constexpr (NB > 1) if (b == 1) { twiddle<N, R, NS, R>(v, j, tw); dft_r<R, R>(v); }
-            if constexpr (NB > 2) if (b == 2) { twiddle<N, R, NS, 2 * R>(v, j, tw); dft_r<R, 2 * R>(v); }
-            if constexpr (NB > 3) if (b == 3) { twiddle<N, R, NS, 3 * R>(v, j, tw); dft_r<R, 3 * R>(v); }
-            if constexpr (NB > 4) if (b == 4) { twiddle<N, R, NS, 4 * R>(v, j, tw); dft_r<R, 4 * R>(v); }
-            if constexpr (NB > 5) if (b == 5) { twiddle<N, R, NS, 5 * R>(v, j, tw); dft_r<R, 5 * R>(v); }
-            if constexpr (NB > 6) if (b == 6) { twiddle<N, R, NS, 6 * R>(v, j, tw); dft_r<R, 6 * R>(v); }
-            if constexpr (NB > 7) if (b == 7) { twiddle<N, R, NS, 7 * R>(v, j, tw); dft_r<R, 7 * R>(v); }
-        }
-        const int base = (j / NS) * NS * R + (j & (NS - 1));
-#pragma unroll
-        for (int r = 0; r < R; r++) L[pad(base + r * NS)] = CONJ ? conjf2(v[b * R + r]) : v[b * R + r];
-    }
-    __syncthreads();
-}
-
-// Passes of span NS .. N after the first; the last one (span N/R) writes natural order.
-template <int N, int NS, bool CONJ>
-__device__ __forceinline__ void passes(float2 (&v)[16], int t, float2 *L, const float2 *__restrict__ tw)
-{
-    if constexpr (NS < N) {
-        constexpr int R = N / NS >= 16 ? 16 : N / NS;
-        pass<N, R, NS, CONJ && NS * R == N>(v, t, L, tw);
-        passes<N, NS * R, CONJ>(v, t, L, tw);
-    }
-}
-
-// First pass (span 1, radix 16) of the thread's butterfly j = t: inputs
-// x[t + r N/16] already in v, outputs at 16 t + r.
-template <int N>
-__device__ __forceinline__ void first_pass(float2 (&v)[16], int t, float2 *L)
-{
-    dft16(v);
-#pragma unroll
-    for (int r = 0; r < 16; r++) L[pad(16 * t + r)] = v[r];
-    __syncthreads();
-}
 
 template <int LOGN>
 __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const KUpDown *__restrict__ shots, int it)
@@ -249,7 +39,7 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const 
     const float *im = (f ? U.ph[slot] : U.dh) + c0 + z0;
     const float2 *__restrict__ tw = U.tw;
 
-    // forward transform of the column, zero padded from ntz to N
+    // transform of the column, zero padded to N, split, inverse transforms
     float2 v[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -258,29 +48,9 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const 
     }
     first_pass<N>(v, t, L);
     passes<N, 16, false>(v, t, L, tw);
-
-    // Y[k] (scaled by 1/(2N), conjugated for the inverse) for the inverse's first butterfly
-    const float sc = 0.5f / static_cast<float>(N);   // a power of two: exact
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        // k = t + r N/16 lies below N/2 exactly when r < 8
-        const int k = t + r * T, m = (N - k) & (N - 1);
-        const float2 xk = L[pad(k)], xm = L[pad(m)];
-        const float ck = U.coef[k] * sc, cm = U.coef[m] * sc;
-        float2 y;
-        if (r < 8) {
-            const float2 u = make_float2(ck * xk.x, ck * xk.y), d = make_float2(cm * xm.x, cm * xm.y);
-            y = make_float2(u.x + d.y, u.y + d.x);   // c_k X_k + i conj(c_m X_m)
-        } else {
-            const float2 u = make_float2(cm * xm.x, cm * xm.y), d = make_float2(ck * xk.x, ck * xk.y);
-            y = make_float2(u.x - d.y, d.x - u.y);   // conj(c_m X_m) + i c_k X_k
-        }
-        if ((r == 0 || r == 8) && t == 0) y = make_float2(2.f * ck * xk.x, 0.f);   // DC and Nyquist are "up"
-        v[r] = conjf2(y);
-    }
-    __syncthreads();   // the spectrum is read before the inverse's first pass overwrites it
+    kz_pack<N>(v, t, L, U.coef, 0.5f / static_cast<float>(N));   // 1/(2N): a power of two, exact
     first_pass<N>(v, t, L);
-    passes<N, 16, true>(v, t, L, tw);   // L now holds (Re up, Re down) of the field in natural order
+    passes<N, 16, true>(v, t, L, tw);
 
     // imaging of the interior rows
     const float2 *S = lds, *Rv = lds + NP;
