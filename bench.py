@@ -424,8 +424,9 @@ def main():
                     help="shots per fused RTM forward/backward loop (0 = sized to the cache)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy 256 MiB device-to-device once (reference for HBM byte counters)")
-    ap.add_argument("--streams", type=int, default=2, choices=(1, 2, 3, 4),
-                    help="RTM shot groups in flight at once (swd_plan_set_streams)")
+    ap.add_argument("--streams", type=int, default=0, choices=(0, 1, 2, 3, 4),
+                    help="RTM shot groups in flight at once (swd_plan_set_streams); 0: 3 for --workload updown, "
+                         "else 2 (the fastest measured)")
     ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig"), default="rtm",
                     help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution; "
                          "updown: the up/down-separated RTM; cig: RTM with angle gathers")
@@ -433,6 +434,8 @@ def main():
     ap.add_argument("--nfftz", type=int, default=0,
                     help="kz transform length of --workload updown (0: nz padded to a power of two, at least 128)")
     a = ap.parse_args()
+    if a.streams == 0:
+        a.streams = 3 if a.workload == "updown" else 2
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU shot sharding is "
               "design-only in this build", file=sys.stderr)

@@ -1,12 +1,14 @@
 // swd_fft.h -- radix-16 Stockham FFT along a column in registers and LDS, and
 // the kz up/down split built on it (device code; included by .hip files).
 //
-// A transform of N points (a power of two, 16 to 8192) runs on N/16 threads.
-// Each thread holds the 16 values of its butterflies in registers; a pass
-// reads them from LDS, waits, applies the twiddles and an in-register DFT
-// (16 = 4 x 4; a last pass of radix 2, 4 or 8 when log2 N is not a multiple
-// of 4), and writes its outputs in place; the last pass writes natural order.
-// An N-point transform thus makes ceil(log2 N / 4) trips through LDS.
+// A transform of N points (a power of two, at least V) runs on N/V threads
+// that hold V = 8 or 16 values each.  Each thread keeps the values of its
+// butterflies in registers; a pass reads them from LDS, waits, applies the
+// twiddles and an in-register DFT (radix V, 16 = 4 x 4; a last pass of a
+// smaller radix when log2 N is not a multiple of log2 V), and writes its
+// outputs in place; the last pass writes natural order.  An N-point transform
+// thus makes ceil(log2 N / log2 V) trips through LDS; V = 8 gives twice the
+// threads for the same transform, for launches too small to fill the GPU.
 // Twiddles come from the table e^{-2 pi i q/N} (q < N/2, evaluated in double
 // and rounded): a radix-R butterfly loads w, w^2, w^4, w^8 and forms the other
 // powers by at most three complex products.  The inverse is
@@ -39,8 +41,8 @@ __device__ __forceinline__ float2 twf(const float2 *__restrict__ tw, int q)
 
 // In-register DFTs (forward, e^{-2 pi i jk/R}), natural order in and out, on
 // v[O .. O+R-1].
-template <int O>
-__device__ __forceinline__ void dft2(float2 (&v)[16])
+template <int V, int O>
+__device__ __forceinline__ void dft2(float2 (&v)[V])
 {
     const float2 a = v[O], b = v[O + 1];
     v[O] = cadd(a, b);
@@ -56,8 +58,8 @@ __device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 
     a3 = csub(t1, t3);
 }
 
-template <int O>
-__device__ __forceinline__ void dft4(float2 (&v)[16])
+template <int V, int O>
+__device__ __forceinline__ void dft4(float2 (&v)[V])
 {
     dft4(v[O], v[O + 1], v[O + 2], v[O + 3]);
 }
@@ -66,8 +68,8 @@ constexpr float kC8 = 0.92387953251128674f;   // cos(pi/8)
 constexpr float kS8 = 0.38268343236508977f;   // sin(pi/8)
 constexpr float kR2 = 0.70710678118654752f;   // sqrt(1/2)
 
-template <int O>
-__device__ __forceinline__ void dft8(float2 (&v)[16])
+template <int V, int O>
+__device__ __forceinline__ void dft8(float2 (&v)[V])
 {
     // even and odd halves, then X[k] = E[k] + W8^k O[k], X[k+4] = E[k] - W8^k O[k]
     float2 e0 = v[O], e1 = v[O + 2], e2 = v[O + 4], e3 = v[O + 6];
@@ -116,19 +118,19 @@ __device__ __forceinline__ void dft16(float2 (&v)[16])
         for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = t[4 * k1 + k2];
 }
 
-template <int R, int O>
-__device__ __forceinline__ void dft_r(float2 (&v)[16])
+template <int V, int R, int O>
+__device__ __forceinline__ void dft_r(float2 (&v)[V])
 {
-    if constexpr (R == 2) dft2<O>(v);
-    else if constexpr (R == 4) dft4<O>(v);
-    else if constexpr (R == 8) dft8<O>(v);
+    if constexpr (R == 2) dft2<V, O>(v);
+    else if constexpr (R == 4) dft4<V, O>(v);
+    else if constexpr (R == 8) dft8<V, O>(v);
     else dft16(v);
 }
 
 // Twiddles of butterfly j of a pass (span NS, radix R): x[r] *= W^(r k),
 // W = e^{-2 pi i / (NS R)}, k = j mod NS.
-template <int N, int R, int NS, int O>
-__device__ __forceinline__ void twiddle(float2 (&v)[16], int j, const float2 *__restrict__ tw)
+template <int N, int V, int R, int NS, int O>
+__device__ __forceinline__ void twiddle(float2 (&v)[V], int j, const float2 *__restrict__ tw)
 {
     const int q1 = (j & (NS - 1)) * (N / (NS * R));
     const float2 w1 = twf<N>(tw, q1);
@@ -158,13 +160,13 @@ __device__ __forceinline__ void twiddle(float2 (&v)[16], int j, const float2 *__
     }
 }
 
-// One Stockham pass after the first (span NS > 1) of the thread's 16/R
-// butterflies j = t + (N/16) b: inputs x[j + r N/R], outputs at
+// One Stockham pass after the first (span NS > 1) of the thread's V/R
+// butterflies j = t + (N/V) b: inputs x[j + r N/R], outputs at
 // (j / NS) NS R + (j mod NS) + r NS.  CONJ conjugates what is stored.
-template <int N, int R, int NS, bool CONJ>
-__device__ __forceinline__ void pass(float2 (&v)[16], int t, float2 *L, const float2 *__restrict__ tw)
+template <int N, int V, int R, int NS, bool CONJ>
+__device__ __forceinline__ void pass(float2 (&v)[V], int t, float2 *L, const float2 *__restrict__ tw)
 {
-    constexpr int T = N / 16, NB = 16 / R;
+    constexpr int T = N / V, NB = V / R;
 #pragma unroll
     for (int b = 0; b < NB; b++)
 #pragma unroll
@@ -173,19 +175,19 @@ __device__ __forceinline__ void pass(float2 (&v)[16], int t, float2 *L, const fl
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const int j = t + T * b;
-        if constexpr (R == 16) {
-            twiddle<N, 16, NS, 0>(v, j, tw);
-            dft16(v);
+        if constexpr (R == V) {
+            twiddle<N, V, R, NS, 0>(v, j, tw);
+            dft_r<V, R, 0>(v);
         } else {
             // b is a compile-time constant after unrolling; dispatch on it
-            if (b == 0) { twiddle<N, R, NS, 0>(v, j, tw); dft_r<R, 0>(v); }
-            if constexpr (NB > 1) if (b == 1) { twiddle<N, R, NS, R>(v, j, tw); dft_r<R, R>(v); }
-            if constexpr (NB > 2) if (b == 2) { twiddle<N, R, NS, 2 * R>(v, j, tw); dft_r<R, 2 * R>(v); }
-            if constexpr (NB > 3) if (b == 3) { twiddle<N, R, NS, 3 * R>(v, j, tw); dft_r<R, 3 * R>(v); }
-            if constexpr (NB > 4) if (b == 4) { twiddle<N, R, NS, 4 * R>(v, j, tw); dft_r<R, 4 * R>(v); }
-            if constexpr (NB > 5) if (b == 5) { twiddle<N, R, NS, 5 * R>(v, j, tw); dft_r<R, 5 * R>(v); }
-            if constexpr (NB > 6) if (b == 6) { twiddle<N, R, NS, 6 * R>(v, j, tw); dft_r<R, 6 * R>(v); }
-            if constexpr (NB > 7) if (b == 7) { twiddle<N, R, NS, 7 * R>(v, j, tw); dft_r<R, 7 * R>(v); }
+            if (b == 0) { twiddle<N, V, R, NS, 0>(v, j, tw); dft_r<V, R, 0>(v); }
+            if constexpr (NB > 1) if (b == 1) { twiddle<N, V, R, NS, R>(v, j, tw); dft_r<V, R, R>(v); }
+            if constexpr (NB > 2) if (b == 2) { twiddle<N, V, R, NS, 2 * R>(v, j, tw); dft_r<V, R, 2 * R>(v); }
+            if constexpr (NB > 3) if (b == 3) { twiddle<N, V, R, NS, 3 * R>(v, j, tw); dft_r<V, R, 3 * R>(v); }
+            if constexpr (NB > 4) if (b == 4) { twiddle<N, V, R, NS, 4 * R>(v, j, tw); dft_r<V, R, 4 * R>(v); }
+            if constexpr (NB > 5) if (b == 5) { twiddle<N, V, R, NS, 5 * R>(v, j, tw); dft_r<V, R, 5 * R>(v); }
+            if constexpr (NB > 6) if (b == 6) { twiddle<N, V, R, NS, 6 * R>(v, j, tw); dft_r<V, R, 6 * R>(v); }
+            if constexpr (NB > 7) if (b == 7) { twiddle<N, V, R, NS, 7 * R>(v, j, tw); dft_r<V, R, 7 * R>(v); }
         }
         const int base = (j / NS) * NS * R + (j & (NS - 1));
 #pragma unroll
@@ -195,31 +197,31 @@ __device__ __forceinline__ void pass(float2 (&v)[16], int t, float2 *L, const fl
 }
 
 // Passes of span NS .. N after the first; the last one (span N/R) writes natural order.
-template <int N, int NS, bool CONJ>
-__device__ __forceinline__ void passes(float2 (&v)[16], int t, float2 *L, const float2 *__restrict__ tw)
+template <int N, int V, int NS, bool CONJ>
+__device__ __forceinline__ void passes(float2 (&v)[V], int t, float2 *L, const float2 *__restrict__ tw)
 {
     if constexpr (NS < N) {
-        constexpr int R = N / NS >= 16 ? 16 : N / NS;
-        pass<N, R, NS, CONJ && NS * R == N>(v, t, L, tw);
-        passes<N, NS * R, CONJ>(v, t, L, tw);
+        constexpr int R = N / NS >= V ? V : N / NS;
+        pass<N, V, R, NS, CONJ && NS * R == N>(v, t, L, tw);
+        passes<N, V, NS * R, CONJ>(v, t, L, tw);
     }
 }
 
-// First pass (span 1, radix 16) of the thread's butterfly j = t: inputs
-// x[t + r N/16] already in v, outputs at 16 t + r.
-template <int N>
-__device__ __forceinline__ void first_pass(float2 (&v)[16], int t, float2 *L)
+// First pass (span 1, radix V) of the thread's butterfly j = t: inputs
+// x[t + r N/V] already in v, outputs at V t + r.
+template <int N, int V>
+__device__ __forceinline__ void first_pass(float2 (&v)[V], int t, float2 *L)
 {
-    dft16(v);
+    dft_r<V, V, 0>(v);
 #pragma unroll
-    for (int r = 0; r < 16; r++) L[pad(16 * t + r)] = v[r];
+    for (int r = 0; r < V; r++) L[pad(V * t + r)] = v[r];
     __syncthreads();
 }
 
 // The kz up/down split of one column (the AFor separation, FK:52-138) is
-//   first_pass, passes<N, 16, false>   forward transform, X in L
-//   kz_pack                            v = conj(Y) for the inverse
-//   first_pass, passes<N, 16, true>    L = (Re up, Re down), natural order
+//   first_pass, passes<N, V, V, false>   forward transform, X in L
+//   kz_pack                              v = conj(Y) for the inverse
+//   first_pass, passes<N, V, V, true>    L = (Re up, Re down), natural order
 // The spectrum X is weighted by the taper c and split (bins 0..N/2 "up",
 // the rest "down").  Only real parts are needed, so both halves share one
 // inverse transform: with m = -k mod N,
@@ -229,25 +231,25 @@ __device__ __forceinline__ void first_pass(float2 (&v)[16], int t, float2 *L)
 // the block takes part (barriers).  The kernels call the steps in sequence:
 // one function around all of them made the compiler keep about 40 more
 // registers live.
-template <int N>
-__device__ __forceinline__ void kz_pack(float2 (&v)[16], int t, const float2 *L, const float *coef, float sc)
+template <int N, int V>
+__device__ __forceinline__ void kz_pack(float2 (&v)[V], int t, const float2 *L, const float *coef, float sc)
 {
-    constexpr int T = N / 16;
+    constexpr int T = N / V;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        // k = t + r N/16 lies below N/2 exactly when r < 8
+    for (int r = 0; r < V; r++) {
+        // k = t + r N/V lies below N/2 exactly when r < V/2
         const int k = t + r * T, m = (N - k) & (N - 1);
         const float2 xk = L[pad(k)], xm = L[pad(m)];
         const float ck = coef[k] * sc, cm = coef[m] * sc;
         float2 y;
-        if (r < 8) {
+        if (r < V / 2) {
             const float2 u = make_float2(ck * xk.x, ck * xk.y), d = make_float2(cm * xm.x, cm * xm.y);
             y = make_float2(u.x + d.y, u.y + d.x);   // c_k X_k + i conj(c_m X_m)
         } else {
             const float2 u = make_float2(cm * xm.x, cm * xm.y), d = make_float2(ck * xk.x, ck * xk.y);
             y = make_float2(u.x - d.y, d.x - u.y);   // conj(c_m X_m) + i c_k X_k
         }
-        if ((r == 0 || r == 8) && t == 0) y = make_float2(2.f * ck * xk.x, 0.f);   // DC and Nyquist are "up"
+        if ((r == 0 || r == V / 2) && t == 0) y = make_float2(2.f * ck * xk.x, 0.f);   // DC and Nyquist are "up"
         v[r] = conjf2(y);
     }
     __syncthreads();   // the spectrum is read before the inverse's first pass overwrites it

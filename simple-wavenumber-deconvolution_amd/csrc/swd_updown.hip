@@ -22,10 +22,10 @@
 
 namespace {
 
-template <int LOGN>
-__global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const KUpDown *__restrict__ shots, int it)
+template <int LOGN, int V>
+__global__ __launch_bounds__(2 * (1 << LOGN) / V) void k_updown(KGrid g, const KUpDown *__restrict__ shots, int it)
 {
-    constexpr int N = 1 << LOGN, T = N / 16, NP = N + N / 16;
+    constexpr int N = 1 << LOGN, T = N / V, NP = N + N / 16;
     __shared__ float2 lds[2 * NP];
     const int col = g.pmlc + blockIdx.x;
     const KUpDown &U = shots[blockIdx.y];
@@ -40,17 +40,17 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const 
     const float2 *__restrict__ tw = U.tw;
 
     // transform of the column, zero padded to N, split, inverse transforms
-    float2 v[16];
+    float2 v[V];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
+    for (int r = 0; r < V; r++) {
         const int i = t + r * T;
         v[r] = i < nzw ? make_float2(re[i], im[i]) : make_float2(0.f, 0.f);
     }
-    first_pass<N>(v, t, L);
-    passes<N, 16, false>(v, t, L, tw);
-    kz_pack<N>(v, t, L, U.coef, 0.5f / static_cast<float>(N));   // 1/(2N): a power of two, exact
-    first_pass<N>(v, t, L);
-    passes<N, 16, true>(v, t, L, tw);
+    first_pass<N, V>(v, t, L);
+    passes<N, V, V, false>(v, t, L, tw);
+    kz_pack<N, V>(v, t, L, U.coef, 0.5f / static_cast<float>(N));   // 1/(2N): a power of two, exact
+    first_pass<N, V>(v, t, L);
+    passes<N, V, V, true>(v, t, L, tw);
 
     // imaging of the interior rows
     const float2 *S = lds, *Rv = lds + NP;
@@ -72,12 +72,12 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / 16) void k_updown(KGrid g, const 
 hipError_t launch_updown(const KGrid &g, const KUpDown *shots, int nshot, int nfftz, int it, hipStream_t s)
 {
     const dim3 grid(g.nx, nshot);
-#define UD(LOGN)                                                                                         \
-    case 1 << LOGN:                                                                                      \
-        hipLaunchKernelGGL((k_updown<LOGN>), grid, dim3(2 * (1 << LOGN) / 16), 0, s, g, shots, it); \
+#define UD(LOGN, V)                                                                                   \
+    case 1 << LOGN:                                                                                       \
+        hipLaunchKernelGGL((k_updown<LOGN, V>), grid, dim3(2 * (1 << LOGN) / V), 0, s, g, shots, it); \
         break;
     switch (nfftz) {
-        UD(7) UD(8) UD(9) UD(10) UD(11) UD(12)
+        UD(7, 8) UD(8, 8) UD(9, 8) UD(10, 16) UD(11, 16) UD(12, 16)
     default:
         return hipErrorInvalidValue;
     }
