@@ -26,11 +26,9 @@
 
 namespace {
 
-constexpr double kRefPi = 3.1415926;   // RK:10
-
 template <int LC>
 __global__ __launch_bounds__(256) void k_cig(KGrid g, const KCig *__restrict__ shots, int thetan, int delt, float w1,
-                                             float w2, int it)
+                                             float w2, float thr, int it)
 {
     const int nint = g.nx * g.nz;
     // consecutive cells on one XCD: the x stencil's neighbouring columns then
@@ -46,11 +44,10 @@ __global__ __launch_bounds__(256) void k_cig(KGrid g, const KCig *__restrict__ s
     const float vxs = __builtin_fmaf(-g.Bx[o], dplus<LC>(P, o, ld, g.rc, g.idx), K.vxs[s1][o]);
     const float vzs = __builtin_fmaf(-g.Bz[o], dplus<LC>(P, o, 1, g.rc, g.idz), K.vzs[s1][o]);
     const float pr = K.pr[s][o], vxr = K.vxr[s][o], vzr = K.vzr[s][o];
-    const float fx = -ps * vxs, fz = -ps * vzs;
-    const float fxr = pr * vxr, fzr = pr * vzr;
-    const float F = __builtin_fmaf(fx, fxr, fz * fzr);
-    const float G = static_cast<float>(sqrt(static_cast<double>(__builtin_fmaf(fx, fx, fz * fz)))) *
-                    static_cast<float>(sqrt(static_cast<double>(__builtin_fmaf(fxr, fxr, fzr * fzr))));
+    float F, a, b;
+    cig_products(ps, vxs, vzs, pr, vxr, vzr, F, a, b);
+    if (delt == 1 && !cig_may_bin(F, a, b, thr)) return;
+    const float G = cig_norm(a, b);
     float pre = F, pst = G;
     if (delt > 1) {
         float *rF = K.ring, *rG = K.ring + static_cast<size_t>(delt) * nint;
@@ -64,26 +61,16 @@ __global__ __launch_bounds__(256) void k_cig(KGrid g, const KCig *__restrict__ s
             pst += rG[static_cast<size_t>(k) * nint + c];
         }
     }
-    const float tht = pre / pst;
-    const float ac = static_cast<float>(acos(static_cast<double>(tht)));
-    const float ftheta = static_cast<float>(0.5 * static_cast<double>(ac) * 180.0 / kRefPi);
-    if (ftheta != ftheta) return;
-    const float pp = ps * pr;
-    float *bins = K.cig + static_cast<size_t>(c) * thetan;
-#pragma unroll
-    for (int ii = -2; ii <= 2; ii++) {
-        const int itheta = static_cast<int>(ceilf(static_cast<float>(static_cast<double>(ftheta + static_cast<float>(ii)) - 0.5)));
-        const float w = ii == 0 ? 1.f : (ii == 1 || ii == -1) ? w1 : w2;
-        if (itheta >= 2 && itheta < thetan) bins[itheta] = __builtin_fmaf(pp, w, bins[itheta]);
-    }
+    cig_bins(pre, pst, ps * pr, thetan, w1, w2, thr, K.cig + static_cast<size_t>(c) * thetan);
 }
 
 }  // namespace
 
-hipError_t launch_cig(const KGrid &g, const KCig *shots, int nshot, int thetan, int delt, float w1, float w2, int it,
-                      hipStream_t s)
+hipError_t launch_cig(const KGrid &g, const KCig *shots, int nshot, int thetan, int delt, float w1, float w2, float thr,
+                      int it, hipStream_t s)
 {
     const dim3 grid(((g.nx * g.nz + 255) / 256 + 7) / 8 * 8, nshot);   // a multiple of 8 for xcd_block
-    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_cig<L>), grid, dim3(256), 0, s, g, shots, thetan, delt, w1, w2, it));
+    DISPATCH_LC(g.Lc,
+                hipLaunchKernelGGL((k_cig<L>), grid, dim3(256), 0, s, g, shots, thetan, delt, w1, w2, thr, it));
     return hipGetLastError();
 }

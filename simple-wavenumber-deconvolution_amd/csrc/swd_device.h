@@ -47,6 +47,58 @@ __device__ __forceinline__ float dminus(const float *__restrict__ f, size_t ip, 
     return d;
 }
 
+// Angle-gather pieces of one cell (the reference's sum_image_GPU_kernel_imageCIG,
+// RK:859-901).  cig_products: F = S.R and the squared norms a = |S|^2,
+// b = |R|^2 of the source vector S = (-p_s vx_s, -p_s vz_s) and the receiver
+// vector R = (p_r vx_r, p_r vz_r) (+p v: the receiver field is stepped with
+// (-dx, -dz)).
+__device__ __forceinline__ void cig_products(float ps, float vxs, float vzs, float pr, float vxr, float vzr, float &F,
+                                             float &a, float &b)
+{
+    const float fx = -ps * vxs, fz = -ps * vzs;
+    const float fxr = pr * vxr, fzr = pr * vzr;
+    F = __builtin_fmaf(fx, fxr, fz * fzr);
+    a = __builtin_fmaf(fx, fx, fz * fz);
+    b = __builtin_fmaf(fxr, fxr, fzr * fzr);
+}
+
+// G = |S||R| with sqrt in double rounded to float.
+__device__ __forceinline__ float cig_norm(float a, float b)
+{
+    return static_cast<float>(sqrt(static_cast<double>(a))) * static_cast<float>(sqrt(static_cast<double>(b)));
+}
+
+// A cheap test that the angle of F/G cannot reach a bin: thr is a lower
+// bound, with margin, of the cosine the last bin needs (-2 when every angle
+// can bin).  Rejected cells bin nothing either way, so results do not change.
+__device__ __forceinline__ bool cig_may_bin(float F, float a, float b, float thr)
+{
+    const float t = F / (__builtin_sqrtf(a) * __builtin_sqrtf(b));
+    return t >= thr;
+}
+
+// Binning of summed products: the half-angle in degrees
+// 0.5 acos(pre/pst) 180 / 3.1415926 (acos in double rounded to float), and
+// the bins ceil(angle + ii - 0.5), ii = -2..2, inside [2, thetan), each taking
+// fma(pp, w_|ii|, bin) with w = 1, w1, w2 (exp(-ii^2/18) rounded to float).
+// An undefined angle adds nothing.
+__device__ __forceinline__ void cig_bins(float pre, float pst, float pp, int thetan, float w1, float w2, float thr,
+                                         float *bins)
+{
+    const float tht = pre / pst;
+    if (!(tht >= thr)) return;   // no bin in reach (or undefined)
+    const float ac = static_cast<float>(acos(static_cast<double>(tht)));
+    const float ftheta = static_cast<float>(0.5 * static_cast<double>(ac) * 180.0 / 3.1415926);
+    if (ftheta != ftheta) return;
+#pragma unroll
+    for (int ii = -2; ii <= 2; ii++) {
+        const int itheta =
+            static_cast<int>(ceilf(static_cast<float>(static_cast<double>(ftheta + static_cast<float>(ii)) - 0.5)));
+        const float w = ii == 0 ? 1.f : (ii == 1 || ii == -1) ? w1 : w2;
+        if (itheta >= 2 && itheta < thetan) bins[itheta] = __builtin_fmaf(pp, w, bins[itheta]);
+    }
+}
+
 typedef __attribute__((address_space(1))) float gfl;
 typedef __attribute__((address_space(1))) const float gcfl;
 

@@ -31,7 +31,11 @@
 // A backward launch pairs the SRC and RCV wavefields of a shot in one
 // workgroup per tile: the reconstruction's dp/dt stays in registers and the
 // imaging condition of step it (RK:785-817) is applied right after the
-// receiver pressure of step it, so dp/dt never goes to memory.
+// receiver pressure of step it, so dp/dt never goes to memory.  The CIG
+// variant also bins the angle gathers of step it+1 (RK:859-901): the
+// reconstruction's p and velocities at that level are in LDS during the
+// source half (kept in registers), and the receiver field's are its previous
+// slot.
 #include <algorithm>
 
 #include "swd_device.h"
@@ -102,10 +106,10 @@ struct FGeo {
 // memory, a frame index or an interior test; the per-cell arithmetic is the
 // same as in the general case (psi = 0 there; only the sign of an exact zero
 // may differ, through the omitted "+ 0").
-template <int LC, int TX, bool FULL>
+template <int LC, int TX, bool FULL, bool CIG>
 __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X0, int Z0, int it, int first,
                                            float4 *__restrict__ sP, float4 *__restrict__ sVX, float4 *__restrict__ sVZ,
-                                           float4 (&dpa)[FGeo<LC, TX>::NOWN])
+                                           float4 (&dpa)[FGeo<LC, TX>::NOWN], float4 (&cgk)[3][FGeo<LC, TX>::NOWN])
 {
     using G = FGeo<LC, TX>;
     constexpr int R = G::R, RP = G::RP, OWN = G::OWN, NPQ = G::NPQ, VZQ = G::VZQ;
@@ -385,6 +389,13 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                 }
                 dpa[r] = dp;
                 if (I.dpdt) st4((gfl *)I.dpdt, o, dp);   // for the up/down imaging that follows the step
+                if constexpr (CIG) {
+                    // the reconstruction at step it+1 for the receiver half's angle gathers:
+                    // p as staged, the velocities just updated to that level
+                    cgk[0][r] = pold;
+                    cgk[1][r] = sVX[(c + LC) * 16 + q];
+                    cgk[2][r] = sVZ[c * VZQ + q + R];
+                }
             }
             if (hassrc) add_at(pv, I.s_iz - z, -ldg(I.wavelet, it + 1));   // RK:478-481
             if (!FULL) {
@@ -498,12 +509,30 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
             st4((gfl *)I.illum_s, o, is);
             st4((gfl *)I.illum_r, o, ir);
         }
+        if constexpr (CIG) {
+            // angle gathers of step it+1 (RK:859-901) from both fields at that level: the
+            // reconstruction kept from the source half, the receiver field's previous slot
+            if (neg && !first && I.cig && colint) {
+                const float4 vxr = ld4((gcfl *)I.vx[po], o), vzr = ld4((gcfl *)I.vz[po], o);
+                const float4 ps = cgk[0][r], vxs = cgk[1][r], vzs = cgk[2][r];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int zk = z + k;
+                    if (!(FULL || (zk >= pmlc && zk < ntz - pmlc))) continue;
+                    float F, a, b;
+                    cig_products(elc(ps, k), elc(vxs, k), elc(vzs, k), elc(pold, k), elc(vxr, k), elc(vzr, k), F, a, b);
+                    if (!cig_may_bin(F, a, b, I.cthr)) continue;
+                    float *bins = I.cig + (static_cast<size_t>(x - pmlc) * g.nz + (zk - pmlc)) * I.thetan;
+                    cig_bins(F, cig_norm(a, b), elc(ps, k) * elc(pold, k), I.thetan, I.cw1, I.cw2, I.cthr, bins);
+                }
+            }
+        }
     }
 }
 
 // PAIR = false: one FWD wavefield per instance.  PAIR = true: instances
 // 2k (SRC) and 2k+1 (RCV) of one shot share each workgroup.
-template <int LC, int TX, bool PAIR>
+template <int LC, int TX, bool PAIR, bool CIG>
 __global__ __launch_bounds__(fstep_threads<TX>()) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb,
                                                               int nxb,
                                                int it, int first)
@@ -520,13 +549,13 @@ __global__ __launch_bounds__(fstep_threads<TX>()) void k_fstep(KGrid g, const KS
     const int X0 = xb * TX, Z0 = zb * 64;
     const bool full = X0 - G::OWN >= g.fx0 && X0 + TX + 2 * LC - 2 <= g.fx1 && Z0 - 4 * G::RP >= g.fz0 &&
                       Z0 + 64 + 4 * G::RP - 1 <= g.fz1;
-    float4 dpa[G::NOWN];
+    float4 dpa[G::NOWN], cgk[3][G::NOWN];
 #pragma unroll
-    for (int r = 0; r < G::NOWN; r++) dpa[r] = zero4();
+    for (int r = 0; r < G::NOWN; r++) dpa[r] = cgk[0][r] = cgk[1][r] = cgk[2][r] = zero4();
     if (!PAIR) {
         const KStep &I = insts[inst];
-        if (full) fstep_tile<LC, TX, true>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa);
-        else fstep_tile<LC, TX, false>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa);
+        if (full) fstep_tile<LC, TX, true, false>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        else fstep_tile<LC, TX, false, false>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
         return;
     }
     const KStep &S = insts[2 * inst], &Rv = insts[2 * inst + 1];
@@ -534,12 +563,12 @@ __global__ __launch_bounds__(fstep_threads<TX>()) void k_fstep(KGrid g, const KS
     const bool any = X0 + TX > g.pmlc && X0 < g.ntx - g.pmlc && Z0 + 64 > g.pmlc && Z0 < g.ntz - g.pmlc;
     const bool src = S.s_ix >= X0 && S.s_ix < X0 + TX && S.s_iz >= Z0 && S.s_iz < Z0 + 64;
     if (any || src) {
-        if (full) fstep_tile<LC, TX, true>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa);
-        else fstep_tile<LC, TX, false>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa);
+        if (full) fstep_tile<LC, TX, true, CIG>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        else fstep_tile<LC, TX, false, CIG>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
         __syncthreads();   // LDS is staged again for the receiver wavefield
     }
-    if (full) fstep_tile<LC, TX, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa);
-    else fstep_tile<LC, TX, false>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa);
+    if (full) fstep_tile<LC, TX, true, CIG>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+    else fstep_tile<LC, TX, false, CIG>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
 }
 
 // Multi-grid launch: instances of different grid sizes (same Lc) in one
@@ -566,10 +595,10 @@ __global__ __launch_bounds__(256) void k_fstep_multi(const KStep *__restrict__ i
     const int X0 = xb * TX, Z0 = zb * 64;
     const bool full = X0 - G::OWN >= g.fx0 && X0 + TX + 2 * LC - 2 <= g.fx1 && Z0 - 4 * G::RP >= g.fz0 &&
                       Z0 + 64 + 4 * G::RP - 1 <= g.fz1;
-    float4 dpa[G::NOWN];
+    float4 dpa[G::NOWN], cgk[3][G::NOWN];
     dpa[0] = zero4();
-    if (full) fstep_tile<LC, TX, true>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa);
-    else fstep_tile<LC, TX, false>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa);
+    if (full) fstep_tile<LC, TX, true, false>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa, cgk);
+    else fstep_tile<LC, TX, false, false>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa, cgk);
 }
 
 }  // namespace
@@ -589,27 +618,39 @@ hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, b
 hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair,
                         hipStream_t s)
 {
+    return launch_fstep(g, insts, ninst, it, first, tx, pair, false, s);
+}
+
+hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair, bool cig,
+                        hipStream_t s)
+{
     const int n = pair ? ninst / 2 : ninst;
     const int nzb = (g.ntz + 63) / 64;
     const int nxb = (g.ntx + tx - 1) / tx;
     const int nblocks = (n * nzb * nxb + 7) / 8 * 8;
     const int f = first ? 1 : 0;
-#define LF(TXV, PV)                                                                                                   \
-    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_fstep<L, TXV, PV>), dim3(nblocks), dim3(FGeo<L, TXV>::NT), 0, s, g, insts, n, \
-                                         nzb, nxb, \
-                                         it, f))
-    if (pair) {
+#define LF(TXV, PV, CV)                                                                                               \
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_fstep<L, TXV, PV, CV>), dim3(nblocks), dim3(FGeo<L, TXV>::NT), 0, s, g, \
+                                         insts, n, nzb, nxb, it, f))
+    if (pair && cig) {
         switch (tx) {
-        case 16: LF(16, true); break;
-        case 32: LF(32, true); break;
-        case 64: LF(64, true); break;
+        case 16: LF(16, true, true); break;
+        case 32: LF(32, true, true); break;
+        case 64: LF(64, true, true); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else if (pair) {
+        switch (tx) {
+        case 16: LF(16, true, false); break;
+        case 32: LF(32, true, false); break;
+        case 64: LF(64, true, false); break;
         default: return hipErrorInvalidValue;
         }
     } else {
         switch (tx) {
-        case 16: LF(16, false); break;
-        case 32: LF(32, false); break;
-        case 64: LF(64, false); break;
+        case 16: LF(16, false, false); break;
+        case 32: LF(32, false, false); break;
+        case 64: LF(64, false, false); break;
         default: return hipErrorInvalidValue;
         }
     }

@@ -66,6 +66,10 @@ struct KStep {
     int capcomp, ncap, nfftz;
     float *image, *illum_s, *illum_r;   // RCV (paired with the SRC of the same shot)
     float *dpdt;                 // SRC: dp/dt of step it stored on the interior [ntx][ldz], or null
+    float *cig;                  // RCV, angle gathers in the paired launch: [nx*nz][thetan], or null
+    int thetan;
+    float cw1, cw2;              // bin weights exp(-1/18), exp(-4/18) rounded to float
+    float cthr;                  // cosine below which no bin is in reach
 };
 
 // One shot of the angle-gather update that follows a backward step: the

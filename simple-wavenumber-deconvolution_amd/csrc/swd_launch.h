@@ -17,6 +17,9 @@ hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, 
 hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, hipStream_t s);
 hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair,
                         hipStream_t s);
+// cig: the paired backward also bins the angle gathers of step it+1 (RCV instances' cig)
+hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair, bool cig,
+                        hipStream_t s);
 // multi-grid forward step: btab[nblk] = (instance, tile), tile width 16
 hipError_t launch_fstep_multi(int Lc, const KStep *insts, const int2 *btab, int nblk, int it, hipStream_t s);
 hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2 *tw, const float *coef,
@@ -24,6 +27,7 @@ hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2
 // up/down imaging of step it for nshot shots (KUpDown each) over the interior columns
 hipError_t launch_updown(const KGrid &g, const KUpDown *shots, int nshot, int nfftz, int it, hipStream_t s);
 // angle-gather update of step it for nshot shots (KCig each) over the interior cells;
-// w1, w2: the bin weights exp(-1/18), exp(-4/18) rounded to float
-hipError_t launch_cig(const KGrid &g, const KCig *shots, int nshot, int thetan, int delt, float w1, float w2, int it,
-                      hipStream_t s);
+// w1, w2: the bin weights exp(-1/18), exp(-4/18) rounded to float; thr: the
+// cosine below which no bin is in reach (cig_bins)
+hipError_t launch_cig(const KGrid &g, const KCig *shots, int nshot, int thetan, int delt, float w1, float w2, float thr,
+                      int it, hipStream_t s);

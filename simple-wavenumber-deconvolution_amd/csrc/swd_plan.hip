@@ -95,6 +95,7 @@ struct Plan {
     int tile_req = 0;           // fused tile width asked for (0 = tune)
     int group_fwd = 0, group_bwd = 0;     // RTM shots per fused forward / backward loop (0 = sized to the cache)
     std::map<int, int> tuned_fwd, tuned_bwd;   // tuned tile width by instances per launch
+    std::map<int, int> tuned_cig;              // the same for the backward with angle gathers
     int streams = 2;            // RTM shot groups run on this many streams at once (1 .. kMaxStreams)
     hipStream_t side[3] = {};   // the other streams (default priority)
     hipEvent_t fork = nullptr, join[3] = {};
@@ -471,6 +472,7 @@ extern "C" int swd_plan_set_tile(void *plan, int tile_x)
     P->tile_req = tile_x;
     P->tuned_fwd.clear();
     P->tuned_bwd.clear();
+    P->tuned_cig.clear();
     return SWD_OK;
 }
 
@@ -585,7 +587,8 @@ struct StepHook {
 };
 
 int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int first_it, std::map<int, int> &tuned,
-                long long *launches, bool pair = false, hipStream_t st = nullptr, const StepHook *hook = nullptr)
+                long long *launches, bool pair = false, hipStream_t st = nullptr, const StepHook *hook = nullptr,
+                bool cig = false)
 {
     if (!st) st = P.stream;
     int tx = P.tile_req;
@@ -596,7 +599,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     size_t k = 0;
     auto launch = [&](int w) -> int {
         const int it = its[k++];
-        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, pair, st));
+        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, pair, cig, st));
         (*launches)++;
         if (hook) {
             SWD_TRY(hook->run(it, st));
@@ -631,14 +634,14 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     }
     if (k < its.size() && P.graphs) {
         std::vector<long long> key = {1, reinterpret_cast<long long>(d), n, its[k], static_cast<long long>(its.size() - k),
-                                      first_it, tx, pair ? 1 : 0, reinterpret_cast<long long>(P.ws)};
+                                      first_it, tx, (pair ? 1 : 0) + (cig ? 2 : 0), reinterpret_cast<long long>(P.ws)};
         if (hook) key.insert(key.end(), hook->key.begin(), hook->key.end());
         const size_t k0 = k;
         SWD_TRY(graph_run(
             P, key,
             [&]() -> int {
                 for (size_t j = k0; j < its.size(); j++) {
-                    HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, tx, pair, st));
+                    HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, tx, pair, cig, st));
                     if (hook) SWD_TRY(hook->run(its[j], st));
                 }
                 return SWD_OK;
@@ -1093,6 +1096,29 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
         R.illum_s = im + nf;
         R.illum_r = im + 2 * nf;
     }
+    // bin weights exp(-ii^2/18) of the angle gathers, rounded to float
+    const float w1 = static_cast<float>(std::exp(-static_cast<double>(static_cast<float>(1.0 / 18.0))));
+    const float w2 = static_cast<float>(std::exp(-static_cast<double>(static_cast<float>(4.0 / 18.0))));
+    // With delt = 1 the paired backward launch of step it bins the gathers of
+    // step it+1 itself (the fields of that level are at hand there); k_cig
+    // then only adds step 0 after the loop.  Averaging over delt > 1 steps
+    // runs k_cig after every launch.
+    const bool fuse_cig = cig && cig->delt == 1;
+    // A bin needs half-angle <= thetan + 1.5 degrees, that is cos(full angle)
+    // >= cos((thetan + 1.5) pi / 90); the test uses thetan + 2 and a margin far
+    // above the float error of its estimate.
+    const float cthr = !cig || cig->thetan + 2 >= 90
+                           ? -2.f
+                           : static_cast<float>(std::cos((cig->thetan + 2) * 3.14159265358979323846 / 90.0) - 1e-3);
+    if (fuse_cig)
+        for (int k = 0; k < nshots; k++) {
+            KStep &R = hb[2 * k + 1];
+            R.cig = at<float>(P, o_cig) + nint * cig->thetan * k;
+            R.thetan = cig->thetan;
+            R.cw1 = w1;
+            R.cw2 = w2;
+            R.cthr = cthr;
+        }
     SWD_TRY(upload_steps(P, hf, o_fwd));
     SWD_TRY(upload_steps(P, hb, o_bwd));
     if (cig) {
@@ -1155,24 +1181,27 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     stats_begin(P, 1);
     SWD_TRY(fork());
     const std::vector<int> gbb = group_bounds(nshots, gb);
-    // bin weights exp(-ii^2/18) of the angle gathers, rounded to float
-    const float w1 = static_cast<float>(std::exp(-static_cast<double>(static_cast<float>(1.0 / 18.0))));
-    const float w2 = static_cast<float>(std::exp(-static_cast<double>(static_cast<float>(4.0 / 18.0))));
     for (size_t g = 0; g + 1 < gbb.size(); g++) {
         const int k0 = gbb[g], n = gbb[g + 1] - gbb[g];
         const KStep *d = at<KStep>(P, o_bwd) + 2 * k0;
+        const hipStream_t gs = group_stream(static_cast<int>(g));
+        if (fuse_cig) {
+            SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tuned_cig, &launches, true, gs, nullptr, true));
+            HIP_TRY(launch_cig(P.kg, at<KCig>(P, o_kcig) + k0, n, cig->thetan, 1, w1, w2, cthr, 0, gs));   // step 0
+            launches++;
+            continue;
+        }
         StepHook hook;
         if (cig) {
             const KCig *dk = at<KCig>(P, o_kcig) + k0;
             const int thetan = cig->thetan, delt = cig->delt;
-            hook.run = [&P, dk, n, thetan, delt, w1, w2](int it, hipStream_t st) -> int {
-                HIP_TRY(launch_cig(P.kg, dk, n, thetan, delt, w1, w2, it, st));
+            hook.run = [&P, dk, n, thetan, delt, w1, w2, cthr](int it, hipStream_t st) -> int {
+                HIP_TRY(launch_cig(P.kg, dk, n, thetan, delt, w1, w2, cthr, it, st));
                 return SWD_OK;
             };
             hook.key = {3, reinterpret_cast<long long>(dk), thetan, delt};
         }
-        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tuned_bwd, &launches, true, group_stream(static_cast<int>(g)),
-                            cig ? &hook : nullptr));
+        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tuned_bwd, &launches, true, gs, cig ? &hook : nullptr));
     }
     SWD_TRY(join());
     P.stats.bwd_ms = stats_end(P, 1);
