@@ -533,7 +533,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 // PAIR = false: one FWD wavefield per instance.  PAIR = true: instances
 // 2k (SRC) and 2k+1 (RCV) of one shot share each workgroup.
 template <int LC, int TX, bool PAIR, bool CIG>
-__global__ __launch_bounds__(fstep_threads<TX>()) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb,
+__global__ __launch_bounds__(fstep_threads<TX>(), CIG ? 4 : 1) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb,
                                                               int nxb,
                                                int it, int first)
 {
