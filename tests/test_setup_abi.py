@@ -48,6 +48,21 @@ def test_argument_errors():
         swd.kz_taper(100, 10.0)   # not a power of two
 
 
+def test_null_plan_rejected_by_every_plan_entry_point():
+    """Entry points of the later imaging modes refuse a null plan (no device needed)."""
+    L = swd.load()
+    null = C.c_void_p(0)
+    out = swd.UpDownImages()
+    s = C.c_void_p()
+    assert L.swd_rtm_updown(null, 1, null, null, null, null, null, null, 128, C.byref(out)) < 0
+    assert L.swd_rtm_updown_dev(null, 1, null, null, null, null, null, null, 128, C.byref(out)) < 0
+    assert L.swd_updown_image_dev(null, 1, null, null, null, null, null, 128, C.byref(out)) < 0
+    assert L.swd_rtm_cig(null, 1, null, null, null, 10, 1, null, null, null, null) < 0
+    assert L.swd_rtm_cig_dev(null, 1, null, null, null, 10, 1, null, null, null, null) < 0
+    assert L.swd_plan_get_stream(null, C.byref(s)) < 0
+    assert "null plan" in swd.last_error()
+
+
 PARAMS = [("--x grid number: nx", "845"), ("--z grid number: nz", "702"), ("--PML numbers: pml", "16"),
           ("--FD order: Lc", "5"), ("--x grid interval (m): dx", "10.0"), ("--z grid interval (m): dz", "10.0"),
           ("--recording time: itmax", "7000"), ("--time sample (s): dt", "1.0e-3"),
