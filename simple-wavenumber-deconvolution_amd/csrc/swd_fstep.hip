@@ -532,8 +532,11 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 
 // PAIR = false: one FWD wavefield per instance.  PAIR = true: instances
 // 2k (SRC) and 2k+1 (RCV) of one shot share each workgroup.
+// The angle-gather variant at Lc 4 and tiles up to 32 wide is held to four
+// wavefronts per SIMD (128 VGPRs, 8-12 bytes of spill, faster than three
+// wavefronts); elsewhere that bound would spill heavily.
 template <int LC, int TX, bool PAIR, bool CIG>
-__global__ __launch_bounds__(fstep_threads<TX>(), CIG ? 4 : 1) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb,
+__global__ __launch_bounds__(fstep_threads<TX>(), (CIG && LC == 4 && TX <= 32) ? 4 : 1) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb,
                                                               int nxb,
                                                int it, int first)
 {
