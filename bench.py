@@ -408,6 +408,132 @@ def bench_cig(a):
     print(json.dumps(out), flush=True)
 
 
+def afor_bytes_per_step(g, nrec):
+    """Algorithmic HBM bytes of one AFor step of one shot: the p and Hilbert
+    wavefields' state and CPML memories, the model read once for both, the
+    gathers (the kz transforms of the receiver columns add under 0.1%)."""
+    N = g.ntx * g.ntz
+    P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
+    return 56 * N + 32 * P + 8 * nrec
+
+
+def bench_afor(a):
+    """AFor of the c2 batch: one step = the p and Hilbert-companion forward
+    wavefields of every shot (nt steps each), the receiver-column capture and
+    the kz up/down separation of the gathers (nfftz by the reference's rule,
+    the padded column length rounded up to a power of two), gathers left in HBM."""
+    import torch
+    import swd
+
+    log(f"bench: c2 AFor, batch {a.batch} shots, nt {a.nt}")
+    case = build_case(a.batch, a.nt)
+    g = case.grid
+    plan = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0, tile=a.tile)
+    plan.set_groups(*a.groups)
+    plan.set_streams(a.streams)
+    shots = swd.ShotSet(case.shots)
+    nfftz = swd.nfft(g.ntz)
+    kz, _ = swd.kz_taper(nfftz, g.dz)
+    hw = swd.hilbert1d(case.wavelet)
+    dev = torch.device("cuda:0")
+    d = torch.zeros((3, shots.ntr, g.nt), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def step(pl, sh, out):
+        pl.afor_dev(sh, case.wavelet, hw, kz, out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr())
+
+    copy_gbs = copy_bandwidth(torch, dev)
+    for i in range(a.warmup):
+        step(plan, shots, d)
+        log(f"bench: warmup {i + 1}/{a.warmup} done")
+    torch.cuda.synchronize()
+    fwd_ms = fft_ms = 0.0
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        step(plan, shots, d)
+        st = plan.stats()
+        fwd_ms += st["fwd_ms"]
+        fft_ms += st["fft_ms"]
+        log(f"bench: step {i + 1}/{a.steps} wavefields {st['fwd_ms']:.1f} ms, kz separation {st['fft_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    N = g.ntx * g.ntz
+    cells = 2 * g.nt * N * a.batch
+    shots_per_s = a.batch * a.steps / elapsed
+    nrec = len(case.shots[0][2])
+    st = plan.stats()
+    launch_ms = fwd_ms / a.steps / max(1, st["fwd_launches"])
+    launches_per_step = st["fwd_launches"] / g.nt
+    bytes_per_launch = afor_bytes_per_step(g, nrec) * a.batch / launches_per_step
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    up0 = d[1, :nrec].cpu().numpy()
+
+    plain = None
+    if not a.no_plain:
+        one = swd.ShotSet(case.shots[:1])
+        pp = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0, path=swd.SWD_PATH_PLAIN)
+        d1 = torch.zeros((3, one.ntr, g.nt), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        step(pp, one, d1)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t1
+        pp.close()
+        rel = float(np.linalg.norm(d1[1].cpu().numpy() - up0) / max(1e-30, np.linalg.norm(up0)))
+        plain = {"shots_per_s": 1.0 / tp, "gcell_per_s": 2 * g.nt * N / tp / 1e9, "up_rel_l2_vs_batched": rel}
+        log(f"bench: plain path one shot {tp:.2f}s, up-going gather rel diff vs batched {rel:.2e}")
+        del d1
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        from helpers import oracle_problem
+
+        threads = min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        true_case = swd.configs.Case(case.name, g, case.vp_true, case.rho, case.coefs, case.wavelet,
+                                     case.shots[:2], case.f0)
+        t1 = time.perf_counter()
+        for k in range(2):
+            oracle_problem(true_case, k).afor(hw, kz)
+        dt = time.perf_counter() - t1
+        cpu = {"value": 2 * 2 * g.nt * N / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"oracle fp32 host loops (OpenMP) on 2 c2 shots with nt={g.nt} (p and Hilbert "
+                         f"wavefields, capture, kz separation), {dt:.1f}s"}
+        log(f"bench: cpu baseline {cpu['value']:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
+
+    out = {
+        "metric": "AFor Gcell-updates/s (c2 batch, p and Hilbert wavefields, kz up/down gathers)",
+        "value": cells * a.steps / elapsed / 1e9,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered model with dipping reflectors, c2 geometry)",
+        "config": {"workload": f"c2 batched AFor: 502x845 interior (590x933 padded), nt {g.nt}, Lc 4, "
+                               f"{nrec} borehole receivers, nfftz {nfftz}",
+                   "shots_per_step": a.batch, "shots_per_s": shots_per_s, "plain_path": plain,
+                   "speedup_vs_plain_shots_per_s": (shots_per_s / plain["shots_per_s"]) if plain else None,
+                   "wavefield_ms_per_step": fwd_ms / a.steps, "kz_separation_ms_per_step": fft_ms / a.steps,
+                   "up_finite": bool(np.isfinite(up0).all())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "kernel": "k_fstep (forward step of a group of p and Hilbert wavefields with capture)",
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": launch_ms,
+                     "timing": "HIP events on the plan's stream around the wavefield loop (joining the other "
+                               "streams) / launches: the interval between launch completions"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,9 +553,10 @@ def main():
     ap.add_argument("--streams", type=int, default=0, choices=(0, 1, 2, 3, 4),
                     help="RTM shot groups in flight at once (swd_plan_set_streams); 0: 3 for --workload updown, "
                          "else 2 (the fastest measured)")
-    ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig"), default="rtm",
+    ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig", "afor"), default="rtm",
                     help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution; "
-                         "updown: the up/down-separated RTM; cig: RTM with angle gathers")
+                         "updown: the up/down-separated RTM; cig: RTM with angle gathers; afor: AFor forward "
+                         "modelling with the kz up/down separation")
     ap.add_argument("--thetan", type=int, default=10, help="angle bins of --workload cig (RC:235: 10)")
     ap.add_argument("--nfftz", type=int, default=0,
                     help="kz transform length of --workload updown (0: nz padded to a power of two, at least 128)")
@@ -449,6 +576,9 @@ def main():
         return
     if a.workload == "cig":
         bench_cig(a)
+        return
+    if a.workload == "afor":
+        bench_afor(a)
         return
 
     import torch

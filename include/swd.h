@@ -175,6 +175,11 @@ int swd_forward(void *plan, int nshots, const swd_shot *shots, const float *wave
 int swd_afor(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
              const float *hwavelet, const float *kzcoef, int nfftz,
              float *gather, float *up, float *down);
+/* Same as swd_afor with gather, up and down in device memory of the plan's
+ * device ([sum nrec][nt] each). */
+int swd_afor_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+                 const float *hwavelet, const float *kzcoef, int nfftz,
+                 float *gather, float *up, float *down);
 
 /* RTM / PSF imaging of a batch: forward with boundary strips (inv_flag=1,
  * RK:239-432), then source reconstruction, receiver back-propagation of

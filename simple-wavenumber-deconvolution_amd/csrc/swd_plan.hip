@@ -661,12 +661,89 @@ std::vector<int> steps_up(int nt)
     return v;
 }
 
-// Forward time loop of the fused path.
+// Shots per group of the fused RTM loops.  The per-shot working set of a
+// step (double-buffered state, CPML memories in the frame, and on the
+// backward the reconstruction, images and dp/dt) is kept within a budget of
+// the 256 MB last-level cache, leaving room for the model, the halos and the
+// streamed strips; groups are then evened out.  The forward, which also
+// writes its strips every step, runs best with a smaller budget (measured at
+// c2: 4-shot groups at 287-290 ms per 16-shot step, 6-shot at 298-313,
+// 8-shot at 317).
+constexpr double kCacheBudgetBwd = 192.0 * (1 << 20);
+constexpr double kCacheBudgetFwd = 128.0 * (1 << 20);
+
+int even_groups(int n, int g)
+{
+    g = std::max(1, std::min(n, g));
+    const int ng = (n + g - 1) / g;
+    return (n + ng - 1) / ng;
+}
+
+// Group boundaries of n shots in groups of at most g: ceil(n/g) groups whose
+// sizes differ by at most one (the larger ones first).
+std::vector<int> group_bounds(int n, int g)
+{
+    const int ng = (n + g - 1) / g, base = n / ng, extra = n % ng;
+    std::vector<int> b(1, 0);
+    for (int i = 0; i < ng; i++) b.push_back(b.back() + base + (i < extra ? 1 : 0));
+    return b;
+}
+
+void rtm_groups(const Plan &P, int nshots, int &gf, int &gb)
+{
+    const double N = static_cast<double>(P.ntx) * P.ntz, Nint = static_cast<double>(P.g.nx) * P.g.nz;
+    const double Ncpml = N - Nint;
+    const double fwd = 4.0 * (6 * N + 6 * Ncpml);
+    const double bwd = 4.0 * (9 * Nint + 6 * N + 6 * Ncpml);
+    // groups in flight on the plan's streams share the cache
+    gf = P.group_fwd > 0 ? P.group_fwd : static_cast<int>(kCacheBudgetFwd / P.streams / fwd);
+    gb = P.group_bwd > 0 ? P.group_bwd : static_cast<int>(kCacheBudgetBwd / P.streams / bwd);
+    gf = even_groups(nshots, gf);
+    gb = even_groups(nshots, gb);
+}
+
+// With several streams, consecutive shot groups take them in turn so that
+// one group's launch gaps and tails overlap the others' work.
+hipStream_t rtm_group_stream(const Plan &P, int g)
+{
+    const int ns = P.streams;
+    return g % ns == 0 ? P.stream : P.side[g % ns - 1];
+}
+
+// The other streams wait for the work queued so far on the plan's stream.
+int fork_streams(Plan &P)
+{
+    if (P.streams == 1) return SWD_OK;
+    HIP_TRY(hipEventRecord(P.fork, P.stream));
+    for (int i = 0; i < P.streams - 1; i++) HIP_TRY(hipStreamWaitEvent(P.side[i], P.fork, 0));
+    return SWD_OK;
+}
+
+// The plan's stream waits for the work queued so far on the other streams.
+int join_streams(Plan &P)
+{
+    for (int i = 0; i < P.streams - 1; i++) {
+        HIP_TRY(hipEventRecord(P.join[i], P.side[i]));
+        HIP_TRY(hipStreamWaitEvent(P.stream, P.join[i], 0));
+    }
+    return SWD_OK;
+}
+
+// Forward time loop of the fused path: n independent wavefield instances in
+// groups whose per-step working set stays in the last-level cache, the groups
+// taking the plan's streams in turn (as the RTM forward).
 int fused_forward_loop(Plan &P, const KStep *d, int n)
 {
     long long launches = 0;
+    int gf = 0, gb = 0;
+    rtm_groups(P, n, gf, gb);
     stats_begin(P, 0);
-    SWD_TRY(fused_steps(P, d, n, steps_up(P.g.nt), -1, P.tuned_fwd, &launches));
+    SWD_TRY(fork_streams(P));
+    const std::vector<int> bounds = group_bounds(n, gf);
+    for (size_t g = 0; g + 1 < bounds.size(); g++)
+        SWD_TRY(fused_steps(P, d + bounds[g], bounds[g + 1] - bounds[g], steps_up(P.g.nt), -1, P.tuned_fwd, &launches,
+                            false, rtm_group_stream(P, static_cast<int>(g))));
+    SWD_TRY(join_streams(P));
     P.stats.fwd_ms = stats_end(P, 0);
     P.stats.fwd_launches = launches;
     P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * P.g.nt * n;
@@ -823,15 +900,13 @@ extern "C" int swd_forward(void *plan, int nshots, const swd_shot *shots, const 
     return SWD_OK;
 }
 
-extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *hwavelet,
-                        const float *kzcoef, int nfftz, float *gather, float *up, float *down)
+namespace {
+
+// AFor of a batch with the three gathers in device memory ([sum nrec][nt]
+// each); returns when they are complete.
+int afor_dev(Plan *P, int nshots, const swd_shot *shots, const float *wavelet, const float *hwavelet,
+             const float *kzcoef, int nfftz, float *d_gather, float *d_up, float *d_down)
 {
-    Plan *P;
-    SWD_TRY(check_plan(plan, P));
-    if (!wavelet || !hwavelet || !kzcoef || !gather || !up || !down || !shots || nshots < 1)
-        return swd_fail(SWD_E_ARG, "swd_afor: null argument");
-    if (nfftz < P->ntz || (nfftz & (nfftz - 1)) || nfftz > 8192)
-        return swd_fail(SWD_E_ARG, "nfftz=%d must be a power of two in [ntz=%d, 8192]", nfftz, P->ntz);
     Batch b;
     SWD_TRY(build_batch(*P, nshots, shots, b));
     const int nt = P->g.nt;
@@ -859,7 +934,6 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
         const double ang = -1 * 2.0 * 3.14159265358979323846 * k / nfftz;
         tw[k] = make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
     }
-    const size_t ng = static_cast<size_t>(b.ntr) * nt;
     Carver c;
     const size_t o_ints = c.add<int>(b.ints.size());
     const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
@@ -872,7 +946,6 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     const size_t o_tr = c.add<int2>(std::max<size_t>(1, traces.size()));
     const size_t o_tw = c.add<float2>(nfftz / 2);
     const size_t o_coef = c.add<float>(nfftz);
-    const size_t o_gat = c.add<float>(3 * std::max<size_t>(ng, 1));
     const size_t o_cap = c.add<float2>(static_cast<size_t>(nt) * ncap * nfftz);
     const size_t o_f = c.add<float>(per_wf * P->nfield * 2 * nshots);
     SWD_TRY(ensure_ws(*P, c.off));
@@ -888,7 +961,6 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     HIP_TRY(hipMemcpyAsync(P->ws + o_coef, kzcoef, sizeof(float) * nfftz, hipMemcpyHostToDevice, P->stream));
     HIP_TRY(hipMemsetAsync(P->ws + o_cap, 0, sizeof(float2) * static_cast<size_t>(nt) * ncap * nfftz, P->stream));
     zero_fields(*P, at<float>(*P, o_f), per_wf * P->nfield * 2 * nshots);
-    float *d_gather = at<float>(*P, o_gat), *d_up = d_gather + ng, *d_down = d_up + ng;
     if (fused) {
         std::vector<KStep> hs(2 * nshots);
         for (int k = 0; k < nshots; k++)
@@ -947,82 +1019,50 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
                              at<KCapCol>(*P, o_cols), at<int2>(*P, o_tr), d_up, d_down, nt, P->stream));
         P->stats.fft_ms = stats_end(*P, 1);
     }
-    HIP_TRY(hipMemcpyAsync(gather, d_gather, sizeof(float) * ng, hipMemcpyDeviceToHost, P->stream));
-    HIP_TRY(hipMemcpyAsync(up, d_up, sizeof(float) * ng, hipMemcpyDeviceToHost, P->stream));
-    HIP_TRY(hipMemcpyAsync(down, d_down, sizeof(float) * ng, hipMemcpyDeviceToHost, P->stream));
     HIP_TRY(hipStreamSynchronize(P->stream));
     return SWD_OK;
 }
 
+int afor_args(Plan *P, const char *fn, int nshots, const swd_shot *shots, const float *wavelet, const float *hwavelet,
+              const float *kzcoef, int nfftz, const float *gather, const float *up, const float *down)
+{
+    if (!wavelet || !hwavelet || !kzcoef || !gather || !up || !down || !shots || nshots < 1)
+        return swd_fail(SWD_E_ARG, "%s: null argument", fn);
+    if (nfftz < P->ntz || (nfftz & (nfftz - 1)) || nfftz > 8192)
+        return swd_fail(SWD_E_ARG, "nfftz=%d must be a power of two in [ntz=%d, 8192]", nfftz, P->ntz);
+    return SWD_OK;
+}
+
+}  // namespace
+
+extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *hwavelet,
+                        const float *kzcoef, int nfftz, float *gather, float *up, float *down)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    SWD_TRY(afor_args(P, "swd_afor", nshots, shots, wavelet, hwavelet, kzcoef, nfftz, gather, up, down));
+    const size_t ng = static_cast<size_t>(total_traces(nshots, shots)) * P->g.nt;
+    DevBuf d;
+    SWD_TRY(d.alloc(sizeof(float) * 3 * ng));
+    float *d_gather = d.as<float>(), *d_up = d_gather + ng, *d_down = d_up + ng;
+    SWD_TRY(afor_dev(P, nshots, shots, wavelet, hwavelet, kzcoef, nfftz, d_gather, d_up, d_down));
+    HIP_TRY(hipMemcpy(gather, d_gather, sizeof(float) * ng, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(up, d_up, sizeof(float) * ng, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(down, d_down, sizeof(float) * ng, hipMemcpyDeviceToHost));
+    return SWD_OK;
+}
+
+extern "C" int swd_afor_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+                            const float *hwavelet, const float *kzcoef, int nfftz, float *gather, float *up,
+                            float *down)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    SWD_TRY(afor_args(P, "swd_afor_dev", nshots, shots, wavelet, hwavelet, kzcoef, nfftz, gather, up, down));
+    return afor_dev(P, nshots, shots, wavelet, hwavelet, kzcoef, nfftz, gather, up, down);
+}
+
 namespace {
-
-// Shots per group of the fused RTM loops.  The per-shot working set of a
-// step (double-buffered state, CPML memories in the frame, and on the
-// backward the reconstruction, images and dp/dt) is kept within a budget of
-// the 256 MB last-level cache, leaving room for the model, the halos and the
-// streamed strips; groups are then evened out.  The forward, which also
-// writes its strips every step, runs best with a smaller budget (measured at
-// c2: 4-shot groups at 287-290 ms per 16-shot step, 6-shot at 298-313,
-// 8-shot at 317).
-constexpr double kCacheBudgetBwd = 192.0 * (1 << 20);
-constexpr double kCacheBudgetFwd = 128.0 * (1 << 20);
-
-int even_groups(int n, int g)
-{
-    g = std::max(1, std::min(n, g));
-    const int ng = (n + g - 1) / g;
-    return (n + ng - 1) / ng;
-}
-
-// Group boundaries of n shots in groups of at most g: ceil(n/g) groups whose
-// sizes differ by at most one (the larger ones first).
-std::vector<int> group_bounds(int n, int g)
-{
-    const int ng = (n + g - 1) / g, base = n / ng, extra = n % ng;
-    std::vector<int> b(1, 0);
-    for (int i = 0; i < ng; i++) b.push_back(b.back() + base + (i < extra ? 1 : 0));
-    return b;
-}
-
-void rtm_groups(const Plan &P, int nshots, int &gf, int &gb)
-{
-    const double N = static_cast<double>(P.ntx) * P.ntz, Nint = static_cast<double>(P.g.nx) * P.g.nz;
-    const double Ncpml = N - Nint;
-    const double fwd = 4.0 * (6 * N + 6 * Ncpml);
-    const double bwd = 4.0 * (9 * Nint + 6 * N + 6 * Ncpml);
-    // groups in flight on the plan's streams share the cache
-    gf = P.group_fwd > 0 ? P.group_fwd : static_cast<int>(kCacheBudgetFwd / P.streams / fwd);
-    gb = P.group_bwd > 0 ? P.group_bwd : static_cast<int>(kCacheBudgetBwd / P.streams / bwd);
-    gf = even_groups(nshots, gf);
-    gb = even_groups(nshots, gb);
-}
-
-// With several streams, consecutive shot groups take them in turn so that
-// one group's launch gaps and tails overlap the others' work.
-hipStream_t rtm_group_stream(const Plan &P, int g)
-{
-    const int ns = P.streams;
-    return g % ns == 0 ? P.stream : P.side[g % ns - 1];
-}
-
-// The other streams wait for the work queued so far on the plan's stream.
-int fork_streams(Plan &P)
-{
-    if (P.streams == 1) return SWD_OK;
-    HIP_TRY(hipEventRecord(P.fork, P.stream));
-    for (int i = 0; i < P.streams - 1; i++) HIP_TRY(hipStreamWaitEvent(P.side[i], P.fork, 0));
-    return SWD_OK;
-}
-
-// The plan's stream waits for the work queued so far on the other streams.
-int join_streams(Plan &P)
-{
-    for (int i = 0; i < P.streams - 1; i++) {
-        HIP_TRY(hipEventRecord(P.join[i], P.side[i]));
-        HIP_TRY(hipStreamWaitEvent(P.stream, P.join[i], 0));
-    }
-    return SWD_OK;
-}
 
 // Angle gathers asked of an RTM call: thetan one-degree bins, delt-step
 // averaging, output [nshots][ntx*ntz][thetan] in device memory.

@@ -85,7 +85,7 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
            "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
-           "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor",
+           "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor", "swd_afor_dev",
            "swd_rtm", "swd_rtm_dev", "swd_rtm_cig", "swd_rtm_cig_dev", "swd_rtm_updown", "swd_rtm_updown_dev", "swd_updown_image_dev",
            "swd_plan_get_stream", "swd_born", "swd_born_adjoint", "swd_born_dev",
            "swd_born_adjoint_dev", "swd_multi_forward", "swd_decon_read_parameters", "swd_decon_geometry",
@@ -362,6 +362,14 @@ class Plan:
         check(self.L.swd_afor(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(hwavelet)), _p(kz), kz.size,
                               *[_p(o) for o in outs]))
         return outs
+
+    def afor_dev(self, shots, wavelet, hwavelet, kzcoef, gather_ptr, up_ptr, down_ptr):
+        """Device-pointer AFor: the three [sum nrec][nt] gathers at the given addresses."""
+        S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
+        kz = f32(kzcoef)
+        vp = C.c_void_p
+        check(self.L.swd_afor_dev(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(hwavelet)), _p(kz), kz.size,
+                                  vp(int(gather_ptr)), vp(int(up_ptr)), vp(int(down_ptr))))
 
     def rtm(self, shots, wavelet, adj, want_gather=False):
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)

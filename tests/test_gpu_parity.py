@@ -101,6 +101,42 @@ def test_afor_small(path):
         assert e32 <= TOL
 
 
+def test_forward_and_afor_groups_streams_device():
+    """One-shot groups on two streams give the gathers of one group of the whole
+    batch bit for bit, and the device-pointer AFor those of the host one."""
+    import torch
+
+    c = configs.small(nshots=3, dup=True, nt=300)
+    print("case small batch of 3 shots: forward and afor in one-shot groups on two streams", flush=True)
+    nfftz = swd.nfft(c.grid.ntz)
+    kz, _ = swd.kz_taper(nfftz, c.grid.dz)
+    hw = swd.hilbert1d(c.wavelet)
+    one = plan_of(c)
+    ref_g = one.forward(c.shots, c.wavelet)
+    ref_a = one.afor(c.shots, c.wavelet, hw, kz)
+    grp = plan_of(c)
+    grp.set_groups(1, 0)
+    grp.set_streams(2)
+    assert np.array_equal(grp.forward(c.shots, c.wavelet), ref_g)
+    for a, b in zip(grp.afor(c.shots, c.wavelet, hw, kz), ref_a):
+        assert np.array_equal(a, b)
+    S = swd.ShotSet(c.shots)
+    d = torch.zeros((3, S.ntr, c.grid.nt), dtype=torch.float32, device="cuda:0")
+    grp.afor_dev(S, c.wavelet, hw, kz, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr())
+    torch.cuda.synchronize()
+    for a, b in zip(d.cpu().numpy(), ref_a):
+        assert np.array_equal(a, b)
+    # and the oracle, shot by shot
+    row = 0
+    for k in range(3):
+        og, ou, od = oracle_problem(c, k).afor(hw, kz)
+        n = og.shape[0]
+        for name, a, b in (("gather", ref_a[0], og), ("up", ref_a[1], ou), ("down", ref_a[2], od)):
+            e32, _ = report(f"afor shot {k} {name}", a[row:row + n], b)
+            assert e32 <= TOL
+        row += n
+
+
 def test_born_and_adjoint_small():
     c = configs.small(nt=300)
     print("case small born", flush=True)

@@ -60,6 +60,7 @@ def test_null_plan_rejected_by_every_plan_entry_point():
     assert L.swd_rtm_cig(null, 1, null, null, null, 10, 1, null, null, null, null) < 0
     assert L.swd_rtm_cig_dev(null, 1, null, null, null, 10, 1, null, null, null, null) < 0
     assert L.swd_plan_get_stream(null, C.byref(s)) < 0
+    assert L.swd_afor_dev(null, 1, null, null, null, null, 128, null, null, null) < 0
     assert "null plan" in swd.last_error()
 
 
