@@ -139,11 +139,14 @@ int swd_plan_set_path(void *plan, int path);
  * first fused loop for a batch size and keep the fastest.  Results do not
  * depend on it. */
 int swd_plan_set_tile(void *plan, int tile_x);
-/* Shots per fused RTM loop: the forward and the backward run their batch in
- * groups of these sizes (0, the default: sized so a group's working set stays
- * in the last-level cache).  Results do not depend on them. */
+/* Group sizes of the fused loops: the RTM forward and backward run their
+ * batch in groups of at most fwd_shots and bwd_shots shots, and swd_forward
+ * and swd_afor theirs in groups of at most fwd_shots wavefields (0, the
+ * default: sized so a group's working set stays in the last-level cache).
+ * The group count is raised to a multiple of the streams where the batch
+ * allows.  Results do not depend on them. */
 int swd_plan_set_groups(void *plan, int fwd_shots, int bwd_shots);
-/* Streams the shot groups of swd_rtm run on at once, 1 to 4 (default 2):
+/* Streams the shot groups of the fused loops run on at once, 1 to 4 (default 2):
  * consecutive groups take the plan's stream and its other streams (default
  * priority) in turn, so one group's launch gaps and tails overlap the others'
  * work, and the cache budget of a group is divided by the count.  Results do

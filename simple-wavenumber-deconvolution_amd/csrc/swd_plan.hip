@@ -679,21 +679,36 @@ int even_groups(int n, int g)
     return (n + ng - 1) / ng;
 }
 
-// Group boundaries of n shots in groups of at most g: ceil(n/g) groups whose
-// sizes differ by at most one (the larger ones first).
-std::vector<int> group_bounds(int n, int g)
+// Group boundaries of n instances in groups of at most g: ceil(n/g) groups,
+// their count raised to a multiple of the streams where there are instances
+// enough, so that every stream gets as many groups (with an odd count on two
+// streams one of them runs alone at the end: c2 AFor, 32 wavefields, five
+// groups of 6-7 took 555 ms a batch, six groups of 5-6 504 ms); sizes differ
+// by at most one (the larger ones first).
+std::vector<int> group_bounds(int n, int g, int streams)
 {
-    const int ng = (n + g - 1) / g, base = n / ng, extra = n % ng;
+    int ng = (n + g - 1) / g;
+    const int up = (ng + streams - 1) / streams * streams;
+    if (up <= n) ng = up;
+    const int base = n / ng, extra = n % ng;
     std::vector<int> b(1, 0);
     for (int i = 0; i < ng; i++) b.push_back(b.back() + base + (i < extra ? 1 : 0));
     return b;
+}
+
+// Per-step working set of one forward wavefield: double-buffered state and
+// the CPML memories of the frame.
+double fwd_group_bytes(const Plan &P)
+{
+    const double N = static_cast<double>(P.ntx) * P.ntz, Ncpml = N - static_cast<double>(P.g.nx) * P.g.nz;
+    return 4.0 * (6 * N + 6 * Ncpml);
 }
 
 void rtm_groups(const Plan &P, int nshots, int &gf, int &gb)
 {
     const double N = static_cast<double>(P.ntx) * P.ntz, Nint = static_cast<double>(P.g.nx) * P.g.nz;
     const double Ncpml = N - Nint;
-    const double fwd = 4.0 * (6 * N + 6 * Ncpml);
+    const double fwd = fwd_group_bytes(P);
     const double bwd = 4.0 * (9 * Nint + 6 * N + 6 * Ncpml);
     // groups in flight on the plan's streams share the cache
     gf = P.group_fwd > 0 ? P.group_fwd : static_cast<int>(kCacheBudgetFwd / P.streams / fwd);
@@ -735,11 +750,13 @@ int join_streams(Plan &P)
 int fused_forward_loop(Plan &P, const KStep *d, int n)
 {
     long long launches = 0;
-    int gf = 0, gb = 0;
-    rtm_groups(P, n, gf, gb);
+    // with no strips to write, the backward's budget (c2 AFor, 32 wavefields
+    // on two streams: groups of 5-6 504 ms a batch, of 4 550 ms)
+    int gf = P.group_fwd > 0 ? P.group_fwd : static_cast<int>(kCacheBudgetBwd / P.streams / fwd_group_bytes(P));
+    gf = even_groups(n, gf);
     stats_begin(P, 0);
     SWD_TRY(fork_streams(P));
-    const std::vector<int> bounds = group_bounds(n, gf);
+    const std::vector<int> bounds = group_bounds(n, gf, P.streams);
     for (size_t g = 0; g + 1 < bounds.size(); g++)
         SWD_TRY(fused_steps(P, d + bounds[g], bounds[g + 1] - bounds[g], steps_up(P.g.nt), -1, P.tuned_fwd, &launches,
                             false, rtm_group_stream(P, static_cast<int>(g))));
@@ -1193,7 +1210,7 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
         long long launches = 0;
         stats_begin(P, 0);
         SWD_TRY(fork());
-        const std::vector<int> gbf = group_bounds(nshots, gf);
+        const std::vector<int> gbf = group_bounds(nshots, gf, P.streams);
         for (size_t g = 0; g + 1 < gbf.size(); g++) {
             const int k0 = gbf[g], n = gbf[g + 1] - gbf[g];
             SWD_TRY(fused_steps(P, at<KStep>(P, o_fwd) + k0, n, steps_up(nt), -1, P.tuned_fwd, &launches, false,
@@ -1220,7 +1237,7 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     long long launches = 0;
     stats_begin(P, 1);
     SWD_TRY(fork());
-    const std::vector<int> gbb = group_bounds(nshots, gb);
+    const std::vector<int> gbb = group_bounds(nshots, gb, P.streams);
     for (size_t g = 0; g + 1 < gbb.size(); g++) {
         const int k0 = gbb[g], n = gbb[g + 1] - gbb[g];
         const KStep *d = at<KStep>(P, o_bwd) + 2 * k0;
@@ -1572,7 +1589,7 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
         long long launches = 0;
         stats_begin(P, 0);
         SWD_TRY(fork_streams(P));
-        const std::vector<int> gbf = group_bounds(nshots, gf);
+        const std::vector<int> gbf = group_bounds(nshots, gf, P.streams);
         for (size_t g = 0; g + 1 < gbf.size(); g++) {
             const int k0 = gbf[g], n = gbf[g + 1] - gbf[g];
             SWD_TRY(fused_steps(P, at<KStep>(P, o_fwd) + 2 * k0, 2 * n, steps_up(nt), -1, P.tuned_fwd, &launches, false,
@@ -1595,7 +1612,7 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
     long long launches = 0;
     stats_begin(P, 1);
     SWD_TRY(fork_streams(P));
-    const std::vector<int> gbb = group_bounds(nshots, gb);
+    const std::vector<int> gbb = group_bounds(nshots, gb, P.streams);
     for (size_t g = 0; g + 1 < gbb.size(); g++) {
         const int k0 = gbb[g], n = gbb[g + 1] - gbb[g];
         const KUpDown *du = at<KUpDown>(P, o_ud) + k0;
