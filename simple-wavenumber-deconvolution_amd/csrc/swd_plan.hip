@@ -618,11 +618,17 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
                 HIP_TRY(hipEventRecord(P.tev[c + 1], st));
             }
             HIP_TRY(hipEventSynchronize(P.tev[3]));
+            // A wider tile must be more than 10% faster than the narrower
+            // one: narrow tiles keep more workgroups in flight, which the
+            // groups overlapping on the plan's streams need (c2, 16 shots on
+            // two streams, whole runs at widths 16/32/64: AFor 505/548/686 ms,
+            // RTM 785/820/1043 ms), while a short timing of one launch
+            // sequence can favour a wider one.
             float best = 0.f;
             for (int c = 0; c < 3; c++) {
                 float ms = 0.f;
                 HIP_TRY(hipEventElapsedTime(&ms, P.tev[c], P.tev[c + 1]));
-                if (c == 0 || ms < best) {
+                if (c == 0 || ms < 0.9f * best) {
                     best = ms;
                     tx = cand[c];
                 }
