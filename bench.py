@@ -467,6 +467,11 @@ def bench_afor(a):
     bytes_per_launch = afor_bytes_per_step(g, nrec) * a.batch / launches_per_step
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     up0 = d[1, :nrec].cpu().numpy()
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "afor_c2_traffic.json")
+    if os.path.exists(tfile) and (g.nx, g.nz, g.pml, g.Lc) == (845, 502, 40, 4):
+        with open(tfile) as f:
+            traffic = json.load(f)["hbm_bytes_per_wavefield_step"] * 2 * a.batch / launches_per_step
 
     plain = None
     if not a.no_plain:
@@ -523,12 +528,14 @@ def bench_afor(a):
                    "wavefield_ms_per_step": fwd_ms / a.steps, "kz_separation_ms_per_step": fft_ms / a.steps,
                    "up_finite": bool(np.isfinite(up0).all())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
                      "kernel": "k_fstep (forward step of a group of p and Hilbert wavefields with capture)",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": launch_ms,
                      "timing": "HIP events on the plan's stream around the wavefield loop (joining the other "
-                               "streams) / launches: the interval between launch completions"},
+                               "streams) / launches: the interval between launch completions",
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per wavefield-step, "
+                                       "profiles/afor_c2_16shots.md" if traffic else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
