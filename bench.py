@@ -541,6 +541,128 @@ def bench_afor(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_born(a):
+    """Born modelling d = L m and its exact adjoint m_grad = L^T d on the c5 grid
+    (1001 x 1690 interior, 5 m, dt 0.4 ms, nt 8000, 60-cell CPML, 801 receivers
+    on 401 nodes): one step = the Born forward of the batch, then its adjoint
+    with sqrt(nt) checkpoints, m and d resident in HBM."""
+    import torch
+    import swd
+
+    from swd import configs
+
+    nshots = a.born_shots
+    log(f"bench: c5 Born forward + adjoint, {nshots} shots, nt {a.nt_born}")
+    idx = [int(v) for v in np.linspace(0, 127, nshots)]
+    case = configs.c5(nt=a.nt_born, shot_index=idx)
+    g = case.grid
+    plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0)
+    shots = swd.ShotSet(case.shots)
+    rng = np.random.default_rng(3)
+    nint = g.nx * g.nz
+    dev = torch.device("cuda:0")
+    d_m = torch.from_numpy(rng.standard_normal(nshots * nint).astype(np.float32)).to(dev)
+    d_d = torch.zeros((shots.ntr, g.nt), dtype=torch.float32, device=dev)
+    d_res = torch.from_numpy(rng.standard_normal((shots.ntr, g.nt)).astype(np.float32)).to(dev)
+    d_g = torch.zeros(nint, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    copy_gbs = copy_bandwidth(torch, dev)
+
+    def fwd():
+        plan.born_dev(shots, case.wavelet, d_m.data_ptr(), d_d.data_ptr())
+
+    def adj():
+        plan.born_adjoint_dev(shots, case.wavelet, d_res.data_ptr(), d_g.data_ptr())
+
+    for i in range(a.warmup):
+        fwd()
+        adj()
+        log(f"bench: warmup {i + 1}/{a.warmup} done")
+    torch.cuda.synchronize()
+    t_f = t_a = f_ms = 0.0
+    f_launches = 0
+    for i in range(a.steps):
+        t0 = time.perf_counter()
+        fwd()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        st = plan.stats()
+        f_ms += st["fwd_ms"]
+        f_launches = st["fwd_launches"]
+        adj()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_f += t1 - t0
+        t_a += t2 - t1
+        log(f"bench: step {i + 1}/{a.steps} Born forward {1e3 * (t1 - t0):.0f} ms, adjoint {1e3 * (t2 - t1):.0f} ms")
+    st = plan.stats()
+    N = g.ntx * g.ntz
+    cells = 2 * g.nt * N * nshots   # per operator
+    P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
+    nrec = len(case.shots[0][2])
+    step_bytes = (56 * N + 32 * P + 4 * nint + 4 * nrec) * nshots   # background + scattered wavefields, m read
+    step_ms = f_ms / a.steps / g.nt
+    achieved = step_bytes / (step_ms * 1e-3) / 1e9
+    grad_finite = bool(torch.isfinite(d_g).all().item())
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        from helpers import oracle_problem
+
+        threads = min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        nts = min(g.nt, a.cpu_nt_born)
+        sub = configs.Case(case.name, swd.Grid(g.nx, g.nz, g.pml, g.Lc, g.dx, g.dz, g.dt, nts), case.vp,
+                               case.rho, case.coefs, case.wavelet[:nts], case.shots[:1], case.f0)
+        Pq = oracle_problem(sub, 0)
+        m0 = d_m[:nint].cpu().numpy()
+        r0 = rng.standard_normal((nrec, nts)).astype(np.float32)
+        t0 = time.perf_counter()
+        Pq.born(m0)
+        Pq.born_adjoint(r0)
+        dt = time.perf_counter() - t0
+        cpu = {"value": 2 * 2 * nts * N / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"oracle fp32 host loops (OpenMP) on 1 c5 shot with nt={nts}: Born forward + adjoint "
+                         f"(the oracle keeps every step's background, no recompute), {dt:.1f}s"}
+        log(f"bench: cpu baseline {cpu['value']:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
+
+    out = {
+        "metric": "Born forward + adjoint Gcell-updates/s (c5 grid, sqrt(nt) checkpoints)",
+        "value": 2 * cells * a.steps / (t_f + t_a) / 1e9,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": (t_f + t_a) * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered c5 model, Gaussian m = dK/K, random residual d)",
+        "config": {"workload": f"c5 batched Born forward + exact adjoint: 1001x1690 interior (1129x1818 padded), "
+                               f"nt {g.nt}, Lc 4, {nrec} receivers, {nshots} shots",
+                   "shots_per_step": nshots, "born_fwd_gcell_per_s": cells * a.steps / t_f / 1e9,
+                   "born_adj_gcell_per_s": cells * a.steps / t_a / 1e9,
+                   "adj_recompute_cells_per_step": g.nt * N * nshots,
+                   "born_fwd_ms": t_f * 1e3 / a.steps, "born_adj_ms": t_a * 1e3 / a.steps,
+                   "ckpt_GB": st["ckpt_bytes"] / 1e9, "full_storage_GB": st["full_bytes"] / 1e9,
+                   "waves": st["waves"], "grad_finite": grad_finite},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "kernel": "k_v + k_p (batched Born forward step: background and scattered wavefields "
+                               "of every shot, two launches a step)",
+                     "bytes_per_launch": step_bytes, "avg_launch_ms": step_ms,
+                     "launches_per_step": f_launches / g.nt,
+                     "timing": "HIP events on the plan's stream around the Born forward loop / nt steps "
+                               "(a step is the velocity and the pressure launch)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -560,10 +682,13 @@ def main():
     ap.add_argument("--streams", type=int, default=0, choices=(0, 1, 2, 3, 4),
                     help="RTM shot groups in flight at once (swd_plan_set_streams); 0: 3 for --workload updown, "
                          "else 2 (the fastest measured)")
-    ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig", "afor"), default="rtm",
+    ap.add_argument("--born-shots", type=int, default=8, help="shots of --workload born")
+    ap.add_argument("--nt-born", type=int, default=8000, help="time steps of --workload born (c5: 8000)")
+    ap.add_argument("--cpu-nt-born", type=int, default=1000, help="time steps of the born CPU sample")
+    ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig", "afor", "born"), default="rtm",
                     help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution; "
                          "updown: the up/down-separated RTM; cig: RTM with angle gathers; afor: AFor forward "
-                         "modelling with the kz up/down separation")
+                         "modelling with the kz up/down separation; born: Born forward + adjoint on the c5 grid")
     ap.add_argument("--thetan", type=int, default=10, help="angle bins of --workload cig (RC:235: 10)")
     ap.add_argument("--nfftz", type=int, default=0,
                     help="kz transform length of --workload updown (0: nz padded to a power of two, at least 128)")
@@ -586,6 +711,9 @@ def main():
         return
     if a.workload == "afor":
         bench_afor(a)
+        return
+    if a.workload == "born":
+        bench_born(a)
         return
 
     import torch

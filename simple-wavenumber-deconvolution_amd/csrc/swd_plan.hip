@@ -514,33 +514,6 @@ int upload_insts(Plan &P, const std::vector<KInst> &h, size_t o_inst)
 // update), pvx, pvz (CPML memory of the pressure update).
 constexpr int kFieldArrays = 7;
 
-// Forward time loop of nshots shots with per_shot wavefields each.  Plain
-// path: shot after shot, one kernel per sub-step.  Batched path: every
-// wavefield of the batch in each launch, a velocity and a pressure kernel.
-int forward_loop(Plan &P, const KInst *d_inst, int per_shot, int nshots)
-{
-    const KGrid &g = P.kg;
-    long long launches = 0;
-    stats_begin(P, 0);
-    if (P.path == SWD_PATH_PLAIN) {
-        for (int s = 0; s < nshots; s++) {
-            const KInst *di = d_inst + s * per_shot;
-            for (int it = 0; it < g.nt; it++) {
-                HIP_TRY(launch_v(g, di, per_shot, false, 1, P.stream));
-                HIP_TRY(launch_v(g, di, per_shot, false, 2, P.stream));
-                HIP_TRY(launch_p(g, di, per_shot, false, it, P.stream));
-                launches += 3;
-            }
-        }
-    } else {
-        SWD_TRY(batched_forward_steps(g, d_inst, per_shot * nshots, 0, g.nt, P.stream, &launches));
-    }
-    P.stats.fwd_ms = stats_end(P, 0);
-    P.stats.fwd_launches = launches;
-    P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * g.nt * per_shot * nshots;
-    return SWD_OK;
-}
-
 void zero_fields(Plan &P, float *base, size_t count) { (void)hipMemsetAsync(base, 0, sizeof(float) * count, P.stream); }
 
 void bind_fields(KInst &I, float *f, size_t nfield)
@@ -747,6 +720,49 @@ int join_streams(Plan &P)
         HIP_TRY(hipEventRecord(P.join[i], P.side[i]));
         HIP_TRY(hipStreamWaitEvent(P.stream, P.join[i], 0));
     }
+    return SWD_OK;
+}
+
+// Forward time loop of nshots shots with per_shot wavefields each.  Plain
+// path: shot after shot, one kernel per sub-step.  Batched path: every
+// wavefield of a group of shots in each launch, a velocity and a pressure
+// kernel.
+int forward_loop(Plan &P, const KInst *d_inst, int per_shot, int nshots)
+{
+    const KGrid &g = P.kg;
+    long long launches = 0;
+    stats_begin(P, 0);
+    if (P.path == SWD_PATH_PLAIN) {
+        for (int s = 0; s < nshots; s++) {
+            const KInst *di = d_inst + s * per_shot;
+            for (int it = 0; it < g.nt; it++) {
+                HIP_TRY(launch_v(g, di, per_shot, false, 1, P.stream));
+                HIP_TRY(launch_v(g, di, per_shot, false, 2, P.stream));
+                HIP_TRY(launch_p(g, di, per_shot, false, it, P.stream));
+                launches += 3;
+            }
+        }
+    } else {
+        // shots in groups whose per-step working set stays in the last-level
+        // cache, the groups taking the plan's streams in turn; a shot's
+        // wavefields stay in one group (Born's scattered field reads the
+        // background's velocities of the same step).  c5 Born forward, 8 shots
+        // on one stream: one group 931 ms per 2000 steps, groups of 4 814 ms,
+        // of 2 710 ms, of 1 797 ms.
+        const double per = 4.0 * kFieldArrays * static_cast<double>(P.nfield) * per_shot;
+        int gs = P.group_fwd > 0 ? std::max(1, P.group_fwd / per_shot)
+                                 : static_cast<int>(kCacheBudgetBwd / P.streams / per);
+        gs = even_groups(nshots, gs);
+        SWD_TRY(fork_streams(P));
+        const std::vector<int> b = group_bounds(nshots, gs, P.streams);
+        for (size_t i = 0; i + 1 < b.size(); i++)
+            SWD_TRY(batched_forward_steps(g, d_inst + per_shot * b[i], per_shot * (b[i + 1] - b[i]), 0, g.nt,
+                                          rtm_group_stream(P, static_cast<int>(i)), &launches));
+        SWD_TRY(join_streams(P));
+    }
+    P.stats.fwd_ms = stats_end(P, 0);
+    P.stats.fwd_launches = launches;
+    P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * g.nt * per_shot * nshots;
     return SWD_OK;
 }
 
@@ -1968,6 +1984,15 @@ extern "C" int swd_born_adjoint_dev(void *plan, int nshots, const swd_shot *shot
     const double per = 4.0 * (7.0 * nf * nseg + nint * seg + 16.0 * nf + nint + static_cast<double>(ntr_max) * nt);
     int wave = nshots;
     SWD_TRY(wave_shots(*P, nshots, per, 4.0 * nt * 4, wave));
+    // Waves also hold their per-step working set (both wavefields, the
+    // background sums, the gradient) to about 768 MB, in waves of even size
+    // (c5, 8 shots, nt 2000: one wave 1476 ms, waves of 4 1300 ms, of 2
+    // 1390 ms, of 1 1669 ms).
+    const double step_bytes = 4.0 * (16.0 * nf + 3.0 * nint);
+    wave = std::min(wave, std::max(1, static_cast<int>(768.0 * (1 << 20) / step_bytes)));
+    const int nwave = (nshots + wave - 1) / wave;
+    wave = (nshots + nwave - 1) / nwave;
+    P->stats.waves = nwave;
     for (int s0 = 0; s0 < nshots; s0 += wave) {
         const int n = std::min(wave, nshots - s0);
         SWD_TRY(born_adjoint_dev(*P, n, shots + s0, wavelet, d + static_cast<size_t>(traces_before(shots, s0)) * nt, m_grad,
