@@ -557,6 +557,8 @@ def bench_born(a):
     case = configs.c5(nt=a.nt_born, shot_index=idx)
     g = case.grid
     plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0)
+    plan.set_groups(*a.groups)
+    plan.set_streams(a.streams)
     shots = swd.ShotSet(case.shots)
     rng = np.random.default_rng(3)
     nint = g.nx * g.nz
