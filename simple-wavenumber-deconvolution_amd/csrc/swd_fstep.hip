@@ -106,7 +106,19 @@ struct FGeo {
 // memory, a frame index or an interior test; the per-cell arithmetic is the
 // same as in the general case (psi = 0 there; only the sign of an exact zero
 // may differ, through the omitted "+ 0").
-template <int LC, int TX, bool FULL, bool CIG>
+// Born pair (background first, scattered field second, in one workgroup and
+// on the same own cells): the background keeps the divergence sum S of each
+// interior cell in acc; the scattered field adds m S_background to its own sum
+// (the Born source of the batched k_p, in the same operation order).
+__device__ __forceinline__ void born_sum(const KGrid &g, const KStep &I, int x, int zk, float4 &acc, int k, float &S)
+{
+    if (I.born_m)
+        S = __builtin_fmaf(ldg(I.born_m, static_cast<unsigned>((x - g.pmlc) * g.nz + (zk - g.pmlc))), elc(acc, k), S);
+    else
+        el(acc, k) = S;
+}
+
+template <int LC, int TX, bool FULL, bool CIG, bool BORN = false>
 __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X0, int Z0, int it, int first,
                                            float4 *__restrict__ sP, float4 *__restrict__ sVX, float4 *__restrict__ sVZ,
                                            float4 (&dpa)[FGeo<LC, TX>::NOWN], float4 (&cgk)[3][FGeo<LC, TX>::NOWN])
@@ -418,7 +430,9 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 #pragma unroll
                     for (int ic = 0; ic < LC; ic++)
                         dvz = __builtin_fmaf(g.rc[ic] * (w[4 * R + k + ic] - w[4 * R + k - ic - 1]), idz, dvz);
-                    el(pv, k) = __builtin_fmaf(elc(K, k), elc(dvx, k) + dvz, elc(pold, k));
+                    float S = elc(dvx, k) + dvz;
+                    if constexpr (BORN) born_sum(g, I, x, z + k, dpa[r], k, S);   // the tile is interior
+                    el(pv, k) = __builtin_fmaf(elc(K, k), S, elc(pold, k));
                 }
             } else {
                 const float ax = ldg(g.ax, x), bx = ldg(g.bx, x);
@@ -447,7 +461,9 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                     const float qz = azr[k] != 0.f ? __builtin_fmaf(bzr[k], elc(qzo, k), azr[k] * dvz) : 0.f;
                     el(qxn, k) = (in && ax != 0.f) ? qx : elc(qxo, k);
                     el(qzn, k) = (in && azr[k] != 0.f) ? qz : elc(qzo, k);
-                    const float S = ((elc(dvx, k) + qx) + dvz) + qz;
+                    float S = ((elc(dvx, k) + qx) + dvz) + qz;
+                    if constexpr (BORN)
+                        if (colint && zk >= pmlc && zk < ntz - pmlc) born_sum(g, I, x, zk, dpa[r], k, S);
                     const float np = __builtin_fmaf(elc(K, k), S, elc(pold, k));
                     el(pv, k) = in ? np : elc(pold, k);
                 }
@@ -531,11 +547,12 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 }
 
 // PAIR = false: one FWD wavefield per instance.  PAIR = true: instances
-// 2k (SRC) and 2k+1 (RCV) of one shot share each workgroup.
+// 2k (SRC) and 2k+1 (RCV) of one shot share each workgroup; with BORN, 2k is
+// the background and 2k+1 the scattered field of a Born forward (both FWD).
 // The angle-gather variant at Lc 4 and tiles up to 32 wide is held to four
 // wavefronts per SIMD (128 VGPRs, 8-12 bytes of spill, faster than three
 // wavefronts); elsewhere that bound would spill heavily.
-template <int LC, int TX, bool PAIR, bool CIG>
+template <int LC, int TX, bool PAIR, bool CIG, bool BORN>
 __global__ __launch_bounds__(fstep_threads<TX>(), (CIG && LC == 4 && TX <= 32) ? 4 : 1) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb,
                                                               int nxb,
                                                int it, int first)
@@ -562,6 +579,15 @@ __global__ __launch_bounds__(fstep_threads<TX>(), (CIG && LC == 4 && TX <= 32) ?
         return;
     }
     const KStep &S = insts[2 * inst], &Rv = insts[2 * inst + 1];
+    if constexpr (BORN) {
+        // Born: background, then the scattered field with the background's sums in dpa
+        if (full) fstep_tile<LC, TX, true, false, true>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        else fstep_tile<LC, TX, false, false, true>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        __syncthreads();   // LDS is staged again for the scattered field
+        if (full) fstep_tile<LC, TX, true, false, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        else fstep_tile<LC, TX, false, false, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        return;
+    }
     // the reconstruction writes only interior cells and the source cell
     const bool any = X0 + TX > g.pmlc && X0 < g.ntx - g.pmlc && Z0 + 64 > g.pmlc && Z0 < g.ntz - g.pmlc;
     const bool src = S.s_ix >= X0 && S.s_ix < X0 + TX && S.s_iz >= Z0 && S.s_iz < Z0 + 64;
@@ -627,33 +653,47 @@ hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, b
 hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair, bool cig,
                         hipStream_t s)
 {
+    return launch_fstep(g, insts, ninst, it, first, tx, pair, cig, false, s);
+}
+
+hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair, bool cig,
+                        bool born, hipStream_t s)
+{
     const int n = pair ? ninst / 2 : ninst;
     const int nzb = (g.ntz + 63) / 64;
     const int nxb = (g.ntx + tx - 1) / tx;
     const int nblocks = (n * nzb * nxb + 7) / 8 * 8;
     const int f = first ? 1 : 0;
-#define LF(TXV, PV, CV)                                                                                               \
-    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_fstep<L, TXV, PV, CV>), dim3(nblocks), dim3(FGeo<L, TXV>::NT), 0, s, g, \
-                                         insts, n, nzb, nxb, it, f))
+#define LF(TXV, PV, CV, BV)                                                                                           \
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_fstep<L, TXV, PV, CV, BV>), dim3(nblocks), dim3(FGeo<L, TXV>::NT), 0, s, \
+                                         g, insts, n, nzb, nxb, it, f))
+    if (born && (!pair || cig)) return hipErrorInvalidValue;
     if (pair && cig) {
         switch (tx) {
-        case 16: LF(16, true, true); break;
-        case 32: LF(32, true, true); break;
-        case 64: LF(64, true, true); break;
+        case 16: LF(16, true, true, false); break;
+        case 32: LF(32, true, true, false); break;
+        case 64: LF(64, true, true, false); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else if (pair && born) {
+        switch (tx) {
+        case 16: LF(16, true, false, true); break;
+        case 32: LF(32, true, false, true); break;
+        case 64: LF(64, true, false, true); break;
         default: return hipErrorInvalidValue;
         }
     } else if (pair) {
         switch (tx) {
-        case 16: LF(16, true, false); break;
-        case 32: LF(32, true, false); break;
-        case 64: LF(64, true, false); break;
+        case 16: LF(16, true, false, false); break;
+        case 32: LF(32, true, false, false); break;
+        case 64: LF(64, true, false, false); break;
         default: return hipErrorInvalidValue;
         }
     } else {
         switch (tx) {
-        case 16: LF(16, false, false); break;
-        case 32: LF(32, false, false); break;
-        case 64: LF(64, false, false); break;
+        case 16: LF(16, false, false, false); break;
+        case 32: LF(32, false, false, false); break;
+        case 64: LF(64, false, false, false); break;
         default: return hipErrorInvalidValue;
         }
     }

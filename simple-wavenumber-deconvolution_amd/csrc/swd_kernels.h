@@ -70,6 +70,7 @@ struct KStep {
     int thetan;
     float cw1, cw2;              // bin weights exp(-1/18), exp(-4/18) rounded to float
     float cthr;                  // cosine below which no bin is in reach
+    const float *born_m;         // Born scattered field (paired after its background): dK/K [nx][nz], or null
 };
 
 // One shot of the angle-gather update that follows a backward step: the

@@ -20,6 +20,10 @@ hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, b
 // cig: the paired backward also bins the angle gathers of step it+1 (RCV instances' cig)
 hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair, bool cig,
                         hipStream_t s);
+// Born forward pairs (born, with pair and without cig): instances 2k and 2k+1
+// are the background and scattered fields of shot k.
+hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair, bool cig,
+                        bool born, hipStream_t s);
 // multi-grid forward step: btab[nblk] = (instance, tile), tile width 16
 hipError_t launch_fstep_multi(int Lc, const KStep *insts, const int2 *btab, int nblk, int it, hipStream_t s);
 hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2 *tw, const float *coef,

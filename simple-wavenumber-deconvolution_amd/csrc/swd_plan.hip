@@ -96,6 +96,7 @@ struct Plan {
     int group_fwd = 0, group_bwd = 0;     // RTM shots per fused forward / backward loop (0 = sized to the cache)
     std::map<int, int> tuned_fwd, tuned_bwd;   // tuned tile width by instances per launch
     std::map<int, int> tuned_cig;              // the same for the backward with angle gathers
+    std::map<int, int> tuned_born;             // the same for the Born forward pairs
     int streams = 2;            // RTM shot groups run on this many streams at once (1 .. kMaxStreams)
     hipStream_t side[3] = {};   // the other streams (default priority)
     hipEvent_t fork = nullptr, join[3] = {};
@@ -473,6 +474,7 @@ extern "C" int swd_plan_set_tile(void *plan, int tile_x)
     P->tuned_fwd.clear();
     P->tuned_bwd.clear();
     P->tuned_cig.clear();
+    P->tuned_born.clear();
     return SWD_OK;
 }
 
@@ -561,7 +563,7 @@ struct StepHook {
 
 int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int first_it, std::map<int, int> &tuned,
                 long long *launches, bool pair = false, hipStream_t st = nullptr, const StepHook *hook = nullptr,
-                bool cig = false)
+                bool cig = false, bool born = false)
 {
     if (!st) st = P.stream;
     int tx = P.tile_req;
@@ -572,7 +574,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     size_t k = 0;
     auto launch = [&](int w) -> int {
         const int it = its[k++];
-        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, pair, cig, st));
+        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, pair, cig, born, st));
         (*launches)++;
         if (hook) {
             SWD_TRY(hook->run(it, st));
@@ -613,14 +615,15 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     }
     if (k < its.size() && P.graphs) {
         std::vector<long long> key = {1, reinterpret_cast<long long>(d), n, its[k], static_cast<long long>(its.size() - k),
-                                      first_it, tx, (pair ? 1 : 0) + (cig ? 2 : 0), reinterpret_cast<long long>(P.ws)};
+                                      first_it, tx, (pair ? 1 : 0) + (cig ? 2 : 0) + (born ? 4 : 0),
+                                      reinterpret_cast<long long>(P.ws)};
         if (hook) key.insert(key.end(), hook->key.begin(), hook->key.end());
         const size_t k0 = k;
         SWD_TRY(graph_run(
             P, key,
             [&]() -> int {
                 for (size_t j = k0; j < its.size(); j++) {
-                    HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, tx, pair, cig, st));
+                    HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, tx, pair, cig, born, st));
                     if (hook) SWD_TRY(hook->run(its[j], st));
                 }
                 return SWD_OK;
@@ -1807,10 +1810,70 @@ namespace {
 // Born forward of a batch: m on the device [nshots][nx][nz], d on the device
 // [sum nrec][nt].  Each shot runs the background wavefield and the scattered
 // wavefield as two instances of the same launches.
+// Born forward on the fused path: the background (2k) and scattered (2k+1)
+// fields of shot k in one workgroup per tile, the Born source from the
+// background's divergence sums in registers; shots in cache-sized groups on
+// the plan's streams.
+int born_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, const float *wavelet, const float *m,
+                   float *d)
+{
+    const int nt = P.g.nt;
+    const size_t nf = P.nfield, nint = static_cast<size_t>(P.g.nx) * P.g.nz;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KStep>(2 * nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_f = c.add<float>(kStepArrays * nf * 2 * nshots);
+    SWD_TRY(ensure_ws(P, c.off));
+    SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
+    zero_fields(P, at<float>(P, o_f), kStepArrays * nf * 2 * nshots);
+    std::vector<KStep> h(2 * nshots);
+    for (int k = 0; k < nshots; k++) {
+        KStep &B = h[2 * k], &U = h[2 * k + 1];
+        std::memset(&B, 0, sizeof B);
+        std::memset(&U, 0, sizeof U);
+        B.mode = U.mode = KSTEP_FWD;
+        bind_step(B, at<float>(P, o_f) + kStepArrays * nf * (2 * k), nf);
+        bind_step(U, at<float>(P, o_f) + kStepArrays * nf * (2 * k + 1), nf);
+        B.wavelet = at<float>(P, o_wav);
+        B.s_ix = U.s_ix = shots[k].s_ix;
+        B.s_iz = U.s_iz = shots[k].s_iz;
+        U.born_m = m + nint * k;
+        if (shots[k].nrec > 0) {
+            U.colmap = at<int>(P, o_ints) + b.col_off[k];
+            U.rowmap = at<int>(P, o_ints) + b.row_off[k];
+            U.gather = d + static_cast<size_t>(b.toff[k]) * nt;
+        }
+    }
+    SWD_TRY(upload_steps(P, h, o_inst));
+    long long launches = 0;
+    int gs = P.group_fwd > 0 ? std::max(1, P.group_fwd / 2)
+                             : static_cast<int>(kCacheBudgetBwd / P.streams / (2 * fwd_group_bytes(P)));
+    gs = even_groups(nshots, gs);
+    stats_begin(P, 0);
+    SWD_TRY(fork_streams(P));
+    const std::vector<int> bounds = group_bounds(nshots, gs, P.streams);
+    for (size_t i = 0; i + 1 < bounds.size(); i++)
+        SWD_TRY(fused_steps(P, at<KStep>(P, o_inst) + 2 * bounds[i], 2 * (bounds[i + 1] - bounds[i]), steps_up(nt), -1,
+                            P.tuned_born, &launches, true, rtm_group_stream(P, static_cast<int>(i)), nullptr, false,
+                            true));
+    SWD_TRY(join_streams(P));
+    P.stats.fwd_ms = stats_end(P, 0);
+    P.stats.fwd_launches = launches;
+    P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * nt * 2 * nshots;
+    HIP_TRY(launch_copy_rows(d, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
+    HIP_TRY(hipStreamSynchronize(P.stream));
+    P.stats.ws_bytes = static_cast<long long>(P.ws_cap);
+    return SWD_OK;
+}
+
 int born_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet, const float *m, float *d)
 {
     Batch b;
     SWD_TRY(build_batch(P, nshots, shots, b));
+    if (P.path == SWD_PATH_FUSED) return born_dev_fused(P, nshots, shots, b, wavelet, m, d);
     const int nt = P.g.nt;
     const size_t nf = P.nfield, nint = static_cast<size_t>(P.g.nx) * P.g.nz;
     Carver c;
@@ -1961,7 +2024,8 @@ extern "C" int swd_born_dev(void *plan, int nshots, const swd_shot *shots, const
     if (!wavelet || !m || !d || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_born_dev: null argument");
     const size_t nint = static_cast<size_t>(P->g.nx) * P->g.nz;
     int wave = nshots;
-    SWD_TRY(wave_shots(*P, nshots, 4.0 * 14.0 * P->nfield, 4.0 * P->g.nt * 4, wave));
+    SWD_TRY(wave_shots(*P, nshots, 4.0 * (P->path == SWD_PATH_FUSED ? 2 * kStepArrays : 14.0) * P->nfield,
+                       4.0 * P->g.nt * 4, wave));
     for (int s0 = 0; s0 < nshots; s0 += wave) {
         const int n = std::min(wave, nshots - s0);
         SWD_TRY(born_dev(*P, n, shots + s0, wavelet, m + nint * s0, d + static_cast<size_t>(traces_before(shots, s0)) * P->g.nt));
