@@ -602,9 +602,16 @@ def bench_born(a):
     cells = 2 * g.nt * N * nshots   # per operator
     P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
     nrec = len(case.shots[0][2])
-    step_bytes = (56 * N + 32 * P + 4 * nint + 4 * nrec) * nshots   # background + scattered wavefields, m read
-    step_ms = f_ms / a.steps / g.nt
-    achieved = step_bytes / (step_ms * 1e-3) / 1e9
+    shot_step_bytes = 56 * N + 32 * P + 4 * nint + 4 * nrec   # background + scattered wavefields, m read
+    launches_per_step = f_launches / g.nt
+    bytes_per_launch = shot_step_bytes * nshots / launches_per_step
+    launch_ms = f_ms / a.steps / max(1, f_launches)
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "born_c5_traffic.json")
+    if os.path.exists(tfile) and (g.nx, g.nz, g.pml, g.Lc) == (1690, 1001, 60, 4):
+        with open(tfile) as f:
+            traffic = json.load(f)["hbm_bytes_per_shot_step"] * nshots / launches_per_step
     grad_finite = bool(torch.isfinite(d_g).all().item())
 
     cpu = None
@@ -652,14 +659,16 @@ def bench_born(a):
                    "ckpt_GB": st["ckpt_bytes"] / 1e9, "full_storage_GB": st["full_bytes"] / 1e9,
                    "waves": st["waves"], "grad_finite": grad_finite},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
-                     "kernel": "k_v + k_p (batched Born forward step: background and scattered wavefields "
-                               "of every shot, two launches a step)",
-                     "bytes_per_launch": step_bytes, "avg_launch_ms": step_ms,
-                     "launches_per_step": f_launches / g.nt,
-                     "timing": "HIP events on the plan's stream around the Born forward loop / nt steps "
-                               "(a step is the velocity and the pressure launch)"},
+                     "kernel": "k_fstep Born pair (fused Born forward step: background and scattered field of "
+                               "each shot of a group in one workgroup per tile)",
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": launch_ms,
+                     "launches_per_step": launches_per_step,
+                     "timing": "HIP events on the plan's stream around the Born forward loop (joining the other "
+                               "streams) / launches: the interval between launch completions",
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per shot-step, "
+                                       "profiles/born_c5_8shots.md" if traffic else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
