@@ -674,6 +674,105 @@ def bench_born(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_multi(a):
+    """Forward modelling of the c4 batch: 64 surface shots on the c1 grid with
+    point scatterers plus 32 single-shot grids of log-uniform sizes from 128x128
+    to 502x845 (all Lc 4, nt 1000), every grid's shots in one fused launch per
+    time step (swd_multi_forward), against one call per grid."""
+    import torch
+    import swd
+    from swd import configs
+
+    cases = configs.c4(nt=a.nt_multi)
+    log(f"bench: c4 multi-grid forward, {len(cases)} grids, {sum(len(c.shots) for c in cases)} shots, "
+        f"nt {a.nt_multi}")
+    plans = [swd.Plan(c.grid, c.coefs, c.vp, c.rho, device=0) for c in cases]
+    sets = [swd.ShotSet(c.shots) for c in cases]
+    wavs = [c.wavelet for c in cases]
+    cells = sum(c.grid.ntx * c.grid.ntz * c.grid.nt * len(c.shots) for c in cases)
+    bytes_per_step = 0
+    for c in cases:
+        g = c.grid
+        P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
+        bytes_per_step += sum(32 * g.ntx * g.ntz + 16 * P + 4 * len(sh[2]) for sh in c.shots)
+    dev = torch.device("cuda:0")
+    copy_gbs = copy_bandwidth(torch, dev)
+    for i in range(a.warmup):
+        swd.multi_forward(plans, sets, wavs)
+        log(f"bench: warmup {i + 1}/{a.warmup} done")
+    t_multi = dev_ms = 0.0
+    for i in range(a.steps):
+        t0 = time.perf_counter()
+        outs = swd.multi_forward(plans, sets, wavs)
+        t_multi += time.perf_counter() - t0
+        dev_ms += plans[0].stats()["fwd_ms"]
+    # one call per grid (each plan's own fused loop, its tile tuned on its first call)
+    for pl, S, w in zip(plans, sets, wavs):
+        pl.forward(S, w)
+    t_each = 0.0
+    rel = 0.0
+    for i in range(a.steps):
+        t0 = time.perf_counter()
+        each = [pl.forward(S, w) for pl, S, w in zip(plans, sets, wavs)]
+        t_each += time.perf_counter() - t0
+    for x, y in zip(outs, each):
+        rel = max(rel, float(np.linalg.norm(x - y) / max(1e-30, np.linalg.norm(y))))
+    launch_ms = dev_ms / a.steps / cases[0].grid.nt
+    achieved = bytes_per_step / (launch_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        from helpers import oracle_problem
+
+        threads = min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        ccells = 0
+        for k in range(8):
+            oracle_problem(cases[0], k).forward()
+            ccells += cases[0].grid.ntx * cases[0].grid.ntz * cases[0].grid.nt
+        for c in cases[1:]:
+            oracle_problem(c, 0).forward()
+            ccells += c.grid.ntx * c.grid.ntz * c.grid.nt
+        dt = time.perf_counter() - t0
+        cpu = {"value": ccells / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"oracle fp32 host loops (OpenMP) on 8 of the c1-grid shots and the 32 uneven grids, "
+                         f"nt {cases[0].grid.nt}, {dt:.1f}s"}
+        log(f"bench: cpu baseline {cpu['value']:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
+
+    out = {
+        "metric": "multi-grid forward Gcell-updates/s (c4: uneven grid sizes in one launch per step)",
+        "value": cells * a.steps / t_multi / 1e9,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t_multi * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (c4: c1 layered model with point scatterers, 32 seeded uneven grids)",
+        "config": {"workload": f"c4 multi-grid forward: 64 shots on the c1 grid (200x300) + 32 single-shot grids "
+                               f"from 128x128 to 502x845, Lc 4, nt {cases[0].grid.nt}",
+                   "grids": len(cases), "shots": sum(len(c.shots) for c in cases),
+                   "one_call_per_grid_gcell_per_s": cells * a.steps / t_each / 1e9,
+                   "speedup_vs_one_call_per_grid": t_each / t_multi,
+                   "gather_rel_l2_vs_one_call_per_grid": rel,
+                   "timing_note": "wall clock per call, the gathers' copy to the host included"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "kernel": "k_fstep_multi (forward step of every grid's shots in one launch)",
+                     "bytes_per_launch": bytes_per_step, "avg_launch_ms": launch_ms,
+                     "timing": "HIP events on the first plan's stream around the time loop / nt launches"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -696,10 +795,11 @@ def main():
     ap.add_argument("--born-shots", type=int, default=8, help="shots of --workload born")
     ap.add_argument("--nt-born", type=int, default=8000, help="time steps of --workload born (c5: 8000)")
     ap.add_argument("--cpu-nt-born", type=int, default=1000, help="time steps of the born CPU sample")
-    ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig", "afor", "born"), default="rtm",
+    ap.add_argument("--nt-multi", type=int, default=1000, help="time steps of --workload multi (c4: 1000)")
+    ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig", "afor", "born", "multi"), default="rtm",
                     help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution; "
                          "updown: the up/down-separated RTM; cig: RTM with angle gathers; afor: AFor forward "
-                         "modelling with the kz up/down separation; born: Born forward + adjoint on the c5 grid")
+                         "modelling with the kz up/down separation; born: Born forward + adjoint on the c5 grid; multi: the c4 uneven-grid forward")
     ap.add_argument("--thetan", type=int, default=10, help="angle bins of --workload cig (RC:235: 10)")
     ap.add_argument("--nfftz", type=int, default=0,
                     help="kz transform length of --workload updown (0: nz padded to a power of two, at least 128)")
@@ -725,6 +825,9 @@ def main():
         return
     if a.workload == "born":
         bench_born(a)
+        return
+    if a.workload == "multi":
+        bench_multi(a)
         return
 
     import torch
