@@ -2144,10 +2144,47 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     Carver ca;
     const size_t oa_grid = ca.add<KGrid>(nplans);
     const size_t oa_inst = ca.add<KStep>(h.size());
-    for (int i = 0; i < nplans; i++) {
-        const int nzb = (Ps[i]->ntz + 63) / 64, nxb = (Ps[i]->ntx + 15) / 16;
-        for (int t = 0; t < nzb * nxb; t++)   // shots of one tile next to each other (shared model in L2)
-            for (int k = 0; k < nshots[i]; k++) btab.push_back(make_int2(inst0[i] + k, t));
+    // Groups of whole instances (an instance's tiles exchange halos between
+    // steps) whose per-step working set fits the cache budget, about six at most;
+    // each group runs its own time loop on the plan's streams in turn.  Within
+    // a group the shots of one tile sit next to each other (shared model in L2).
+    struct Group {
+        size_t off;
+        int n;
+    };
+    std::vector<Group> groups;
+    {
+        double total = 0.0;
+        for (int i = 0; i < nplans; i++) total += fwd_group_bytes(*Ps[i]) * nshots[i];
+        const double budget = std::max(kCacheBudgetBwd / P0.streams, total / 6.0);
+        int i = 0, k = 0;
+        while (i < nplans) {
+            const size_t off = btab.size();
+            double used = 0.0;
+            while (i < nplans) {
+                if (k >= nshots[i]) {   // (every plan has a shot; kept for safety)
+                    i++;
+                    k = 0;
+                    continue;
+                }
+                const int nzb = (Ps[i]->ntz + 63) / 64, nxb = (Ps[i]->ntx + 15) / 16;
+                const double per = fwd_group_bytes(*Ps[i]);
+                int k1 = k;
+                while (k1 < nshots[i] && (used == 0.0 || used + per <= budget)) {
+                    used += per;
+                    k1++;
+                }
+                if (k1 == k) break;   // the group is full
+                for (int t = 0; t < nzb * nxb; t++)
+                    for (int q = k; q < k1; q++) btab.push_back(make_int2(inst0[i] + q, t));
+                k = k1;
+                if (k == nshots[i]) {
+                    i++;
+                    k = 0;
+                }
+            }
+            if (btab.size() > off) groups.push_back(Group{off, static_cast<int>(btab.size() - off)});
+        }
     }
     const size_t oa_btab = ca.add<int2>(btab.size());
     SWD_TRY(ensure_aux(P0, ca.off));
@@ -2178,21 +2215,27 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     HIP_TRY(hipMemcpyAsync(P0.aux + oa_btab, btab.data(), sizeof(int2) * btab.size(), hipMemcpyHostToDevice, P0.stream));
     const KStep *d_inst = reinterpret_cast<const KStep *>(P0.aux + oa_inst);
     const int2 *d_btab = reinterpret_cast<const int2 *>(P0.aux + oa_btab);
-    const int nblk = static_cast<int>(btab.size());
     stats_begin(P0, 0);
-    auto record = [&]() -> int {
-        for (int it = 0; it < nt; it++) HIP_TRY(launch_fstep_multi(P0.g.Lc, d_inst, d_btab, nblk, it, P0.stream));
-        return SWD_OK;
-    };
-    if (P0.graphs) {
-        std::vector<long long> key = {2, reinterpret_cast<long long>(P0.aux), static_cast<long long>(h.size()), nblk, nt};
-        for (int i = 0; i < nplans; i++) key.push_back(reinterpret_cast<long long>(Ps[i]->ws));
-        SWD_TRY(graph_run(P0, key, record));
-    } else {
-        SWD_TRY(record());
+    SWD_TRY(fork_streams(P0));
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const Group G = groups[gi];
+        const hipStream_t st = rtm_group_stream(P0, static_cast<int>(gi));
+        auto record = [&]() -> int {
+            for (int it = 0; it < nt; it++) HIP_TRY(launch_fstep_multi(P0.g.Lc, d_inst, d_btab + G.off, G.n, it, st));
+            return SWD_OK;
+        };
+        if (P0.graphs) {
+            std::vector<long long> key = {2, reinterpret_cast<long long>(P0.aux), static_cast<long long>(h.size()),
+                                          static_cast<long long>(G.off), G.n, nt};
+            for (int i = 0; i < nplans; i++) key.push_back(reinterpret_cast<long long>(Ps[i]->ws));
+            SWD_TRY(graph_run(P0, key, record, st));
+        } else {
+            SWD_TRY(record());
+        }
     }
+    SWD_TRY(join_streams(P0));
     P0.stats.fwd_ms = stats_end(P0, 0);
-    P0.stats.fwd_launches = nt;
+    P0.stats.fwd_launches = static_cast<long long>(nt) * static_cast<long long>(groups.size());
     long long cells = 0;
     for (int i = 0; i < nplans; i++) cells += static_cast<long long>(Ps[i]->ntx) * Ps[i]->ntz * nt * nshots[i];
     P0.stats.fwd_cells = cells;
