@@ -14,6 +14,14 @@ dominant kernel (the fused backward step: algorithmic bytes per launch over the
 HIP-event launch time; HBM traffic per launch from the committed rocprofv3
 counter summary) and a CPU baseline (the oracle's fp32 host loops with OpenMP,
 on a bounded sample of the same workload).
+
+Other workloads (--workload), each one JSON line of the same form:
+  decon   the d1 PSF deconvolution (845 x 702 image, 2484 windows)
+  updown  the c2 up/down-separated RTM (du, ud images)
+  cig     the c2 RTM with angle-domain common-image gathers
+  afor    the c2 AFor forward (p and Hilbert wavefields, kz up/down gathers)
+  born    the c5 Born forward and exact adjoint (sqrt(nt) checkpoints)
+  multi   the c4 uneven-grid forward (33 grids in one launch per step and group)
 """
 import argparse
 import json
