@@ -140,9 +140,10 @@ int swd_plan_set_path(void *plan, int path);
  * depend on it. */
 int swd_plan_set_tile(void *plan, int tile_x);
 /* Group sizes of the fused loops: the RTM forward and backward run their
- * batch in groups of at most fwd_shots and bwd_shots shots, and swd_forward
- * and swd_afor theirs in groups of at most fwd_shots wavefields (0, the
- * default: sized so a group's working set stays in the last-level cache).
+ * batch in groups of at most fwd_shots and bwd_shots shots, and swd_forward,
+ * swd_afor and the Born forward theirs in groups of at most fwd_shots
+ * wavefields, a shot's wavefields kept together (0, the default: sized so a
+ * group's working set stays in the last-level cache).
  * The group count is raised to a multiple of the streams where the batch
  * allows.  Results do not depend on them. */
 int swd_plan_set_groups(void *plan, int fwd_shots, int bwd_shots);
