@@ -31,7 +31,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp",
              "-Wall", f"-I{INC}", f"-I{CSRC}"]
 
-HIP_SOURCES = ["swd_plan.hip", "swd_kernels.hip", "swd_loops.hip", "swd_fstep.hip", "swd_decon.hip", "swd_updown.hip", "swd_cig.hip"]
+HIP_SOURCES = ["swd_plan.hip", "swd_kernels.hip", "swd_loops.hip", "swd_fstep.hip", "swd_decon.hip", "swd_updown.hip", "swd_cig.hip", "swd_adjoint.hip"]
 CXX_SOURCES = ["swd_setup.cpp"]
 DRIVERS = {"AFor": ["afor.cpp", "driver_common.cpp"], "AVSPRTM": ["avsprtm.cpp", "driver_common.cpp"],
            "Decon": ["decon.cpp", "driver_common.cpp"]}

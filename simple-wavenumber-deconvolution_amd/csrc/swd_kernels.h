@@ -38,7 +38,8 @@ struct KInst {
     float *s0_out;               // Born adjoint recompute: store background sums [slot][nx][nz]
     int s0_t0;                   // time step of slot 0 of s0 / s0_out
     float *grad;                 // Born adjoint: gradient accumulator [nx][nz]
-    float *ex, *ez;              // Born adjoint scratch [ntx][ldz]
+    float *ex, *ez;              // Born adjoint scratch [ntx][ldz]; the tiled adjoint's second slot of pvx, pvz
+    float *e2, *e3;              // the tiled adjoint's second slot of psx, psz [ntx][ldz]
     const float *dsum;           // Born adjoint: data summed per receiver node [ntr][nt]
 };
 

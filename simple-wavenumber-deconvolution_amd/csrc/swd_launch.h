@@ -9,6 +9,9 @@ hipError_t launch_p(const KGrid &g, const KInst *insts, int ninst, bool neg, int
 hipError_t launch_src_p(const KGrid &g, const KInst *insts, int ninst, int it, hipStream_t s);
 hipError_t launch_src_v(const KGrid &g, const KInst *insts, int ninst, int comp, hipStream_t s);
 hipError_t launch_adjoint_step(const KGrid &g, const KInst *insts, int ninst, int it, bool first, hipStream_t s);
+// The same step in two tiled launches (the instances' ex, ez, e2, e3 hold the
+// second slot of their CPML memories, all zero before the first step).
+hipError_t launch_adjoint_step_tiled(const KGrid &g, const KInst *insts, int ninst, int it, bool first, hipStream_t s);
 hipError_t launch_copy_rows(float *gather, const int2 *pairs, int npairs, int nt, hipStream_t s);
 hipError_t launch_sum_rows(float *dst, const float *src, const int2 *pairs, int npairs, int nt, hipStream_t s);
 hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, bool accumulate, hipStream_t s);

@@ -1937,7 +1937,10 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
     const size_t o_grad = c.add<float>(nint * nshots);
     const size_t o_s0 = c.add<float>(nint * seg * nshots);
     const size_t o_snap = c.add<float>(kFieldArrays * nf * static_cast<size_t>(nseg) * nshots);
-    const size_t per_a = kFieldArrays + 2;   // adjoint fields + ex, ez scratch
+    // adjoint fields + ex, ez scratch (the tiled step of the fused path: the
+    // second slots of pvx, pvz, psx, psz)
+    const bool tiled = P.path == SWD_PATH_FUSED;
+    const size_t per_a = kFieldArrays + (tiled ? 4 : 2);
     const size_t o_f = c.add<float>((kFieldArrays + per_a) * nf * nshots);
     SWD_TRY(ensure_ws(P, c.off));
     SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
@@ -1958,6 +1961,10 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
         bind_fields(A, fa + per_a * nf * k, nf);
         A.ex = A.p + kFieldArrays * nf;
         A.ez = A.ex + nf;
+        if (tiled) {
+            A.e2 = A.ez + nf;
+            A.e3 = A.e2 + nf;
+        }
         B.wavelet = at<float>(P, o_wav);
         B.s_ix = A.s_ix = shots[k].s_ix;
         B.s_iz = A.s_iz = shots[k].s_iz;
@@ -1998,8 +2005,11 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
         SWD_TRY(upload_insts(P, h, o_inst));
         SWD_TRY(steps(t0, t1));
         for (int it = t1 - 1; it >= t0; it--) {
-            HIP_TRY(launch_adjoint_step(g, dA, nshots, it, it == nt - 1, P.stream));
-            bl += it == nt - 1 ? 1 : 4;
+            if (tiled)
+                HIP_TRY(launch_adjoint_step_tiled(g, dA, nshots, it, it == nt - 1, P.stream));
+            else
+                HIP_TRY(launch_adjoint_step(g, dA, nshots, it, it == nt - 1, P.stream));
+            bl += it == nt - 1 ? 1 : (tiled ? 2 : 4);
         }
     }
     P.stats.fwd_ms = stats_end(P, 0);
@@ -2045,14 +2055,14 @@ extern "C" int swd_born_adjoint_dev(void *plan, int nshots, const swd_shot *shot
     const int nseg = (nt + seg - 1) / seg;
     int ntr_max = 0;
     for (int k = 0; k < nshots; k++) ntr_max = std::max(ntr_max, shots[k].nrec);
-    const double per = 4.0 * (7.0 * nf * nseg + nint * seg + 16.0 * nf + nint + static_cast<double>(ntr_max) * nt);
+    const double per = 4.0 * (7.0 * nf * nseg + nint * seg + 18.0 * nf + nint + static_cast<double>(ntr_max) * nt);
     int wave = nshots;
     SWD_TRY(wave_shots(*P, nshots, per, 4.0 * nt * 4, wave));
     // Waves also hold their per-step working set (both wavefields, the
     // background sums, the gradient) to about 768 MB, in waves of even size
     // (c5, 8 shots, nt 2000: one wave 1476 ms, waves of 4 1300 ms, of 2
     // 1390 ms, of 1 1669 ms).
-    const double step_bytes = 4.0 * (16.0 * nf + 3.0 * nint);
+    const double step_bytes = 4.0 * (18.0 * nf + 3.0 * nint);
     wave = std::min(wave, std::max(1, static_cast<int>(768.0 * (1 << 20) / step_bytes)));
     const int nwave = (nshots + wave - 1) / wave;
     wave = (nshots + nwave - 1) / nwave;
