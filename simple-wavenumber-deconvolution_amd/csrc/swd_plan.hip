@@ -2155,9 +2155,12 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     const size_t oa_grid = ca.add<KGrid>(nplans);
     const size_t oa_inst = ca.add<KStep>(h.size());
     // Groups of whole instances (an instance's tiles exchange halos between
-    // steps) whose per-step working set fits the cache budget, about six at most;
-    // each group runs its own time loop on the plan's streams in turn.  Within
-    // a group the shots of one tile sit next to each other (shared model in L2).
+    // steps): as many as the cache budget asks, rounded up to a multiple of the
+    // streams (at most eight), each closed once it holds its share of the
+    // per-step working set, so the streams carry even loads (c4 on two streams:
+    // four groups of 72-82 MB).  Each group runs its own time loop on the
+    // plan's streams in turn.  Within a group the shots of one tile sit next to
+    // each other (shared model in L2).
     struct Group {
         size_t off;
         int n;
@@ -2166,35 +2169,32 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     {
         double total = 0.0;
         for (int i = 0; i < nplans; i++) total += fwd_group_bytes(*Ps[i]) * nshots[i];
-        const double budget = std::max(kCacheBudgetBwd / P0.streams, total / 6.0);
-        int i = 0, k = 0;
-        while (i < nplans) {
-            const size_t off = btab.size();
-            double used = 0.0;
-            while (i < nplans) {
-                if (k >= nshots[i]) {   // (every plan has a shot; kept for safety)
-                    i++;
-                    k = 0;
-                    continue;
+        const int ns = P0.streams;
+        int ng = std::max(1, static_cast<int>(std::ceil(total / (kCacheBudgetBwd / ns))));
+        ng = std::min(8, (ng + ns - 1) / ns * ns);
+        const double target = total / ng;
+        double used = 0.0;
+        size_t off = 0;
+        for (int i = 0; i < nplans; i++) {
+            const int nzb = (Ps[i]->ntz + 63) / 64, nxb = (Ps[i]->ntx + 15) / 16;
+            const double per = fwd_group_bytes(*Ps[i]);
+            int k0 = 0;
+            for (int k = 0; k < nshots[i]; k++) {
+                used += per;
+                const bool close = used >= target && static_cast<int>(groups.size()) + 1 < ng;
+                if (close || k == nshots[i] - 1) {
+                    for (int t = 0; t < nzb * nxb; t++)
+                        for (int q = k0; q <= k; q++) btab.push_back(make_int2(inst0[i] + q, t));
+                    k0 = k + 1;
                 }
-                const int nzb = (Ps[i]->ntz + 63) / 64, nxb = (Ps[i]->ntx + 15) / 16;
-                const double per = fwd_group_bytes(*Ps[i]);
-                int k1 = k;
-                while (k1 < nshots[i] && (used == 0.0 || used + per <= budget)) {
-                    used += per;
-                    k1++;
-                }
-                if (k1 == k) break;   // the group is full
-                for (int t = 0; t < nzb * nxb; t++)
-                    for (int q = k; q < k1; q++) btab.push_back(make_int2(inst0[i] + q, t));
-                k = k1;
-                if (k == nshots[i]) {
-                    i++;
-                    k = 0;
+                if (close) {
+                    groups.push_back(Group{off, static_cast<int>(btab.size() - off)});
+                    off = btab.size();
+                    used = 0.0;
                 }
             }
-            if (btab.size() > off) groups.push_back(Group{off, static_cast<int>(btab.size() - off)});
         }
+        if (btab.size() > off) groups.push_back(Group{off, static_cast<int>(btab.size() - off)});
     }
     const size_t oa_btab = ca.add<int2>(btab.size());
     SWD_TRY(ensure_aux(P0, ca.off));
