@@ -326,7 +326,8 @@ class Plan:
         check(self.L.swd_plan_set_graphs(self.h, 1 if on else 0))
 
     def set_streams(self, streams):
-        """Shot groups of an RTM call in flight at once (1 or 2); results do not change."""
+        """Groups of a fused loop (RTM, forward, AFor, Born, multi-grid) in flight at once on the
+        plan's streams (1 to 4); results do not change."""
         check(self.L.swd_plan_set_streams(self.h, int(streams)))
 
     def set_tile(self, tile):

@@ -56,6 +56,26 @@ def test_multi_forward_c4_matches_single_plans_and_oracle():
         p.close()
 
 
+@pytest.mark.parametrize("streams", [1, 3])
+def test_multi_forward_groups_on_other_stream_counts(streams):
+    """The multi-grid groups (whole instances, their count a multiple of the
+    streams) give the per-grid gathers on one and on three streams."""
+    cases = configs.c4(nt=120)
+    print(f"case c4 nt 120 on {streams} streams", flush=True)
+    plans = plans_of(cases)
+    plans[0].set_streams(streams)
+    got = swd.multi_forward(plans, [c.shots for c in cases], [c.wavelet for c in cases])
+    again = swd.multi_forward(plans, [c.shots for c in cases], [c.wavelet for c in cases])   # graph replays
+    for p in plans[1:]:
+        p.set_streams(streams)
+    ref = [p.forward(c.shots, c.wavelet) for p, c in zip(plans, cases)]
+    for k, (a, b, r) in enumerate(zip(got, again, ref)):
+        assert np.array_equal(a, r), k
+        assert np.array_equal(b, r), k
+    for p in plans:
+        p.close()
+
+
 def test_graph_replays_match_eager():
     """Several replays of a captured loop, with eager work in between, all
     equal to the eager (graphs off) result."""
