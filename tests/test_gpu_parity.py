@@ -137,6 +137,29 @@ def test_forward_and_afor_groups_streams_device():
         row += n
 
 
+@pytest.mark.parametrize("Lc", [2, 8])
+def test_afor_orders_in_groups_on_three_streams(Lc):
+    """AFor at other FD orders, its wavefields in groups of two on three
+    streams, against the oracle shot by shot."""
+    c = configs.small(nshots=2, dup=True, nt=240, Lc=Lc)
+    print(f"case small afor Lc={Lc}, 2 shots, groups of 2 wavefields on 3 streams", flush=True)
+    nfftz = swd.nfft(c.grid.ntz)
+    kz, _ = swd.kz_taper(nfftz, c.grid.dz)
+    hw = swd.hilbert1d(c.wavelet)
+    plan = plan_of(c)
+    plan.set_groups(2, 0)
+    plan.set_streams(3)
+    g, up, dn = plan.afor(c.shots, c.wavelet, hw, kz)
+    row = 0
+    for k in range(2):
+        og, ou, od = oracle_problem(c, k).afor(hw, kz)
+        n = og.shape[0]
+        for name, a, b in (("gather", g, og), ("up", up, ou), ("down", dn, od)):
+            e32, _ = report(f"afor Lc={Lc} shot {k} {name}", a[row:row + n], b)
+            assert e32 <= TOL
+        row += n
+
+
 def test_born_and_adjoint_small():
     c = configs.small(nt=300)
     print("case small born", flush=True)
