@@ -289,6 +289,11 @@ def bench_updown(a):
     k_flops = n_k * g.nx * 4 * 5 * nfftz * int(np.log2(nfftz))
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     del f, im
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "updown_c2_traffic.json")
+    if os.path.exists(tfile) and nfftz == 512 and (g.nx, g.nz, g.pml, g.Lc) == (845, 502, 40, 4):
+        with open(tfile) as fh:
+            traffic = json.load(fh)["hbm_bytes_per_shot"]["4-shot launch"] * n_k
 
     out = {
         "metric": "up/down-separated RTM Gcell-updates/s (c2 batch, du and ud images)",
@@ -311,7 +316,7 @@ def bench_updown(a):
                    "fwd_ms_per_step": fwd_ms / a.steps, "bwd_ms_per_step": bwd_ms / a.steps,
                    "du_finite": bool(np.isfinite(du).all())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "copy_GBs": copy_gbs,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "copy_GBs": copy_gbs,
                      "kernel": f"k_updown (kz split + imaging of one backward step, {n_k} c2 shots, "
                                f"{g.nx} columns each, nfftz {nfftz})",
                      "bytes_per_launch": k_bytes, "avg_launch_ms": k_ms,
