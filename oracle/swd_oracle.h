@@ -94,6 +94,27 @@ int or_rtm_backward_f64(const or_grid *g, const float *vp, const float *rho, con
                         const double *strips, const float *adj,
                         double *image, double *illum_s, double *illum_r);
 
+/* The same two runs with per-step histories for the reconstruction tests
+ * ([nt][ntp] each, any may be NULL; gather may be NULL too).
+ * or_forward_hist: phist[it] = p after the source injection of step it.
+ * or_rtm_backward_hist: at backward step it, shist[it] = the reconstructed p
+ * after the strip restore, rhist[it] = the receiver p after the hard
+ * replacement, dhist[it] = dp/dt of the step (rows it = nt-1 are untouched). */
+int or_forward_hist_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                        const or_shot *s, const float *wavelet, float *gather, float *strips, float *final_state,
+                        float *phist);
+int or_forward_hist_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                        const or_shot *s, const float *wavelet, double *gather, double *strips, double *final_state,
+                        double *phist);
+int or_rtm_backward_hist_f32(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                             const or_shot *s, const float *wavelet, const float *final_state,
+                             const float *strips, const float *adj, float *image, float *illum_s, float *illum_r,
+                             float *shist, float *rhist, float *dhist);
+int or_rtm_backward_hist_f64(const or_grid *g, const float *vp, const float *rho, const or_coefs *c,
+                             const or_shot *s, const float *wavelet, const double *final_state,
+                             const double *strips, const float *adj, double *image, double *illum_s,
+                             double *illum_r, double *shist, double *rhist, double *dhist);
+
 /* AFor forward (FK:576-863): p driven by wavelet and htp driven by hwavelet,
  * gather of p, and per-step kz up/down separation of p + i*htp on the receiver
  * columns (FK:705, 762-777, 52-138).  coef is the kz taper [nfftz]. */

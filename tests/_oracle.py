@@ -45,37 +45,54 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        L.or_cal_xishu.argtypes = [C.c_int, f32p]
-        L.or_ricker.argtypes = [f32p, C.c_int, C.c_float, C.c_float, C.c_float]
-        L.or_cpml.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float] + [f32p] * 8
-        L.or_model_in_pml.argtypes = [f32p, C.c_int, C.c_int, C.c_int]
-        L.or_smooth.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int]
-        L.or_nfft.argtypes = [C.c_int]
-        L.or_nfft.restype = C.c_int
-        L.or_kz_taper.argtypes = [f32p, C.c_int, C.c_float]
-        L.or_kz_taper.restype = C.c_int
-        L.or_hilbert1d.argtypes = [f32p, f32p, C.c_int]
-        L.or_geometry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
-                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
-                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_int]
-        L.or_geometry.restype = C.c_int
-        L.or_adjoint_source.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_float]
-        L.or_scatterers.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_int]
-        L.or_image_normalize.argtypes = [f32p, f32p, C.c_int]
-        L.or_image_laplacian.argtypes = [f32p, f32p, f32p, C.c_void_p, C.c_int, C.c_int, C.c_int,
-                                         C.c_float, C.c_float]
-        L.or_edge_zero.argtypes = [f32p, C.c_int, C.c_int, C.c_int]
-        L.or_frame_size.argtypes = [C.POINTER(Grid)]
-        L.or_frame_size.restype = C.c_int
-        L.or_set_threads.argtypes = [C.c_int]
-        for name in ("or_forward_f32", "or_forward_f64", "or_rtm_backward_f32", "or_rtm_backward_f64",
-                     "or_afor_f32", "or_afor_f64", "or_born_f32", "or_born_f64",
-                     "or_born_adjoint_f32", "or_born_adjoint_f64", "or_rtm_updown_f32", "or_rtm_updown_f64",
-                     "or_rtm_cig_f32", "or_rtm_cig_f64"):
-            getattr(L, name).restype = C.c_int
-        _lib = L
+        _lib = _bind(C.CDLL(LIB_PATH))
     return _lib
+
+
+def build_variant(out_dir, plant):
+    """Builds the oracle with the deliberate mistake OR_PLANT=plant (see
+    swd_oracle_prop.inc) into out_dir and returns the bound library: the
+    reconstruction tests check that each such mistake fails them."""
+    so = os.path.join(out_dir, f"libswd_oracle_plant{plant}.so")
+    subprocess.run(["gcc", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+                    "-w", f"-DOR_PLANT={int(plant)}", "-shared", "-o", so, os.path.join(ORACLE_DIR, "swd_oracle.c"),
+                    "-lm"], check=True)
+    return _bind(C.CDLL(so))
+
+
+def _bind(L):
+    """Sets the argument and result types of the oracle's entry points on L."""
+    L.or_cal_xishu.argtypes = [C.c_int, f32p]
+    L.or_ricker.argtypes = [f32p, C.c_int, C.c_float, C.c_float, C.c_float]
+    L.or_cpml.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float] + [f32p] * 8
+    L.or_model_in_pml.argtypes = [f32p, C.c_int, C.c_int, C.c_int]
+    L.or_smooth.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.or_nfft.argtypes = [C.c_int]
+    L.or_nfft.restype = C.c_int
+    L.or_kz_taper.argtypes = [f32p, C.c_int, C.c_float]
+    L.or_kz_taper.restype = C.c_int
+    L.or_hilbert1d.argtypes = [f32p, f32p, C.c_int]
+    L.or_geometry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                              C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                              C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_int]
+    L.or_geometry.restype = C.c_int
+    L.or_adjoint_source.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_float]
+    L.or_scatterers.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.or_image_normalize.argtypes = [f32p, f32p, C.c_int]
+    L.or_image_laplacian.argtypes = [f32p, f32p, f32p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                     C.c_float, C.c_float]
+    L.or_edge_zero.argtypes = [f32p, C.c_int, C.c_int, C.c_int]
+    L.or_frame_size.argtypes = [C.POINTER(Grid)]
+    L.or_frame_size.restype = C.c_int
+    L.or_set_threads.argtypes = [C.c_int]
+    for name in ("or_forward_f32", "or_forward_f64", "or_rtm_backward_f32", "or_rtm_backward_f64",
+                 "or_afor_f32", "or_afor_f64", "or_born_f32", "or_born_f64",
+                 "or_born_adjoint_f32", "or_born_adjoint_f64", "or_rtm_updown_f32", "or_rtm_updown_f64",
+                 "or_rtm_cig_f32", "or_rtm_cig_f64", "or_forward_hist_f32", "or_forward_hist_f64",
+                 "or_rtm_backward_hist_f32", "or_rtm_backward_hist_f64"):
+        if hasattr(L, name):
+            getattr(L, name).restype = C.c_int
+    return L
 
 
 def ptr(a):
@@ -108,15 +125,19 @@ class Problem:
             wavelet = ricker(nt, f0, 1.0 / f0, dt)
         self.wavelet = np.ascontiguousarray(wavelet, np.float32)
         self.nt = nt
+        self.L = None   # a library from build_variant() in place of the oracle
+
+    def _lib(self):
+        return self.L if self.L is not None else lib()
 
     def frame_size(self):
-        return lib().or_frame_size(C.byref(self.g))
+        return self._lib().or_frame_size(C.byref(self.g))
 
     def _args(self):
         return (C.byref(self.g), ptr(self.vp), ptr(self.rho), C.byref(self.c), C.byref(self.shot))
 
     def forward(self, dtype=np.float32, strips=False, final=False, wavelet=None):
-        L = lib()
+        L = self._lib()
         w = self.wavelet if wavelet is None else np.ascontiguousarray(wavelet, np.float32)
         gather = np.zeros((self.nrec, self.nt), dtype)
         st = np.zeros((self.nt, self.frame_size()), dtype) if strips else None
@@ -127,7 +148,7 @@ class Problem:
         return gather, st, fs
 
     def rtm_backward(self, final_state, strips, adj, dtype=np.float32):
-        L = lib()
+        L = self._lib()
         adj = np.ascontiguousarray(adj, np.float32)
         out = [np.zeros(self.ntp, dtype) for _ in range(3)]
         fn = L.or_rtm_backward_f32 if dtype == np.float32 else L.or_rtm_backward_f64
@@ -137,9 +158,37 @@ class Problem:
         assert rc == 0, rc
         return out
 
+    def forward_hist(self, dtype=np.float32):
+        """(gather, strips, final_state, phist): phist[it] is p after step it, [nt][ntp]."""
+        L = self._lib()
+        gather = np.zeros((self.nrec, self.nt), dtype)
+        st = np.zeros((self.nt, self.frame_size()), dtype)
+        fs = np.zeros((3, self.ntp), dtype)
+        ph = np.zeros((self.nt, self.ntp), dtype)
+        fn = L.or_forward_hist_f32 if dtype == np.float32 else L.or_forward_hist_f64
+        rc = fn(*self._args(), ptr(self.wavelet), ptr(gather), ptr(st), ptr(fs), ptr(ph))
+        assert rc == 0, rc
+        return gather, st, fs, ph
+
+    def rtm_backward_hist(self, final_state, strips, adj, dtype=np.float32):
+        """(image, illum_s, illum_r, shist, rhist, dhist): at backward step it,
+        shist[it] is the reconstructed p, rhist[it] the receiver p after the hard
+        replacement, dhist[it] dp/dt ([nt][ntp]; row nt-1 stays zero)."""
+        L = self._lib()
+        adj = np.ascontiguousarray(adj, np.float32)
+        out = [np.zeros(self.ntp, dtype) for _ in range(3)]
+        hist = [np.zeros((self.nt, self.ntp), dtype) for _ in range(3)]
+        fn = L.or_rtm_backward_hist_f32 if dtype == np.float32 else L.or_rtm_backward_hist_f64
+        fs = np.ascontiguousarray(final_state, dtype)
+        st = np.ascontiguousarray(strips, dtype)
+        rc = fn(*self._args(), ptr(self.wavelet), ptr(fs), ptr(st), ptr(adj), *[ptr(o) for o in out],
+                *[ptr(h) for h in hist])
+        assert rc == 0, rc
+        return (*out, *hist)
+
     def rtm_cig(self, final_state, strips, adj, thetan=10, delt=1, dtype=np.float32):
         """RTM with angle gathers: image, illum_s, illum_r [ntp] and cig [ntp][thetan]."""
-        L = lib()
+        L = self._lib()
         adj = np.ascontiguousarray(adj, np.float32)
         out = [np.zeros(self.ntp, dtype) for _ in range(3)]
         cig = np.zeros((self.ntp, thetan), dtype)
@@ -155,7 +204,7 @@ class Problem:
         """Up/down-separated images (du, ud, uu, dd), each [ntp].  fwd and hfwd
         are the (final_state, strips) of the forward runs with the wavelet and
         with hwavelet, in dtype."""
-        L = lib()
+        L = self._lib()
         nfftz = kzcoef.size
         out = [np.zeros(self.ntp, dtype) for _ in range(4)]
         fn = L.or_rtm_updown_f32 if dtype == np.float32 else L.or_rtm_updown_f64
@@ -167,7 +216,7 @@ class Problem:
         return out
 
     def afor(self, hwavelet, kzcoef, dtype=np.float32):
-        L = lib()
+        L = self._lib()
         nfftz = kzcoef.size
         outs = [np.zeros((self.nrec, self.nt), dtype) for _ in range(3)]
         fn = L.or_afor_f32 if dtype == np.float32 else L.or_afor_f64
@@ -177,7 +226,7 @@ class Problem:
         return outs
 
     def born(self, m, dtype=np.float32):
-        L = lib()
+        L = self._lib()
         m = np.ascontiguousarray(m, np.float32).reshape(-1)
         d = np.zeros((self.nrec, self.nt), dtype)
         fn = L.or_born_f32 if dtype == np.float32 else L.or_born_f64
@@ -186,7 +235,7 @@ class Problem:
         return d
 
     def born_adjoint(self, d, dtype=np.float32):
-        L = lib()
+        L = self._lib()
         d = np.ascontiguousarray(d, np.float32)
         g = np.zeros(self.g.nx * self.g.nz, dtype)
         fn = L.or_born_adjoint_f32 if dtype == np.float32 else L.or_born_adjoint_f64
