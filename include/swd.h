@@ -42,7 +42,7 @@ extern "C" {
 #define SWD_E_NOMEM (-3)    /* allocation failed */
 #define SWD_E_IO (-4)       /* file missing, short or unreadable */
 #define SWD_E_STATE (-5)    /* call order (e.g. model not set) */
-#define SWD_E_RANGE (-6)    /* configuration outside reference limits (layerz, CFL, ...) */
+#define SWD_E_RANGE (-6)    /* configuration outside the limits (layerz, CFL, 32-bit field offsets, ...) */
 
 #define SWD_MAX_LC 8
 
@@ -115,6 +115,15 @@ int swd_cpml(int ntx, int ntz, int pml_arg, float dx, float dz, float dt,
 int swd_model_in_pml(float *vp, int ntz, int ntx, int pml);    /* model_in_pml (RC:1428-1468) */
 int swd_read_model(const char *path, float *vp, float *rho, int ntx, int ntz, int pml); /* get_acc_model (RC:1136-1204) */
 int swd_smooth_model(float *vp, int ntz, int ntx, int pml, int window); /* ini_model_mine (RC:1473-1569) */
+/* Largest stable velocity of the scheme, 1/(sum|rc| dt sqrt(1/dx^2 + 1/dz^2)), the
+ * limit the reference prints (RC:110-118, dx/(sum|rc| dt sqrt 2) for dx = dz);
+ * rc = NULL takes swd_cal_xishu(Lc). */
+int swd_cfl_vmax(const swd_grid *g, const float *rc, double *vmax);
+/* The checks swd_plan_set_model makes on a padded model [ntx][ntz] (no
+ * reference counterpart: the reference only prints the limit, RC:117):
+ * SWD_E_ARG for a non-positive or non-finite vp or rho, SWD_E_RANGE for a vp
+ * above swd_cfl_vmax. */
+int swd_check_model(const swd_grid *g, const float *rc, const float *vp, const float *rho);
 int swd_nfft(int n);                                           /* 2^ceil(log2 n) (FK:595-596) */
 int swd_kz_taper(float *coef, int nfftz, float dz);            /* kz taper (FC:354-377); returns layerz */
 int swd_hilbert1d(const float *rick, float *hrick, int nt);    /* hilbert_transform1d (FK:1257-1324) */
@@ -129,10 +138,14 @@ int swd_image_laplacian(const float *image, const float *vp, float *out, float *
 int swd_edge_zero(float *mig, int nx, int nz, int Lc);                           /* RC:754-768 */
 
 /* ---------------- plans ---------------- */
-int swd_plan_create(const swd_grid *g, int device, void **plan); /* variables_malloc (RK:1613, FK:1425) */
+/* variables_malloc (RK:1613, FK:1425).  SWD_E_RANGE when a padded field
+ * (ntx x ntz rounded up to 32 rows) reaches 2^30 floats. */
+int swd_plan_create(const swd_grid *g, int device, void **plan);
 int swd_plan_destroy(void *plan);                                /* variables_free (RK:1732, FK:1565) */
 int swd_plan_set_coefs(void *plan, const swd_coefs *c);          /* H2D of rc and CPML vectors (RK:278-297) */
-int swd_plan_set_model(void *plan, const float *vp, const float *rho); /* H2D of vp, rho (RK:281-282) */
+/* H2D of vp, rho (RK:281-282), after swd_check_model (the plan's rc if its
+ * coefficients are set). */
+int swd_plan_set_model(void *plan, const float *vp, const float *rho);
 int swd_plan_set_path(void *plan, int path);
 /* Tile width (columns of a 64-row workgroup tile) of the fused path: 16, 32
  * or 64, or 0 (default) to time the three on the first steps of a plan's
@@ -158,7 +171,7 @@ int swd_plan_set_streams(void *plan, int streams);
  * later calls with the same batch layout. */
 int swd_plan_set_graphs(void *plan, int on);
 /* Device memory a call may use for its workspace (0, the default: 85% of the
- * free device memory).  swd_rtm*, swd_born* and swd_forward run larger
+ * free device memory).  swd_rtm*, swd_born*, swd_forward and swd_afor run larger
  * batches in consecutive waves of shots that fit; results are the same. */
 int swd_plan_set_memory_limit(void *plan, size_t bytes);
 int swd_plan_get_stats(void *plan, swd_stats *st);

@@ -414,7 +414,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const int f = frame_index(g, x, z + k);
-                    if (f >= 0) el(pv, k) = ldg(I.strips, static_cast<unsigned>(it) * g.nframe + f);   // RK:575-620
+                    if (f >= 0) el(pv, k) = ((gcfl *)I.strips)[static_cast<size_t>(it) * g.nframe + f];   // RK:575-620
                 }
             }
             st4((gfl *)I.p[pn], o, pv);

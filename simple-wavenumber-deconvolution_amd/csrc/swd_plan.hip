@@ -83,7 +83,6 @@ struct Plan {
     size_t nfield = 0;   // ntx*ldz
     int path = SWD_PATH_FUSED;
     bool have_coefs = false, have_model = false;
-    float rc[SWD_MAX_LC] = {};
     float *d_coef = nullptr;    // 4*ntx + 4*ntz
     float *d_model = nullptr;   // 4*nfield
     char *ws = nullptr;
@@ -300,6 +299,14 @@ extern "C" int swd_plan_create(const swd_grid *g, int device, void **out)
     if (g->nx < 2 * g->Lc + 1 || g->nz < 2 * g->Lc + 1 || g->pml < 0 || g->nt < 2 || !(g->dx > 0) ||
         !(g->dz > 0) || !(g->dt > 0))
         return swd_fail(SWD_E_ARG, "invalid grid (nx=%d nz=%d pml=%d Lc=%d nt=%d)", g->nx, g->nz, g->pml, g->Lc, g->nt);
+    {
+        // The kernels form field offsets (ix*ldz + iz, ix*nz + iz over the interior) in 32 bits
+        // and address them as bytes, so a field must stay below 2^30 floats.
+        const long long pmlc = static_cast<long long>(g->pml) + g->Lc;
+        const long long ntx = g->nx + 2 * pmlc, ldz = static_cast<long long>(align_up(g->nz + 2 * pmlc, 32));
+        if (ntx * ldz >= (1LL << 30))
+            return swd_fail(SWD_E_RANGE, "grid of %lld x %lld padded cells: a field exceeds 2^30 floats", ntx, ldz);
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return swd_fail(SWD_E_DEVICE, "no HIP device available");
     if (device < 0 || device >= ndev) return swd_fail(SWD_E_ARG, "device %d outside 0..%d", device, ndev - 1);
@@ -410,6 +417,8 @@ extern "C" int swd_plan_set_model(void *plan, const float *vp, const float *rho)
 {
     if (!plan || !vp || !rho) return swd_fail(SWD_E_ARG, "swd_plan_set_model: null argument");
     Plan *P = static_cast<Plan *>(plan);
+    const int chk = swd_check_model(&P->g, P->have_coefs ? P->kg.rc : nullptr, vp, rho);
+    if (chk != SWD_OK) return chk;
     HIP_TRY(hipSetDevice(P->dev));
     const int ntx = P->ntx, ntz = P->ntz, ldz = P->ldz;
     const float dt = P->g.dt;
@@ -418,8 +427,6 @@ extern "C" int swd_plan_set_model(void *plan, const float *vp, const float *rho)
     for (int ix = 0; ix < ntx; ix++)
         for (int iz = 0; iz < ntz; iz++) {
             const size_t ip = static_cast<size_t>(ix) * ntz + iz, o = static_cast<size_t>(ix) * ldz + iz;
-            if (!(vp[ip] > 0.f) || !(rho[ip] > 0.f))
-                return swd_fail(SWD_E_ARG, "model must be positive (vp=%g rho=%g at ix=%d iz=%d)", vp[ip], rho[ip], ix, iz);
             // 1/(0.5*(rho+rho')) in double, stored as float, times dt (RK:62-64, RK:111-113)
             if (ix + 1 < ntx) Bx[o] = dt * static_cast<float>(1.0 / (0.5 * (static_cast<double>(rho[ip]) + rho[ip + ntz])));
             if (iz + 1 < ntz) Bz[o] = dt * static_cast<float>(1.0 / (0.5 * (static_cast<double>(rho[ip]) + rho[ip + 1])));
@@ -1065,6 +1072,38 @@ int afor_dev(Plan *P, int nshots, const swd_shot *shots, const float *wavelet, c
     return SWD_OK;
 }
 
+// AFor of a batch in waves of shots that fit the memory limit: per shot two
+// wavefields and the column capture ([nt][columns][nfftz] complex).
+int afor_waves(Plan *P, int nshots, const swd_shot *shots, const float *wavelet, const float *hwavelet,
+               const float *kzcoef, int nfftz, float *d_gather, float *d_up, float *d_down)
+{
+    int maxcols = 1;
+    for (int k = 0; k < nshots; k++) {
+        std::vector<char> seen(P->ntx, 0);
+        int n = 0;
+        for (int tr = 0; tr < shots[k].nrec; tr++) {
+            const int ix = shots[k].rec_ix[tr];
+            if (ix >= 0 && ix < P->ntx && !seen[ix]) {
+                seen[ix] = 1;
+                n++;
+            }
+        }
+        maxcols = std::max(maxcols, n);
+    }
+    const int per_wf = P->path == SWD_PATH_FUSED ? kStepArrays : kFieldArrays;
+    const double per_shot = 8.0 * P->g.nt * maxcols * nfftz + 8.0 * per_wf * P->nfield + 4.0 * P->ntx;
+    int wave = nshots;
+    SWD_TRY(wave_shots(*P, nshots, per_shot, 8.0 * P->g.nt + 12.0 * nfftz, wave));
+    const long long waves = P->stats.waves;
+    for (int s0 = 0; s0 < nshots; s0 += wave) {
+        const size_t off = static_cast<size_t>(traces_before(shots, s0)) * P->g.nt;
+        SWD_TRY(afor_dev(P, std::min(wave, nshots - s0), shots + s0, wavelet, hwavelet, kzcoef, nfftz, d_gather + off,
+                         d_up + off, d_down + off));
+    }
+    P->stats.waves = waves;
+    return SWD_OK;
+}
+
 int afor_args(Plan *P, const char *fn, int nshots, const swd_shot *shots, const float *wavelet, const float *hwavelet,
               const float *kzcoef, int nfftz, const float *gather, const float *up, const float *down)
 {
@@ -1087,7 +1126,7 @@ extern "C" int swd_afor(void *plan, int nshots, const swd_shot *shots, const flo
     DevBuf d;
     SWD_TRY(d.alloc(sizeof(float) * 3 * ng));
     float *d_gather = d.as<float>(), *d_up = d_gather + ng, *d_down = d_up + ng;
-    SWD_TRY(afor_dev(P, nshots, shots, wavelet, hwavelet, kzcoef, nfftz, d_gather, d_up, d_down));
+    SWD_TRY(afor_waves(P, nshots, shots, wavelet, hwavelet, kzcoef, nfftz, d_gather, d_up, d_down));
     HIP_TRY(hipMemcpy(gather, d_gather, sizeof(float) * ng, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(up, d_up, sizeof(float) * ng, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(down, d_down, sizeof(float) * ng, hipMemcpyDeviceToHost));
@@ -1101,7 +1140,7 @@ extern "C" int swd_afor_dev(void *plan, int nshots, const swd_shot *shots, const
     Plan *P;
     SWD_TRY(check_plan(plan, P));
     SWD_TRY(afor_args(P, "swd_afor_dev", nshots, shots, wavelet, hwavelet, kzcoef, nfftz, gather, up, down));
-    return afor_dev(P, nshots, shots, wavelet, hwavelet, kzcoef, nfftz, gather, up, down);
+    return afor_waves(P, nshots, shots, wavelet, hwavelet, kzcoef, nfftz, gather, up, down);
 }
 
 namespace {

@@ -91,6 +91,48 @@ extern "C" int swd_cal_xishu(int Lx, float *rx)
     return SWD_OK;
 }
 
+extern "C" int swd_cfl_vmax(const swd_grid *g, const float *rc, double *vmax)
+{
+    if (!g || !vmax || g->Lc < 1 || g->Lc > SWD_MAX_LC || !(g->dx > 0) || !(g->dz > 0) || !(g->dt > 0))
+        return swd_fail(SWD_E_ARG, "swd_cfl_vmax: bad grid");
+    float own[SWD_MAX_LC];
+    if (!rc) {
+        swd_cal_xishu(g->Lc, own);
+        rc = own;
+    }
+    double sum = 0.0;   // tmprc of RC:110-114
+    for (int i = 0; i < g->Lc; i++) sum += std::fabs(static_cast<double>(rc[i]));
+    const double h = std::sqrt(1.0 / (static_cast<double>(g->dx) * g->dx) + 1.0 / (static_cast<double>(g->dz) * g->dz));
+    *vmax = 1.0 / (sum * g->dt * h);
+    return SWD_OK;
+}
+
+extern "C" int swd_check_model(const swd_grid *g, const float *rc, const float *vp, const float *rho)
+{
+    if (!g || !vp || !rho) return swd_fail(SWD_E_ARG, "swd_check_model: null argument");
+    double lim = 0.0;
+    const int e = swd_cfl_vmax(g, rc, &lim);
+    if (e != SWD_OK) return e;
+    const long long pmlc = static_cast<long long>(g->pml) + g->Lc;
+    const long long ntz = g->nz + 2 * pmlc, ntp = (g->nx + 2 * pmlc) * ntz;
+    float vmax = 0.f;
+    long long imax = 0;
+    for (long long ip = 0; ip < ntp; ip++) {
+        if (!(vp[ip] > 0.f) || !(rho[ip] > 0.f) || !std::isfinite(vp[ip]) || !std::isfinite(rho[ip]))
+            return swd_fail(SWD_E_ARG, "model must be positive and finite (vp=%g rho=%g at ix=%lld iz=%lld)", vp[ip],
+                            rho[ip], ip / ntz, ip % ntz);
+        if (vp[ip] > vmax) {
+            vmax = vp[ip];
+            imax = ip;
+        }
+    }
+    if (vmax > lim)
+        return swd_fail(SWD_E_RANGE,
+                        "vp=%g m/s at ix=%lld iz=%lld exceeds the stability limit %.3f m/s of dx=%g dz=%g dt=%g Lc=%d",
+                        vmax, imax / ntz, imax % ntz, lim, g->dx, g->dz, g->dt, g->Lc);
+    return SWD_OK;
+}
+
 extern "C" int swd_ricker(float *rick, int nt, float f0, float t0, float dt)
 {
     if (!rick || nt < 1) return swd_fail(SWD_E_ARG, "swd_ricker: bad arguments");

@@ -81,7 +81,8 @@ class UpDownImages(C.Structure):
 _lib = None
 
 EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd_cal_xishu", "swd_ricker",
-           "swd_cpml", "swd_model_in_pml", "swd_read_model", "swd_smooth_model", "swd_nfft", "swd_kz_taper",
+           "swd_cpml", "swd_model_in_pml", "swd_read_model", "swd_smooth_model", "swd_cfl_vmax", "swd_check_model",
+           "swd_nfft", "swd_kz_taper",
            "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
            "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
@@ -131,6 +132,14 @@ def _p(a):
 
 def f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _need(a, shape, name):
+    """Raises ValueError unless a has the given size (and, for 2-D shapes, that shape):
+    the C-ABI reads these buffers by the sizes the plan implies."""
+    if a.size != int(np.prod(shape)) or (len(shape) > 1 and a.ndim > 1 and a.shape != tuple(shape)):
+        raise ValueError(f"{name}: expected shape {tuple(shape)}, got {a.shape}")
+    return a
 
 
 def i32(a):
@@ -196,6 +205,21 @@ def smooth_model(vp, ntz, ntx, pml, window):
     v = f32(vp).reshape(-1).copy()
     check(load().swd_smooth_model(_p(v), ntz, ntx, pml, window))
     return v
+
+
+def cfl_vmax(grid, rc=None):
+    """Largest stable velocity of the grid's scheme (swd_cfl_vmax)."""
+    v = C.c_double(0.0)
+    check(load().swd_cfl_vmax(C.byref(grid), _p(None if rc is None else f32(rc)), C.byref(v)))
+    return v.value
+
+
+def check_model(grid, vp, rho, rc=None):
+    """The checks a plan makes on a padded model (swd_check_model); raises SwdError."""
+    vp, rho = f32(vp).reshape(-1), f32(rho).reshape(-1)
+    _need(vp, (grid.ntx * grid.ntz,), "vp")
+    _need(rho, (grid.ntx * grid.ntz,), "rho")
+    check(load().swd_check_model(C.byref(grid), _p(None if rc is None else f32(rc)), _p(vp), _p(rho)))
 
 
 def nfft(n):
@@ -305,7 +329,8 @@ class Plan:
 
     def set_model(self, vp, rho):
         vp, rho = f32(vp).reshape(-1), f32(rho).reshape(-1)
-        assert vp.size == self.grid.ntx * self.grid.ntz == rho.size
+        _need(vp, (self.grid.ntx * self.grid.ntz,), "vp")
+        _need(rho, (self.grid.ntx * self.grid.ntz,), "rho")
         check(self.L.swd_plan_set_model(self.h, _p(vp), _p(rho)))
 
     def set_path(self, path):
@@ -350,18 +375,24 @@ class Plan:
         check(self.L.swd_plan_get_stats(self.h, C.byref(s)))
         return s.as_dict()
 
+    def _wav(self, w, name="wavelet"):
+        return _need(f32(w).reshape(-1), (self.grid.nt,), name)
+
+    def _traces(self, S, a, name):
+        return _need(f32(a), (S.ntr, self.grid.nt), name)
+
     def forward(self, shots, wavelet):
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
         g = np.zeros((S.ntr, self.grid.nt), np.float32)
-        check(self.L.swd_forward(self.h, S.n, S.arr, _p(f32(wavelet)), _p(g)))
+        check(self.L.swd_forward(self.h, S.n, S.arr, _p(self._wav(wavelet)), _p(g)))
         return g
 
     def afor(self, shots, wavelet, hwavelet, kzcoef):
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
         kz = f32(kzcoef)
         outs = [np.zeros((S.ntr, self.grid.nt), np.float32) for _ in range(3)]
-        check(self.L.swd_afor(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(hwavelet)), _p(kz), kz.size,
-                              *[_p(o) for o in outs]))
+        check(self.L.swd_afor(self.h, S.n, S.arr, _p(self._wav(wavelet)), _p(self._wav(hwavelet, "hwavelet")), _p(kz),
+                              kz.size, *[_p(o) for o in outs]))
         return outs
 
     def afor_dev(self, shots, wavelet, hwavelet, kzcoef, gather_ptr, up_ptr, down_ptr):
@@ -369,7 +400,8 @@ class Plan:
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
         kz = f32(kzcoef)
         vp = C.c_void_p
-        check(self.L.swd_afor_dev(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(hwavelet)), _p(kz), kz.size,
+        check(self.L.swd_afor_dev(self.h, S.n, S.arr, _p(self._wav(wavelet)), _p(self._wav(hwavelet, "hwavelet")),
+                                  _p(kz), kz.size,
                                   vp(int(gather_ptr)), vp(int(up_ptr)), vp(int(down_ptr))))
 
     def rtm(self, shots, wavelet, adj, want_gather=False):
@@ -377,14 +409,15 @@ class Plan:
         ntp = self.grid.ntx * self.grid.ntz
         outs = [np.zeros((S.n, ntp), np.float32) for _ in range(3)]
         gs = np.zeros((S.ntr, self.grid.nt), np.float32) if want_gather else None
-        check(self.L.swd_rtm(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(adj)), *[_p(o) for o in outs], _p(gs)))
+        check(self.L.swd_rtm(self.h, S.n, S.arr, _p(self._wav(wavelet)), _p(self._traces(S, adj, "adj")),
+                             *[_p(o) for o in outs], _p(gs)))
         return (*outs, gs)
 
     def rtm_dev(self, shots, wavelet, adj_ptr, image_ptr, illum_s_ptr, illum_r_ptr, gather_ptr=None):
         """Device-pointer RTM (pointers are integers, e.g. torch tensor data_ptr())."""
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
         vp = lambda x: None if x is None else C.c_void_p(int(x))
-        check(self.L.swd_rtm_dev(self.h, S.n, S.arr, _p(f32(wavelet)), vp(adj_ptr), vp(image_ptr),
+        check(self.L.swd_rtm_dev(self.h, S.n, S.arr, _p(self._wav(wavelet)), vp(adj_ptr), vp(image_ptr),
                                  vp(illum_s_ptr), vp(illum_r_ptr), vp(gather_ptr)))
 
     def rtm_cig(self, shots, wavelet, adj, thetan=10, delt=1):
@@ -394,7 +427,8 @@ class Plan:
         ntp = self.grid.ntx * self.grid.ntz
         outs = [np.zeros((S.n, ntp), np.float32) for _ in range(3)]
         cig = np.zeros((S.n, ntp, int(thetan)), np.float32)
-        check(self.L.swd_rtm_cig(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(adj)), int(thetan), int(delt),
+        check(self.L.swd_rtm_cig(self.h, S.n, S.arr, _p(self._wav(wavelet)), _p(self._traces(S, adj, "adj")),
+                                 int(thetan), int(delt),
                                  *[_p(o) for o in outs], _p(cig)))
         return (*outs, cig)
 
@@ -402,7 +436,7 @@ class Plan:
         """Device-pointer RTM with angle gathers (pointers are integers)."""
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
         vp = lambda x: C.c_void_p(int(x))
-        check(self.L.swd_rtm_cig_dev(self.h, S.n, S.arr, _p(f32(wavelet)), vp(adj_ptr), int(thetan), int(delt),
+        check(self.L.swd_rtm_cig_dev(self.h, S.n, S.arr, _p(self._wav(wavelet)), vp(adj_ptr), int(thetan), int(delt),
                                      vp(image_ptr), vp(illum_s_ptr), vp(illum_r_ptr), vp(cig_ptr)))
 
     def rtm_updown(self, shots, wavelet, hwavelet, adj, hadj, kzcoef, want_same_dir=False, want_image=False):
@@ -417,8 +451,9 @@ class Plan:
         outs = {k: np.zeros((S.n, ntp), np.float32) for k in names}
         imgs = UpDownImages(*[C.c_void_p(outs[k].ctypes.data) if k in outs else None for k in UpDownImages.NAMES])
         kz = f32(kzcoef)
-        check(self.L.swd_rtm_updown(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(hwavelet)), _p(f32(adj)),
-                                    _p(f32(hadj)), _p(kz), kz.size, C.byref(imgs)))
+        check(self.L.swd_rtm_updown(self.h, S.n, S.arr, _p(self._wav(wavelet)), _p(self._wav(hwavelet, "hwavelet")),
+                                    _p(self._traces(S, adj, "adj")), _p(self._traces(S, hadj, "hadj")), _p(kz),
+                                    kz.size, C.byref(imgs)))
         return outs
 
     def rtm_updown_dev(self, shots, wavelet, hwavelet, adj_ptr, hadj_ptr, kzcoef, out_ptrs):
@@ -427,7 +462,7 @@ class Plan:
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
         imgs = UpDownImages(*[C.c_void_p(int(out_ptrs[k])) if out_ptrs.get(k) else None for k in UpDownImages.NAMES])
         kz = f32(kzcoef)
-        check(self.L.swd_rtm_updown_dev(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(hwavelet)),
+        check(self.L.swd_rtm_updown_dev(self.h, S.n, S.arr, _p(self._wav(wavelet)), _p(self._wav(hwavelet, "hwavelet")),
                                         C.c_void_p(int(adj_ptr)), C.c_void_p(int(hadj_ptr)), _p(kz), kz.size,
                                         C.byref(imgs)))
 
@@ -448,28 +483,28 @@ class Plan:
 
     def born(self, shots, wavelet, m):
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
-        m = f32(m)
-        assert m.size == S.n * self.grid.nx * self.grid.nz
+        m = _need(f32(m).reshape(-1), (S.n * self.grid.nx * self.grid.nz,), "m")
         d = np.zeros((S.ntr, self.grid.nt), np.float32)
-        check(self.L.swd_born(self.h, S.n, S.arr, _p(f32(wavelet)), _p(m), _p(d)))
+        check(self.L.swd_born(self.h, S.n, S.arr, _p(self._wav(wavelet)), _p(m), _p(d)))
         return d
 
     def born_adjoint(self, shots, wavelet, d, nsnap=0):
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
         g = np.zeros(self.grid.nx * self.grid.nz, np.float32)
-        check(self.L.swd_born_adjoint(self.h, S.n, S.arr, _p(f32(wavelet)), _p(f32(d)), _p(g), int(nsnap)))
+        check(self.L.swd_born_adjoint(self.h, S.n, S.arr, _p(self._wav(wavelet)), _p(self._traces(S, d, "d")), _p(g),
+                                      int(nsnap)))
         return g
 
     def born_dev(self, shots, wavelet, m_ptr, d_ptr):
         """Device-pointer Born forward: m [nshots][nx][nz] -> d [sum nrec][nt]."""
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
-        check(self.L.swd_born_dev(self.h, S.n, S.arr, _p(f32(wavelet)), C.c_void_p(int(m_ptr)),
+        check(self.L.swd_born_dev(self.h, S.n, S.arr, _p(self._wav(wavelet)), C.c_void_p(int(m_ptr)),
                                   C.c_void_p(int(d_ptr))))
 
     def born_adjoint_dev(self, shots, wavelet, d_ptr, g_ptr, nsnap=0):
         """Device-pointer Born adjoint: d [sum nrec][nt] -> m_grad [nx][nz] (summed over shots)."""
         S = shots if isinstance(shots, ShotSet) else ShotSet(shots)
-        check(self.L.swd_born_adjoint_dev(self.h, S.n, S.arr, _p(f32(wavelet)), C.c_void_p(int(d_ptr)),
+        check(self.L.swd_born_adjoint_dev(self.h, S.n, S.arr, _p(self._wav(wavelet)), C.c_void_p(int(d_ptr)),
                                           C.c_void_p(int(g_ptr)), int(nsnap)))
 
 
@@ -479,7 +514,7 @@ def multi_forward(plans, shotsets, wavelets):
     n = len(plans)
     L = load()
     sets = [S if isinstance(S, ShotSet) else ShotSet(S) for S in shotsets]
-    wav = [f32(w) for w in wavelets]
+    wav = [_need(f32(w).reshape(-1), (plans[i].grid.nt,), "wavelet") for i, w in enumerate(wavelets)]
     outs = [np.zeros((S.ntr, plans[i].grid.nt), np.float32) for i, S in enumerate(sets)]
     arr_p = (C.c_void_p * n)(*[p.h.value for p in plans])
     arr_n = (C.c_int * n)(*[S.n for S in sets])
