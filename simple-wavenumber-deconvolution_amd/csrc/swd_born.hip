@@ -1,0 +1,309 @@
+// swd_born.hip -- Born modelling and its exact adjoint with sqrt(nt) checkpoints
+// (swd_born, swd_born_adjoint and their device-pointer forms).
+#include "swd_plan.h"
+
+using namespace swd_rt;
+
+// ------------------------------------------------------------------ Born
+namespace {
+
+// Born forward of a batch: m on the device [nshots][nx][nz], d on the device
+// [sum nrec][nt].  Each shot runs the background wavefield and the scattered
+// wavefield as two instances of the same launches.
+// Born forward on the fused path: the background (2k) and scattered (2k+1)
+// fields of shot k in one workgroup per tile, the Born source from the
+// background's divergence sums in registers; shots in cache-sized groups on
+// the plan's streams.
+int born_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, const float *wavelet, const float *m,
+                   float *d)
+{
+    const int nt = P.g.nt;
+    const size_t nf = P.nfield, nint = static_cast<size_t>(P.g.nx) * P.g.nz;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KStep>(2 * nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_f = c.add<float>(kStepArrays * nf * 2 * nshots);
+    SWD_TRY(ensure_ws(P, c.off));
+    SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
+    zero_fields(P, at<float>(P, o_f), kStepArrays * nf * 2 * nshots);
+    std::vector<KStep> h(2 * nshots);
+    for (int k = 0; k < nshots; k++) {
+        KStep &B = h[2 * k], &U = h[2 * k + 1];
+        std::memset(&B, 0, sizeof B);
+        std::memset(&U, 0, sizeof U);
+        B.mode = U.mode = KSTEP_FWD;
+        bind_step(B, at<float>(P, o_f) + kStepArrays * nf * (2 * k), nf);
+        bind_step(U, at<float>(P, o_f) + kStepArrays * nf * (2 * k + 1), nf);
+        B.wavelet = at<float>(P, o_wav);
+        B.s_ix = U.s_ix = shots[k].s_ix;
+        B.s_iz = U.s_iz = shots[k].s_iz;
+        U.born_m = m + nint * k;
+        if (shots[k].nrec > 0) {
+            U.colmap = at<int>(P, o_ints) + b.col_off[k];
+            U.rowmap = at<int>(P, o_ints) + b.row_off[k];
+            U.gather = d + static_cast<size_t>(b.toff[k]) * nt;
+        }
+    }
+    SWD_TRY(upload_steps(P, h, o_inst));
+    long long launches = 0;
+    int gs = P.group_fwd > 0 ? std::max(1, P.group_fwd / 2)
+                             : static_cast<int>(kCacheBudgetBwd / P.streams / (2 * fwd_group_bytes(P)));
+    gs = even_groups(nshots, gs);
+    stats_begin(P, 0);
+    SWD_TRY(fork_streams(P));
+    const std::vector<int> bounds = group_bounds(nshots, gs, P.streams);
+    for (size_t i = 0; i + 1 < bounds.size(); i++)
+        SWD_TRY(fused_steps(P, at<KStep>(P, o_inst) + 2 * bounds[i], 2 * (bounds[i + 1] - bounds[i]), steps_up(nt), -1,
+                            P.tuned_born, &launches, true, rtm_group_stream(P, static_cast<int>(i)), nullptr, false,
+                            true));
+    SWD_TRY(join_streams(P));
+    P.stats.fwd_ms = stats_end(P, 0);
+    P.stats.fwd_launches = launches;
+    P.stats.fwd_cells = static_cast<long long>(P.ntx) * P.ntz * nt * 2 * nshots;
+    HIP_TRY(launch_copy_rows(d, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
+    HIP_TRY(hipStreamSynchronize(P.stream));
+    P.stats.ws_bytes = static_cast<long long>(P.ws_cap);
+    return SWD_OK;
+}
+
+int born_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet, const float *m, float *d)
+{
+    Batch b;
+    SWD_TRY(build_batch(P, nshots, shots, b));
+    if (P.path == SWD_PATH_FUSED) return born_dev_fused(P, nshots, shots, b, wavelet, m, d);
+    const int nt = P.g.nt;
+    const size_t nf = P.nfield, nint = static_cast<size_t>(P.g.nx) * P.g.nz;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KInst>(2 * nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_f = c.add<float>(kFieldArrays * nf * 2 * nshots);
+    SWD_TRY(ensure_ws(P, c.off));
+    SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
+    zero_fields(P, at<float>(P, o_f), kFieldArrays * nf * 2 * nshots);
+    std::vector<KInst> h(2 * nshots);
+    for (int k = 0; k < nshots; k++) {
+        KInst &B = h[2 * k], &U = h[2 * k + 1];
+        std::memset(&B, 0, sizeof B);
+        std::memset(&U, 0, sizeof U);
+        bind_fields(B, at<float>(P, o_f) + kFieldArrays * nf * 2 * k, nf);
+        bind_fields(U, at<float>(P, o_f) + kFieldArrays * nf * (2 * k + 1), nf);
+        B.wavelet = at<float>(P, o_wav);
+        B.s_ix = U.s_ix = shots[k].s_ix;
+        B.s_iz = U.s_iz = shots[k].s_iz;
+        U.bvx = B.vx; U.bvz = B.vz;
+        U.born_m = m + nint * k;
+        if (shots[k].nrec > 0) {
+            U.colmap = at<int>(P, o_ints) + b.col_off[k];
+            U.rowmap = at<int>(P, o_ints) + b.row_off[k];
+            U.gather = d + static_cast<size_t>(b.toff[k]) * nt;
+        }
+    }
+    SWD_TRY(upload_insts(P, h, o_inst));
+    SWD_TRY(forward_loop(P, at<KInst>(P, o_inst), 2, nshots));
+    HIP_TRY(launch_copy_rows(d, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
+    HIP_TRY(hipStreamSynchronize(P.stream));
+    P.stats.ws_bytes = static_cast<long long>(P.ws_cap);
+    return SWD_OK;
+}
+
+// Exact discrete adjoint of born_dev: d on the device [sum nrec][nt], m_grad
+// on the device [nx][nz], summed over the batch in shot order.  The
+// background is checkpointed every seg steps (seg = ceil(nt/nsnap), or
+// ceil(sqrt(nt)) when nsnap <= 0) and recomputed segment by segment in
+// reverse, which stores the background sums of one segment at a time instead
+// of all nt steps.
+int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wavelet, const float *d, float *m_grad,
+                     int nsnap, bool accumulate)
+{
+    Batch b;
+    SWD_TRY(build_batch(P, nshots, shots, b));
+    const int nt = P.g.nt;
+    const size_t nf = P.nfield, nint = static_cast<size_t>(P.g.nx) * P.g.nz;
+    const size_t ng = static_cast<size_t>(b.ntr) * nt;
+    const int seg = nsnap > 0 ? (nt + nsnap - 1) / nsnap : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(nt))));
+    const int nseg = (nt + seg - 1) / seg;
+    Carver c;
+    const size_t o_ints = c.add<int>(b.ints.size());
+    const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
+    const size_t o_inst = c.add<KInst>(2 * nshots);
+    const size_t o_wav = c.add<float>(nt);
+    const size_t o_d = c.add<float>(std::max<size_t>(ng, 1));
+    const size_t o_grad = c.add<float>(nint * nshots);
+    const size_t o_s0 = c.add<float>(nint * seg * nshots);
+    const size_t o_snap = c.add<float>(kFieldArrays * nf * static_cast<size_t>(nseg) * nshots);
+    // adjoint fields + ex, ez scratch (the tiled step of the fused path: the
+    // second slots of pvx, pvz, psx, psz)
+    const bool tiled = P.path == SWD_PATH_FUSED;
+    const size_t per_a = kFieldArrays + (tiled ? 4 : 2);
+    const size_t o_f = c.add<float>((kFieldArrays + per_a) * nf * nshots);
+    SWD_TRY(ensure_ws(P, c.off));
+    SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
+    HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
+    // traces sharing a node are summed into the node's last trace (the adjoint of the shared recording)
+    float *dsum = at<float>(P, o_d);
+    HIP_TRY(hipMemcpyAsync(dsum, d, sizeof(float) * ng, hipMemcpyDeviceToDevice, P.stream));
+    HIP_TRY(launch_sum_rows(dsum, d, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
+    zero_fields(P, at<float>(P, o_f), (kFieldArrays + per_a) * nf * nshots);
+    zero_fields(P, at<float>(P, o_grad), nint * nshots);
+    std::vector<KInst> h(2 * nshots);
+    float *fb = at<float>(P, o_f), *fa = fb + kFieldArrays * nf * nshots;
+    for (int k = 0; k < nshots; k++) {
+        KInst &B = h[k], &A = h[nshots + k];
+        std::memset(&B, 0, sizeof B);
+        std::memset(&A, 0, sizeof A);
+        bind_fields(B, fb + kFieldArrays * nf * k, nf);
+        bind_fields(A, fa + per_a * nf * k, nf);
+        A.ex = A.p + kFieldArrays * nf;
+        A.ez = A.ex + nf;
+        if (tiled) {
+            A.e2 = A.ez + nf;
+            A.e3 = A.e2 + nf;
+        }
+        B.wavelet = at<float>(P, o_wav);
+        B.s_ix = A.s_ix = shots[k].s_ix;
+        B.s_iz = A.s_iz = shots[k].s_iz;
+        if (shots[k].nrec > 0) {
+            A.colmap = at<int>(P, o_ints) + b.col_off[k];
+            A.rowmap = at<int>(P, o_ints) + b.row_off[k];
+            A.dsum = dsum + static_cast<size_t>(b.toff[k]) * nt;
+        }
+        A.grad = at<float>(P, o_grad) + nint * k;
+        A.s0 = at<float>(P, o_s0) + nint * seg * k;
+    }
+    const KInst *dB = at<KInst>(P, o_inst), *dA = dB + nshots;
+    const KGrid &g = P.kg;
+    long long fl = 0, bl = 0;
+    // a snapshot holds the background state before the first step of a segment
+    auto snap = [&](int k, int s) -> float * {
+        return at<float>(P, o_snap) + kFieldArrays * nf * (static_cast<size_t>(nseg) * k + s);
+    };
+    auto steps = [&](int it0, int it1) -> int { return batched_forward_steps(g, dB, nshots, it0, it1, P.stream, &fl); };
+    // checkpoint pass
+    SWD_TRY(upload_insts(P, h, o_inst));
+    stats_begin(P, 0);
+    for (int s = 0; s < nseg; s++) {
+        const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
+        for (int k = 0; k < nshots; k++)
+            HIP_TRY(hipMemcpyAsync(snap(k, s), h[k].p, sizeof(float) * kFieldArrays * nf, hipMemcpyDeviceToDevice, P.stream));
+        SWD_TRY(steps(t0, t1));
+    }
+    // reverse segments: recompute the background sums, then the adjoint steps
+    for (int s = nseg - 1; s >= 0; s--) {
+        const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
+        for (int k = 0; k < nshots; k++) {
+            HIP_TRY(hipMemcpyAsync(h[k].p, snap(k, s), sizeof(float) * kFieldArrays * nf, hipMemcpyDeviceToDevice, P.stream));
+            h[k].s0_out = at<float>(P, o_s0) + nint * seg * k;
+            h[k].s0_t0 = t0;
+            h[nshots + k].s0_t0 = t0;
+        }
+        SWD_TRY(upload_insts(P, h, o_inst));
+        SWD_TRY(steps(t0, t1));
+        for (int it = t1 - 1; it >= t0; it--) {
+            if (tiled)
+                HIP_TRY(launch_adjoint_step_tiled(g, dA, nshots, it, it == nt - 1, P.stream));
+            else
+                HIP_TRY(launch_adjoint_step(g, dA, nshots, it, it == nt - 1, P.stream));
+            bl += it == nt - 1 ? 1 : (tiled ? 2 : 4);
+        }
+    }
+    P.stats.fwd_ms = stats_end(P, 0);
+    P.stats.fwd_launches = fl;
+    P.stats.bwd_launches = bl;
+    HIP_TRY(launch_sum_shots(m_grad, at<float>(P, o_grad), nshots, nint, accumulate, P.stream));
+    HIP_TRY(hipStreamSynchronize(P.stream));
+    P.stats.ws_bytes = static_cast<long long>(P.ws_cap);
+    P.stats.ckpt_bytes = static_cast<long long>(sizeof(float)) * nshots *
+                         (static_cast<long long>(kFieldArrays) * nf * nseg + static_cast<long long>(nint) * seg);
+    P.stats.full_bytes = static_cast<long long>(sizeof(float)) * nshots * static_cast<long long>(nint) * nt;
+    return SWD_OK;
+}
+
+}  // namespace
+
+extern "C" int swd_born_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *m,
+                            float *d)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !m || !d || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_born_dev: null argument");
+    const size_t nint = static_cast<size_t>(P->g.nx) * P->g.nz;
+    int wave = nshots;
+    SWD_TRY(wave_shots(*P, nshots, 4.0 * (P->path == SWD_PATH_FUSED ? 2 * kStepArrays : 14.0) * P->nfield,
+                       4.0 * P->g.nt * 4, wave));
+    for (int s0 = 0; s0 < nshots; s0 += wave) {
+        const int n = std::min(wave, nshots - s0);
+        SWD_TRY(born_dev(*P, n, shots + s0, wavelet, m + nint * s0, d + static_cast<size_t>(traces_before(shots, s0)) * P->g.nt));
+    }
+    return SWD_OK;
+}
+
+extern "C" int swd_born_adjoint_dev(void *plan, int nshots, const swd_shot *shots, const float *wavelet,
+                                    const float *d, float *m_grad, int nsnap)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !m_grad || !d || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_born_adjoint_dev: null argument");
+    const int nt = P->g.nt;
+    const double nf = static_cast<double>(P->nfield), nint = static_cast<double>(P->g.nx) * P->g.nz;
+    const int seg = nsnap > 0 ? (nt + nsnap - 1) / nsnap : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(nt))));
+    const int nseg = (nt + seg - 1) / seg;
+    int ntr_max = 0;
+    for (int k = 0; k < nshots; k++) ntr_max = std::max(ntr_max, shots[k].nrec);
+    const double per = 4.0 * (7.0 * nf * nseg + nint * seg + 18.0 * nf + nint + static_cast<double>(ntr_max) * nt);
+    int wave = nshots;
+    SWD_TRY(wave_shots(*P, nshots, per, 4.0 * nt * 4, wave));
+    // Waves also hold their per-step working set (both wavefields, the
+    // background sums, the gradient) to about 768 MB, in waves of even size
+    // (c5, 8 shots, nt 2000: one wave 1476 ms, waves of 4 1300 ms, of 2
+    // 1390 ms, of 1 1669 ms).
+    const double step_bytes = 4.0 * (18.0 * nf + 3.0 * nint);
+    wave = std::min(wave, std::max(1, static_cast<int>(768.0 * (1 << 20) / step_bytes)));
+    const int nwave = (nshots + wave - 1) / wave;
+    wave = (nshots + nwave - 1) / nwave;
+    P->stats.waves = nwave;
+    for (int s0 = 0; s0 < nshots; s0 += wave) {
+        const int n = std::min(wave, nshots - s0);
+        SWD_TRY(born_adjoint_dev(*P, n, shots + s0, wavelet, d + static_cast<size_t>(traces_before(shots, s0)) * nt, m_grad,
+                                 nsnap, s0 > 0));
+    }
+    return SWD_OK;
+}
+
+extern "C" int swd_born(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *m, float *d)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !m || !d || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_born: null argument");
+    const size_t nint = static_cast<size_t>(P->g.nx) * P->g.nz;
+    const size_t ng = static_cast<size_t>(total_traces(nshots, shots)) * P->g.nt;
+    DevBuf buf;
+    SWD_TRY(buf.alloc(sizeof(float) * (nint * nshots + ng)));
+    float *d_m = buf.as<float>(), *d_d = d_m + nint * nshots;
+    HIP_TRY(hipMemcpy(d_m, m, sizeof(float) * nint * nshots, hipMemcpyHostToDevice));
+    SWD_TRY(swd_born_dev(plan, nshots, shots, wavelet, d_m, d_d));
+    HIP_TRY(hipMemcpy(d, d_d, sizeof(float) * ng, hipMemcpyDeviceToHost));
+    return SWD_OK;
+}
+
+extern "C" int swd_born_adjoint(void *plan, int nshots, const swd_shot *shots, const float *wavelet, const float *d,
+                                float *m_grad, int nsnap)
+{
+    Plan *P;
+    SWD_TRY(check_plan(plan, P));
+    if (!wavelet || !m_grad || !d || !shots || nshots < 1) return swd_fail(SWD_E_ARG, "swd_born_adjoint: null argument");
+    const size_t nint = static_cast<size_t>(P->g.nx) * P->g.nz;
+    const size_t ng = static_cast<size_t>(total_traces(nshots, shots)) * P->g.nt;
+    DevBuf buf;
+    SWD_TRY(buf.alloc(sizeof(float) * (nint + std::max<size_t>(ng, 1))));
+    float *d_g = buf.as<float>(), *d_d = d_g + nint;
+    HIP_TRY(hipMemcpy(d_d, d, sizeof(float) * ng, hipMemcpyHostToDevice));
+    SWD_TRY(swd_born_adjoint_dev(plan, nshots, shots, wavelet, d_d, d_g, nsnap));
+    HIP_TRY(hipMemcpy(m_grad, d_g, sizeof(float) * nint, hipMemcpyDeviceToHost));
+    return SWD_OK;
+}

@@ -1,0 +1,158 @@
+// swd_multi.hip -- forward modelling of several plans of uneven grid sizes in one
+// fused launch per step (swd_multi_forward).
+#include "swd_plan.h"
+
+using namespace swd_rt;
+
+// ------------------------------------------------------------------ multi-grid forward
+extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nshots, const swd_shot *const *shots,
+                                 const float *const *wavelets, float *const *gathers)
+{
+    if (nplans < 1 || !plans || !nshots || !shots || !wavelets || !gathers)
+        return swd_fail(SWD_E_ARG, "swd_multi_forward: null argument");
+    std::vector<Plan *> Ps(nplans);
+    for (int i = 0; i < nplans; i++) {
+        for (int j = 0; j < i; j++)
+            if (plans[j] == plans[i]) return swd_fail(SWD_E_ARG, "swd_multi_forward: plan %d is listed twice", i);
+        SWD_TRY(check_plan(plans[i], Ps[i]));
+        if (nshots[i] < 1 || !shots[i] || !wavelets[i] || !gathers[i])
+            return swd_fail(SWD_E_ARG, "swd_multi_forward: plan %d has no shots or buffers", i);
+        if (Ps[i]->dev != Ps[0]->dev || Ps[i]->g.Lc != Ps[0]->g.Lc || Ps[i]->g.nt != Ps[0]->g.nt)
+            return swd_fail(SWD_E_ARG, "swd_multi_forward: plan %d differs from plan 0 in device, Lc or nt", i);
+    }
+    Plan &P0 = *Ps[0];
+    HIP_TRY(hipSetDevice(P0.dev));
+    const int nt = P0.g.nt;
+    // per plan: its batch, fields and gathers in its own workspace
+    std::vector<Batch> bs(nplans);
+    std::vector<size_t> o_ints(nplans), o_pairs(nplans), o_wav(nplans), o_f(nplans), o_g(nplans);
+    for (int i = 0; i < nplans; i++) {
+        Plan &P = *Ps[i];
+        SWD_TRY(build_batch(P, nshots[i], shots[i], bs[i]));
+        Carver c;
+        o_ints[i] = c.add<int>(bs[i].ints.size());
+        o_pairs[i] = c.add<int2>(std::max<size_t>(1, bs[i].dup_pairs.size()));
+        o_wav[i] = c.add<float>(nt);
+        o_g[i] = c.add<float>(std::max<size_t>(1, static_cast<size_t>(bs[i].ntr) * nt));
+        o_f[i] = c.add<float>(kStepArrays * P.nfield * nshots[i]);
+        SWD_TRY(ensure_ws(P, c.off));
+    }
+    // instance, grid and block tables (tiles of 64 rows x 16 columns)
+    std::vector<KGrid> grids(nplans);
+    std::vector<KStep> h;
+    std::vector<int2> btab;
+    std::vector<int> inst0(nplans);
+    for (int i = 0; i < nplans; i++) {
+        grids[i] = Ps[i]->kg;
+        inst0[i] = static_cast<int>(h.size());
+        for (int k = 0; k < nshots[i]; k++) h.emplace_back();
+    }
+    Carver ca;
+    const size_t oa_grid = ca.add<KGrid>(nplans);
+    const size_t oa_inst = ca.add<KStep>(h.size());
+    // Groups of whole instances (an instance's tiles exchange halos between
+    // steps): as many as the cache budget asks, rounded up to a multiple of the
+    // streams (at most eight), each closed once it holds its share of the
+    // per-step working set, so the streams carry even loads (c4 on two streams:
+    // four groups of 72-82 MB).  Each group runs its own time loop on the
+    // plan's streams in turn.  Within a group the shots of one tile sit next to
+    // each other (shared model in L2).
+    struct Group {
+        size_t off;
+        int n;
+    };
+    std::vector<Group> groups;
+    {
+        double total = 0.0;
+        for (int i = 0; i < nplans; i++) total += fwd_group_bytes(*Ps[i]) * nshots[i];
+        const int ns = P0.streams;
+        int ng = std::max(1, static_cast<int>(std::ceil(total / (kCacheBudgetBwd / ns))));
+        ng = std::min(8, (ng + ns - 1) / ns * ns);
+        const double target = total / ng;
+        double used = 0.0;
+        size_t off = 0;
+        for (int i = 0; i < nplans; i++) {
+            const int nzb = (Ps[i]->ntz + 63) / 64, nxb = (Ps[i]->ntx + 15) / 16;
+            const double per = fwd_group_bytes(*Ps[i]);
+            int k0 = 0;
+            for (int k = 0; k < nshots[i]; k++) {
+                used += per;
+                const bool close = used >= target && static_cast<int>(groups.size()) + 1 < ng;
+                if (close || k == nshots[i] - 1) {
+                    for (int t = 0; t < nzb * nxb; t++)
+                        for (int q = k0; q <= k; q++) btab.push_back(make_int2(inst0[i] + q, t));
+                    k0 = k + 1;
+                }
+                if (close) {
+                    groups.push_back(Group{off, static_cast<int>(btab.size() - off)});
+                    off = btab.size();
+                    used = 0.0;
+                }
+            }
+        }
+        if (btab.size() > off) groups.push_back(Group{off, static_cast<int>(btab.size() - off)});
+    }
+    const size_t oa_btab = ca.add<int2>(btab.size());
+    SWD_TRY(ensure_aux(P0, ca.off));
+    const KGrid *d_grids = reinterpret_cast<const KGrid *>(P0.aux + oa_grid);
+    for (int i = 0; i < nplans; i++) {
+        Plan &P = *Ps[i];
+        SWD_TRY(upload_batch_on(P, bs[i], o_ints[i], o_pairs[i], P0.stream));
+        HIP_TRY(hipMemcpyAsync(P.ws + o_wav[i], wavelets[i], sizeof(float) * nt, hipMemcpyHostToDevice, P0.stream));
+        HIP_TRY(hipMemsetAsync(P.ws + o_f[i], 0, sizeof(float) * kStepArrays * P.nfield * nshots[i], P0.stream));
+        for (int k = 0; k < nshots[i]; k++) {
+            KStep &S = h[inst0[i] + k];
+            std::memset(&S, 0, sizeof S);
+            S.mode = KSTEP_FWD;
+            S.grid = d_grids + i;
+            bind_step(S, at<float>(P, o_f[i]) + kStepArrays * P.nfield * k, P.nfield);
+            S.wavelet = at<float>(P, o_wav[i]);
+            S.s_ix = shots[i][k].s_ix;
+            S.s_iz = shots[i][k].s_iz;
+            if (shots[i][k].nrec > 0) {
+                S.colmap = at<int>(P, o_ints[i]) + bs[i].col_off[k];
+                S.rowmap = at<int>(P, o_ints[i]) + bs[i].row_off[k];
+                S.gather = at<float>(P, o_g[i]) + static_cast<size_t>(bs[i].toff[k]) * nt;
+            }
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(P0.aux + oa_grid, grids.data(), sizeof(KGrid) * nplans, hipMemcpyHostToDevice, P0.stream));
+    HIP_TRY(hipMemcpyAsync(P0.aux + oa_inst, h.data(), sizeof(KStep) * h.size(), hipMemcpyHostToDevice, P0.stream));
+    HIP_TRY(hipMemcpyAsync(P0.aux + oa_btab, btab.data(), sizeof(int2) * btab.size(), hipMemcpyHostToDevice, P0.stream));
+    const KStep *d_inst = reinterpret_cast<const KStep *>(P0.aux + oa_inst);
+    const int2 *d_btab = reinterpret_cast<const int2 *>(P0.aux + oa_btab);
+    stats_begin(P0, 0);
+    SWD_TRY(fork_streams(P0));
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const Group G = groups[gi];
+        const hipStream_t st = rtm_group_stream(P0, static_cast<int>(gi));
+        auto record = [&]() -> int {
+            for (int it = 0; it < nt; it++) HIP_TRY(launch_fstep_multi(P0.g.Lc, d_inst, d_btab + G.off, G.n, it, st));
+            return SWD_OK;
+        };
+        if (P0.graphs) {
+            std::vector<long long> key = {2, reinterpret_cast<long long>(P0.aux), static_cast<long long>(h.size()),
+                                          static_cast<long long>(G.off), G.n, nt};
+            for (int i = 0; i < nplans; i++) key.push_back(reinterpret_cast<long long>(Ps[i]->ws));
+            SWD_TRY(graph_run(P0, key, record, st));
+        } else {
+            SWD_TRY(record());
+        }
+    }
+    SWD_TRY(join_streams(P0));
+    P0.stats.fwd_ms = stats_end(P0, 0);
+    P0.stats.fwd_launches = static_cast<long long>(nt) * static_cast<long long>(groups.size());
+    long long cells = 0;
+    for (int i = 0; i < nplans; i++) cells += static_cast<long long>(Ps[i]->ntx) * Ps[i]->ntz * nt * nshots[i];
+    P0.stats.fwd_cells = cells;
+    for (int i = 0; i < nplans; i++) {
+        Plan &P = *Ps[i];
+        HIP_TRY(launch_copy_rows(at<float>(P, o_g[i]), at<int2>(P, o_pairs[i]), static_cast<int>(bs[i].dup_pairs.size()), nt,
+                                 P0.stream));
+        HIP_TRY(hipMemcpyAsync(gathers[i], at<float>(P, o_g[i]), sizeof(float) * bs[i].ntr * nt, hipMemcpyDeviceToHost,
+                               P0.stream));
+    }
+    HIP_TRY(hipStreamSynchronize(P0.stream));
+    return SWD_OK;
+}
+
