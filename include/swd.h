@@ -267,6 +267,19 @@ int swd_rtm_updown_dev(void *plan, int nshots, const swd_shot *shots, const floa
 int swd_updown_image_dev(void *plan, int nshots, const float *src, const float *hsrc, const float *rcv,
                          const float *hrcv, const float *kzcoef, int nfftz, const swd_updown_images *out);
 
+/* The step of swd_updown_image_dev with no device allocation.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight per-step serving loops, harnesses that forbid
+ * hipMalloc).  Its tables (shot descriptors, twiddles, kz taper) go to a fixed
+ * 64 KiB array of the library on each GPU instead of the plan's workspace:
+ * nshots <= SWD_NOALLOC_MAX_SHOTS (else SWD_E_RANGE), nfftz <= 4096 as for
+ * swd_updown_image_dev.  Every call on a GPU shares that GPU's array, which
+ * keeps the last call's tables: calls queued on one stream are ordered, calls
+ * that overlap on one GPU (other streams or threads) race. */
+#define SWD_NOALLOC_MAX_SHOTS 64
+int swd_updown_image_dev_noalloc(void *plan, int nshots, const float *src, const float *hsrc, const float *rcv,
+                                 const float *hrcv, const float *kzcoef, int nfftz, const swd_updown_images *out);
+
 /* Born modelling d = L m, m = dK/K on the interior [nx][nz] (no reference
  * counterpart: linearisation of the RK:24-221 scheme), and its exact discrete
  * adjoint m_grad = L^T d with sqrt(nt) checkpointing (nsnap <= 0: automatic).

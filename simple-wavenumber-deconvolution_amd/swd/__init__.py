@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libswd.so")
 SWD_PATH_PLAIN = 0
 SWD_PATH_BATCHED = 1
 SWD_PATH_FUSED = 2
+SWD_NOALLOC_MAX_SHOTS = 64
 ERRORS = {-1: "SWD_E_ARG", -2: "SWD_E_DEVICE", -3: "SWD_E_NOMEM", -4: "SWD_E_IO", -5: "SWD_E_STATE",
           -6: "SWD_E_RANGE"}
 
@@ -88,6 +89,7 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
            "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor", "swd_afor_dev",
            "swd_rtm", "swd_rtm_dev", "swd_rtm_cig", "swd_rtm_cig_dev", "swd_rtm_updown", "swd_rtm_updown_dev", "swd_updown_image_dev",
+           "swd_updown_image_dev_noalloc",
            "swd_plan_get_stream", "swd_born", "swd_born_adjoint", "swd_born_dev",
            "swd_born_adjoint_dev", "swd_multi_forward", "swd_decon_read_parameters", "swd_decon_geometry",
            "swd_decon_taper", "swd_decon_plan_create", "swd_decon_plan_destroy", "swd_decon", "swd_decon_dev",
@@ -474,6 +476,16 @@ class Plan:
         vp = lambda x: C.c_void_p(int(x))
         check(self.L.swd_updown_image_dev(self.h, int(nshots), vp(src_ptr), vp(hsrc_ptr), vp(rcv_ptr), vp(hrcv_ptr),
                                           _p(kz), kz.size, C.byref(imgs)))
+
+    def updown_image_dev_noalloc(self, nshots, src_ptr, hsrc_ptr, rcv_ptr, hrcv_ptr, kzcoef, out_ptrs):
+        """The same step with its tables in the library's fixed per-GPU array
+        (swd_updown_image_dev_noalloc): no device allocation, at most
+        SWD_NOALLOC_MAX_SHOTS shots; calls overlapping on one GPU race."""
+        imgs = UpDownImages(*[C.c_void_p(int(out_ptrs[k])) if out_ptrs.get(k) else None for k in UpDownImages.NAMES])
+        kz = f32(kzcoef)
+        vp = lambda x: C.c_void_p(int(x))
+        check(self.L.swd_updown_image_dev_noalloc(self.h, int(nshots), vp(src_ptr), vp(hsrc_ptr), vp(rcv_ptr),
+                                                  vp(hrcv_ptr), _p(kz), kz.size, C.byref(imgs)))
 
     def stream(self):
         """The plan's hipStream_t as an integer (e.g. for torch.cuda.ExternalStream)."""
