@@ -64,25 +64,105 @@ def build_case(nshots, nt):
     return configs.c2(nt=nt, shot_ix=xs)
 
 
-def cpu_baseline(case, nt_sample, threads, nshots=2):
+def host_cpu():
+    """(threads to use, description of the host's CPUs): the threads OMP_NUM_THREADS
+    grants (the CPU share a GPU box gives one GPU), else every CPU this process
+    may run on."""
+    avail = len(os.sched_getaffinity(0))
+    try:
+        threads = min(avail, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        threads = avail
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, f"{model}; nproc {os.cpu_count()}, {avail} in this process's affinity mask"
+
+
+def native_oracle():
+    """The oracle built with -march=native for this host (the CPU baseline only; the
+    parity tests use the portable build), or the portable build if that fails."""
+    import ctypes as C
+    import subprocess
+    import tempfile
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+
+    out = tempfile.mkdtemp(prefix="swd_oracle_native_")
+    so = os.path.join(out, "libswd_oracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC", "-w",
+           "-shared", "-o", so, os.path.join(ROOT, "oracle", "swd_oracle.c"), "-lm"]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+        return O._bind(C.CDLL(so)), "gcc -O3 -march=native -ffp-contract=off"
+    except (OSError, subprocess.SubprocessError):
+        return O.lib(), "gcc -O3 -march=x86-64-v3 -ffp-contract=off (portable build; the native build failed)"
+
+
+def cpu_baseline(case, nt_sample, runs=5):
+    """The oracle's fp32 host loops (OpenMP over x) on one shot of the workload with
+    nt_sample steps (forward with strips, reconstruction, receiver propagation,
+    imaging): one warm-up run, then the median of `runs`."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
     from helpers import oracle_problem
     import swd
 
-    O.set_threads(threads)
+    L, flags = native_oracle()
+    threads, cpus = host_cpu()
+    L.or_set_threads(threads)
     g = case.grid
     sub = swd.configs.Case(case.name, swd.Grid(g.nx, g.nz, g.pml, g.Lc, g.dx, g.dz, g.dt, nt_sample),
-                           case.vp, case.rho, case.coefs, case.wavelet[:nt_sample], case.shots[:nshots], case.f0)
-    t0 = time.perf_counter()
-    for k in range(nshots):
-        P = oracle_problem(sub, k)
+                           case.vp, case.rho, case.coefs, case.wavelet[:nt_sample], case.shots[:1], case.f0)
+    P = oracle_problem(sub, 0)
+    P.L = L
+    times = []
+    for r in range(runs + 1):
+        t0 = time.perf_counter()
         d, st, fs = P.forward(strips=True, final=True)
         adj = O.adjoint_source(d, g.dt)
         P.rtm_backward(fs, st, adj)
-    dt = time.perf_counter() - t0
-    cells = rtm_cells(sub.grid) * nshots
-    return cells / dt / 1e9, dt
+        if r > 0:
+            times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    rate = rtm_cells(sub.grid) / med / 1e9
+    return {"value": rate, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle fp32 host loops (OpenMP over x, {flags}) on 1 c2 shot with nt={nt_sample} "
+                      f"(forward with strips + reconstruction + receiver propagation + imaging): median of "
+                      f"{runs} runs after 1 warm-up, {med:.2f}s each; {threads} threads on {cpus}",
+            "cpu": cpus, "runs_s": times}
+
+
+def backward_launch_ms(swd, torch, case, adj, nshots, group, streams, nt, tile):
+    """HIP-event time per backward launch of an RTM of the first nshots shots of
+    case cut to nt steps, in groups of `group` shots on `streams` streams: the
+    plan's backward-phase events (on its stream) over its launches, on a second
+    call (the first tunes the tile and captures the graphs)."""
+    g = case.grid
+    sub = swd.configs.Case(case.name, swd.Grid(g.nx, g.nz, g.pml, g.Lc, g.dx, g.dz, g.dt, nt), case.vp, case.rho,
+                           case.coefs, case.wavelet[:nt], case.shots[:nshots], case.f0)
+    plan = swd.Plan(sub.grid, sub.coefs, sub.vp, sub.rho, device=0, tile=tile)
+    plan.set_groups(group, group)
+    plan.set_streams(streams)
+    shots = swd.ShotSet(sub.shots)
+    dev = torch.device("cuda:0")
+    d_adj = torch.from_numpy(np.ascontiguousarray(adj[:shots.ntr, :nt])).to(dev)
+    ntp = g.ntx * g.ntz
+    d_img = torch.zeros((3, nshots, ntp), dtype=torch.float32, device=dev)
+    for _ in range(2):
+        plan.rtm_dev(shots, sub.wavelet, d_adj.data_ptr(), d_img[0].data_ptr(), d_img[1].data_ptr(),
+                     d_img[2].data_ptr())
+    st = plan.stats()
+    plan.close()
+    del d_adj, d_img
+    return st["bwd_ms"] / max(1, st["bwd_launches"]), st["bwd_launches"] / (nt - 1)
 
 
 def copy_bandwidth(torch, dev, nbytes=1 << 30, reps=5):
@@ -153,7 +233,7 @@ def bench_decon(a):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import _oracle as O
 
-        threads = min(16, os.cpu_count() or 1)
+        threads, cpus = host_cpu()
         O.set_threads(threads)
         t1 = time.perf_counter()
         odec, _ = O.decon(O.decon_params(nx, nz), img, psf)
@@ -161,7 +241,7 @@ def bench_decon(a):
         same = bool(np.array_equal(odec, td.cpu().numpy()))
         cpu = {"value": windows / dt, "unit": "windows/s", "cores": threads, "kind": "port",
                "sample": f"oracle (direct fp64 DFTs, OpenMP over windows) on the whole d1 image, {dt:.2f}s; "
-                         f"GPU output bit-identical: {same}"}
+                         f"GPU output bit-identical: {same}; {threads} threads on {cpus}"}
     out = {
         "metric": "PSF-deconvolution windows/s (kkdecon, 101x101 windows)",
         "value": windows * a.steps / elapsed,
@@ -293,7 +373,7 @@ def bench_updown(a):
     tfile = os.path.join(ROOT, "profiles", "updown_c2_traffic.json")
     if os.path.exists(tfile) and nfftz == 512 and (g.nx, g.nz, g.pml, g.Lc) == (845, 502, 40, 4):
         with open(tfile) as fh:
-            traffic = json.load(fh)["hbm_bytes_per_shot"]["4-shot launch"] * n_k
+            traffic = json.load(fh)["l2_miss_bytes_per_shot"]["4-shot launch"] * n_k
 
     out = {
         "metric": "up/down-separated RTM Gcell-updates/s (c2 batch, du and ud images)",
@@ -484,7 +564,7 @@ def bench_afor(a):
     tfile = os.path.join(ROOT, "profiles", "afor_c2_traffic.json")
     if os.path.exists(tfile) and (g.nx, g.nz, g.pml, g.Lc) == (845, 502, 40, 4):
         with open(tfile) as f:
-            traffic = json.load(f)["hbm_bytes_per_wavefield_step"] * 2 * a.batch / launches_per_step
+            traffic = json.load(f)["l2_miss_bytes_per_wavefield_step"] * 2 * a.batch / launches_per_step
 
     plain = None
     if not a.no_plain:
@@ -508,7 +588,7 @@ def bench_afor(a):
         import _oracle as O
         from helpers import oracle_problem
 
-        threads = min(16, os.cpu_count() or 1)
+        threads, cpus = host_cpu()
         O.set_threads(threads)
         true_case = swd.configs.Case(case.name, g, case.vp_true, case.rho, case.coefs, case.wavelet,
                                      case.shots[:2], case.f0)
@@ -518,7 +598,7 @@ def bench_afor(a):
         dt = time.perf_counter() - t1
         cpu = {"value": 2 * 2 * g.nt * N / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
                "sample": f"oracle fp32 host loops (OpenMP) on 2 c2 shots with nt={g.nt} (p and Hilbert "
-                         f"wavefields, capture, kz separation), {dt:.1f}s"}
+                         f"wavefields, capture, kz separation), {dt:.1f}s; {threads} threads on {cpus}"}
         log(f"bench: cpu baseline {cpu['value']:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
 
     out = {
@@ -624,7 +704,7 @@ def bench_born(a):
     tfile = os.path.join(ROOT, "profiles", "born_c5_traffic.json")
     if os.path.exists(tfile) and (g.nx, g.nz, g.pml, g.Lc) == (1690, 1001, 60, 4):
         with open(tfile) as f:
-            traffic = json.load(f)["hbm_bytes_per_shot_step"] * nshots / launches_per_step
+            traffic = json.load(f)["l2_miss_bytes_per_shot_step"] * nshots / launches_per_step
     grad_finite = bool(torch.isfinite(d_g).all().item())
 
     cpu = None
@@ -633,7 +713,7 @@ def bench_born(a):
         import _oracle as O
         from helpers import oracle_problem
 
-        threads = min(16, os.cpu_count() or 1)
+        threads, cpus = host_cpu()
         O.set_threads(threads)
         nts = min(g.nt, a.cpu_nt_born)
         sub = configs.Case(case.name, swd.Grid(g.nx, g.nz, g.pml, g.Lc, g.dx, g.dz, g.dt, nts), case.vp,
@@ -647,7 +727,8 @@ def bench_born(a):
         dt = time.perf_counter() - t0
         cpu = {"value": 2 * 2 * nts * N / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
                "sample": f"oracle fp32 host loops (OpenMP) on 1 c5 shot with nt={nts}: Born forward + adjoint "
-                         f"(the oracle keeps every step's background, no recompute), {dt:.1f}s"}
+                         f"(the oracle keeps every step's background, no recompute), {dt:.1f}s; {threads} threads "
+                         f"on {cpus}"}
         log(f"bench: cpu baseline {cpu['value']:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
 
     out = {
@@ -739,7 +820,7 @@ def bench_multi(a):
         import _oracle as O
         from helpers import oracle_problem
 
-        threads = min(16, os.cpu_count() or 1)
+        threads, cpus = host_cpu()
         O.set_threads(threads)
         t0 = time.perf_counter()
         ccells = 0
@@ -752,7 +833,7 @@ def bench_multi(a):
         dt = time.perf_counter() - t0
         cpu = {"value": ccells / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
                "sample": f"oracle fp32 host loops (OpenMP) on 8 of the c1-grid shots and the 32 uneven grids, "
-                         f"nt {cases[0].grid.nt}, {dt:.1f}s"}
+                         f"nt {cases[0].grid.nt}, {dt:.1f}s; {threads} threads on {cpus}"}
         log(f"bench: cpu baseline {cpu['value']:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
 
     out = {
@@ -795,7 +876,9 @@ def main():
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-nt", type=int, default=4000)
+    ap.add_argument("--cpu-nt", type=int, default=2000, help="time steps of the CPU baseline's c2 shot")
+    ap.add_argument("--nt-iso", type=int, default=400,
+                    help="time steps of the isolated and HBM-resident backward-launch measurements")
     ap.add_argument("--tile", type=int, default=0, help="fused tile width (0 = tuned on the first steps)")
     ap.add_argument("--path", choices=("fused", "batched"), default="fused")
     ap.add_argument("--groups", type=int, nargs=2, default=(0, 0), metavar=("FWD", "BWD"),
@@ -901,18 +984,32 @@ def main():
     img = d_img[0].cpu().numpy()
     finite = bool(np.isfinite(img).all())
 
-    # roofline of the dominant kernel (the fused backward step) from HIP-event timing on the plan's stream
+    # the dominant kernel, the fused backward step, timed three ways with HIP events on the plan's stream
     nrec = len(case.shots[0][2])
     nframe = 2 * g.Lc * (g.nx + g.nz - 2 * g.Lc)
     bf, bb = rtm_bytes_per_step(g, nrec, nframe)
     st = plan.stats()
-    bwd_launch_ms = bwd_ms / a.steps / max(1, st["bwd_launches"])
     bwd_launches_per_step = st["bwd_launches"] / (g.nt - 1)
-    bytes_per_launch = bb * a.batch / bwd_launches_per_step
-    achieved = bytes_per_launch / (bwd_launch_ms * 1e-3) / 1e9
+    group = -(-a.batch // int(round(bwd_launches_per_step)))   # the largest backward group of the run
+    # (1) overlapped: the timed region's backward phase over its launches (groups on a.streams streams)
+    ov_ms = bwd_ms / a.steps / max(1, st["bwd_launches"])
+    ov_bytes = bb * a.batch / bwd_launches_per_step
+    ov_achieved = ov_bytes / (ov_ms * 1e-3) / 1e9
     fwd_launch_ms = fwd_ms / a.steps / max(1, st["fwd_launches"])
     fwd_launches_per_step = st["fwd_launches"] / g.nt
     fwd_achieved = bf * a.batch / fwd_launches_per_step / (fwd_launch_ms * 1e-3) / 1e9
+    # (2) isolated: one group of that size alone on one stream, graph-replayed launches back to back
+    iso_ms, _ = backward_launch_ms(swd, torch, case, adj, group, group, 1, a.nt_iso, a.tile)
+    iso_bytes = bb * group
+    iso_achieved = iso_bytes / (iso_ms * 1e-3) / 1e9
+    log(f"bench: isolated {group}-shot backward launch {iso_ms * 1e3:.2f} us, {iso_achieved:.0f} GB/s")
+    # (3) HBM-resident: every shot of the batch in one launch on one stream, a working set twice the
+    # last-level cache (32 MB a shot at c2)
+    res_ms, _ = backward_launch_ms(swd, torch, case, adj, a.batch, a.batch, 1, a.nt_iso, a.tile)
+    res_set = 4.0 * (9 * g.nx * g.nz + 6 * g.ntx * g.ntz + 6 * (g.ntx * g.ntz - g.nx * g.nz)) * a.batch
+    res_achieved = bb * a.batch / (res_ms * 1e-3) / 1e9
+    log(f"bench: {a.batch}-shot backward launch ({res_set / 2**20:.0f} MiB working set) {res_ms * 1e3:.1f} us, "
+        f"{res_achieved:.0f} GB/s")
 
     plain = None
     if not a.no_plain:
@@ -934,19 +1031,15 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        rate, dt = cpu_baseline(case, a.cpu_nt, threads)
-        cpu = {"value": rate, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-               "sample": f"oracle fp32 host loops (OpenMP) on 2 c2 shots with nt={a.cpu_nt} "
-                         f"(forward with strips + backward + imaging), {dt:.1f}s"}
-        log(f"bench: cpu baseline {rate:.3f} Gcell/s in {dt:.1f}s on {threads} threads")
+        cpu = cpu_baseline(case, a.cpu_nt)
+        log(f"bench: cpu baseline {cpu['value']:.3f} Gcell/s ({cpu['sample']})")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "rtm_fused_c2_traffic.json")
     if a.path == "fused" and os.path.exists(tfile) and (g.nx, g.nz, g.pml, g.Lc) == (845, 502, 40, 4):
         with open(tfile) as f:
             t = json.load(f)
-        traffic = t["hbm_bytes_per_shot_step"]["backward"] * a.batch / bwd_launches_per_step
+        traffic = t["l2_miss_bytes_per_shot_step"]["backward"] * group
 
     out = {
         "metric": "Gcell-updates/s and batched shots/s vs plain GPU path at <=1e-5 rel L2",
@@ -969,18 +1062,30 @@ def main():
                    "speedup_vs_plain_shots_per_s": (shots_per_s / plain["shots_per_s"]) if plain else None,
                    "fwd_ms_per_step": fwd_ms / a.steps, "bwd_ms_per_step": bwd_ms / a.steps,
                    "fwd_achieved_GBs": fwd_achieved, "image_finite": finite},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
-                     "kernel": "k_fstep (fused RTM backward step: reconstruction + receiver + imaging)"
-                               if a.path == "fused" else "RTM backward step (4 sub-step kernels)",
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": bwd_launch_ms,
-                     "timing": ("HIP events on the plan's stream around the backward phase (joining the other "
-                                "streams) / launches: with several streams consecutive groups' launches overlap, so this "
-                                "is the interval between launch completions" if a.streams > 1 else
-                                "HIP events on the plan's stream around the backward phase / launches"),
-                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per shot-step, "
-                                       "profiles/rtm_fused_c2_16shots.md" if traffic else None},
+        "roofline": {"bound": "hbm", "achieved": iso_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": iso_achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_note": "FETCH_SIZE x2 + WRITE_SIZE per launch (profiles/rtm_fused_c2_traffic.json): "
+                                     "L2-miss bytes, which include last-level-cache hits, so an upper bound on "
+                                     "HBM bytes" if traffic else None,
+                     "copy_GBs": copy_gbs, "frac_of_copy": iso_achieved / copy_gbs,
+                     "kernel": (f"k_fstep paired backward step (reconstruction + receiver + imaging), {group}-shot "
+                                "group alone on one stream") if a.path == "fused" else
+                               "RTM backward step (4 sub-step kernels)",
+                     "bytes_per_launch": iso_bytes, "avg_launch_ms": iso_ms,
+                     "timing": f"HIP events on the plan's stream around {a.nt_iso - 1} graph-replayed launches of one "
+                               f"{group}-shot group with no other work in flight: launch duration plus launch gap",
+                     "overlapped": {"achieved": ov_achieved, "frac": ov_achieved / HBM_PEAK_GBS,
+                                    "bytes_per_launch": ov_bytes, "avg_launch_ms": ov_ms,
+                                    "timing": f"the timed region: backward phase (HIP events on the plan's stream, "
+                                              f"joining the others) over its launches, groups on {a.streams} "
+                                              "streams overlapping: the interval between launch completions; "
+                                              "the groups' working set is served largely by the last-level cache"},
+                     "hbm_resident": {"achieved": res_achieved, "frac": res_achieved / HBM_PEAK_GBS,
+                                      "shots_per_launch": a.batch, "working_set_MiB": res_set / 2**20,
+                                      "avg_launch_ms": res_ms,
+                                      "timing": f"{a.batch}-shot group alone on one stream, {a.nt_iso - 1} "
+                                                "launches: a per-step working set twice the 256 MB last-level cache"},
+                     "forward_overlapped_GBs": fwd_achieved},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -45,3 +45,22 @@ def test_afor_and_updown_bytes_c2():
     assert bench.updown_kernel_bytes(g, 4, 2, 512) == 4 * 845 * (16 * 502 + 16 * 502)
     # whole padded columns when they fit
     assert bench.updown_kernel_bytes(g, 1, 2, 1024) == 845 * (16 * 590 + 16 * 502)
+
+
+def test_cpu_baseline_host_and_native_oracle():
+    """The CPU baseline's host description and its -march=native oracle build,
+    which must compute what the portable oracle computes."""
+    import numpy as np
+    from swd import configs
+    from helpers import oracle_problem
+
+    threads, cpus = bench.host_cpu()
+    assert threads >= 1 and "nproc" in cpus and "affinity" in cpus
+    L, flags = bench.native_oracle()
+    c = configs.small(nt=120)
+    P = oracle_problem(c)
+    ref = P.forward(strips=True, final=True)
+    P.L = L
+    got = P.forward(strips=True, final=True)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b), flags
