@@ -31,6 +31,9 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp",
              "-Wall", f"-I{INC}", f"-I{CSRC}"]
 
+# extra hipcc flags for experiments (e.g. -DSWD_NT=1)
+HIP_FLAGS += os.environ.get("SWD_EXTRA_HIPFLAGS", "").split()
+
 HIP_SOURCES = ["swd_plan.hip", "swd_forward.hip", "swd_rtm.hip", "swd_rtm_updown.hip", "swd_born.hip", "swd_multi.hip", "swd_noalloc.hip",
                "swd_kernels.hip", "swd_loops.hip", "swd_fstep.hip", "swd_fstep_fwd.hip", "swd_fstep_pair.hip",
                "swd_fstep_pair_cig.hip", "swd_fstep_born.hip", "swd_decon.hip", "swd_updown.hip", "swd_cig.hip",
