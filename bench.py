@@ -22,6 +22,7 @@ Other workloads (--workload), each one JSON line of the same form:
   afor    the c2 AFor forward (p and Hilbert wavefields, kz up/down gathers)
   born    the c5 Born forward and exact adjoint (sqrt(nt) checkpoints)
   multi   the c4 uneven-grid forward (33 grids in one launch per step and group)
+  c5      the c5 batched RTM + up/down RTM + Born misfit gradient (128 shots, nt 8000)
 """
 import argparse
 import json
@@ -867,6 +868,200 @@ def bench_multi(a):
     print(json.dumps(out), flush=True)
 
 
+def hilbert_rows(x, chunk=2048):
+    """Hilbert transform along time of every row of x [rows][nt], the
+    definition of swd_hilbert1d (FK:1257-1324: nt padded to a power of two,
+    -i on bins 1..NZ/2-1, +i from NZ/2, DC zeroed) with numpy's fp64 FFT in
+    chunks of rows: input preparation of the c5 workload, outside the timing."""
+    nt = x.shape[1]
+    nz = 1 << (nt - 1).bit_length()
+    mult = np.zeros(nz, np.complex128)
+    mult[1:nz // 2] = -1j
+    mult[nz // 2:] = 1j
+    out = np.empty_like(x)
+    for r0 in range(0, x.shape[0], chunk):
+        X = np.fft.fft(x[r0:r0 + chunk].astype(np.float64), n=nz, axis=1)
+        out[r0:r0 + chunk] = np.fft.ifft(X * mult, axis=1).real[:, :nt].astype(np.float32)
+    return out
+
+
+def bench_c5(a):
+    """The large batched RTM + adjoint configuration (c5): 1001 x 1690 interior,
+    5 m, dt 0.4 ms, nt 8000, 60-cell CPML, 128 surface shots, 801 borehole
+    receivers.  One step = the batched RTM of all shots, their up/down-separated
+    RTM (du, ud at nfftz 1024 over the interior rows) and one data-domain L2
+    misfit gradient L^T (L m - d) through the batched Born forward and adjoint
+    with sqrt(nt) checkpoints; shots run in memory-sized waves and cache-sized
+    groups.  Inputs (observed data, adjoint sources and their Hilbert
+    transforms, m, d) are resident in HBM before the timed region."""
+    import torch
+    import swd
+    from swd import configs
+
+    nshots, nt = a.c5_shots, a.nt_c5
+    case = configs.c5(nt=nt, nshots=128, shot_index=[int(v) for v in np.linspace(0, 127, nshots)])
+    g = case.grid
+    log(f"bench: c5 RTM + up/down RTM + Born misfit gradient, {nshots} shots, nt {nt}")
+    plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0, tile=a.tile)
+    plan.set_streams(a.streams)
+    shots = swd.ShotSet(case.shots)
+    dev = torch.device("cuda:0")
+    ntp, nint = g.ntx * g.ntz, g.nx * g.nz
+    t0 = time.perf_counter()
+    plan_true = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0)
+    obs = plan_true.forward(shots, case.wavelet)
+    plan_true.close()
+    syn = plan.forward(shots, case.wavelet)
+    log(f"bench: observed and modelled data {obs.shape} in {time.perf_counter() - t0:.1f}s")
+    adj = swd.adjoint_source(obs, g.dt)
+    hadj = hilbert_rows(adj)
+    h0 = swd.hilbert1d(adj[0])
+    log(f"bench: Hilbert transforms of {adj.shape[0]} traces, first trace vs swd_hilbert1d "
+        f"{float(np.abs(hadj[0] - h0).max() / max(1e-30, np.abs(h0).max())):.1e}")
+    hw = swd.hilbert1d(case.wavelet)
+    nfftz = max(128, swd.nfft(g.nz))
+    kz, _ = swd.kz_taper(nfftz, g.dz)
+    m = (0.01 * configs.bandlimited(g.nx, g.nz, g.dx, g.dz, 10.0, 30.0, 1500.0, 3)).astype(np.float32).ravel()
+    d_adj, d_hadj = torch.from_numpy(adj).to(dev), torch.from_numpy(hadj).to(dev)
+    d_scat = torch.from_numpy(obs).to(dev) - torch.from_numpy(syn).to(dev)   # reflection data d
+    del obs, syn, hadj
+    d_m = torch.from_numpy(m).to(dev).repeat(nshots)
+    d_lm = torch.zeros((shots.ntr, nt), dtype=torch.float32, device=dev)
+    d_img = torch.zeros((3, nshots, ntp), dtype=torch.float32, device=dev)
+    d_ud = torch.zeros((2, nshots, ntp), dtype=torch.float32, device=dev)
+    d_grad = torch.zeros(nint, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    copy_gbs = copy_bandwidth(torch, dev)
+
+    def rtm():
+        plan.rtm_dev(shots, case.wavelet, d_adj.data_ptr(), d_img[0].data_ptr(), d_img[1].data_ptr(),
+                     d_img[2].data_ptr())
+
+    def updown():
+        plan.rtm_updown_dev(shots, case.wavelet, hw, d_adj.data_ptr(), d_hadj.data_ptr(), kz,
+                            {"du": d_ud[0].data_ptr(), "ud": d_ud[1].data_ptr()})
+
+    def gradient():
+        plan.born_dev(shots, case.wavelet, d_m.data_ptr(), d_lm.data_ptr())
+        d_lm.sub_(d_scat)   # residual L m - d
+        plan.born_adjoint_dev(shots, case.wavelet, d_lm.data_ptr(), d_grad.data_ptr())
+
+    phases = (("rtm", rtm), ("updown", updown), ("gradient", gradient))
+    for i in range(a.warmup):
+        for name, f in phases:
+            f()
+            torch.cuda.synchronize()
+            log(f"bench: warmup {i + 1}/{a.warmup} {name} done")
+    t = {name: 0.0 for name, _ in phases}
+    stats = {}
+    for i in range(a.steps):
+        for name, f in phases:
+            t1 = time.perf_counter()
+            f()
+            torch.cuda.synchronize()
+            t[name] += time.perf_counter() - t1
+            stats[name] = plan.stats()
+            log(f"bench: step {i + 1}/{a.steps} {name} {t[name] / (i + 1):.1f}s")
+    total = sum(t.values())
+    N = g.ntx * g.ntz
+    P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
+    nrec = len(case.shots[0][2])
+    nframe = 2 * g.Lc * (g.nx + g.nz - 2 * g.Lc)
+    bf, bb = rtm_bytes_per_step(g, nrec, nframe)
+    cells = {"rtm": rtm_cells(g) * nshots, "updown": 2 * rtm_cells(g) * nshots, "gradient": 4 * nt * N * nshots}
+    # algorithmic bytes per shot and step of each phase: the RTM forward and backward; the up/down RTM twice
+    # both plus the split (four field columns and two images per interior row); the Born forward (background
+    # and scattered field, m read) and the adjoint (adjoint and recomputed background state, the background
+    # sums read, the gradient read and written, the data read)
+    ud_split = 16 * g.nx * g.nz + 16 * g.nx * g.nz
+    byts = {"rtm": nt * bf + (nt - 1) * bb, "updown": 2 * (nt * bf + (nt - 1) * bb) + (nt - 1) * ud_split,
+            "gradient": nt * (56 * N + 32 * P + 4 * nint + 4 * nrec) + nt * (2 * (32 * N + 16 * P) + 12 * nint + 4 * nrec)}
+    per_phase = {}
+    for name, _ in phases:
+        sec = t[name] / a.steps
+        ach = byts[name] * nshots / sec / 1e9
+        per_phase[name] = {"s_per_step": sec, "gcell_per_s": cells[name] / sec / 1e9, "shots_per_s": nshots / sec,
+                           "algorithmic_GBs": ach, "frac_of_peak": ach / HBM_PEAK_GBS,
+                           "waves": stats[name]["waves"]}
+    per_phase["gradient"]["ckpt_GB"] = stats["gradient"]["ckpt_bytes"] / 1e9
+    per_phase["gradient"]["full_storage_GB"] = stats["gradient"]["full_bytes"] / 1e9
+    # the dominant kernel (the paired RTM backward step, one-shot groups at c5) alone on one stream
+    iso_ms, _ = backward_launch_ms(swd, torch, case, adj, 1, 1, 1, a.nt_iso, a.tile)
+    iso = bb / (iso_ms * 1e-3) / 1e9
+    log(f"bench: isolated 1-shot backward launch {iso_ms * 1e3:.1f} us, {iso:.0f} GB/s")
+    plain = None
+    if not a.no_plain:
+        one = swd.ShotSet(case.shots[:1])
+        plan.set_path(swd.SWD_PATH_PLAIN)
+        d1 = torch.from_numpy(adj[:nrec].copy()).to(dev)
+        im1 = torch.zeros((3, ntp), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plan.rtm_dev(one, case.wavelet, d1.data_ptr(), im1[0].data_ptr(), im1[1].data_ptr(), im1[2].data_ptr())
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t1
+        plan.set_path(swd.SWD_PATH_FUSED)
+        rel = float(np.linalg.norm(im1[0].cpu().numpy() - d_img[0, 0].cpu().numpy()) /
+                    max(1e-30, float(np.linalg.norm(d_img[0, 0].cpu().numpy()))))
+        plain = {"shots_per_s": 1.0 / tp, "image_rel_l2_vs_batched": rel}
+        log(f"bench: plain path one c5 shot {tp:.2f}s, image rel diff vs batched {rel:.2e}")
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        from helpers import oracle_problem
+
+        L, flags = native_oracle()
+        threads, cpus = host_cpu()
+        L.or_set_threads(threads)
+        nts = min(nt, a.cpu_nt_c5)
+        sub = configs.Case(case.name, swd.Grid(g.nx, g.nz, g.pml, g.Lc, g.dx, g.dz, g.dt, nts), case.vp, case.rho,
+                           case.coefs, case.wavelet[:nts], case.shots[:1], case.f0)
+        Pq = oracle_problem(sub, 0)
+        Pq.L = L
+        t1 = time.perf_counter()
+        d, st, fs = Pq.forward(strips=True, final=True)
+        Pq.rtm_backward(fs, st, O.adjoint_source(d, g.dt))
+        dt = time.perf_counter() - t1
+        rate = rtm_cells(sub.grid) / dt / 1e9
+        cpu = {"value": rate, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"oracle fp32 host loops (OpenMP over x, {flags}) on 1 c5 shot with nt={nts} (RTM: forward "
+                         f"with strips, reconstruction, receiver propagation, imaging), {dt:.1f}s; the full c5 step "
+                         f"would take {rate and sum(cells.values()) / rate / 1e9 / 3600:.0f} h at that rate; "
+                         f"{threads} threads on {cpus}"}
+        log(f"bench: cpu baseline {rate:.3f} Gcell/s in {dt:.1f}s")
+    out = {
+        "metric": "c5 batched RTM + up/down RTM + Born misfit gradient, Gcell-updates/s",
+        "value": sum(cells.values()) * a.steps / total / 1e9,
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": total * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered c5 model with 8 dipping reflectors, band-limited m, c5 geometry)",
+        "config": {"workload": f"c5: 1001x1690 interior (1129x1818 padded), 5 m, dt 0.4 ms, nt {nt}, Lc 4, {nshots} "
+                               f"surface shots, {nrec} borehole receivers; RTM, up/down RTM (nfftz {nfftz}, interior "
+                               "rows), Born forward + adjoint L2 misfit gradient with sqrt(nt) checkpoints",
+                   "shots_per_step": nshots, "phases": per_phase,
+                   "shots_per_s_rtm": nshots / (t["rtm"] / a.steps),
+                   "plain_path": plain,
+                   "speedup_vs_plain_shots_per_s": (nshots / (t["rtm"] / a.steps) / plain["shots_per_s"]) if plain
+                   else None,
+                   "grad_finite": bool(torch.isfinite(d_grad).all().item())},
+        "roofline": {"bound": "hbm", "achieved": iso, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": iso / HBM_PEAK_GBS,
+                     "traffic": None, "copy_GBs": copy_gbs, "frac_of_copy": iso / copy_gbs,
+                     "kernel": "k_fstep paired RTM backward step, one-shot group alone on one stream",
+                     "bytes_per_launch": bb, "avg_launch_ms": iso_ms,
+                     "timing": f"HIP events on the plan's stream around {a.nt_iso - 1} graph-replayed launches"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -892,10 +1087,15 @@ def main():
     ap.add_argument("--nt-born", type=int, default=8000, help="time steps of --workload born (c5: 8000)")
     ap.add_argument("--cpu-nt-born", type=int, default=1000, help="time steps of the born CPU sample")
     ap.add_argument("--nt-multi", type=int, default=1000, help="time steps of --workload multi (c4: 1000)")
-    ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig", "afor", "born", "multi"), default="rtm",
+    ap.add_argument("--c5-shots", type=int, default=128, help="shots of --workload c5 (c5: 128)")
+    ap.add_argument("--nt-c5", type=int, default=8000, help="time steps of --workload c5 (c5: 8000)")
+    ap.add_argument("--cpu-nt-c5", type=int, default=400, help="time steps of the c5 CPU sample")
+    ap.add_argument("--workload", choices=("rtm", "decon", "updown", "cig", "afor", "born", "multi", "c5"),
+                    default="rtm",
                     help="rtm: the batched RTM hot path (default); decon: the PSF deconvolution; "
                          "updown: the up/down-separated RTM; cig: RTM with angle gathers; afor: AFor forward "
-                         "modelling with the kz up/down separation; born: Born forward + adjoint on the c5 grid; multi: the c4 uneven-grid forward")
+                         "modelling with the kz up/down separation; born: Born forward + adjoint on the c5 grid; multi: the c4 uneven-grid forward; "
+                         "c5: the large batched RTM + up/down + Born misfit gradient")
     ap.add_argument("--thetan", type=int, default=10, help="angle bins of --workload cig (RC:235: 10)")
     ap.add_argument("--nfftz", type=int, default=0,
                     help="kz transform length of --workload updown (0: nz padded to a power of two, at least 128)")
@@ -924,6 +1124,9 @@ def main():
         return
     if a.workload == "multi":
         bench_multi(a)
+        return
+    if a.workload == "c5":
+        bench_c5(a)
         return
 
     import torch
