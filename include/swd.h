@@ -99,6 +99,7 @@ typedef struct swd_stats {
     long long ckpt_bytes;   /* last Born adjoint: snapshot + one segment of background sums */
     long long full_bytes;   /* last Born adjoint: background sums of all nt steps (full storage) */
     long long waves;        /* shot waves of the last call (see swd_plan_set_memory_limit) */
+    int tile_x, tile_rows, tile_waves; /* tile of the last fused loop: columns, rows, wavefronts-per-SIMD bound (0: none) */
 } swd_stats;
 
 /* ---------------- errors and devices ---------------- */
@@ -147,10 +148,20 @@ int swd_plan_set_coefs(void *plan, const swd_coefs *c);          /* H2D of rc an
  * coefficients are set). */
 int swd_plan_set_model(void *plan, const float *vp, const float *rho);
 int swd_plan_set_path(void *plan, int path);
-/* Tile width (columns of a 64-row workgroup tile) of the fused path: 16, 32
- * or 64, or 0 (default) to time the three on the first steps of a plan's
- * first fused loop for a batch size and keep the fastest.  Results do not
- * depend on it. */
+/* Tile of the fused path's workgroups (no reference counterpart: the reference
+ * fixes 16 x 16 thread blocks, RK:7, forward/headrtm.h:3): tile_x columns
+ * (16, 32, 64) by tile_rows rows (64; at Lc 4 also 32, and 128 up to 32
+ * columns) with a bound of waves wavefronts per SIMD (0: none; at Lc 4 also
+ * 5).  All zero (the default): the tuner times every configuration the loop's
+ * kernel variant supports on the first steps of a loop, keeps the default 16 x
+ * 64 unless another is more than 10% faster, and remembers the choice for the
+ * plan's later loops with as many instances per launch and on disk (the file
+ * SWD_TUNE_CACHE, else $XDG_CACHE_HOME or $HOME/.cache/swd_tiles.txt) keyed by
+ * device, grid, Lc, instances per launch, variant and streams.  Variants that
+ * lack an asked height or bound run the asked width at 64 rows.  Results do not
+ * depend on the tile. */
+int swd_plan_set_tile_config(void *plan, int tile_x, int tile_rows, int waves);
+/* swd_plan_set_tile_config(plan, tile_x, 64, 0), or the tuner for tile_x = 0. */
 int swd_plan_set_tile(void *plan, int tile_x);
 /* Group sizes of the fused loops: the RTM forward and backward run their
  * batch in groups of at most fwd_shots and bwd_shots shots, and swd_forward,

@@ -2,7 +2,8 @@
 // swd_fstep_*.hip translation units, one per kernel variant): the velocity and the pressure half of
 // one time step of every wavefield of a batch in a single launch.
 //
-// A workgroup owns a tile of 64 rows x TX columns of one wavefield.  It first
+// A workgroup owns a tile of 4 TQ rows (TQ float4 quads per column) x TX
+// columns of one wavefield.  It first
 // updates the velocities its pressure stencil needs -- vx on the tile's rows
 // over columns X0-Lc .. X1+Lc-2 and vz on the tile's columns over quads
 // -R .. 15+R (R = ceil(Lc/4)) -- into LDS, storing only its own cells, then
@@ -69,38 +70,38 @@ __device__ __forceinline__ void lds_window(const float4 *s, float (&a)[4 * (2 * 
     }
 }
 
-// Velocity-z item i of a tile: the 16 own quads of each column first (16
+// Velocity-z item i of a tile: the TQ own quads of each column first (TQ
 // consecutive lanes per column, so LDS accesses of 8-lane groups stay in one
 // column), then the 2R halo quads of each column.
-template <int TX, int R>
+template <int TX, int R, int TQ>
 __device__ __forceinline__ void vz_item(int i, int &c, int &qq)
 {
-    if (i < TX * 16) {
-        c = i >> 4;
-        qq = i & 15;
+    if (i < TX * TQ) {
+        c = i / TQ;
+        qq = i % TQ;
     } else {
-        const int j = i - TX * 16, h = j % (2 * R);
+        const int j = i - TX * TQ, h = j % (2 * R);
         c = j / (2 * R);
-        qq = h < R ? h - R : 16 + h - R;
+        qq = h < R ? h - R : TQ + h - R;
     }
 }
 
-// Threads per workgroup: one own quad per thread up to 32 columns.
-template <int TX>
-constexpr int fstep_threads() { return TX >= 32 ? 512 : 256; }
+// Threads per workgroup: one own quad per thread, 128 to 512 threads.
+template <int TX, int TQ>
+constexpr int fstep_threads() { return TX * TQ < 128 ? 128 : TX * TQ > 512 ? 512 : TX * TQ; }
 
-// Tile geometry: 64 rows (16 quads) x TX columns, Lc-deep stencils.
-template <int LC, int TX>
+// Tile geometry: 4 TQ rows (TQ quads) x TX columns, Lc-deep stencils.
+template <int LC, int TX, int TQ>
 struct FGeo {
     static constexpr int R = (LC + 3) / 4;       // z halo (quads) of a velocity stencil
     static constexpr int RP = 2 * R;             // z halo (quads) of the staged pressure
     static constexpr int OWN = 2 * LC - 1;       // staged column of X0
     static constexpr int NPC = TX + 4 * LC - 2;  // staged pressure columns X0-2Lc+1 .. X1+2Lc-2
-    static constexpr int NPQ = 16 + 2 * RP;      // staged pressure quads -RP .. 15+RP
+    static constexpr int NPQ = TQ + 2 * RP;      // staged pressure quads -RP .. TQ-1+RP
     static constexpr int NVX = TX + 2 * LC - 1;  // vx columns X0-Lc .. X1+Lc-2
-    static constexpr int VZQ = 16 + 2 * R;       // vz quads -R .. 15+R
-    static constexpr int NT = fstep_threads<TX>();       // threads per workgroup
-    static constexpr int NOWN = (TX * 16 + NT - 1) / NT;  // own quads per thread
+    static constexpr int VZQ = TQ + 2 * R;       // vz quads -R .. TQ-1+R
+    static constexpr int NT = fstep_threads<TX, TQ>();   // threads per workgroup
+    static constexpr int NOWN = (TX * TQ + NT - 1) / NT;  // own quads per thread
 };
 
 // One tile of one wavefield.  FULL: the tile and every halo lie inside the
@@ -120,14 +121,14 @@ __device__ __forceinline__ void born_sum(const KGrid &g, const KStep &I, int x, 
         el(acc, k) = S;
 }
 
-template <int LC, int TX, bool FULL, bool CIG, bool BORN = false>
+template <int LC, int TX, int TQ, bool FULL, bool CIG, bool BORN = false>
 __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X0, int Z0, int it, int first,
                                            float4 *__restrict__ sP, float4 *__restrict__ sVX, float4 *__restrict__ sVZ,
-                                           float4 (&dpa)[FGeo<LC, TX>::NOWN], float4 (&cgk)[3][FGeo<LC, TX>::NOWN])
+                                           float4 (&dpa)[FGeo<LC, TX, TQ>::NOWN], float4 (&cgk)[3][FGeo<LC, TX, TQ>::NOWN])
 {
-    using G = FGeo<LC, TX>;
+    using G = FGeo<LC, TX, TQ>;
     constexpr int R = G::R, RP = G::RP, OWN = G::OWN, NPQ = G::NPQ, VZQ = G::VZQ;
-    constexpr int N0 = G::NPC * NPQ, N1 = G::NVX * 16, N2 = TX * VZQ, N3 = TX * 16;
+    constexpr int N0 = G::NPC * NPQ, N1 = G::NVX * TQ, N2 = TX * VZQ, N3 = TX * TQ;
     constexpr int NT = G::NT;
     constexpr int I0 = (N0 + NT - 1) / NT, I1 = (N1 + NT - 1) / NT, I2 = (N2 + NT - 1) / NT, I3 = (N3 + NT - 1) / NT;
     const int tid = threadIdx.x;
@@ -148,7 +149,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     float4 vxo[I1], bxv[I1], vzo[I2], bzv[I2], kv[I3];
 #pragma unroll
     for (int r = 0; r < I1; r++) {
-        const int i = tid + NT * r, q = i & 15, cx = i >> 4;
+        const int i = tid + NT * r, q = i % TQ, cx = i / TQ;
         const int x = X0 - LC + cx, z = Z0 + 4 * q;
         vxo[r] = bxv[r] = zero4();
         if (i < N1 && (FULL || (x >= 0 && x < ntx && z < ntz))) {
@@ -161,7 +162,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     for (int r = 0; r < I2; r++) {
         const int i = tid + NT * r;
         int c, qq;
-        vz_item<TX, R>(i, c, qq);
+        vz_item<TX, R, TQ>(i, c, qq);
         const int x = X0 + c, z = Z0 + 4 * qq;
         vzo[r] = bzv[r] = zero4();
         if (i < N2 && (FULL || (x < ntx && z >= 0 && z < ntz))) {
@@ -174,7 +175,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     int cmv[I3];   // receiver map of each own column
 #pragma unroll
     for (int r = 0; r < I3; r++) {
-        const int i = tid + NT * r, q = i & 15, c = i >> 4;
+        const int i = tid + NT * r, q = i % TQ, c = i / TQ;
         const int x = X0 + c, z = Z0 + 4 * q;
         kv[r] = imv[r] = isv[r] = irv[r] = zero4();
         cmv[r] = -1;
@@ -197,7 +198,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         for (int r = 0; r < I0; r++) {
             const int i = tid + NT * r, col = i / NPQ, qq = i - col * NPQ - RP;
             const int x = X0 - OWN + col, z = Z0 + 4 * qq;
-            const bool need = i < N0 && ((qq >= 0 && qq < 16) || (col >= OWN && col < OWN + TX));
+            const bool need = i < N0 && ((qq >= 0 && qq < TQ) || (col >= OWN && col < OWN + TX));
             st[r] = zero4();
             if (need && (FULL || (x >= 0 && x < ntx && z >= 0 && z < ntz))) st[r] = ld4(P, static_cast<unsigned>(x) * ldz + z);
         }
@@ -214,7 +215,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     for (int r = 0; r < I1; r++) {
         const int i = tid + NT * r;
         if (i >= N1) break;
-        const int q = i & 15, cx = i >> 4;
+        const int q = i % TQ, cx = i / TQ;
         const int x = X0 - LC + cx, z = Z0 + 4 * q;
         float4 v = vxo[r];
         if (FULL || (x >= 0 && x < ntx && z < ntz)) {
@@ -280,12 +281,12 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         const int i = tid + NT * r;
         if (i >= N2) break;
         int c, qq;
-        vz_item<TX, R>(i, c, qq);
+        vz_item<TX, R, TQ>(i, c, qq);
         const int x = X0 + c, z = Z0 + 4 * qq;
         float4 v = vzo[r];
         if (FULL || (x < ntx && z >= 0 && z < ntz)) {
             const unsigned o = static_cast<unsigned>(x) * ldz + z;
-            const bool own = qq >= 0 && qq < 16;
+            const bool own = qq >= 0 && qq < TQ;
             const float4 B = bzv[r];
             float a[4 * (2 * R + 1)];   // p rows z-4R .. z+4R+3
             lds_window<R>(&sP[(c + OWN) * NPQ + qq + RP - R], a);
@@ -355,7 +356,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     for (int r = 0; r < I3; r++) {
         const int i = tid + NT * r;
         if (i >= N3) break;
-        const int q = i & 15, c = i >> 4;
+        const int q = i % TQ, c = i / TQ;
         const int x = X0 + c, z = Z0 + 4 * q;
         if (!FULL && (x >= ntx || z >= ntz)) continue;
         const unsigned o = static_cast<unsigned>(x) * ldz + z;
@@ -368,7 +369,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     {                                                                                                                   \
         _Pragma("unroll") for (int ic = 0; ic < LC; ic++)                                                               \
         {                                                                                                               \
-            const float4 a = sVX[(c + LC + ic) * 16 + q], b = sVX[(c + LC - ic - 1) * 16 + q];                          \
+            const float4 a = sVX[(c + LC + ic) * TQ + q], b = sVX[(c + LC - ic - 1) * TQ + q];                          \
             _Pragma("unroll") for (int k = 0; k < 4; k++) el(dvx, k) =                                                  \
                 __builtin_fmaf(g.rc[ic] * (elc(a, k) - elc(b, k)), sx, el(dvx, k));                                     \
         }                                                                                                               \
@@ -407,7 +408,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                     // the reconstruction at step it+1 for the receiver half's angle gathers:
                     // p as staged, the velocities just updated to that level
                     cgk[0][r] = pold;
-                    cgk[1][r] = sVX[(c + LC) * 16 + q];
+                    cgk[1][r] = sVX[(c + LC) * TQ + q];
                     cgk[2][r] = sVZ[c * VZQ + q + R];
                 }
             }
@@ -551,68 +552,71 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 // PAIR = false: one FWD wavefield per instance.  PAIR = true: instances
 // 2k (SRC) and 2k+1 (RCV) of one shot share each workgroup; with BORN, 2k is
 // the background and 2k+1 the scattered field of a Born forward (both FWD).
-// The angle-gather variant at Lc 4 and tiles up to 32 wide is held to four
-// wavefronts per SIMD (128 VGPRs, 8-12 bytes of spill, faster than three
-// wavefronts); elsewhere that bound would spill heavily.
-template <int LC, int TX, bool PAIR, bool CIG, bool BORN>
-__global__ __launch_bounds__(fstep_threads<TX>(), (CIG && LC == 4 && TX <= 32) ? 4 : 1) void k_fstep(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb,
-                                                              int nxb,
-                                               int it, int first)
+// WB > 0 holds the kernel to at least WB wavefronts per SIMD (a register
+// bound the tile tuner tries).  Without it, the angle-gather variant at Lc 4
+// and tiles up to 32 wide is held to four wavefronts per SIMD (128 VGPRs,
+// 8-12 bytes of spill, faster than three wavefronts); elsewhere that bound
+// would spill heavily.
+template <int LC, int TX, int TQ, int WB, bool PAIR, bool CIG, bool BORN>
+__global__ __launch_bounds__((fstep_threads<TX, TQ>()), (WB > 0 ? WB : (CIG && LC == 4 && TX <= 32) ? 4 : 1)) void k_fstep(
+    KGrid g, const KStep *__restrict__ insts, int ninst, int nzb, int nxb, int it, int first)
 {
-    using G = FGeo<LC, TX>;
+    using G = FGeo<LC, TX, TQ>;
+    constexpr int ROWS = 4 * TQ;
     __shared__ float4 sP[G::NPC * G::NPQ];
-    __shared__ float4 sVX[G::NVX * 16];
+    __shared__ float4 sVX[G::NVX * TQ];
     __shared__ float4 sVZ[TX * G::VZQ];
 
     const int L = xcd_block();
     if (L >= ninst * nzb * nxb) return;
     const int inst = L % ninst;
     const int zb = (L / ninst) % nzb, xb = L / (ninst * nzb);
-    const int X0 = xb * TX, Z0 = zb * 64;
+    const int X0 = xb * TX, Z0 = zb * ROWS;
     const bool full = X0 - G::OWN >= g.fx0 && X0 + TX + 2 * LC - 2 <= g.fx1 && Z0 - 4 * G::RP >= g.fz0 &&
-                      Z0 + 64 + 4 * G::RP - 1 <= g.fz1;
+                      Z0 + ROWS + 4 * G::RP - 1 <= g.fz1;
     float4 dpa[G::NOWN], cgk[3][G::NOWN];
 #pragma unroll
     for (int r = 0; r < G::NOWN; r++) dpa[r] = cgk[0][r] = cgk[1][r] = cgk[2][r] = zero4();
     if (!PAIR) {
         const KStep &I = insts[inst];
-        if (full) fstep_tile<LC, TX, true, false>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
-        else fstep_tile<LC, TX, false, false>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        if (full) fstep_tile<LC, TX, TQ, true, false>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        else fstep_tile<LC, TX, TQ, false, false>(g, I, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
         return;
     }
     const KStep &S = insts[2 * inst], &Rv = insts[2 * inst + 1];
     if constexpr (BORN) {
         // Born: background, then the scattered field with the background's sums in dpa
-        if (full) fstep_tile<LC, TX, true, false, true>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
-        else fstep_tile<LC, TX, false, false, true>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        if (full) fstep_tile<LC, TX, TQ, true, false, true>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        else fstep_tile<LC, TX, TQ, false, false, true>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
         __syncthreads();   // LDS is staged again for the scattered field
-        if (full) fstep_tile<LC, TX, true, false, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
-        else fstep_tile<LC, TX, false, false, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        if (full) fstep_tile<LC, TX, TQ, true, false, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        else fstep_tile<LC, TX, TQ, false, false, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
         return;
     }
     // the reconstruction writes only interior cells and the source cell
-    const bool any = X0 + TX > g.pmlc && X0 < g.ntx - g.pmlc && Z0 + 64 > g.pmlc && Z0 < g.ntz - g.pmlc;
-    const bool src = S.s_ix >= X0 && S.s_ix < X0 + TX && S.s_iz >= Z0 && S.s_iz < Z0 + 64;
+    const bool any = X0 + TX > g.pmlc && X0 < g.ntx - g.pmlc && Z0 + ROWS > g.pmlc && Z0 < g.ntz - g.pmlc;
+    const bool src = S.s_ix >= X0 && S.s_ix < X0 + TX && S.s_iz >= Z0 && S.s_iz < Z0 + ROWS;
     if (any || src) {
-        if (full) fstep_tile<LC, TX, true, CIG>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
-        else fstep_tile<LC, TX, false, CIG>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        if (full) fstep_tile<LC, TX, TQ, true, CIG>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+        else fstep_tile<LC, TX, TQ, false, CIG>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
         __syncthreads();   // LDS is staged again for the receiver wavefield
     }
-    if (full) fstep_tile<LC, TX, true, CIG>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
-    else fstep_tile<LC, TX, false, CIG>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+    if (full) fstep_tile<LC, TX, TQ, true, CIG>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
+    else fstep_tile<LC, TX, TQ, false, CIG>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
 }
 
 // Multi-grid launch: instances of different grid sizes (same Lc) in one
-// launch.  btab maps each logical workgroup to (instance, tile of that
-// instance's grid, numbered z-fastest); each instance reads its own grid.
-template <int LC>
-__global__ __launch_bounds__(256) void k_fstep_multi(const KStep *__restrict__ insts, const int2 *__restrict__ btab,
-                                                     int nblk, int it)
+// launch, tiles 16 columns wide and 4 TQ rows high.  btab maps each logical
+// workgroup to (instance, tile of that instance's grid, numbered z-fastest);
+// each instance reads its own grid.
+template <int LC, int TQ, int WB>
+__global__ __launch_bounds__((fstep_threads<16, TQ>()), (WB > 0 ? WB : 1)) void k_fstep_multi(
+    const KStep *__restrict__ insts, const int2 *__restrict__ btab, int nblk, int it)
 {
-    constexpr int TX = 16;
-    using G = FGeo<LC, TX>;
+    constexpr int TX = 16, ROWS = 4 * TQ;
+    using G = FGeo<LC, TX, TQ>;
     __shared__ float4 sP[G::NPC * G::NPQ];
-    __shared__ float4 sVX[G::NVX * 16];
+    __shared__ float4 sVX[G::NVX * TQ];
     __shared__ float4 sVZ[TX * G::VZQ];
 
     const int L = xcd_block();
@@ -621,42 +625,70 @@ __global__ __launch_bounds__(256) void k_fstep_multi(const KStep *__restrict__ i
     const KStep &I = insts[bt.x];
     const KGrid &g = *I.grid;
     if (it >= g.nt) return;
-    const int nzb = (g.ntz + 63) / 64;
+    const int nzb = (g.ntz + ROWS - 1) / ROWS;
     const int zb = bt.y % nzb, xb = bt.y / nzb;
-    const int X0 = xb * TX, Z0 = zb * 64;
+    const int X0 = xb * TX, Z0 = zb * ROWS;
     const bool full = X0 - G::OWN >= g.fx0 && X0 + TX + 2 * LC - 2 <= g.fx1 && Z0 - 4 * G::RP >= g.fz0 &&
-                      Z0 + 64 + 4 * G::RP - 1 <= g.fz1;
+                      Z0 + ROWS + 4 * G::RP - 1 <= g.fz1;
     float4 dpa[G::NOWN], cgk[3][G::NOWN];
-    dpa[0] = zero4();
-    if (full) fstep_tile<LC, TX, true, false>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa, cgk);
-    else fstep_tile<LC, TX, false, false>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa, cgk);
+#pragma unroll
+    for (int r = 0; r < G::NOWN; r++) dpa[r] = zero4();
+    if (full) fstep_tile<LC, TX, TQ, true, false>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa, cgk);
+    else fstep_tile<LC, TX, TQ, false, false>(g, I, X0, Z0, it, 0, sP, sVX, sVZ, dpa, cgk);
 }
 
-
-// One k_fstep variant at tile width tx over n instances (pairs count once).
+// Launch of one k_fstep variant with tile configuration cfg (fstep_cfg) over
+// n instances (pairs count once).  Every Lc has the 64-row tiles 16, 32 and 64
+// wide; Lc 4 also 32- and 128-row tiles and the five-wavefront bound (for the
+// forward and paired RTM variants).
 template <bool PAIR, bool CIG, bool BORN>
-hipError_t launch_fstep_variant(const KGrid &g, const KStep *insts, int n, int it, int first, int tx, hipStream_t s)
+hipError_t launch_fstep_variant(const KGrid &g, const KStep *insts, int n, int it, int first, int cfg, hipStream_t s)
 {
-    const int nzb = (g.ntz + 63) / 64;
+    const int tx = fstep_cfg_tx(cfg), tq = fstep_cfg_tq(cfg), wb = fstep_cfg_wb(cfg);
+    const int nzb = (g.ntz + 4 * tq - 1) / (4 * tq);
     const int nxb = (g.ntx + tx - 1) / tx;
     const int nblocks = (n * nzb * nxb + 7) / 8 * 8;
-#define LF(TXV)                                                                                                     \
-    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_fstep<L, TXV, PAIR, CIG, BORN>), dim3(nblocks), dim3(FGeo<L, TXV>::NT), 0, \
-                                         s, g, insts, n, nzb, nxb, it, first))
-    switch (tx) {
-    case 16: LF(16); break;
-    case 32: LF(32); break;
-    case 64: LF(64); break;
-    default: return hipErrorInvalidValue;
+#define LK(LV, TXV, TQV, WBV)                                                                                       \
+    hipLaunchKernelGGL((k_fstep<LV, TXV, TQV, WBV, PAIR, CIG, BORN>), dim3(nblocks), dim3(FGeo<LV, TXV, TQV>::NT), 0, \
+                       s, g, insts, n, nzb, nxb, it, first)
+    if (tq == 16 && wb == 0) {
+        switch (tx) {
+        case 16: DISPATCH_LC(g.Lc, LK(L, 16, 16, 0)); break;
+        case 32: DISPATCH_LC(g.Lc, LK(L, 32, 16, 0)); break;
+        case 64: DISPATCH_LC(g.Lc, LK(L, 64, 16, 0)); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
     }
-#undef LF
-    return hipGetLastError();
+    if constexpr (!CIG && !BORN) {
+        if (g.Lc != 4 || (wb != 0 && wb != 5)) return hipErrorInvalidValue;
+#define LW(TXV, TQV)                          \
+    {                                         \
+        if (wb == 5) LK(4, TXV, TQV, 5);      \
+        else LK(4, TXV, TQV, 0);              \
+    }
+        switch (tq * 1000 + tx) {
+        case 8016: LW(16, 8); break;
+        case 8032: LW(32, 8); break;
+        case 8064: LW(64, 8); break;
+        case 16016: LK(4, 16, 16, 5); break;
+        case 16032: LK(4, 32, 16, 5); break;
+        case 16064: LK(4, 64, 16, 5); break;
+        case 32016: LW(16, 32); break;
+        case 32032: LW(32, 32); break;
+        default: return hipErrorInvalidValue;
+        }
+#undef LW
+        return hipGetLastError();
+    }
+#undef LK
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
 
 // The variants, each compiled in its own translation unit (swd_fstep_*.hip).
-hipError_t launch_fstep_fwd(const KGrid &g, const KStep *insts, int n, int it, int first, int tx, hipStream_t s);
-hipError_t launch_fstep_pair(const KGrid &g, const KStep *insts, int n, int it, int first, int tx, hipStream_t s);
-hipError_t launch_fstep_pair_cig(const KGrid &g, const KStep *insts, int n, int it, int first, int tx, hipStream_t s);
-hipError_t launch_fstep_born(const KGrid &g, const KStep *insts, int n, int it, int first, int tx, hipStream_t s);
+hipError_t launch_fstep_fwd(const KGrid &g, const KStep *insts, int n, int it, int first, int cfg, hipStream_t s);
+hipError_t launch_fstep_pair(const KGrid &g, const KStep *insts, int n, int it, int first, int cfg, hipStream_t s);
+hipError_t launch_fstep_pair_cig(const KGrid &g, const KStep *insts, int n, int it, int first, int cfg, hipStream_t s);
+hipError_t launch_fstep_born(const KGrid &g, const KStep *insts, int n, int it, int first, int cfg, hipStream_t s);

@@ -1,5 +1,7 @@
 // swd_launch.h -- host-side launchers of the kernels in swd_kernels.hip.
 #pragma once
+#include <vector>
+
 #include <hip/hip_runtime.h>
 #include "swd_kernels.h"
 
@@ -15,20 +17,32 @@ hipError_t launch_adjoint_step_tiled(const KGrid &g, const KInst *insts, int nin
 hipError_t launch_copy_rows(float *gather, const int2 *pairs, int npairs, int nt, hipStream_t s);
 hipError_t launch_sum_rows(float *dst, const float *src, const int2 *pairs, int npairs, int nt, hipStream_t s);
 hipError_t launch_sum_shots(float *dst, const float *src, int nshots, size_t n, bool accumulate, hipStream_t s);
-// fused step (swd_fstep.hip): tx = tile width in columns (16, 32 or 64);
-// pair: instances are (SRC, RCV) pairs of the RTM backward, ninst counts both
-hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, hipStream_t s);
-hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair,
-                        hipStream_t s);
-// cig: the paired backward also bins the angle gathers of step it+1 (RCV instances' cig)
-hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair, bool cig,
-                        hipStream_t s);
-// Born forward pairs (born, with pair and without cig): instances 2k and 2k+1
-// are the background and scattered fields of shot k.
-hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int tx, bool pair, bool cig,
+// Tile configuration of the fused step: width tx (columns: 16, 32, 64),
+// height tq (float4 quads per column: 4 tq rows; 8, 16, 32) and a bound of
+// wb wavefronts per SIMD (0: none; 5).
+constexpr int fstep_cfg(int tx, int tq, int wb) { return tx | (tq << 8) | (wb << 16); }
+constexpr int fstep_cfg_tx(int cfg) { return cfg & 0xff; }
+constexpr int fstep_cfg_tq(int cfg) { return (cfg >> 8) & 0xff; }
+constexpr int fstep_cfg_wb(int cfg) { return (cfg >> 16) & 0xff; }
+constexpr int kFstepDefault = fstep_cfg(16, 16, 0);
+// The configurations a fused launch supports: 64-row tiles 16, 32 or 64 wide
+// for every variant and Lc; at Lc 4, for the forward and the paired RTM
+// backward (not the angle-gather or Born variants), also 32- and 128-row tiles
+// (128 rows up to 32 wide) and the five-wavefront bound.  multi: the
+// multi-grid launch, whose tiles are 16 wide.
+bool fstep_cfg_supported(int Lc, bool pair, bool cig, bool born, bool multi, int cfg);
+// The configurations the tuner times, the default (16 x 64 rows, no bound) first.
+std::vector<int> fstep_candidates(int Lc, bool pair, bool cig, bool born, bool multi);
+
+// fused step (swd_fstep.hip) with tile configuration cfg; pair: instances are
+// (SRC, RCV) pairs of the RTM backward, ninst counts both; cig: the paired
+// backward also bins the angle gathers of step it+1 (RCV instances' cig);
+// born (with pair, without cig): instances 2k and 2k+1 are the background and
+// scattered fields of the Born forward of shot k.
+hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, bool first, int cfg, bool pair, bool cig,
                         bool born, hipStream_t s);
-// multi-grid forward step: btab[nblk] = (instance, tile), tile width 16
-hipError_t launch_fstep_multi(int Lc, const KStep *insts, const int2 *btab, int nblk, int it, hipStream_t s);
+// multi-grid forward step: btab[nblk] = (instance, tile numbered for cfg's tiles)
+hipError_t launch_fstep_multi(int Lc, int cfg, const KStep *insts, const int2 *btab, int nblk, int it, hipStream_t s);
 hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2 *tw, const float *coef,
                         const KCapCol *cols, const int2 *traces, float *up, float *down, int nt, hipStream_t s);
 // up/down imaging of step it for nshot shots (KUpDown each) over the interior columns

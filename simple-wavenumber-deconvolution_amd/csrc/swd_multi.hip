@@ -37,31 +37,25 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
         o_f[i] = c.add<float>(kStepArrays * P.nfield * nshots[i]);
         SWD_TRY(ensure_ws(P, c.off));
     }
-    // instance, grid and block tables (tiles of 64 rows x 16 columns)
+    // instance and grid tables
     std::vector<KGrid> grids(nplans);
     std::vector<KStep> h;
-    std::vector<int2> btab;
     std::vector<int> inst0(nplans);
     for (int i = 0; i < nplans; i++) {
         grids[i] = Ps[i]->kg;
         inst0[i] = static_cast<int>(h.size());
         for (int k = 0; k < nshots[i]; k++) h.emplace_back();
     }
-    Carver ca;
-    const size_t oa_grid = ca.add<KGrid>(nplans);
-    const size_t oa_inst = ca.add<KStep>(h.size());
     // Groups of whole instances (an instance's tiles exchange halos between
     // steps): as many as the cache budget asks, rounded up to a multiple of the
     // streams (at most eight), each closed once it holds its share of the
     // per-step working set, so the streams carry even loads (c4 on two streams:
     // four groups of 72-82 MB).  Each group runs its own time loop on the
-    // plan's streams in turn.  Within a group the shots of one tile sit next to
-    // each other (shared model in L2).
-    struct Group {
-        size_t off;
-        int n;
+    // plan's streams in turn.  A group is a list of (plan, first shot, end).
+    struct Seg {
+        int plan, k0, k1;
     };
-    std::vector<Group> groups;
+    std::vector<std::vector<Seg>> groups(1);
     {
         double total = 0.0;
         for (int i = 0; i < nplans; i++) total += fwd_group_bytes(*Ps[i]) * nshots[i];
@@ -70,29 +64,72 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
         ng = std::min(8, (ng + ns - 1) / ns * ns);
         const double target = total / ng;
         double used = 0.0;
-        size_t off = 0;
         for (int i = 0; i < nplans; i++) {
-            const int nzb = (Ps[i]->ntz + 63) / 64, nxb = (Ps[i]->ntx + 15) / 16;
             const double per = fwd_group_bytes(*Ps[i]);
             int k0 = 0;
             for (int k = 0; k < nshots[i]; k++) {
                 used += per;
-                const bool close = used >= target && static_cast<int>(groups.size()) + 1 < ng;
+                const bool close = used >= target && static_cast<int>(groups.size()) < ng;
                 if (close || k == nshots[i] - 1) {
-                    for (int t = 0; t < nzb * nxb; t++)
-                        for (int q = k0; q <= k; q++) btab.push_back(make_int2(inst0[i] + q, t));
+                    groups.back().push_back(Seg{i, k0, k + 1});
                     k0 = k + 1;
                 }
                 if (close) {
-                    groups.push_back(Group{off, static_cast<int>(btab.size() - off)});
-                    off = btab.size();
+                    groups.emplace_back();
                     used = 0.0;
                 }
             }
         }
-        if (btab.size() > off) groups.push_back(Group{off, static_cast<int>(btab.size() - off)});
+        if (groups.back().empty()) groups.pop_back();
     }
-    const size_t oa_btab = ca.add<int2>(btab.size());
+    // Block table of tile configuration cfg (16 columns, 4 tq rows): per group,
+    // per segment, its grid's tiles numbered z-fastest, the shots of one tile
+    // next to each other (shared model in L2); off[g], cnt[g] locate group g.
+    struct BTab {
+        int cfg;
+        std::vector<int2> t;
+        std::vector<size_t> off;
+        std::vector<int> cnt;
+    };
+    auto make_btab = [&](int cfg) {
+        BTab B;
+        B.cfg = cfg;
+        const int rows = 4 * fstep_cfg_tq(cfg);
+        for (const auto &G : groups) {
+            B.off.push_back(B.t.size());
+            for (const Seg &sg : G) {
+                const int nzb = (Ps[sg.plan]->ntz + rows - 1) / rows, nxb = (Ps[sg.plan]->ntx + 15) / 16;
+                for (int t = 0; t < nzb * nxb; t++)
+                    for (int q = sg.k0; q < sg.k1; q++) B.t.push_back(make_int2(inst0[sg.plan] + q, t));
+            }
+            B.cnt.push_back(static_cast<int>(B.t.size() - B.off.back()));
+        }
+        return B;
+    };
+    // the tile: the plan's, else remembered, else tuned on group 0's first steps
+    const int ninst = static_cast<int>(h.size());
+    long long cells_all = 0;
+    for (int i = 0; i < nplans; i++) cells_all += static_cast<long long>(Ps[i]->ntx) * Ps[i]->ntz * nshots[i];
+    const std::string tkey = tune_key(P0, "multi", ninst) + " plans " + std::to_string(nplans) + " cells " +
+                             std::to_string(cells_all);
+    int cfg = P0.tile_cfg;
+    if (cfg != 0 && !fstep_cfg_supported(P0.g.Lc, false, false, false, true, cfg)) cfg = kFstepDefault;
+    if (cfg == 0) {
+        cfg = tune_lookup(tkey);
+        if (cfg != 0 && !fstep_cfg_supported(P0.g.Lc, false, false, false, true, cfg)) cfg = 0;
+    }
+    constexpr int kTry = 5;
+    std::vector<int> cand = fstep_candidates(P0.g.Lc, false, false, false, true);
+    const bool tune = cfg == 0 && nt >= static_cast<int>(cand.size()) * (kTry + 1) + 1;
+    if (cfg == 0 && !tune) cfg = kFstepDefault;
+    if (!tune) cand.assign(1, cfg);
+    std::vector<BTab> tabs;
+    for (int c : cand) tabs.push_back(make_btab(c));
+    Carver ca;
+    const size_t oa_grid = ca.add<KGrid>(nplans);
+    const size_t oa_inst = ca.add<KStep>(h.size());
+    std::vector<size_t> oa_btab;
+    for (const BTab &B : tabs) oa_btab.push_back(ca.add<int2>(B.t.size()));
     SWD_TRY(ensure_aux(P0, ca.off));
     const KGrid *d_grids = reinterpret_cast<const KGrid *>(P0.aux + oa_grid);
     for (int i = 0; i < nplans; i++) {
@@ -118,21 +155,54 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     }
     HIP_TRY(hipMemcpyAsync(P0.aux + oa_grid, grids.data(), sizeof(KGrid) * nplans, hipMemcpyHostToDevice, P0.stream));
     HIP_TRY(hipMemcpyAsync(P0.aux + oa_inst, h.data(), sizeof(KStep) * h.size(), hipMemcpyHostToDevice, P0.stream));
-    HIP_TRY(hipMemcpyAsync(P0.aux + oa_btab, btab.data(), sizeof(int2) * btab.size(), hipMemcpyHostToDevice, P0.stream));
+    for (size_t c = 0; c < tabs.size(); c++)
+        HIP_TRY(hipMemcpyAsync(P0.aux + oa_btab[c], tabs[c].t.data(), sizeof(int2) * tabs[c].t.size(),
+                               hipMemcpyHostToDevice, P0.stream));
     const KStep *d_inst = reinterpret_cast<const KStep *>(P0.aux + oa_inst);
-    const int2 *d_btab = reinterpret_cast<const int2 *>(P0.aux + oa_btab);
     stats_begin(P0, 0);
     SWD_TRY(fork_streams(P0));
+    int it0 = 0;   // first step of group 0 after the tuning launches
+    if (tune) {
+        // group 0 runs kTry + 1 of its steps with every candidate (the first untimed)
+        const hipStream_t st = rtm_group_stream(P0, 0);
+        std::vector<hipEvent_t> ev(cand.size() + 1);
+        for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+        std::vector<float> ms(cand.size(), 0.f);
+        int rc = SWD_OK;
+        for (size_t c = 0; c < cand.size() && rc == SWD_OK; c++) {
+            const int2 *bt = reinterpret_cast<const int2 *>(P0.aux + oa_btab[c]) + tabs[c].off[0];
+            for (int j = 0; j <= kTry && rc == SWD_OK; j++) {
+                if (j == 1 && hipEventRecord(ev[c], st) != hipSuccess) rc = SWD_E_DEVICE;
+                if (rc == SWD_OK && launch_fstep_multi(P0.g.Lc, cand[c], d_inst, bt, tabs[c].cnt[0], it0++, st) != hipSuccess)
+                    rc = SWD_E_DEVICE;
+            }
+            if (rc == SWD_OK && hipEventRecord(ev[cand.size()], st) != hipSuccess) rc = SWD_E_DEVICE;
+            if (rc == SWD_OK && hipEventSynchronize(ev[cand.size()]) != hipSuccess) rc = SWD_E_DEVICE;
+            if (rc == SWD_OK && hipEventElapsedTime(&ms[c], ev[c], ev[cand.size()]) != hipSuccess) rc = SWD_E_DEVICE;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        if (rc != SWD_OK) return swd_fail(rc, "swd_multi_forward: a tuning launch failed");
+        cfg = choose_cfg(ms, cand);
+        tune_store(tkey, cfg);
+    }
+    size_t ci = 0;
+    while (cand[ci] != cfg) ci++;
+    const BTab &B = tabs[ci];
+    const int2 *d_btab = reinterpret_cast<const int2 *>(P0.aux + oa_btab[ci]);
+    P0.stats.tile_x = fstep_cfg_tx(cfg);
+    P0.stats.tile_rows = 4 * fstep_cfg_tq(cfg);
+    P0.stats.tile_waves = fstep_cfg_wb(cfg);
     for (size_t gi = 0; gi < groups.size(); gi++) {
-        const Group G = groups[gi];
         const hipStream_t st = rtm_group_stream(P0, static_cast<int>(gi));
+        const int first = gi == 0 ? it0 : 0;
         auto record = [&]() -> int {
-            for (int it = 0; it < nt; it++) HIP_TRY(launch_fstep_multi(P0.g.Lc, d_inst, d_btab + G.off, G.n, it, st));
+            for (int it = first; it < nt; it++)
+                HIP_TRY(launch_fstep_multi(P0.g.Lc, cfg, d_inst, d_btab + B.off[gi], B.cnt[gi], it, st));
             return SWD_OK;
         };
         if (P0.graphs) {
             std::vector<long long> key = {2, reinterpret_cast<long long>(P0.aux), static_cast<long long>(h.size()),
-                                          static_cast<long long>(G.off), G.n, nt};
+                                          static_cast<long long>(B.off[gi]), B.cnt[gi], nt, first, cfg};
             for (int i = 0; i < nplans; i++) key.push_back(reinterpret_cast<long long>(Ps[i]->ws));
             SWD_TRY(graph_run(P0, key, record, st));
         } else {

@@ -62,9 +62,11 @@ struct Plan {
     hipEvent_t tev[4] = {};     // autotuner
     swd_stats stats{};
     KGrid kg{};
-    int tile_req = 0;           // fused tile width asked for (0 = tune)
+    int tile_cfg = 0;           // fused tile configuration asked for (fstep_cfg; 0 = tune)
+    std::string arch;           // device architecture and compute units (the tuner cache's key)
+    int cus = 0;
     int group_fwd = 0, group_bwd = 0;     // RTM shots per fused forward / backward loop (0 = sized to the cache)
-    std::map<int, int> tuned_fwd, tuned_bwd;   // tuned tile width by instances per launch
+    std::map<int, int> tuned_fwd, tuned_bwd;   // tuned tile configuration by instances per launch
     std::map<int, int> tuned_cig;              // the same for the backward with angle gathers
     std::map<int, int> tuned_born;             // the same for the Born forward pairs
     int streams = 2;            // RTM shot groups run on this many streams at once (1 .. kMaxStreams)
@@ -233,6 +235,15 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
                 bool cig = false, bool born = false);
 
 std::vector<int> steps_up(int nt);
+
+// The tile tuner's cache: key of a loop (device, grid, Lc, instances per
+// launch n, kernel variant mode, streams), the configuration stored for it on
+// disk (0: none) and a new choice to store.
+std::string tune_key(const Plan &P, const char *mode, int n);
+int tune_lookup(const std::string &key);
+void tune_store(const std::string &key, int cfg);
+// The tuner's pick from candidate timings ms[c] (candidate 0 the default).
+int choose_cfg(const std::vector<float> &ms, const std::vector<int> &cand);
 
 int even_groups(int n, int g);
 

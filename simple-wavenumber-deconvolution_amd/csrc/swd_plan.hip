@@ -1,6 +1,11 @@
 // swd_plan.hip -- plans (creation, coefficients, model, settings), the
 // workspace, graph replay, the fused and batched time loops, shot groups on
 // the plan's streams and memory-sized shot waves.
+#include <sys/stat.h>
+
+#include <cstdlib>
+#include <mutex>
+
 #include "swd_plan.h"
 
 // ------------------------------------------------------------------ errors
@@ -197,20 +202,115 @@ int upload_steps(Plan &P, const std::vector<KStep> &h, size_t o)
     return SWD_OK;
 }
 
+// ---- tile tuner and its cache on disk
+//
+// Chosen tile configurations are kept per plan by instances per launch and
+// in a text file shared by processes: one line per choice, "<key>\t<cfg>",
+// the key naming the device, the padded grid, Lc, the instances per launch,
+// the kernel variant and the stream count.  The file is SWD_TUNE_CACHE if
+// set (empty: no file), else $XDG_CACHE_HOME/swd_tiles.txt or
+// $HOME/.cache/swd_tiles.txt.  Results never depend on the configuration.
+namespace {
+std::mutex g_tune_lock;
+std::map<std::string, int> g_tune_disk;
+bool g_tune_loaded = false;
+
+std::string tune_path()
+{
+    if (const char *e = std::getenv("SWD_TUNE_CACHE")) return e;
+    const char *x = std::getenv("XDG_CACHE_HOME"), *h = std::getenv("HOME");
+    if (x && *x) return std::string(x) + "/swd_tiles.txt";
+    if (h && *h) return std::string(h) + "/.cache/swd_tiles.txt";
+    return "";
+}
+
+void tune_load_locked()
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const std::string path = tune_path();
+    if (path.empty()) return;
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[512];
+    while (std::fgets(line, sizeof line, f)) {
+        char *tab = std::strrchr(line, '\t');
+        if (!tab) continue;
+        *tab = 0;
+        g_tune_disk[line] = std::atoi(tab + 1);
+    }
+    std::fclose(f);
+}
+}  // namespace
+
+std::string tune_key(const Plan &P, const char *mode, int n)
+{
+    char k[256];
+    std::snprintf(k, sizeof k, "%s cu%d grid %dx%d Lc %d n %d %s streams %d", P.arch.c_str(), P.cus, P.ntx, P.ntz,
+                  P.g.Lc, n, mode, P.streams);
+    return k;
+}
+
+int tune_lookup(const std::string &key)
+{
+    std::lock_guard<std::mutex> hold(g_tune_lock);
+    tune_load_locked();
+    const auto f = g_tune_disk.find(key);
+    return f == g_tune_disk.end() ? 0 : f->second;
+}
+
+void tune_store(const std::string &key, int cfg)
+{
+    std::lock_guard<std::mutex> hold(g_tune_lock);
+    tune_load_locked();
+    g_tune_disk[key] = cfg;
+    const std::string path = tune_path();
+    if (path.empty()) return;
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos && slash > 0) ::mkdir(path.substr(0, slash).c_str(), 0755);   // one level, as ~/.cache
+    FILE *f = std::fopen(path.c_str(), "a");
+    if (!f) return;
+    std::fprintf(f, "%s\t%d\n", key.c_str(), cfg);
+    std::fclose(f);
+}
+
+int choose_cfg(const std::vector<float> &ms, const std::vector<int> &cand)
+{
+    // The default (first) configuration stays unless another is more than
+    // 10% faster: the timing is a short sample of one launch sequence, while
+    // narrow 64-row tiles keep the most workgroups in flight, which the groups
+    // overlapping on the plan's streams need (c2, 16 shots on two streams,
+    // whole runs at widths 16/32/64: AFor 505/548/686 ms, RTM 785/820/1043 ms).
+    int best = 0;
+    for (size_t c = 1; c < cand.size(); c++)
+        if (ms[c] < 0.9f * ms[0] && ms[c] < ms[best]) best = static_cast<int>(c);
+    return cand[best];
+}
+
 int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int first_it, std::map<int, int> &tuned,
                 long long *launches, bool pair, hipStream_t st, const StepHook *hook,
                 bool cig, bool born)
 {
     if (!st) st = P.stream;
-    int tx = P.tile_req;
-    if (tx == 0) {
+    const char *mode = born ? "born" : cig ? "cig" : pair ? "pair" : "fwd";
+    int cfg = P.tile_cfg;
+    if (cfg != 0 && !fstep_cfg_supported(P.g.Lc, pair, cig, born, false, cfg))
+        cfg = fstep_cfg(fstep_cfg_tx(cfg), 16, 0);   // the asked width with the variant's 64-row tiles
+    std::string key;
+    if (cfg == 0) {
         const auto f = tuned.find(n);
-        if (f != tuned.end()) tx = f->second;
+        if (f != tuned.end()) cfg = f->second;
+    }
+    if (cfg == 0) {
+        key = tune_key(P, mode, n);
+        cfg = tune_lookup(key);
+        if (cfg != 0 && !fstep_cfg_supported(P.g.Lc, pair, cig, born, false, cfg)) cfg = 0;
+        if (cfg != 0) tuned[n] = cfg;
     }
     size_t k = 0;
-    auto launch = [&](int w) -> int {
+    auto launch = [&](int c) -> int {
         const int it = its[k++];
-        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, w, pair, cig, born, st));
+        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, c, pair, cig, born, st));
         (*launches)++;
         if (hook) {
             SWD_TRY(hook->run(it, st));
@@ -218,40 +318,40 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         }
         return SWD_OK;
     };
-    if (tx == 0) {
-        constexpr int kWarm = 2, kTry = 6;
-        const int cand[3] = {16, 32, 64};
-        if (its.size() >= static_cast<size_t>(kWarm + 3 * kTry)) {
-            for (int j = 0; j < kWarm; j++) SWD_TRY(launch(32));
-            HIP_TRY(hipEventRecord(P.tev[0], st));
-            for (int c = 0; c < 3; c++) {
-                for (int j = 0; j < kTry; j++) SWD_TRY(launch(cand[c]));
-                HIP_TRY(hipEventRecord(P.tev[c + 1], st));
+    if (cfg == 0) {
+        // time kTry launches of every candidate after a warm-up launch of each,
+        // on the loop's own steps, and keep the choice for later loops
+        constexpr int kTry = 5;
+        const std::vector<int> cand = fstep_candidates(P.g.Lc, pair, cig, born, false);
+        if (its.size() >= cand.size() * (kTry + 1) + 1) {
+            std::vector<hipEvent_t> ev(cand.size() + 1);
+            for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+            int rc = SWD_OK;
+            std::vector<float> ms(cand.size(), 0.f);
+            for (size_t c = 0; c < cand.size() && rc == SWD_OK; c++) {
+                rc = launch(cand[c]);   // first use of the configuration, untimed
+                if (rc == SWD_OK) rc = hipEventRecord(ev[c], st) == hipSuccess ? SWD_OK : SWD_E_DEVICE;
+                for (int j = 0; j < kTry && rc == SWD_OK; j++) rc = launch(cand[c]);
+                if (rc == SWD_OK) rc = hipEventRecord(ev[cand.size()], st) == hipSuccess ? SWD_OK : SWD_E_DEVICE;
+                if (rc == SWD_OK) rc = hipEventSynchronize(ev[cand.size()]) == hipSuccess ? SWD_OK : SWD_E_DEVICE;
+                if (rc == SWD_OK) rc = hipEventElapsedTime(&ms[c], ev[c], ev[cand.size()]) == hipSuccess ? SWD_OK
+                                                                                                       : SWD_E_DEVICE;
             }
-            HIP_TRY(hipEventSynchronize(P.tev[3]));
-            // A wider tile must be more than 10% faster than the narrower
-            // one: narrow tiles keep more workgroups in flight, which the
-            // groups overlapping on the plan's streams need (c2, 16 shots on
-            // two streams, whole runs at widths 16/32/64: AFor 505/548/686 ms,
-            // RTM 785/820/1043 ms), while a short timing of one launch
-            // sequence can favour a wider one.
-            float best = 0.f;
-            for (int c = 0; c < 3; c++) {
-                float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, P.tev[c], P.tev[c + 1]));
-                if (c == 0 || ms < 0.9f * best) {
-                    best = ms;
-                    tx = cand[c];
-                }
-            }
-            tuned[n] = tx;
+            for (auto &e : ev) (void)hipEventDestroy(e);
+            if (rc != SWD_OK) return swd_fail(rc, "tile tuner: a timed launch failed");
+            cfg = choose_cfg(ms, cand);
+            tuned[n] = cfg;
+            tune_store(key, cfg);
         } else {
-            tx = 32;
+            cfg = kFstepDefault;   // too few steps to time the candidates
         }
     }
+    P.stats.tile_x = fstep_cfg_tx(cfg);
+    P.stats.tile_rows = 4 * fstep_cfg_tq(cfg);
+    P.stats.tile_waves = fstep_cfg_wb(cfg);
     if (k < its.size() && P.graphs) {
         std::vector<long long> key = {1, reinterpret_cast<long long>(d), n, its[k], static_cast<long long>(its.size() - k),
-                                      first_it, tx, (pair ? 1 : 0) + (cig ? 2 : 0) + (born ? 4 : 0),
+                                      first_it, cfg, (pair ? 1 : 0) + (cig ? 2 : 0) + (born ? 4 : 0),
                                       reinterpret_cast<long long>(P.ws)};
         if (hook) key.insert(key.end(), hook->key.begin(), hook->key.end());
         const size_t k0 = k;
@@ -259,7 +359,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
             P, key,
             [&]() -> int {
                 for (size_t j = k0; j < its.size(); j++) {
-                    HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, tx, pair, cig, born, st));
+                    HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, cfg, pair, cig, born, st));
                     if (hook) SWD_TRY(hook->run(its[j], st));
                 }
                 return SWD_OK;
@@ -268,7 +368,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         *launches += static_cast<long long>(its.size() - k0) * (hook ? 2 : 1);
         return SWD_OK;
     }
-    while (k < its.size()) SWD_TRY(launch(tx));
+    while (k < its.size()) SWD_TRY(launch(cfg));
     return SWD_OK;
 }
 
@@ -464,6 +564,14 @@ extern "C" int swd_plan_create(const swd_grid *g, int device, void **out)
     P->nfield = static_cast<size_t>(P->ntx) * P->ldz;
     P->nframe = 2 * g->Lc * (g->nx + g->nz - 2 * g->Lc);
     hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, device);
+        if (e == hipSuccess) {
+            P->arch = prop.gcnArchName;
+            P->cus = prop.multiProcessorCount;
+        }
+    }
     if (e == hipSuccess) e = hipStreamCreate(&P->stream);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&P->ev[i]);
     for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreate(&P->tev[i]);
@@ -616,17 +724,32 @@ extern "C" int swd_plan_set_streams(void *plan, int streams)
     return SWD_OK;
 }
 
-extern "C" int swd_plan_set_tile(void *plan, int tile_x)
+extern "C" int swd_plan_set_tile_config(void *plan, int tile_x, int tile_rows, int waves)
 {
-    if (!plan || (tile_x != 0 && tile_x != 16 && tile_x != 32 && tile_x != 64))
-        return swd_fail(SWD_E_ARG, "swd_plan_set_tile: tile width %d not in {0, 16, 32, 64}", tile_x);
+    if (!plan) return swd_fail(SWD_E_ARG, "swd_plan_set_tile_config: null plan");
     Plan *P = static_cast<Plan *>(plan);
-    P->tile_req = tile_x;
+    int cfg = 0;
+    if (tile_x != 0 || tile_rows != 0 || waves != 0) {
+        cfg = fstep_cfg(tile_x, tile_rows / 4, waves);
+        if (tile_rows % 4 != 0 || !fstep_cfg_supported(P->g.Lc, false, false, false, false, cfg))
+            return swd_fail(SWD_E_ARG,
+                            "swd_plan_set_tile_config: %d x %d rows, %d wavefronts per SIMD is not a configuration of "
+                            "the fused step at Lc %d",
+                            tile_x, tile_rows, waves, P->g.Lc);
+    }
+    P->tile_cfg = cfg;
     P->tuned_fwd.clear();
     P->tuned_bwd.clear();
     P->tuned_cig.clear();
     P->tuned_born.clear();
     return SWD_OK;
+}
+
+extern "C" int swd_plan_set_tile(void *plan, int tile_x)
+{
+    if (!plan || (tile_x != 0 && tile_x != 16 && tile_x != 32 && tile_x != 64))
+        return swd_fail(SWD_E_ARG, "swd_plan_set_tile: tile width %d not in {0, 16, 32, 64}", tile_x);
+    return swd_plan_set_tile_config(plan, tile_x, tile_x ? 64 : 0, 0);
 }
 
 extern "C" int swd_plan_get_stats(void *plan, swd_stats *st)

@@ -67,7 +67,7 @@ class Stats(C.Structure):
                 ("fwd_launches", C.c_longlong), ("bwd_launches", C.c_longlong),
                 ("fwd_cells", C.c_longlong), ("bwd_cells", C.c_longlong),
                 ("ws_bytes", C.c_longlong), ("ckpt_bytes", C.c_longlong), ("full_bytes", C.c_longlong),
-                ("waves", C.c_longlong)]
+                ("waves", C.c_longlong), ("tile_x", C.c_int), ("tile_rows", C.c_int), ("tile_waves", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -86,7 +86,7 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_nfft", "swd_kz_taper",
            "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
-           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
+           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_tile_config", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
            "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor", "swd_afor_dev",
            "swd_rtm", "swd_rtm_dev", "swd_rtm_cig", "swd_rtm_cig_dev", "swd_rtm_updown", "swd_rtm_updown_dev", "swd_updown_image_dev",
            "swd_updown_image_dev_noalloc",
@@ -360,6 +360,11 @@ class Plan:
     def set_tile(self, tile):
         """Fused-path tile width (16, 32, 64) or 0 to tune it on the first steps."""
         check(self.L.swd_plan_set_tile(self.h, int(tile)))
+
+    def set_tile_config(self, tile_x=0, tile_rows=0, waves=0):
+        """Fused-path tile: columns (16, 32, 64), rows (64; at Lc 4 also 32, 128)
+        and a wavefronts-per-SIMD bound (0; at Lc 4 also 5); all 0 to tune."""
+        check(self.L.swd_plan_set_tile_config(self.h, int(tile_x), int(tile_rows), int(waves)))
 
     def close(self):
         if self.h:

@@ -27,13 +27,23 @@ def plans_of(cases, graphs=True):
     return ps
 
 
-def test_multi_forward_c4_matches_single_plans_and_oracle():
+def test_multi_forward_c4_matches_single_plans_and_oracle(tmp_path, monkeypatch):
+    """c4 with the tile tuned (tile = 0): the pick is recorded in the cache file."""
+    cache = tmp_path / "tiles.txt"
+    monkeypatch.setenv("SWD_TUNE_CACHE", str(cache))
     cases = configs.c4(nt=300)
     print(f"case c4: {len(cases)} grids, {sum(len(c.shots) for c in cases)} shots, nt 300", flush=True)
     plans = plans_of(cases)
+    plans[0].set_tile(0)
     t0 = time.perf_counter()
     got = swd.multi_forward(plans, [c.shots for c in cases], [c.wavelet for c in cases])
     t_multi = time.perf_counter() - t0
+    st = plans[0].stats()
+    print(f"c4 tuned tile {st['tile_x']} x {st['tile_rows']} rows, bound {st['tile_waves']}", flush=True)
+    assert st["tile_x"] == 16 and st["tile_rows"] in (32, 64, 128) and st["tile_waves"] in (0, 5)
+    rec = [ln for ln in cache.read_text().splitlines() if " multi " in ln]
+    print(rec, flush=True)
+    assert len(rec) == 1 and rec[0].endswith(str(16 | (st["tile_rows"] // 4) << 8 | st["tile_waves"] << 16))
     t0 = time.perf_counter()
     ref = [p.forward(c.shots, c.wavelet) for p, c in zip(plans, cases)]
     t_single = time.perf_counter() - t0

@@ -256,6 +256,63 @@ def test_fused_tiles_forward_and_rtm(Lc, tile):
         row += n
 
 
+# Lc 4 tiles of the forward and paired RTM kernels: width x rows x wavefront bound
+TILE_CONFIGS = [(w, r, b) for r in (32, 64, 128) for b in (0, 5) for w in (16, 32, 64) if not (r == 128 and w == 64)]
+
+
+@pytest.mark.parametrize("w,r,b", TILE_CONFIGS)
+def test_tile_configs_forward_and_rtm(w, r, b):
+    """Every tile height and wavefront bound the tuner may pick, against the
+    oracle: forward gathers and the RTM image and illuminations, a second shot
+    with its source in the absorbing frame."""
+    c = configs.small(nx=100, nz=70, pml=8, Lc=4, nt=160, nshots=2)
+    s_ix, s_iz, rx, rz = c.shots[1]
+    c.shots[1] = (3, 5, rx, rz)
+    print(f"case tile {w} x {r} rows, bound {b}", flush=True)
+    Ps = [oracle_problem(c, k) for k in range(2)]
+    outs = [P.forward(strips=True, final=True) for P in Ps]
+    adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
+    plan = plan_of(c)
+    plan.set_tile_config(w, r, b)
+    g = plan.forward(c.shots, c.wavelet)
+    st = plan.stats()
+    assert (st["tile_x"], st["tile_rows"], st["tile_waves"]) == (w, r, b)
+    img, ils, ilr, gs = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
+    row = 0
+    for k, P in enumerate(Ps):
+        d, st_, fs = outs[k]
+        n = d.shape[0]
+        o = P.rtm_backward(fs, st_, adj[row:row + n])
+        assert np.array_equal(g[row:row + n], d) and np.array_equal(gs[row:row + n], d)
+        for name, a, ref in (("image", img[k], o[0]), ("illum_src", ils[k], o[1]), ("illum_rcv", ilr[k], o[2])):
+            e32, _ = report(f"tile {w}x{r}/{b} shot {k} {name}", a, ref)
+            assert e32 <= TOL
+        row += n
+    with pytest.raises(swd.SwdError):
+        plan.set_tile_config(64, 128, 0)
+    plan.close()
+
+
+def test_tuner_picks_and_records_a_configuration(tmp_path, monkeypatch):
+    """tile = 0: the tuner times every configuration on the loop's first steps,
+    stores its pick in the cache file and gives the bits of a fixed tile."""
+    cache = tmp_path / "tiles.txt"
+    monkeypatch.setenv("SWD_TUNE_CACHE", str(cache))
+    c = configs.small(nx=110, nz=90, nt=260, nshots=2)   # a grid no other test tunes (the picks persist in-process)
+    print("case tuner: 110 x 90, 2 shots, nt 260", flush=True)
+    ref = plan_of(c, tile=16).forward(c.shots, c.wavelet)
+    plan = plan_of(c, tile=0)
+    got = plan.forward(c.shots, c.wavelet)
+    st = plan.stats()
+    print(f"tuned tile {st['tile_x']} x {st['tile_rows']} rows, bound {st['tile_waves']}", flush=True)
+    assert np.array_equal(got, ref)
+    assert (st["tile_x"], st["tile_rows"], st["tile_waves"]) != (0, 0, 0)
+    lines = cache.read_text().splitlines()
+    print(lines, flush=True)
+    assert any(" fwd " in ln and "Lc 4" in ln for ln in lines)
+    plan.close()
+
+
 def test_fused_autotuned_tile_matches_fixed():
     """The tuned tile width (chosen on the first steps of the loop) gives the
     same bits as a fixed width."""
