@@ -171,7 +171,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
             bzv[r] = ld4((gcfl *)g.Bz, o);
         }
     }
-    float4 imv[I3], isv[I3], irv[I3];
+    float4 imv[I3], isv[I3], irv[I3];   // receiver field: image accumulators; reconstruction: imv holds K0
     int cmv[I3];   // receiver map of each own column
 #pragma unroll
     for (int r = 0; r < I3; r++) {
@@ -183,6 +183,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
             if (I.colmap) cmv[r] = ldgi(I.colmap, x);
             const unsigned o = static_cast<unsigned>(x) * ldz + z;
             kv[r] = ld4((gcfl *)g.K, o);
+            if (srcm) imv[r] = ld4((gcfl *)g.K0, o);
             if (neg && I.image && (FULL || (x >= pmlc && x < ntx - pmlc))) {
                 imv[r] = ld4((gcfl *)I.image, o);
                 isv[r] = ld4((gcfl *)I.illum_s, o);
@@ -253,9 +254,10 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
 #pragma unroll
                         for (int k = 0; k < 4; k++) el(v, k) = __builtin_fmaf(elc(B, k), elc(d, k), elc(v, k));
                     } else {
+                        // the memory is read with the coefficients (ah is zero inside the damping-free box)
                         const float ah = ldg(g.axh, x), bh = ldg(g.bxh, x);
                         float4 pso = zero4();
-                        if (ah != 0.f) pso = ld4((gcfl *)I.psx[vo], o);
+                        if (x < g.fx0 || x > g.fx1) pso = ld4((gcfl *)I.psx[vo], o);
                         float4 psn = pso;
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
@@ -326,7 +328,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                             anyps |= ahz[k] != 0.f;
                         }
                         float4 pso = zero4();
-                        if (anyps) pso = ld4((gcfl *)I.psz[vo], o);
+                        if (z < g.fz0 || z + 3 > g.fz1) pso = ld4((gcfl *)I.psz[vo], o);
                         float4 psn = pso;
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
@@ -384,10 +386,19 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
             const bool rowint = FULL || (z + 3 >= pmlc && z < ntz - pmlc);
             const bool hassrc = x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4;
             if (!(colint && rowint) && !hassrc) continue;
+            float rst[4];   // frame values of step it, read before the derivatives (RK:575-620)
+            int fr[4];
+            if (!FULL) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    fr[k] = frame_index(g, x, z + k);
+                    rst[k] = fr[k] >= 0 ? ((gcfl *)I.strips)[static_cast<size_t>(it) * g.nframe + fr[k]] : 0.f;
+                }
+            }
             float4 pv = pold;
             if (colint && rowint) {
                 FSTEP_DERIVS
-                const float4 K0 = ld4((gcfl *)g.K0, o);
+                const float4 K0 = imv[r];
                 float4 dp = zero4();
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -415,10 +426,8 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
             if (hassrc) add_at(pv, I.s_iz - z, -ldg(I.wavelet, it + 1));   // RK:478-481
             if (!FULL) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int f = frame_index(g, x, z + k);
-                    if (f >= 0) el(pv, k) = ((gcfl *)I.strips)[static_cast<size_t>(it) * g.nframe + f];   // RK:575-620
-                }
+                for (int k = 0; k < 4; k++)
+                    if (fr[k] >= 0) el(pv, k) = rst[k];   // RK:575-620
             }
             st4((gfl *)I.p[pn], o, pv);
             continue;
@@ -449,8 +458,8 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
                     anyz |= azr[k] != 0.f;
                 }
                 float4 qxo = zero4(), qzo = zero4();
-                if (ax != 0.f) qxo = ld4((gcfl *)I.pvx, o);
-                if (anyz) qzo = ld4((gcfl *)I.pvz, o);
+                if (x < g.fx0 || x > g.fx1) qxo = ld4((gcfl *)I.pvx, o);
+                if (z < g.fz0 || z + 3 > g.fz1) qzo = ld4((gcfl *)I.pvz, o);
                 float4 qxn = qxo, qzn = qzo;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
