@@ -70,6 +70,15 @@ __device__ __forceinline__ void lds_window(const float4 *s, float (&a)[4 * (2 * 
     }
 }
 
+// Direct-to-LDS load of one float4 a lane: lane l of the wavefront writes
+// dst[l], dst being the same for the whole wavefront.  The data arrive with
+// the wavefront's vector-memory counter; the reader waits for it and a barrier.
+__device__ __forceinline__ void ld4_lds(gcfl *f, unsigned o, float4 *dst)
+{
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)((gcch *)f + (o << 2)),
+                                     (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+}
+
 // Velocity-z item i of a tile: the TQ own quads of each column first (TQ
 // consecutive lanes per column, so LDS accesses of 8-lane groups stay in one
 // column), then the 2R halo quads of each column.
@@ -102,6 +111,19 @@ struct FGeo {
     static constexpr int VZQ = TQ + 2 * R;       // vz quads -R .. TQ-1+R
     static constexpr int NT = fstep_threads<TX, TQ>();   // threads per workgroup
     static constexpr int NOWN = (TX * TQ + NT - 1) / NT;  // own quads per thread
+    static constexpr int N0 = NPC * NPQ, N1 = NVX * TQ, N2 = TX * VZQ, N3 = TX * TQ;   // staged p, vx, vz, own items
+    static constexpr int I0 = (N0 + NT - 1) / NT, I1 = (N1 + NT - 1) / NT, I2 = (N2 + NT - 1) / NT,
+                         I3 = (N3 + NT - 1) / NT;   // items per thread
+};
+
+// What the reconstruction half of a fast paired tile prepares for the receiver
+// half: the model (the same cells), read once for both, and the receiver's
+// image accumulators, read during the reconstruction's pressure update.  The
+// receiver's previous p, vx and vz come in by direct-to-LDS loads at that time.
+template <int LC, int TX, int TQ>
+struct FPre {
+    using G = FGeo<LC, TX, TQ>;
+    float4 bx[G::I1], bz[G::I2], k[G::I3], im[G::I3], is[G::I3], ir[G::I3];
 };
 
 // One tile of one wavefield.  FULL: the tile and every halo lie inside the
@@ -121,16 +143,23 @@ __device__ __forceinline__ void born_sum(const KGrid &g, const KStep &I, int x, 
         el(acc, k) = S;
 }
 
-template <int LC, int TX, int TQ, bool FULL, bool CIG, bool BORN = false>
+// PF = 1: a fast reconstruction tile that also prefetches the receiver tile
+// (rcv) into pf and the second LDS set (sP2, sVXo, sVZo).  PF = 2: that
+// receiver tile, run from the prefetch: its staged p is sP (the caller passes
+// sP2), its previous velocities sVXo and sVZo.
+template <int LC, int TX, int TQ, bool FULL, bool CIG, bool BORN = false, int PF = 0>
 __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X0, int Z0, int it, int first,
                                            float4 *__restrict__ sP, float4 *__restrict__ sVX, float4 *__restrict__ sVZ,
-                                           float4 (&dpa)[FGeo<LC, TX, TQ>::NOWN], float4 (&cgk)[3][FGeo<LC, TX, TQ>::NOWN])
+                                           float4 (&dpa)[FGeo<LC, TX, TQ>::NOWN], float4 (&cgk)[3][FGeo<LC, TX, TQ>::NOWN],
+                                           FPre<LC, TX, TQ> *pf = nullptr, const KStep *rcv = nullptr,
+                                           float4 *sP2 = nullptr, float4 *sVXo = nullptr, float4 *sVZo = nullptr)
 {
     using G = FGeo<LC, TX, TQ>;
     constexpr int R = G::R, RP = G::RP, OWN = G::OWN, NPQ = G::NPQ, VZQ = G::VZQ;
-    constexpr int N0 = G::NPC * NPQ, N1 = G::NVX * TQ, N2 = TX * VZQ, N3 = TX * TQ;
+    constexpr int N0 = G::N0, N1 = G::N1, N2 = G::N2, N3 = G::N3;
     constexpr int NT = G::NT;
-    constexpr int I0 = (N0 + NT - 1) / NT, I1 = (N1 + NT - 1) / NT, I2 = (N2 + NT - 1) / NT, I3 = (N3 + NT - 1) / NT;
+    constexpr int I0 = G::I0, I1 = G::I1, I2 = G::I2, I3 = G::I3;
+    static_assert(PF == 0 || (FULL && !CIG && !BORN), "the prefetch runs on fast plain pairs");
     const int tid = threadIdx.x;
     const int mode = I.mode;
     const int pmlc = g.pmlc, ntx = g.ntx, ntz = g.ntz;
@@ -152,7 +181,8 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         const int i = tid + NT * r, q = i % TQ, cx = i / TQ;
         const int x = X0 - LC + cx, z = Z0 + 4 * q;
         vxo[r] = bxv[r] = zero4();
-        if (i < N1 && (FULL || (x >= 0 && x < ntx && z < ntz))) {
+        if constexpr (PF == 2) bxv[r] = pf->bx[r];
+        else if (i < N1 && (FULL || (x >= 0 && x < ntx && z < ntz))) {
             const unsigned o = static_cast<unsigned>(x) * ldz + z;
             vxo[r] = ld4(VXs, o);
             bxv[r] = ld4((gcfl *)g.Bx, o);
@@ -165,7 +195,8 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         vz_item<TX, R, TQ>(i, c, qq);
         const int x = X0 + c, z = Z0 + 4 * qq;
         vzo[r] = bzv[r] = zero4();
-        if (i < N2 && (FULL || (x < ntx && z >= 0 && z < ntz))) {
+        if constexpr (PF == 2) bzv[r] = pf->bz[r];
+        else if (i < N2 && (FULL || (x < ntx && z >= 0 && z < ntz))) {
             const unsigned o = static_cast<unsigned>(x) * ldz + z;
             vzo[r] = ld4(VZs, o);
             bzv[r] = ld4((gcfl *)g.Bz, o);
@@ -179,7 +210,13 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         const int x = X0 + c, z = Z0 + 4 * q;
         kv[r] = imv[r] = isv[r] = irv[r] = zero4();
         cmv[r] = -1;
-        if (i < N3 && (FULL || (x < ntx && z < ntz))) {
+        if constexpr (PF == 2) {
+            kv[r] = pf->k[r];
+            imv[r] = pf->im[r];
+            isv[r] = pf->is[r];
+            irv[r] = pf->ir[r];
+            if (i < N3 && I.colmap) cmv[r] = ldgi(I.colmap, x);
+        } else if (i < N3 && (FULL || (x < ntx && z < ntz))) {
             if (I.colmap) cmv[r] = ldgi(I.colmap, x);
             const unsigned o = static_cast<unsigned>(x) * ldz + z;
             kv[r] = ld4((gcfl *)g.K, o);
@@ -193,7 +230,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
     }
 
     // ---- stage the previous pressure: own columns with z halo, x halo columns on the tile's rows
-    {
+    if constexpr (PF != 2) {
         float4 st[I0];
 #pragma unroll
         for (int r = 0; r < I0; r++) {
@@ -208,8 +245,8 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
             const int i = tid + NT * r;
             if (i < N0) sP[i] = st[r];
         }
+        __syncthreads();
     }
-    __syncthreads();
 
     // ---- velocity x on columns X0-Lc .. X1+Lc-2 of the tile's rows
 #pragma unroll
@@ -219,6 +256,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         const int q = i % TQ, cx = i / TQ;
         const int x = X0 - LC + cx, z = Z0 + 4 * q;
         float4 v = vxo[r];
+        if constexpr (PF == 2) v = sVXo[i];
         if (FULL || (x >= 0 && x < ntx && z < ntz)) {
             const unsigned o = static_cast<unsigned>(x) * ldz + z;
             const bool own = cx >= LC && cx < LC + TX;
@@ -286,6 +324,7 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         vz_item<TX, R, TQ>(i, c, qq);
         const int x = X0 + c, z = Z0 + 4 * qq;
         float4 v = vzo[r];
+        if constexpr (PF == 2) v = sVZo[i];
         if (FULL || (x < ntx && z >= 0 && z < ntz)) {
             const unsigned o = static_cast<unsigned>(x) * ldz + z;
             const bool own = qq >= 0 && qq < TQ;
@@ -352,6 +391,45 @@ __device__ __forceinline__ void fstep_tile(const KGrid &g, const KStep &I, int X
         sVZ[c * VZQ + qq + R] = v;
     }
     __syncthreads();
+
+    if constexpr (PF == 1) {
+        // the receiver tile's previous p (staged as above), vx and vz, straight into the second LDS set,
+        // and its image accumulators; the model stays in registers for it
+        gcfl *RPp = (gcfl *)rcv->p[po], *RVX = (gcfl *)rcv->vx[po], *RVZ = (gcfl *)rcv->vz[po];
+        const int wv = tid & ~63;
+#pragma unroll
+        for (int r = 0; r < I0; r++) {
+            const int i = tid + NT * r, col = i / NPQ, qq = i - col * NPQ - RP;
+            const bool need = i < N0 && ((qq >= 0 && qq < TQ) || (col >= OWN && col < OWN + TX));
+            if (need) ld4_lds(RPp, static_cast<unsigned>(X0 - OWN + col) * ldz + (Z0 + 4 * qq), sP2 + NT * r + wv);
+        }
+#pragma unroll
+        for (int r = 0; r < I1; r++) {
+            const int i = tid + NT * r, q = i % TQ, cx = i / TQ;
+            if (i < N1) ld4_lds(RVX, static_cast<unsigned>(X0 - LC + cx) * ldz + (Z0 + 4 * q), sVXo + NT * r + wv);
+            pf->bx[r] = bxv[r];
+        }
+#pragma unroll
+        for (int r = 0; r < I2; r++) {
+            const int i = tid + NT * r;
+            int c, qq;
+            vz_item<TX, R, TQ>(i, c, qq);
+            if (i < N2) ld4_lds(RVZ, static_cast<unsigned>(X0 + c) * ldz + (Z0 + 4 * qq), sVZo + NT * r + wv);
+            pf->bz[r] = bzv[r];
+        }
+#pragma unroll
+        for (int r = 0; r < I3; r++) {
+            const int i = tid + NT * r, q = i % TQ, c = i / TQ;
+            pf->k[r] = kv[r];
+            pf->im[r] = pf->is[r] = pf->ir[r] = zero4();
+            if (i < N3 && rcv->image) {
+                const unsigned o = static_cast<unsigned>(X0 + c) * ldz + (Z0 + 4 * q);
+                pf->im[r] = ld4((gcfl *)rcv->image, o);
+                pf->is[r] = ld4((gcfl *)rcv->illum_s, o);
+                pf->ir[r] = ld4((gcfl *)rcv->illum_r, o);
+            }
+        }
+    }
 
     // ---- pressure of the tile's own cells from the velocities in LDS
 #pragma unroll
@@ -575,6 +653,12 @@ __global__ __launch_bounds__((fstep_threads<TX, TQ>()), (WB > 0 ? WB : (CIG && L
     __shared__ float4 sP[G::NPC * G::NPQ];
     __shared__ float4 sVX[G::NVX * TQ];
     __shared__ float4 sVZ[TX * G::VZQ];
+    // the second set of a fast plain pair: the receiver tile's prefetched p, vx, vz (up to Lc 4, where
+    // two sets leave the occupancy the registers allow)
+    constexpr bool PFV = PAIR && !CIG && !BORN && LC <= 4;
+    __shared__ float4 sP2[PFV ? G::NPC * G::NPQ : 1];
+    __shared__ float4 sVXo[PFV ? G::NVX * TQ : 1];
+    __shared__ float4 sVZo[PFV ? TX * G::VZQ : 1];
 
     const int L = xcd_block();
     if (L >= ninst * nzb * nxb) return;
@@ -601,6 +685,19 @@ __global__ __launch_bounds__((fstep_threads<TX, TQ>()), (WB > 0 ? WB : (CIG && L
         if (full) fstep_tile<LC, TX, TQ, true, false, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
         else fstep_tile<LC, TX, TQ, false, false, true>(g, Rv, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk);
         return;
+    }
+    if constexpr (PFV) {
+        if (full) {
+            // a fast tile is interior: the reconstruction always runs, and prefetches the receiver tile
+            FPre<LC, TX, TQ> pf;
+            fstep_tile<LC, TX, TQ, true, false, false, 1>(g, S, X0, Z0, it, first, sP, sVX, sVZ, dpa, cgk, &pf, &Rv,
+                                                          sP2, sVXo, sVZo);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this wavefront's direct-to-LDS loads
+            __syncthreads();
+            fstep_tile<LC, TX, TQ, true, false, false, 2>(g, Rv, X0, Z0, it, first, sP2, sVX, sVZ, dpa, cgk, &pf, nullptr,
+                                                          nullptr, sVXo, sVZo);
+            return;
+        }
     }
     // the reconstruction writes only interior cells and the source cell
     const bool any = X0 + TX > g.pmlc && X0 < g.ntx - g.pmlc && Z0 + ROWS > g.pmlc && Z0 < g.ntz - g.pmlc;
