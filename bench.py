@@ -42,14 +42,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def rtm_bytes_per_step(g, nrec, nframe):
-    """Algorithmic HBM bytes of one forward and one backward RTM step of one shot."""
+def rtm_bytes(g, nrec, nframe):
+    """Algorithmic HBM bytes of the RTM steps: (forward per shot, backward per shot, forward per launch,
+    backward per launch).  Per shot and step: the wavefield state (p, vx, vz read and written), the CPML
+    memories on the frame, the strips and the receiver traces; the backward adds the reconstruction's
+    interior state and the image and both illuminations, read and written.  Per launch, shared by the shots
+    of its group: the model, K, Bx and Bz, in the backward also K0."""
     N = g.ntx * g.ntz
     Nint = g.nx * g.nz
     P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
-    fwd = 32 * N + 16 * P + 4 * nrec + 4 * nframe
-    bwd = 24 * Nint + 32 * N + 16 * P + 24 * Nint + 4 * nframe + 4 * nrec
-    return fwd, bwd
+    fwd = 24 * N + 16 * P + 4 * nrec + 4 * nframe
+    bwd = 24 * Nint + 24 * N + 16 * P + 24 * Nint + 4 * nframe + 4 * nrec
+    return fwd, bwd, 12 * N, 16 * N
 
 
 def rtm_cells(g):
@@ -502,13 +506,14 @@ def bench_cig(a):
     print(json.dumps(out), flush=True)
 
 
-def afor_bytes_per_step(g, nrec):
-    """Algorithmic HBM bytes of one AFor step of one shot: the p and Hilbert
-    wavefields' state and CPML memories, the model read once for both, the
-    gathers (the kz transforms of the receiver columns add under 0.1%)."""
+def afor_bytes(g, nrec):
+    """Algorithmic HBM bytes of an AFor step: (per shot, per launch).  Per shot:
+    the p and Hilbert wavefields' state and CPML memories and the gathers (the
+    kz transforms of the receiver columns add under 0.1%); per launch the model
+    (K, Bx, Bz), shared by the launch's wavefields."""
     N = g.ntx * g.ntz
     P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
-    return 56 * N + 32 * P + 8 * nrec
+    return 48 * N + 32 * P + 8 * nrec, 12 * N
 
 
 def bench_afor(a):
@@ -558,7 +563,8 @@ def bench_afor(a):
     st = plan.stats()
     launch_ms = fwd_ms / a.steps / max(1, st["fwd_launches"])
     launches_per_step = st["fwd_launches"] / g.nt
-    bytes_per_launch = afor_bytes_per_step(g, nrec) * a.batch / launches_per_step
+    afs, afm = afor_bytes(g, nrec)
+    bytes_per_launch = (afs * a.batch + afm * launches_per_step) / launches_per_step
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     up0 = d[1, :nrec].cpu().numpy()
     traffic = None
@@ -696,9 +702,10 @@ def bench_born(a):
     cells = 2 * g.nt * N * nshots   # per operator
     P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
     nrec = len(case.shots[0][2])
-    shot_step_bytes = 56 * N + 32 * P + 4 * nint + 4 * nrec   # background + scattered wavefields, m read
+    # per shot: background and scattered wavefields, m read, the gather; per launch: the model (K, Bx, Bz)
+    shot_step_bytes = 48 * N + 32 * P + 4 * nint + 4 * nrec
     launches_per_step = f_launches / g.nt
-    bytes_per_launch = shot_step_bytes * nshots / launches_per_step
+    bytes_per_launch = (shot_step_bytes * nshots + 12 * N * launches_per_step) / launches_per_step
     launch_ms = f_ms / a.steps / max(1, f_launches)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     traffic = None
@@ -789,7 +796,8 @@ def bench_multi(a):
     for c in cases:
         g = c.grid
         P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
-        bytes_per_step += sum(32 * g.ntx * g.ntz + 16 * P + 4 * len(sh[2]) for sh in c.shots)
+        # per shot the state, CPML memories and gather; the grid's model (K, Bx, Bz) once a step
+        bytes_per_step += sum(24 * g.ntx * g.ntz + 16 * P + 4 * len(sh[2]) for sh in c.shots) + 12 * g.ntx * g.ntz
     dev = torch.device("cuda:0")
     copy_gbs = copy_bandwidth(torch, dev)
     for i in range(a.warmup):
@@ -967,15 +975,18 @@ def bench_c5(a):
     P = 2 * g.pmlc * g.ntz + 2 * g.pmlc * g.ntx
     nrec = len(case.shots[0][2])
     nframe = 2 * g.Lc * (g.nx + g.nz - 2 * g.Lc)
-    bf, bb = rtm_bytes_per_step(g, nrec, nframe)
+    bf, bb, mf, mb = rtm_bytes(g, nrec, nframe)
     cells = {"rtm": rtm_cells(g) * nshots, "updown": 2 * rtm_cells(g) * nshots, "gradient": 4 * nt * N * nshots}
-    # algorithmic bytes per shot and step of each phase: the RTM forward and backward; the up/down RTM twice
-    # both plus the split (four field columns and two images per interior row); the Born forward (background
-    # and scattered field, m read) and the adjoint (adjoint and recomputed background state, the background
-    # sums read, the gradient read and written, the data read)
+    # algorithmic bytes per shot of each phase, with the model read once a shot and step (the cache budget
+    # gives one-shot groups at c5; a launch's wavefields share the model): the RTM forward and backward; the
+    # up/down RTM twice both plus the split (four field columns and two images per interior row); the Born
+    # forward (background and scattered field, m read) and the adjoint (adjoint and recomputed background
+    # state, the background sums read, the gradient read and written, the data read)
     ud_split = 16 * g.nx * g.nz + 16 * g.nx * g.nz
-    byts = {"rtm": nt * bf + (nt - 1) * bb, "updown": 2 * (nt * bf + (nt - 1) * bb) + (nt - 1) * ud_split,
-            "gradient": nt * (56 * N + 32 * P + 4 * nint + 4 * nrec) + nt * (2 * (32 * N + 16 * P) + 12 * nint + 4 * nrec)}
+    byts = {"rtm": nt * (bf + mf) + (nt - 1) * (bb + mb),
+            "updown": nt * (2 * bf + mf) + (nt - 1) * (2 * bb + mb) + (nt - 1) * ud_split,
+            "gradient": nt * (48 * N + 32 * P + 4 * nint + 4 * nrec + 12 * N) +
+                        nt * (2 * (24 * N + 16 * P) + 12 * nint + 4 * nrec + 12 * N)}
     per_phase = {}
     for name, _ in phases:
         sec = t[name] / a.steps
@@ -987,7 +998,7 @@ def bench_c5(a):
     per_phase["gradient"]["full_storage_GB"] = stats["gradient"]["full_bytes"] / 1e9
     # the dominant kernel (the paired RTM backward step, one-shot groups at c5) alone on one stream
     iso_ms, _ = backward_launch_ms(swd, torch, case, adj, 1, 1, 1, a.nt_iso, a.tile)
-    iso = bb / (iso_ms * 1e-3) / 1e9
+    iso = (bb + mb) / (iso_ms * 1e-3) / 1e9
     log(f"bench: isolated 1-shot backward launch {iso_ms * 1e3:.1f} us, {iso:.0f} GB/s")
     plain = None
     if not a.no_plain:
@@ -1055,7 +1066,7 @@ def bench_c5(a):
         "roofline": {"bound": "hbm", "achieved": iso, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": iso / HBM_PEAK_GBS,
                      "traffic": None, "copy_GBs": copy_gbs, "frac_of_copy": iso / copy_gbs,
                      "kernel": "k_fstep paired RTM backward step, one-shot group alone on one stream",
-                     "bytes_per_launch": bb, "avg_launch_ms": iso_ms,
+                     "bytes_per_launch": bb + mb, "avg_launch_ms": iso_ms,
                      "timing": f"HIP events on the plan's stream around {a.nt_iso - 1} graph-replayed launches"},
         "cpu_baseline": cpu,
     }
@@ -1190,27 +1201,28 @@ def main():
     # the dominant kernel, the fused backward step, timed three ways with HIP events on the plan's stream
     nrec = len(case.shots[0][2])
     nframe = 2 * g.Lc * (g.nx + g.nz - 2 * g.Lc)
-    bf, bb = rtm_bytes_per_step(g, nrec, nframe)
+    bf, bb, mf, mb = rtm_bytes(g, nrec, nframe)
     st = plan.stats()
     bwd_launches_per_step = st["bwd_launches"] / (g.nt - 1)
     group = -(-a.batch // int(round(bwd_launches_per_step)))   # the largest backward group of the run
     # (1) overlapped: the timed region's backward phase over its launches (groups on a.streams streams)
     ov_ms = bwd_ms / a.steps / max(1, st["bwd_launches"])
-    ov_bytes = bb * a.batch / bwd_launches_per_step
+    ov_bytes = (bb * a.batch + mb * bwd_launches_per_step) / bwd_launches_per_step
     ov_achieved = ov_bytes / (ov_ms * 1e-3) / 1e9
     fwd_launch_ms = fwd_ms / a.steps / max(1, st["fwd_launches"])
     fwd_launches_per_step = st["fwd_launches"] / g.nt
-    fwd_achieved = bf * a.batch / fwd_launches_per_step / (fwd_launch_ms * 1e-3) / 1e9
+    fwd_achieved = (bf * a.batch + mf * fwd_launches_per_step) / fwd_launches_per_step / (fwd_launch_ms * 1e-3) / 1e9
     # (2) isolated: one group of that size alone on one stream, graph-replayed launches back to back
     iso_ms, _ = backward_launch_ms(swd, torch, case, adj, group, group, 1, a.nt_iso, a.tile)
-    iso_bytes = bb * group
+    iso_bytes = bb * group + mb
     iso_achieved = iso_bytes / (iso_ms * 1e-3) / 1e9
     log(f"bench: isolated {group}-shot backward launch {iso_ms * 1e3:.2f} us, {iso_achieved:.0f} GB/s")
     # (3) HBM-resident: every shot of the batch in one launch on one stream, a working set twice the
     # last-level cache (32 MB a shot at c2)
     res_ms, _ = backward_launch_ms(swd, torch, case, adj, a.batch, a.batch, 1, a.nt_iso, a.tile)
     res_set = 4.0 * (9 * g.nx * g.nz + 6 * g.ntx * g.ntz + 6 * (g.ntx * g.ntz - g.nx * g.nz)) * a.batch
-    res_achieved = bb * a.batch / (res_ms * 1e-3) / 1e9
+    res_bytes = bb * a.batch + mb
+    res_achieved = res_bytes / (res_ms * 1e-3) / 1e9
     log(f"bench: {a.batch}-shot backward launch ({res_set / 2**20:.0f} MiB working set) {res_ms * 1e3:.1f} us, "
         f"{res_achieved:.0f} GB/s")
 
@@ -1285,7 +1297,7 @@ def main():
                                               "the groups' working set is served largely by the last-level cache"},
                      "hbm_resident": {"achieved": res_achieved, "frac": res_achieved / HBM_PEAK_GBS,
                                       "shots_per_launch": a.batch, "working_set_MiB": res_set / 2**20,
-                                      "avg_launch_ms": res_ms,
+                                      "bytes_per_launch": res_bytes, "avg_launch_ms": res_ms,
                                       "timing": f"{a.batch}-shot group alone on one stream, {a.nt_iso - 1} "
                                                 "launches: a per-step working set twice the 256 MB last-level cache"},
                      "forward_overlapped_GBs": fwd_achieved},

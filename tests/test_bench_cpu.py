@@ -29,18 +29,31 @@ def test_more_than_one_gpu_is_refused():
 def test_rtm_bytes_c2():
     g = c2_grid()
     assert (g.ntx, g.ntz) == (933, 590)
-    N, Nint, P = 933 * 590, 845 * 502, 2 * 44 * 590 + 2 * 44 * 933
+    N, Nint = 933 * 590, 845 * 502
+    Px, Pz = 2 * 44 * 590, 2 * 44 * 933   # the CPML bands of x and z
     nframe = 2 * 4 * (845 + 502 - 8)
-    fwd, bwd = bench.rtm_bytes_per_step(g, 401, nframe)
-    assert fwd == 32 * N + 16 * P + 4 * 401 + 4 * nframe
-    assert bwd == 48 * Nint + 32 * N + 16 * P + 4 * nframe + 4 * 401
-    assert fwd == pytest.approx(19.8e6, rel=0.01) and bwd == pytest.approx(40.2e6, rel=0.01)
+    rw = 2   # read and written
+    # (arrays, cells, passes) of one shot-step: forward state, x-band memories (psx, pvx), z-band memories
+    # (psz, pvz), the gather written, the strips written
+    fwd_inv = [(3, N, rw), (2, Px, rw), (2, Pz, rw), (1, 401, 1), (1, nframe, 1)]
+    # backward: the reconstruction's interior state, image and both illuminations, the receiver field's
+    # state and memories, the strips and the adjoint source read
+    bwd_inv = [(3, Nint, rw), (3, Nint, rw), (3, N, rw), (2, Px, rw), (2, Pz, rw), (1, nframe, 1), (1, 401, 1)]
+    fwd, bwd, mf, mb = bench.rtm_bytes(g, 401, nframe)
+    assert fwd == sum(4 * a * n * k for a, n, k in fwd_inv)
+    assert bwd == sum(4 * a * n * k for a, n, k in bwd_inv)
+    assert (mf, mb) == (4 * 3 * N, 4 * 4 * N)   # K, Bx, Bz (+ K0) once per launch
+    # a 3-shot backward launch moves 116.1 MB, a one-shot launch 44.6 MB; a 4-shot forward launch 68.2 MB
+    assert 3 * bwd + mb == pytest.approx(116.09e6, rel=1e-3) and bwd + mb == pytest.approx(44.57e6, rel=1e-3)
+    assert 4 * fwd + mf == pytest.approx(68.21e6, rel=1e-3)
     assert bench.rtm_cells(g) == 4000 * N + 3999 * (N + Nint)
 
 
 def test_afor_and_updown_bytes_c2():
     g = c2_grid()
-    assert bench.afor_bytes_per_step(g, 401) == pytest.approx(35.12e6, rel=0.005)
+    per_shot, per_launch = bench.afor_bytes(g, 401)
+    assert per_shot + per_launch == pytest.approx(37.32e6, rel=0.005)
+    assert per_launch == 12 * 933 * 590
     # k_updown, interior rows (nfftz 512 < ntz): four field columns over nz rows, two images read and written
     assert bench.updown_kernel_bytes(g, 4, 2, 512) == 4 * 845 * (16 * 502 + 16 * 502)
     # whole padded columns when they fit
