@@ -36,7 +36,7 @@ HIP_FLAGS += os.environ.get("SWD_EXTRA_HIPFLAGS", "").split()
 
 HIP_SOURCES = ["swd_plan.hip", "swd_forward.hip", "swd_rtm.hip", "swd_rtm_updown.hip", "swd_born.hip", "swd_multi.hip", "swd_noalloc.hip",
                "swd_kernels.hip", "swd_loops.hip", "swd_fstep.hip", "swd_fstep_fwd.hip", "swd_fstep_pair.hip",
-               "swd_fstep_pair_cig.hip", "swd_fstep_born.hip", "swd_decon.hip", "swd_updown.hip", "swd_cig.hip",
+               "swd_fstep_pair_cig.hip", "swd_fstep_born.hip", "swd_fstep2.hip", "swd_decon.hip", "swd_updown.hip", "swd_cig.hip",
                "swd_adjoint.hip"]
 CXX_SOURCES = ["swd_setup.cpp"]
 DRIVERS = {"AFor": ["afor.cpp", "driver_common.cpp"], "AVSPRTM": ["avsprtm.cpp", "driver_common.cpp"],

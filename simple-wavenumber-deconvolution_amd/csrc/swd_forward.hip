@@ -33,7 +33,7 @@ static int forward_dev_fused(Plan &P, int nshots, const swd_shot *shots, const B
         }
     }
     SWD_TRY(upload_steps(P, h, o_inst));
-    SWD_TRY(fused_forward_loop(P, at<KStep>(P, o_inst), nshots));
+    SWD_TRY(fused_forward_loop(P, at<KStep>(P, o_inst), nshots, true));
     HIP_TRY(launch_copy_rows(d_gather, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
     HIP_TRY(hipStreamSynchronize(P.stream));
     return SWD_OK;
@@ -88,7 +88,7 @@ extern "C" int swd_forward(void *plan, int nshots, const swd_shot *shots, const 
     DevBuf d;
     SWD_TRY(d.alloc(sizeof(float) * n));
     int wave = nshots;
-    SWD_TRY(wave_shots(*P, nshots, 4.0 * (P->path == SWD_PATH_FUSED ? 12.0 : 7.0) * P->nfield, 4.0 * P->g.nt * 4, wave));
+    SWD_TRY(wave_shots(*P, nshots, 4.0 * (P->path == SWD_PATH_FUSED ? double(kStepArrays) : 7.0) * P->nfield, 4.0 * P->g.nt * 4, wave));
     for (int s0 = 0; s0 < nshots; s0 += wave)
         SWD_TRY(forward_dev(*P, std::min(wave, nshots - s0), shots + s0, wavelet,
                             d.as<float>() + static_cast<size_t>(traces_before(shots, s0)) * P->g.nt));
@@ -181,7 +181,7 @@ int afor_dev(Plan *P, int nshots, const swd_shot *shots, const float *wavelet, c
                 }
             }
         SWD_TRY(upload_steps(*P, hs, o_inst));
-        SWD_TRY(fused_forward_loop(*P, at<KStep>(*P, o_inst), 2 * nshots));
+        SWD_TRY(fused_forward_loop(*P, at<KStep>(*P, o_inst), 2 * nshots, false));
     }
     std::vector<KInst> h(fused ? 0 : 2 * nshots);
     for (int k = 0; !fused && k < nshots; k++)

@@ -57,6 +57,7 @@ struct KStep {
     float *p[2], *vx[2], *vz[2];
     float *psx[2], *psz[2];      // CPML memory of dp/dx, dp/dz (FWD, RCV)
     float *pvx, *pvz;            // CPML memory of dvx/dx, dvz/dz, own cells only (FWD, RCV)
+    float *pvxb, *pvzb;          // their second slots (the two-step kernel reads one slot, writes the other)
     const float *wavelet;        // FWD: added at it; SRC: subtracted at it+1
     const int *colmap, *rowmap;  // receiver lookup (FWD records, RCV replaces)
     float *gather;               // FWD

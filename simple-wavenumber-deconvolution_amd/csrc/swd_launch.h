@@ -25,6 +25,20 @@ constexpr int fstep_cfg_tx(int cfg) { return cfg & 0xff; }
 constexpr int fstep_cfg_tq(int cfg) { return (cfg >> 8) & 0xff; }
 constexpr int fstep_cfg_wb(int cfg) { return (cfg >> 16) & 0xff; }
 constexpr int kFstepDefault = fstep_cfg(16, 16, 0);
+// Two time steps per launch (swd_fstep2.h): bit 24 of a configuration.
+constexpr int kFstepTwo = 1 << 24;
+constexpr int fstep_cfg2(int tx, int tq) { return fstep_cfg(tx, tq, 0) | kFstepTwo; }
+constexpr bool fstep_cfg_two(int cfg) { return (cfg & kFstepTwo) != 0; }
+constexpr int kFstep2Default = fstep_cfg2(32, 8);
+// Tile shapes of the two-step kernel (tx columns x 4 tq rows) and the
+// configurations the tuner times, the default first.
+bool fstep2_shape_supported(int tx, int tq);
+std::vector<int> fstep2_candidates();
+// Two steps (it, it+1 forward; it, it-1 for the paired backward) of every
+// instance in one launch, Lc 4, no capture, dp/dt output, angle gathers or
+// Born; ph: the launch's place in its pair (launches come in pairs, see
+// swd_fstep2.h).  Not for the first reconstruction step.
+hipError_t launch_fstep2(const KGrid &g, const KStep *insts, int ninst, int it, int ph, int cfg, bool pair, hipStream_t s);
 // The configurations a fused launch supports: 64-row tiles 16, 32 or 64 wide
 // for every variant and Lc; at Lc 4, for the forward and the paired RTM
 // backward (not the angle-gather or Born variants), also 32- and 128-row tiles

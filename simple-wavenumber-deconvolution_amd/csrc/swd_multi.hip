@@ -192,6 +192,7 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     P0.stats.tile_x = fstep_cfg_tx(cfg);
     P0.stats.tile_rows = 4 * fstep_cfg_tq(cfg);
     P0.stats.tile_waves = fstep_cfg_wb(cfg);
+    P0.stats.steps_per_launch = 1;
     for (size_t gi = 0; gi < groups.size(); gi++) {
         const hipStream_t st = rtm_group_stream(P0, static_cast<int>(gi));
         const int first = gi == 0 ? it0 : 0;

@@ -63,6 +63,7 @@ struct Plan {
     swd_stats stats{};
     KGrid kg{};
     int tile_cfg = 0;           // fused tile configuration asked for (fstep_cfg; 0 = tune)
+    int steps_per_launch = 0;   // fused loops: 1 or 2 time steps per launch where allowed (0 = tune)
     std::string arch;           // device architecture and compute units (the tuner cache's key)
     int cus = 0;
     int group_fwd = 0, group_bwd = 0;     // RTM shots per fused forward / backward loop (0 = sized to the cache)
@@ -157,9 +158,9 @@ struct Batch {
 // Arrays per wavefield: p, vx, vz, psx, psz (CPML memory of the velocity
 // update), pvx, pvz (CPML memory of the pressure update).
 constexpr int kFieldArrays = 7;
-// Arrays per wavefield of the fused path: p, vx, vz, psx, psz in two slots,
-// then pvx, pvz.
-constexpr int kStepArrays = 12;
+// Arrays per wavefield of the fused path: p, vx, vz, psx, psz, pvx, pvz in
+// two slots.
+constexpr int kStepArrays = 14;
 // Shots per group of the fused RTM loops.  The per-shot working set of a
 // step (double-buffered state, CPML memories in the frame, and on the
 // backward the reconstruction, images and dp/dt) is kept within a budget of
@@ -224,15 +225,21 @@ void bind_step(KStep &S, float *f, size_t nf);
 
 int upload_steps(Plan &P, const std::vector<KStep> &h, size_t o);
 
-// Runs the fused launches its[0], its[1], ... (time steps in execution order)
-// of n instances on stream st (the plan's by default).  The tile width is the
-// plan's requested one, else the one tuned earlier for n instances, else it is
-// tuned here: after two warm-up launches, six launches with each candidate
-// width are timed and the fastest width runs the rest (the results do not
-// depend on the width).
+// Runs the fused time steps its[0], its[1], ... (in execution order) of n
+// instances on stream st (the plan's by default).  The tile configuration is
+// the plan's requested one, else the one tuned earlier for n instances, else
+// it is tuned here: each candidate runs one warm-up unit, then kTry timed
+// units of the loop's own steps, and the fastest per step runs the rest (the
+// results do not depend on the configuration).  two: the steps may run two
+// per launch (swd_fstep2.h; the caller's instances have no AFor capture and
+// no dp/dt output, there is no hook, and consecutive steps follow each other
+// in its): the two-step configurations join the candidates (or alone run the
+// loop, if the plan asks for two steps per launch).  A two-step schedule runs
+// the first reconstruction step (first_it) and the remainder of the step
+// count modulo 4 one step per launch first, then pairs of two-step launches.
 int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int first_it, std::map<int, int> &tuned,
                 long long *launches, bool pair = false, hipStream_t st = nullptr, const StepHook *hook = nullptr,
-                bool cig = false, bool born = false);
+                bool cig = false, bool born = false, bool two = false);
 
 std::vector<int> steps_up(int nt);
 
@@ -279,8 +286,9 @@ int forward_loop(Plan &P, const KInst *d_inst, int per_shot, int nshots);
 
 // Forward time loop of the fused path: n independent wavefield instances in
 // groups whose per-step working set stays in the last-level cache, the groups
-// taking the plan's streams in turn (as the RTM forward).
-int fused_forward_loop(Plan &P, const KStep *d, int n);
+// taking the plan's streams in turn (as the RTM forward).  two: the instances
+// may run two steps per launch (no AFor capture; see fused_steps).
+int fused_forward_loop(Plan &P, const KStep *d, int n, bool two);
 
 int total_traces(int nshots, const swd_shot *shots);
 

@@ -194,6 +194,8 @@ void bind_step(KStep &S, float *f, size_t nf)
     }
     S.pvx = f + 10 * nf;
     S.pvz = f + 11 * nf;
+    S.pvxb = f + 12 * nf;
+    S.pvzb = f + 13 * nf;
 }
 
 int upload_steps(Plan &P, const std::vector<KStep> &h, size_t o)
@@ -289,86 +291,130 @@ int choose_cfg(const std::vector<float> &ms, const std::vector<int> &cand)
 
 int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int first_it, std::map<int, int> &tuned,
                 long long *launches, bool pair, hipStream_t st, const StepHook *hook,
-                bool cig, bool born)
+                bool cig, bool born, bool two)
 {
     if (!st) st = P.stream;
-    const char *mode = born ? "born" : cig ? "cig" : pair ? "pair" : "fwd";
+    // two steps per launch: the forward goes up in time, the paired backward down, one step at a time
+    two = two && !hook && !cig && !born && P.g.Lc == 4 && P.steps_per_launch != 1;
+    const int dir = pair ? -1 : 1;
+    for (size_t j = 1; two && j < its.size(); j++)
+        if (its[j] != its[j - 1] + dir) two = false;
+    const bool only2 = two && P.steps_per_launch == 2;
+    const int variant = two ? (only2 ? 2 : 1) : 0;
+    const std::string mode = std::string(born ? "born" : cig ? "cig" : pair ? "pair" : "fwd") +
+                             (variant == 2 ? " two" : variant == 1 ? " one+two" : "");
     int cfg = P.tile_cfg;
-    if (cfg != 0 && !fstep_cfg_supported(P.g.Lc, pair, cig, born, false, cfg))
-        cfg = fstep_cfg(fstep_cfg_tx(cfg), 16, 0);   // the asked width with the variant's 64-row tiles
+    if (cfg != 0) {
+        const int tx = fstep_cfg_tx(cfg), tq = fstep_cfg_tq(cfg);
+        if (only2) cfg = fstep2_shape_supported(tx, tq) ? fstep_cfg2(tx, tq) : kFstep2Default;
+        else if (!fstep_cfg_supported(P.g.Lc, pair, cig, born, false, cfg))
+            cfg = fstep_cfg(tx, 16, 0);   // the asked width with the variant's 64-row tiles
+    }
+    const int tkey = 4 * n + variant;
     std::string key;
     if (cfg == 0) {
-        const auto f = tuned.find(n);
+        const auto f = tuned.find(tkey);
         if (f != tuned.end()) cfg = f->second;
     }
+    auto usable = [&](int c) {
+        return fstep_cfg_two(c) ? two && fstep2_shape_supported(fstep_cfg_tx(c), fstep_cfg_tq(c))
+                                : !only2 && fstep_cfg_supported(P.g.Lc, pair, cig, born, false, c);
+    };
     if (cfg == 0) {
-        key = tune_key(P, mode, n);
+        key = tune_key(P, mode.c_str(), n);
         cfg = tune_lookup(key);
-        if (cfg != 0 && !fstep_cfg_supported(P.g.Lc, pair, cig, born, false, cfg)) cfg = 0;
-        if (cfg != 0) tuned[n] = cfg;
+        if (cfg != 0 && !usable(cfg)) cfg = 0;
+        if (cfg != 0) tuned[tkey] = cfg;
     }
     size_t k = 0;
-    auto launch = [&](int c) -> int {
+    long long nl = 0;
+    // one step (it == first_it: the reconstruction keeps the forward's velocities)
+    auto one = [&](int c) -> int {
         const int it = its[k++];
-        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, c, pair, cig, born, st));
-        (*launches)++;
+        HIP_TRY(launch_fstep(P.kg, d, n, it, it == first_it, fstep_cfg_two(c) ? kFstepDefault : c, pair, cig, born, st));
+        nl++;
         if (hook) {
             SWD_TRY(hook->run(it, st));
-            (*launches)++;
+            nl++;
         }
         return SWD_OK;
     };
+    // four steps as a pair of two-step launches (slot parity as before afterwards)
+    auto quad = [&](int c) -> int {
+        HIP_TRY(launch_fstep2(P.kg, d, n, its[k], 0, c, pair, st));
+        HIP_TRY(launch_fstep2(P.kg, d, n, its[k + 2], 1, c, pair, st));
+        k += 4;
+        nl += 2;
+        return SWD_OK;
+    };
+    auto unit = [&](int c) -> int { return fstep_cfg_two(c) ? quad(c) : one(c); };
+    auto unit_steps = [](int c) -> size_t { return fstep_cfg_two(c) ? 4 : 1; };
+    if (two && k < its.size() && its[k] == first_it) SWD_TRY(one(kFstepDefault));
     if (cfg == 0) {
-        // time kTry launches of every candidate after a warm-up launch of each,
-        // on the loop's own steps, and keep the choice for later loops
+        // time kTry units of every candidate after a warm-up unit of each, on
+        // the loop's own steps, and keep the choice for later loops
         constexpr int kTry = 5;
-        const std::vector<int> cand = fstep_candidates(P.g.Lc, pair, cig, born, false);
-        if (its.size() >= cand.size() * (kTry + 1) + 1) {
+        std::vector<int> cand;
+        if (!only2) cand = fstep_candidates(P.g.Lc, pair, cig, born, false);
+        if (two)
+            for (int c : fstep2_candidates()) cand.push_back(c);
+        size_t need = 1;
+        for (int c : cand) need += unit_steps(c) * (kTry + 1);
+        if (its.size() - k >= need) {
             std::vector<hipEvent_t> ev(cand.size() + 1);
             for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
             int rc = SWD_OK;
             std::vector<float> ms(cand.size(), 0.f);
             for (size_t c = 0; c < cand.size() && rc == SWD_OK; c++) {
-                rc = launch(cand[c]);   // first use of the configuration, untimed
+                rc = unit(cand[c]);   // first use of the configuration, untimed
                 if (rc == SWD_OK) rc = hipEventRecord(ev[c], st) == hipSuccess ? SWD_OK : SWD_E_DEVICE;
-                for (int j = 0; j < kTry && rc == SWD_OK; j++) rc = launch(cand[c]);
+                for (int j = 0; j < kTry && rc == SWD_OK; j++) rc = unit(cand[c]);
                 if (rc == SWD_OK) rc = hipEventRecord(ev[cand.size()], st) == hipSuccess ? SWD_OK : SWD_E_DEVICE;
                 if (rc == SWD_OK) rc = hipEventSynchronize(ev[cand.size()]) == hipSuccess ? SWD_OK : SWD_E_DEVICE;
                 if (rc == SWD_OK) rc = hipEventElapsedTime(&ms[c], ev[c], ev[cand.size()]) == hipSuccess ? SWD_OK
                                                                                                        : SWD_E_DEVICE;
+                ms[c] /= static_cast<float>(kTry * unit_steps(cand[c]));   // per step
             }
             for (auto &e : ev) (void)hipEventDestroy(e);
             if (rc != SWD_OK) return swd_fail(rc, "tile tuner: a timed launch failed");
             cfg = choose_cfg(ms, cand);
-            tuned[n] = cfg;
+            tuned[tkey] = cfg;
             tune_store(key, cfg);
         } else {
-            cfg = kFstepDefault;   // too few steps to time the candidates
+            cfg = only2 ? kFstep2Default : kFstepDefault;   // too few steps to time the candidates
         }
     }
     P.stats.tile_x = fstep_cfg_tx(cfg);
     P.stats.tile_rows = 4 * fstep_cfg_tq(cfg);
     P.stats.tile_waves = fstep_cfg_wb(cfg);
+    P.stats.steps_per_launch = fstep_cfg_two(cfg) ? 2 : 1;
+    // the rest: with two steps per launch, the remainder modulo 4 one step at a time, then pairs
+    auto rest = [&]() -> int {
+        if (fstep_cfg_two(cfg)) {
+            const size_t r = (its.size() - k) % 4;
+            for (size_t j = 0; j < r; j++) SWD_TRY(one(kFstepDefault));
+            while (k < its.size()) SWD_TRY(quad(cfg));
+        } else {
+            while (k < its.size()) SWD_TRY(one(cfg));
+        }
+        return SWD_OK;
+    };
     if (k < its.size() && P.graphs) {
-        std::vector<long long> key = {1, reinterpret_cast<long long>(d), n, its[k], static_cast<long long>(its.size() - k),
-                                      first_it, cfg, (pair ? 1 : 0) + (cig ? 2 : 0) + (born ? 4 : 0),
-                                      reinterpret_cast<long long>(P.ws)};
-        if (hook) key.insert(key.end(), hook->key.begin(), hook->key.end());
-        const size_t k0 = k;
-        SWD_TRY(graph_run(
-            P, key,
-            [&]() -> int {
-                for (size_t j = k0; j < its.size(); j++) {
-                    HIP_TRY(launch_fstep(P.kg, d, n, its[j], its[j] == first_it, cfg, pair, cig, born, st));
-                    if (hook) SWD_TRY(hook->run(its[j], st));
-                }
-                return SWD_OK;
-            },
-            st));
-        *launches += static_cast<long long>(its.size() - k0) * (hook ? 2 : 1);
+        std::vector<long long> gkey = {1, reinterpret_cast<long long>(d), n, its[k], static_cast<long long>(its.size() - k),
+                                       first_it, cfg, (pair ? 1 : 0) + (cig ? 2 : 0) + (born ? 4 : 0),
+                                       reinterpret_cast<long long>(P.ws)};
+        if (hook) gkey.insert(gkey.end(), hook->key.begin(), hook->key.end());
+        // launches of the rest, counted here since a replay does not record them
+        const size_t left = its.size() - k;
+        const long long nrest = fstep_cfg_two(cfg) ? static_cast<long long>(left % 4 + left / 4 * 2)
+                                                   : static_cast<long long>(left) * (hook ? 2 : 1);
+        const long long before = nl;   // the first step's and the tuner's launches
+        SWD_TRY(graph_run(P, gkey, rest, st));
+        *launches += before + nrest;
         return SWD_OK;
     }
-    while (k < its.size()) SWD_TRY(launch(cfg));
+    SWD_TRY(rest());
+    *launches += nl;
     return SWD_OK;
 }
 
@@ -478,7 +524,7 @@ int forward_loop(Plan &P, const KInst *d_inst, int per_shot, int nshots)
     return SWD_OK;
 }
 
-int fused_forward_loop(Plan &P, const KStep *d, int n)
+int fused_forward_loop(Plan &P, const KStep *d, int n, bool two)
 {
     long long launches = 0;
     // with no strips to write, the backward's budget (c2 AFor, 32 wavefields
@@ -490,7 +536,7 @@ int fused_forward_loop(Plan &P, const KStep *d, int n)
     const std::vector<int> bounds = group_bounds(n, gf, P.streams);
     for (size_t g = 0; g + 1 < bounds.size(); g++)
         SWD_TRY(fused_steps(P, d + bounds[g], bounds[g + 1] - bounds[g], steps_up(P.g.nt), -1, P.tuned_fwd, &launches,
-                            false, rtm_group_stream(P, static_cast<int>(g))));
+                            false, rtm_group_stream(P, static_cast<int>(g)), nullptr, false, false, two));
     SWD_TRY(join_streams(P));
     P.stats.fwd_ms = stats_end(P, 0);
     P.stats.fwd_launches = launches;
@@ -742,6 +788,17 @@ extern "C" int swd_plan_set_tile_config(void *plan, int tile_x, int tile_rows, i
     P->tuned_bwd.clear();
     P->tuned_cig.clear();
     P->tuned_born.clear();
+    return SWD_OK;
+}
+
+extern "C" int swd_plan_set_steps_per_launch(void *plan, int steps)
+{
+    if (!plan || steps < 0 || steps > 2)
+        return swd_fail(SWD_E_ARG, "swd_plan_set_steps_per_launch: %d steps per launch, not 0, 1 or 2", steps);
+    Plan *P = static_cast<Plan *>(plan);
+    P->steps_per_launch = steps;
+    P->tuned_fwd.clear();
+    P->tuned_bwd.clear();
     return SWD_OK;
 }
 

@@ -139,7 +139,7 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
         for (size_t g = 0; g + 1 < gbf.size(); g++) {
             const int k0 = gbf[g], n = gbf[g + 1] - gbf[g];
             SWD_TRY(fused_steps(P, at<KStep>(P, o_fwd) + k0, n, steps_up(nt), -1, P.tuned_fwd, &launches, false,
-                                group_stream(static_cast<int>(g))));
+                                group_stream(static_cast<int>(g)), nullptr, false, false, true));
         }
         SWD_TRY(join());
         P.stats.fwd_ms = stats_end(P, 0);
@@ -183,7 +183,8 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
             };
             hook.key = {3, reinterpret_cast<long long>(dk), thetan, delt};
         }
-        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tuned_bwd, &launches, true, gs, cig ? &hook : nullptr));
+        SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tuned_bwd, &launches, true, gs, cig ? &hook : nullptr, false,
+                            false, !cig));
     }
     SWD_TRY(join());
     P.stats.bwd_ms = stats_end(P, 1);
@@ -311,7 +312,7 @@ extern "C" int swd_rtm_dev(void *plan, int nshots, const swd_shot *shots, const 
     if (!wavelet || !adj || !image || !illum_src || !illum_rcv || !shots || nshots < 1)
         return swd_fail(SWD_E_ARG, "swd_rtm: null argument");
     const double nt = P->g.nt, nf = static_cast<double>(P->nfield);
-    const double per = 4.0 * (nt * P->nframe + (P->path == SWD_PATH_FUSED ? 27.0 : 18.0) * nf);
+    const double per = 4.0 * (nt * P->nframe + (P->path == SWD_PATH_FUSED ? 2.0 * kStepArrays + 3.0 : 18.0) * nf);
     int wave = nshots;
     SWD_TRY(wave_shots(*P, nshots, per, 4.0 * nt * 4, wave));
     const size_t ntp = static_cast<size_t>(P->ntx) * P->ntz;
@@ -357,7 +358,7 @@ extern "C" int swd_rtm_cig_dev(void *plan, int nshots, const swd_shot *shots, co
         return swd_fail(SWD_E_ARG, "swd_rtm_cig: thetan=%d must be in [1, 1024] and delt=%d in [1, 64]", thetan, delt);
     const double nt = P->g.nt, nf = static_cast<double>(P->nfield);
     CigSpec spec{thetan, delt, nullptr};
-    const double per = 4.0 * (nt * P->nframe + 27.0 * nf) + cig_bytes(*P, &spec);
+    const double per = 4.0 * (nt * P->nframe + (2.0 * kStepArrays + 3.0) * nf) + cig_bytes(*P, &spec);
     int wave = nshots;
     SWD_TRY(wave_shots(*P, nshots, per, 4.0 * nt * 4, wave));
     const size_t ntp = static_cast<size_t>(P->ntx) * P->ntz;

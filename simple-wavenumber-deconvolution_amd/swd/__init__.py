@@ -67,7 +67,8 @@ class Stats(C.Structure):
                 ("fwd_launches", C.c_longlong), ("bwd_launches", C.c_longlong),
                 ("fwd_cells", C.c_longlong), ("bwd_cells", C.c_longlong),
                 ("ws_bytes", C.c_longlong), ("ckpt_bytes", C.c_longlong), ("full_bytes", C.c_longlong),
-                ("waves", C.c_longlong), ("tile_x", C.c_int), ("tile_rows", C.c_int), ("tile_waves", C.c_int)]
+                ("waves", C.c_longlong), ("tile_x", C.c_int), ("tile_rows", C.c_int), ("tile_waves", C.c_int),
+                ("steps_per_launch", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -86,7 +87,7 @@ EXPORTS = ("swd_last_error", "swd_get_device_count", "swd_read_parameters", "swd
            "swd_nfft", "swd_kz_taper",
            "swd_hilbert1d", "swd_geometry", "swd_adjoint_source", "swd_scatterers", "swd_image_normalize",
            "swd_image_laplacian", "swd_edge_zero", "swd_plan_create", "swd_plan_destroy", "swd_plan_set_coefs",
-           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_tile_config", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
+           "swd_plan_set_model", "swd_plan_set_path", "swd_plan_set_tile", "swd_plan_set_tile_config", "swd_plan_set_steps_per_launch", "swd_plan_set_streams", "swd_plan_set_groups", "swd_plan_set_graphs",
            "swd_plan_set_memory_limit", "swd_plan_get_stats", "swd_forward", "swd_afor", "swd_afor_dev",
            "swd_rtm", "swd_rtm_dev", "swd_rtm_cig", "swd_rtm_cig_dev", "swd_rtm_updown", "swd_rtm_updown_dev", "swd_updown_image_dev",
            "swd_updown_image_dev_noalloc",
@@ -365,6 +366,11 @@ class Plan:
         """Fused-path tile: columns (16, 32, 64), rows (64; at Lc 4 also 32, 128)
         and a wavefronts-per-SIMD bound (0; at Lc 4 also 5); all 0 to tune."""
         check(self.L.swd_plan_set_tile_config(self.h, int(tile_x), int(tile_rows), int(waves)))
+
+    def set_steps_per_launch(self, steps):
+        """Time steps per launch of the fused forward and paired RTM backward at Lc 4:
+        1, 2 (temporal blocking) or 0 to let the tuner pick; results do not change."""
+        check(self.L.swd_plan_set_steps_per_launch(self.h, int(steps)))
 
     def close(self):
         if self.h:

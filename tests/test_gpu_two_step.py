@@ -1,0 +1,128 @@
+"""Two time steps per launch (temporal blocking) on the fused path, against
+the one-step kernel and the CPU oracle.
+
+Bar: the gathers, RTM images and illuminations equal the one-step kernel's
+bit for bit (both make the same per-cell operations), and the fp32 oracle's
+within 1e-5 relative L2 (the forward gathers bit for bit).  Step counts are
+odd and even so that the one-step remainder of a two-step schedule is
+exercised; sources sit in the absorbing frame, on the interior frame and in
+the fast interior, receivers cross tile edges.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import swd
+from swd import configs
+from helpers import oracle_problem, rel_l2, report
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+SHAPES = [(32, 64), (64, 32), (16, 64), (32, 32)]   # columns x rows of the two-step kernel
+
+
+def plan_of(c, steps, shape=None):
+    p = swd.Plan(c.grid, c.coefs, c.vp, c.rho, path=swd.SWD_PATH_FUSED)
+    p.set_steps_per_launch(steps)
+    if shape:
+        p.set_tile_config(shape[0], shape[1], 0)
+    else:
+        p.set_tile_config(16, 64, 0)
+    return p
+
+
+def run_both(c, shape, nshots):
+    """Forward and RTM with one and with two steps per launch, and the oracle's."""
+    Ps = [oracle_problem(c, k) for k in range(nshots)]
+    outs = [P.forward(strips=True, final=True) for P in Ps]
+    adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
+    res = {}
+    for steps in (1, 2):
+        plan = plan_of(c, steps, shape if steps == 2 else None)
+        g = plan.forward(c.shots, c.wavelet)
+        assert plan.stats()["steps_per_launch"] == steps
+        r = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
+        st = plan.stats()
+        assert st["steps_per_launch"] == steps
+        res[steps] = (g,) + tuple(r)
+        plan.close()
+    return Ps, outs, adj, res
+
+
+def check(c, shape, nshots, label):
+    Ps, outs, adj, res = run_both(c, shape, nshots)
+    for a, b in zip(res[1], res[2]):
+        assert np.array_equal(a, b), label
+    g, img, ils, ilr, gs = res[2]
+    row = 0
+    for k, P in enumerate(Ps):
+        d, st, fs = outs[k]
+        n = d.shape[0]
+        o = P.rtm_backward(fs, st, adj[row:row + n])
+        assert np.array_equal(g[row:row + n], d), f"{label} shot {k} forward"
+        assert np.array_equal(gs[row:row + n], d), f"{label} shot {k} RTM forward"
+        for name, a, ref in (("image", img[k], o[0]), ("illum_src", ils[k], o[1]), ("illum_rcv", ilr[k], o[2])):
+            e32, _ = report(f"{label} shot {k} {name}", a, ref)
+            assert e32 <= TOL
+        row += n
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("nt", [160, 163])
+def test_two_step_small(shape, nt):
+    """A grid smaller than most tiles' margins (frame tiles everywhere), a second
+    shot with its source in the absorbing frame."""
+    c = configs.small(nx=100, nz=70, pml=8, Lc=4, nt=nt, nshots=2)
+    s_ix, s_iz, rx, rz = c.shots[1]
+    c.shots[1] = (3, 5, rx, rz)
+    print(f"case two-step small {shape} nt={nt}", flush=True)
+    check(c, shape, 2, f"small {shape} nt {nt}")
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_two_step_interior_sources_and_receivers(shape):
+    """A grid with fast (damping- and frame-free) tiles: one source deep in the
+    interior, one on the interior frame, receiver columns at tile edges."""
+    c = configs.small(nx=300, nz=260, pml=10, Lc=4, nt=161, nshots=2)
+    pm = c.pmlc
+    rz = np.arange(pm + 2, pm + 250, 3)
+    c.shots = [(pm + 150, pm + 130, np.full(rz.size, pm + 96), rz),
+               (pm + 1, pm + 60, np.full(rz.size, pm + 127), rz)]
+    print(f"case two-step interior {shape}", flush=True)
+    check(c, shape, 2, f"interior {shape}")
+
+
+def test_two_step_c2_grid():
+    """The c2 grid (590 x 933 padded, 401 receivers) for 161 steps, two shots,
+    one of them in the receivers' column."""
+    c = configs.c2(nt=161, shot_ix=(300, 420))
+    print("case two-step c2 grid nt=161", flush=True)
+    check(c, (32, 64), 2, "c2")
+
+
+def test_two_step_tuner_and_groups():
+    """steps = 0 (the tuner times one- and two-step configurations) and shot
+    groups on two streams give the one-step bits."""
+    c = configs.small(nx=180, nz=150, pml=10, nt=300, nshots=4, dup=True)
+    print("case two-step tuner and groups", flush=True)
+    outs = [oracle_problem(c, k).forward() for k in range(4)]
+    adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
+    ref_plan = plan_of(c, 1)
+    ref = ref_plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
+    ref_plan.close()
+    plan = swd.Plan(c.grid, c.coefs, c.vp, c.rho, path=swd.SWD_PATH_FUSED)
+    got = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
+    print(f"tuned: {plan.stats()['steps_per_launch']} steps per launch, tile {plan.stats()['tile_x']} x "
+          f"{plan.stats()['tile_rows']}", flush=True)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    plan.set_steps_per_launch(2)
+    for gf, gb, streams in ((1, 1, 2), (3, 2, 1)):
+        plan.set_groups(gf, gb)
+        plan.set_streams(streams)
+        got = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), (gf, gb, streams)
+    with pytest.raises(swd.SwdError):
+        plan.set_steps_per_launch(3)
+    plan.close()
