@@ -101,6 +101,7 @@ typedef struct swd_stats {
     long long waves;        /* shot waves of the last call (see swd_plan_set_memory_limit) */
     int tile_x, tile_rows, tile_waves; /* tile of the last fused loop: columns, rows, wavefronts-per-SIMD bound (0: none) */
     int steps_per_launch;   /* time steps per launch of the last fused loop (1 or 2) */
+    int tile_group_waves;   /* wavefronts per workgroup of the last fused loop */
 } swd_stats;
 
 /* ---------------- errors and devices ---------------- */
@@ -159,7 +160,12 @@ int swd_plan_set_path(void *plan, int path);
  * plan's later loops with as many instances per launch and on disk (the file
  * SWD_TUNE_CACHE, else $XDG_CACHE_HOME or $HOME/.cache/swd_tiles.txt) keyed by
  * device, grid, Lc, instances per launch, variant and streams.  Variants that
- * lack an asked height or bound run the asked width at 64 rows.  Results do not
+ * lack an asked height or bound run the asked width at 64 rows.  For two steps
+ * per launch (swd_plan_set_steps_per_launch) waves is the wavefronts per
+ * workgroup instead, negative for split launches (the tiles free of damping
+ * and frame cells, then the others): 32 x 32 rows with 4 or -4, 64 x 32 with 8
+ * or -8, 16 x 64 with 4, 32 x 64 with -8 (0: the first of the shape).
+ * Results do not
  * depend on the tile. */
 int swd_plan_set_tile_config(void *plan, int tile_x, int tile_rows, int waves);
 /* swd_plan_set_tile_config(plan, tile_x, 64, 0), or the tuner for tile_x = 0. */

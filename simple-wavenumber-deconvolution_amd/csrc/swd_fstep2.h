@@ -37,6 +37,8 @@
 // condition).  Not here: AFor capture, dp/dt output, angle gathers, Born, the
 // first reconstruction step (which keeps the forward's velocities).
 #pragma once
+#include <algorithm>
+
 #include "swd_device.h"
 #include "swd_kernels.h"
 
@@ -44,11 +46,11 @@ namespace {
 
 __device__ __forceinline__ float4 f2zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 
-template <int TX, int TQ>
+template <int TX, int TQ, int NW>
 struct T2 {
     static constexpr int LC = 4;
-    static constexpr int NT = TX * TQ;   // one own quad per thread
-    static_assert(NT >= 256 && NT <= 512, "256 to 512 own quads per tile");
+    static constexpr int NT = 64 * NW;   // threads; thread t < TX TQ owns quad t
+    static_assert(NT >= TX * TQ && NT <= 1024, "a thread for every own quad");
     // LDS boxes, column-major (a column's quads contiguous)
     static constexpr int PNQ = TQ + 8, PNC = TX + 28;   // sP: columns -14 .. TX+13, quads -4 .. TQ+3
     static constexpr int XNQ = TQ + 4, XNC = TX + 21;   // sVX: columns -11 .. TX+9, quads -2 .. TQ+1
@@ -72,7 +74,7 @@ struct T2 {
 template <int TX, int TQ>
 __device__ __forceinline__ void f2_p0(int i, int &c, int &q)
 {
-    using G = T2<TX, TQ>;
+    using G = T2<TX, TQ, 16>;   // region sizes do not depend on the thread count
     if (i < G::P0A) {
         c = i / TQ - 14;
         q = i % TQ;
@@ -89,7 +91,7 @@ __device__ __forceinline__ void f2_p0(int i, int &c, int &q)
 template <int TX, int TQ>
 __device__ __forceinline__ void f2_vx(int i, int &c, int &q)
 {
-    using G = T2<TX, TQ>;
+    using G = T2<TX, TQ, 16>;   // region sizes do not depend on the thread count
     if (i < G::XCORE) {
         c = i / TQ - 4;
         q = i % TQ;
@@ -106,7 +108,7 @@ __device__ __forceinline__ void f2_vx(int i, int &c, int &q)
 template <int TX, int TQ>
 __device__ __forceinline__ void f2_vz(int i, int &c, int &q)
 {
-    using G = T2<TX, TQ>;
+    using G = T2<TX, TQ, 16>;   // region sizes do not depend on the thread count
     if (i < G::ZOWN) {
         c = i / TQ;
         q = i % TQ;
@@ -131,7 +133,7 @@ __device__ __forceinline__ void f2_vz(int i, int &c, int &q)
 template <int TX, int TQ>
 __device__ __forceinline__ void f2_p(int i, int &c, int &q)
 {
-    using G = T2<TX, TQ>;
+    using G = T2<TX, TQ, 16>;   // region sizes do not depend on the thread count
     if (i < G::PCORE) {
         c = i / TQ;
         q = i % TQ;
@@ -425,12 +427,12 @@ __device__ __forceinline__ void f2_image(const KGrid &g, int z, float4 dp, float
 
 // Two steps of one wavefield on one tile.  dps: dp/dt of the own quad at the
 // two levels (SRC writes them, RCV images with them).
-template <int TX, int TQ, bool FULL, int MODE>
+template <int TX, int TQ, int NW, bool FULL, int MODE>
 __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0, int Z0, int it, int ph,
                                          float4 *__restrict__ sP, float4 *__restrict__ sVX, float4 *__restrict__ sVZ,
                                          float4 (&dps)[2])
 {
-    using G = T2<TX, TQ>;
+    using G = T2<TX, TQ, NW>;
     constexpr int NT = G::NT, PNQ = G::PNQ, XNQ = G::XNQ, ZNQ = G::ZNQ;
     constexpr bool SRC = MODE == KSTEP_SRC, NEG = MODE == KSTEP_RCV, CPML = !FULL && !SRC;
     const int tid = threadIdx.x;
@@ -513,7 +515,8 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
     }
     // own quad (item tid of the core regions): receiver map, K0, image accumulators
     const int oc = tid / TQ, oq = tid % TQ, ox = X0 + oc, oz = Z0 + 4 * oq;
-    const bool oin = FULL || (ox < ntx && oz < ntz);
+    const bool own = NT == TX * TQ || tid < TX * TQ;
+    const bool oin = own && (FULL || (ox < ntx && oz < ntz));
     const bool ocolint = FULL || (ox >= g.pmlc && ox < ntx - g.pmlc);
     int ocm = -1;
     if (!SRC && I.colmap && oin) ocm = ldgi(I.colmap, ox);
@@ -598,7 +601,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
         }
         const bool wr = f2_p_quad<FULL, MODE, XNQ>(g, I, x, z, lv1, &sVX[G::sx(c, q)], w, pold, kv[r], k0, qxo, qzo, pv,
                                                    dp, qxn, qzn, axo, anyz);
-        if (r == 0) {   // the own quad (core = item tid)
+        if (i < G::PCORE) {   // the own quad (core = item tid)
             if (SRC) dps[0] = dp;
             if (!SRC) f2_receivers<FULL, MODE>(g, I, ocm, z, lv1, pv);
             if (doimg) f2_image<FULL>(g, z, dps[0], pv, im, is, ir);
@@ -705,33 +708,61 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
 // One launch: two steps of every instance (PAIR = false: FWD instances; PAIR
 // = true: (SRC, RCV) pairs of one shot per workgroup, the reconstruction
 // first).  it: the first step's time index; ph: the launch's place in its pair.
-// Wavefronts per SIMD the kernels are held to (1: no bound): the forward fits
-// four in its registers without spilling.
+// The tiles whose stencils reach no damping or frame cell (the "fast" tiles)
+// form a rectangle of tile columns fr.x .. fr.x+fr.y-1 and tile rows fr.z ..
+// fr.z+fr.w-1; SEL = 1 launches cover those, SEL = 2 launches the others (the
+// two kernels differ in registers: the fast one holds more wavefronts), SEL = 0
+// launches every tile and picks the path per tile.
 #ifndef SWD_F2_WAVES_FWD
 #define SWD_F2_WAVES_FWD 4
 #endif
-#ifndef SWD_F2_WAVES_PAIR
-#define SWD_F2_WAVES_PAIR 1
+#ifndef SWD_F2_WAVES_FAST
+#define SWD_F2_WAVES_FAST 4
 #endif
-template <int TX, int TQ, bool PAIR>
-__global__ __launch_bounds__(TX *TQ, PAIR ? SWD_F2_WAVES_PAIR : SWD_F2_WAVES_FWD) void k_fstep2(KGrid g, const KStep *__restrict__ insts, int ninst, int nzb,
-                                                   int nxb, int it, int ph)
+#ifndef SWD_F2_WAVES_FRAME
+#define SWD_F2_WAVES_FRAME 1
+#endif
+template <int TX, int TQ, int NW, bool PAIR, int SEL>
+__global__ __launch_bounds__(64 * NW, SEL == 1 ? (PAIR ? SWD_F2_WAVES_FAST : SWD_F2_WAVES_FWD) : (PAIR ? SWD_F2_WAVES_FRAME : SWD_F2_WAVES_FWD)) void k_fstep2(
+    KGrid g, const KStep *__restrict__ insts, int ninst, int nzb, int nxb, int4 fr, int it, int ph)
 {
-    using G = T2<TX, TQ>;
+    using G = T2<TX, TQ, NW>;
     __shared__ float4 sP[G::PNC * G::PNQ];
     __shared__ float4 sVX[G::XNC * G::XNQ];
     __shared__ float4 sVZ[G::ZNC * G::ZNQ];
     const int L = xcd_block();
-    if (L >= ninst * nzb * nxb) return;
-    const int inst = L % ninst;
-    const int zb = (L / ninst) % nzb, xb = L / (ninst * nzb);
+    const int ntile = SEL == 0 ? nzb * nxb : SEL == 1 ? fr.y * fr.w : nzb * nxb - fr.y * fr.w;
+    if (L >= ninst * ntile) return;
+    const int inst = L % ninst, t = L / ninst;
+    int xb, zb;
+    if (SEL == 0) {
+        zb = t % nzb;
+        xb = t / nzb;
+    } else if (SEL == 1) {
+        zb = fr.z + t % fr.w;
+        xb = fr.x + t / fr.w;
+    } else {
+        // tile columns left of the rectangle, the rectangle's columns above and below it, the columns right of it
+        const int a = fr.x * nzb, m = nzb - fr.w, b = a + fr.y * m;
+        if (t < a) {
+            xb = t / nzb;
+            zb = t % nzb;
+        } else if (t < b) {
+            const int u = t - a, k = u % m;
+            xb = fr.x + u / m;
+            zb = k < fr.z ? k : k + fr.w;
+        } else {
+            const int u = t - b;
+            xb = fr.x + fr.y + u / nzb;
+            zb = u % nzb;
+        }
+    }
     const int X0 = xb * TX, Z0 = zb * 4 * TQ;
-    const bool full = X0 - 14 >= g.fx0 && X0 + TX + 13 <= g.fx1 && Z0 - 16 >= g.fz0 && Z0 + 4 * TQ + 15 <= g.fz1;
+    const bool full = SEL == 1 || (SEL == 0 && xb >= fr.x && xb < fr.x + fr.y && zb >= fr.z && zb < fr.z + fr.w);
     float4 dps[2] = {f2zero(), f2zero()};
     if (!PAIR) {
-        const KStep &I = insts[inst];
-        if (full) f2_field<TX, TQ, true, KSTEP_FWD>(g, I, X0, Z0, it, ph, sP, sVX, sVZ, dps);
-        else f2_field<TX, TQ, false, KSTEP_FWD>(g, I, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+        if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_FWD>(g, insts[inst], X0, Z0, it, ph, sP, sVX, sVZ, dps);
+        else f2_field<TX, TQ, NW, SEL == 1, KSTEP_FWD>(g, insts[inst], X0, Z0, it, ph, sP, sVX, sVZ, dps);
         return;
     }
     const KStep &S = insts[2 * inst], &Rv = insts[2 * inst + 1];
@@ -740,12 +771,25 @@ __global__ __launch_bounds__(TX *TQ, PAIR ? SWD_F2_WAVES_PAIR : SWD_F2_WAVES_FWD
     const bool any = X0 + TX > g.pmlc && X0 < g.ntx - g.pmlc && Z0 + rows > g.pmlc && Z0 < g.ntz - g.pmlc;
     const bool src = S.s_ix >= X0 && S.s_ix < X0 + TX && S.s_iz >= Z0 && S.s_iz < Z0 + rows;
     if (any || src) {
-        if (full) f2_field<TX, TQ, true, KSTEP_SRC>(g, S, X0, Z0, it, ph, sP, sVX, sVZ, dps);
-        else f2_field<TX, TQ, false, KSTEP_SRC>(g, S, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+        if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_SRC>(g, S, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+        else f2_field<TX, TQ, NW, SEL == 1, KSTEP_SRC>(g, S, X0, Z0, it, ph, sP, sVX, sVZ, dps);
         __syncthreads();   // LDS is staged again for the receiver wavefield
     }
-    if (full) f2_field<TX, TQ, true, KSTEP_RCV>(g, Rv, X0, Z0, it, ph, sP, sVX, sVZ, dps);
-    else f2_field<TX, TQ, false, KSTEP_RCV>(g, Rv, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+    if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_RCV>(g, Rv, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+    else f2_field<TX, TQ, NW, SEL == 1, KSTEP_RCV>(g, Rv, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+}
+
+// The fast rectangle of a grid for TX x 4TQ tiles (empty: y = w = 0): a tile
+// is fast when its staged region (columns X0-14 .. X0+TX+13, rows Z0-16 ..
+// Z0+4TQ+15) lies inside the damping-free box.
+inline int4 f2_fast_rect(const KGrid &g, int tx, int tq, int nxb, int nzb)
+{
+    const int rows = 4 * tq;
+    const int xa = std::max(0, (g.fx0 + 14 + tx - 1) / tx), xz = (g.fx1 - 13 - tx) >= 0 ? (g.fx1 - 13 - tx) / tx : -1;
+    const int za = std::max(0, (g.fz0 + 16 + rows - 1) / rows), zz = (g.fz1 - 15 - rows) >= 0 ? (g.fz1 - 15 - rows) / rows : -1;
+    const int nx = std::min(xz, nxb - 1) - xa + 1, nz = std::min(zz, nzb - 1) - za + 1;
+    if (g.fx1 < g.fx0 || g.fz1 < g.fz0 || nx <= 0 || nz <= 0) return make_int4(0, 0, 0, 0);
+    return make_int4(xa, nx, za, nz);
 }
 
 }  // namespace

@@ -2,40 +2,62 @@
 #include "swd_fstep2.h"
 #include "swd_launch.h"
 
-bool fstep2_shape_supported(int tx, int tq)
-{
-    return (tx == 32 && tq == 16) || (tx == 64 && tq == 8) || (tx == 16 && tq == 16) || (tx == 32 && tq == 8);
-}
-
 std::vector<int> fstep2_candidates()
 {
-    return {fstep_cfg2(32, 8), fstep_cfg2(16, 16), fstep_cfg2(64, 8), fstep_cfg2(32, 16)};
+    return {fstep_cfg2(32, 8, 4), fstep_cfg2(32, 8, 4) | kFstepSplit, fstep_cfg2(64, 8, 8) | kFstepSplit,
+            fstep_cfg2(64, 8, 8), fstep_cfg2(16, 16, 4), fstep_cfg2(32, 16, 8) | kFstepSplit};
 }
 
+bool fstep2_cfg_supported(int cfg)
+{
+    for (int c : fstep2_candidates())
+        if (c == cfg) return true;
+    return false;
+}
+
+int fstep2_cfg_of_shape(int tx, int tq)
+{
+    for (int c : fstep2_candidates())
+        if (fstep_cfg_tx(c) == tx && fstep_cfg_tq(c) == tq) return c;
+    return 0;
+}
+
+// One launch over every tile, or (split) the fast tiles then the frame tiles:
+// both read slot sa and write the other, so their order does not matter.
 hipError_t launch_fstep2(const KGrid &g, const KStep *insts, int ninst, int it, int ph, int cfg, bool pair, hipStream_t s)
 {
-    const int tx = fstep_cfg_tx(cfg), tq = fstep_cfg_tq(cfg);
-    if (g.Lc != 4 || !fstep_cfg_two(cfg) || !fstep2_shape_supported(tx, tq) || (ph != 0 && ph != 1))
-        return hipErrorInvalidValue;
+    const int tx = fstep_cfg_tx(cfg), tq = fstep_cfg_tq(cfg), nw = fstep_cfg_wb(cfg);
+    if (g.Lc != 4 || !fstep2_cfg_supported(cfg) || (ph != 0 && ph != 1)) return hipErrorInvalidValue;
+    const bool split = (cfg & kFstepSplit) != 0;
     const int n = pair ? ninst / 2 : ninst;
     const int nzb = (g.ntz + 4 * tq - 1) / (4 * tq), nxb = (g.ntx + tx - 1) / tx;
-    const int nblocks = (n * nzb * nxb + 7) / 8 * 8;
-#define L2K(TXV, TQV)                                                                                                  \
+    const int4 fr = f2_fast_rect(g, tx, tq, nxb, nzb);
+    const int nfast = fr.y * fr.w, nframe = nzb * nxb - nfast;
+    const int b0 = (n * nzb * nxb + 7) / 8 * 8, b1 = (n * nfast + 7) / 8 * 8, b2 = (n * nframe + 7) / 8 * 8;
+#define L2K(TXV, TQV, NWV, PV, SV, NB)                                                                                 \
+    hipLaunchKernelGGL((k_fstep2<TXV, TQV, NWV, PV, SV>), dim3(NB), dim3(64 * NWV), 0, s, g, insts, n, nzb, nxb, fr, it, ph)
+#define L2P(TXV, TQV, NWV, PV)                                                                                         \
     {                                                                                                                  \
-        if (pair)                                                                                                      \
-            hipLaunchKernelGGL((k_fstep2<TXV, TQV, true>), dim3(nblocks), dim3(TXV * TQV), 0, s, g, insts, n, nzb, nxb, \
-                               it, ph);                                                                                \
-        else                                                                                                           \
-            hipLaunchKernelGGL((k_fstep2<TXV, TQV, false>), dim3(nblocks), dim3(TXV * TQV), 0, s, g, insts, n, nzb,    \
-                               nxb, it, ph);                                                                           \
+        if (!split) {                                                                                                  \
+            L2K(TXV, TQV, NWV, PV, 0, b0);                                                                             \
+        } else {                                                                                                       \
+            if (b1) L2K(TXV, TQV, NWV, PV, 1, b1);                                                                     \
+            if (b2) L2K(TXV, TQV, NWV, PV, 2, b2);                                                                     \
+        }                                                                                                              \
     }
-    switch (tq * 1000 + tx) {
-    case 16032: L2K(32, 16); break;
-    case 8064: L2K(64, 8); break;
-    case 16016: L2K(16, 16); break;
-    case 8032: L2K(32, 8); break;
+#define L2S(TXV, TQV, NWV)                                                                                             \
+    {                                                                                                                  \
+        if (pair) L2P(TXV, TQV, NWV, true) else L2P(TXV, TQV, NWV, false)                                              \
+    }
+    switch (nw * 100000 + tq * 1000 + tx) {
+    case 408032: L2S(32, 8, 4); break;
+    case 416016: L2S(16, 16, 4); break;
+    case 808064: L2S(64, 8, 8); break;
+    case 816032: L2S(32, 16, 8); break;
     default: return hipErrorInvalidValue;
     }
+#undef L2S
+#undef L2P
 #undef L2K
     return hipGetLastError();
 }

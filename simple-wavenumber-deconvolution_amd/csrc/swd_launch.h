@@ -25,15 +25,29 @@ constexpr int fstep_cfg_tx(int cfg) { return cfg & 0xff; }
 constexpr int fstep_cfg_tq(int cfg) { return (cfg >> 8) & 0xff; }
 constexpr int fstep_cfg_wb(int cfg) { return (cfg >> 16) & 0xff; }
 constexpr int kFstepDefault = fstep_cfg(16, 16, 0);
-// Two time steps per launch (swd_fstep2.h): bit 24 of a configuration.
+// Two time steps per launch (swd_fstep2.h): bit 24 of a configuration, whose
+// third field is then the wavefronts per workgroup (64 nw threads for tx tq
+// own quads).
 constexpr int kFstepTwo = 1 << 24;
-constexpr int fstep_cfg2(int tx, int tq) { return fstep_cfg(tx, tq, 0) | kFstepTwo; }
+// Bit 25 of a two-step configuration: the fast tiles (stencils free of damping
+// and frame cells) and the others in two launches of kernels compiled apart.
+constexpr int kFstepSplit = 1 << 25;
+constexpr int fstep_cfg2(int tx, int tq, int nw) { return fstep_cfg(tx, tq, nw) | kFstepTwo; }
 constexpr bool fstep_cfg_two(int cfg) { return (cfg & kFstepTwo) != 0; }
-constexpr int kFstep2Default = fstep_cfg2(32, 8);
-// Tile shapes of the two-step kernel (tx columns x 4 tq rows) and the
-// configurations the tuner times, the default first.
-bool fstep2_shape_supported(int tx, int tq);
+constexpr int kFstep2Default = fstep_cfg2(32, 8, 4);
+// Threads per workgroup of a one-step configuration (one own quad per thread, 128 to 512).
+constexpr int fstep_threads_of(int cfg)
+{
+    return fstep_cfg_tx(cfg) * fstep_cfg_tq(cfg) < 128   ? 128
+           : fstep_cfg_tx(cfg) * fstep_cfg_tq(cfg) > 512 ? 512
+                                                          : fstep_cfg_tx(cfg) * fstep_cfg_tq(cfg);
+}
+// The two-step kernel's configurations (tx columns x 4 tq rows, nw
+// wavefronts), the tuner's candidates (the default first) and the first
+// candidate of a tile shape (0: none).
+bool fstep2_cfg_supported(int cfg);
 std::vector<int> fstep2_candidates();
+int fstep2_cfg_of_shape(int tx, int tq);
 // Two steps (it, it+1 forward; it, it-1 for the paired backward) of every
 // instance in one launch, Lc 4, no capture, dp/dt output, angle gathers or
 // Born; ph: the launch's place in its pair (launches come in pairs, see

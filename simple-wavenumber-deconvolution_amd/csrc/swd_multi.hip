@@ -193,6 +193,7 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     P0.stats.tile_rows = 4 * fstep_cfg_tq(cfg);
     P0.stats.tile_waves = fstep_cfg_wb(cfg);
     P0.stats.steps_per_launch = 1;
+    P0.stats.tile_group_waves = fstep_threads_of(cfg) / 64;
     for (size_t gi = 0; gi < groups.size(); gi++) {
         const hipStream_t st = rtm_group_stream(P0, static_cast<int>(gi));
         const int first = gi == 0 ? it0 : 0;

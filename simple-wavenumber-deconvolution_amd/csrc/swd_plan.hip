@@ -306,7 +306,12 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     int cfg = P.tile_cfg;
     if (cfg != 0) {
         const int tx = fstep_cfg_tx(cfg), tq = fstep_cfg_tq(cfg);
-        if (only2) cfg = fstep2_shape_supported(tx, tq) ? fstep_cfg2(tx, tq) : kFstep2Default;
+        if (only2)
+            cfg = fstep2_cfg_supported(cfg | kFstepTwo) ? cfg | kFstepTwo
+                  : fstep2_cfg_of_shape(tx, tq)         ? fstep2_cfg_of_shape(tx, tq)
+                                                        : kFstep2Default;
+        else if (fstep_cfg_two(cfg))
+            cfg = fstep_cfg(tx, 16, 0);   // a split two-step configuration asked of a one-step loop
         else if (!fstep_cfg_supported(P.g.Lc, pair, cig, born, false, cfg))
             cfg = fstep_cfg(tx, 16, 0);   // the asked width with the variant's 64-row tiles
     }
@@ -317,7 +322,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         if (f != tuned.end()) cfg = f->second;
     }
     auto usable = [&](int c) {
-        return fstep_cfg_two(c) ? two && fstep2_shape_supported(fstep_cfg_tx(c), fstep_cfg_tq(c))
+        return fstep_cfg_two(c) ? two && fstep2_cfg_supported(c)
                                 : !only2 && fstep_cfg_supported(P.g.Lc, pair, cig, born, false, c);
     };
     if (cfg == 0) {
@@ -386,8 +391,9 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     }
     P.stats.tile_x = fstep_cfg_tx(cfg);
     P.stats.tile_rows = 4 * fstep_cfg_tq(cfg);
-    P.stats.tile_waves = fstep_cfg_wb(cfg);
+    P.stats.tile_waves = fstep_cfg_two(cfg) ? 0 : fstep_cfg_wb(cfg);
     P.stats.steps_per_launch = fstep_cfg_two(cfg) ? 2 : 1;
+    P.stats.tile_group_waves = fstep_cfg_two(cfg) ? fstep_cfg_wb(cfg) : fstep_threads_of(cfg) / 64;
     // the rest: with two steps per launch, the remainder modulo 4 one step at a time, then pairs
     auto rest = [&]() -> int {
         if (fstep_cfg_two(cfg)) {
@@ -776,10 +782,13 @@ extern "C" int swd_plan_set_tile_config(void *plan, int tile_x, int tile_rows, i
     Plan *P = static_cast<Plan *>(plan);
     int cfg = 0;
     if (tile_x != 0 || tile_rows != 0 || waves != 0) {
-        cfg = fstep_cfg(tile_x, tile_rows / 4, waves);
-        if (tile_rows % 4 != 0 || !fstep_cfg_supported(P->g.Lc, false, false, false, false, cfg))
+        // two-step shapes: waves is the wavefronts per workgroup, negative for split launches
+        const int c2 = fstep_cfg2(tile_x, tile_rows / 4, waves < 0 ? -waves : waves) | (waves < 0 ? kFstepSplit : 0);
+        const bool two = P->g.Lc == 4 && tile_x > 0 && tile_rows > 0 && fstep2_cfg_supported(c2);
+        cfg = two && waves < 0 ? c2 : fstep_cfg(tile_x, tile_rows / 4, waves);
+        if (tile_rows % 4 != 0 || (!two && !fstep_cfg_supported(P->g.Lc, false, false, false, false, cfg)))
             return swd_fail(SWD_E_ARG,
-                            "swd_plan_set_tile_config: %d x %d rows, %d wavefronts per SIMD is not a configuration of "
+                            "swd_plan_set_tile_config: %d x %d rows, %d wavefronts is not a configuration of "
                             "the fused step at Lc %d",
                             tile_x, tile_rows, waves, P->g.Lc);
     }

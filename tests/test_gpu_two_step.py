@@ -18,14 +18,15 @@ from helpers import oracle_problem, rel_l2, report
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
-SHAPES = [(32, 64), (64, 32), (16, 64), (32, 32)]   # columns x rows of the two-step kernel
+# configurations of the two-step kernel: columns, rows, wavefronts per workgroup
+SHAPES = [(32, 32, 4), (32, 32, -4), (64, 32, -8), (64, 32, 8), (16, 64, 4), (32, 64, -8)]   # negative: split launches
 
 
 def plan_of(c, steps, shape=None):
     p = swd.Plan(c.grid, c.coefs, c.vp, c.rho, path=swd.SWD_PATH_FUSED)
     p.set_steps_per_launch(steps)
     if shape:
-        p.set_tile_config(shape[0], shape[1], 0)
+        p.set_tile_config(*shape)
     else:
         p.set_tile_config(16, 64, 0)
     return p
@@ -40,7 +41,10 @@ def run_both(c, shape, nshots):
     for steps in (1, 2):
         plan = plan_of(c, steps, shape if steps == 2 else None)
         g = plan.forward(c.shots, c.wavelet)
-        assert plan.stats()["steps_per_launch"] == steps
+        st = plan.stats()
+        assert st["steps_per_launch"] == steps
+        if shape and steps == 2:
+            assert (st["tile_x"], st["tile_rows"], st["tile_group_waves"]) == shape[:2] + (abs(shape[2]),)
         r = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
         st = plan.stats()
         assert st["steps_per_launch"] == steps
@@ -97,7 +101,7 @@ def test_two_step_c2_grid():
     one of them in the receivers' column."""
     c = configs.c2(nt=161, shot_ix=(300, 420))
     print("case two-step c2 grid nt=161", flush=True)
-    check(c, (32, 64), 2, "c2")
+    check(c, (32, 32, 4), 2, "c2")
 
 
 def test_two_step_tuner_and_groups():
