@@ -145,15 +145,26 @@ def cpu_baseline(case, nt_sample, runs=5):
             "cpu": cpus, "runs_s": times}
 
 
-def backward_launch_ms(swd, torch, case, adj, nshots, group, streams, nt, tile):
+def launch_config(st):
+    """(steps per launch, tile columns, rows, waves argument) of a plan's last fused loop, as
+    swd_plan_set_steps_per_launch and swd_plan_set_tile_config take them."""
+    two = st["steps_per_launch"] == 2
+    return (st["steps_per_launch"], st["tile_x"], st["tile_rows"], st["tile_group_waves"] if two else st["tile_waves"])
+
+
+def backward_launch_ms(swd, torch, case, adj, nshots, group, streams, nt, tile, cfg=None):
     """HIP-event time per backward launch of an RTM of the first nshots shots of
     case cut to nt steps, in groups of `group` shots on `streams` streams: the
     plan's backward-phase events (on its stream) over its launches, on a second
-    call (the first tunes the tile and captures the graphs)."""
+    call (the first tunes the tile and captures the graphs), and the launches
+    per step.  cfg (launch_config) pins the launch configuration."""
     g = case.grid
     sub = swd.configs.Case(case.name, swd.Grid(g.nx, g.nz, g.pml, g.Lc, g.dx, g.dz, g.dt, nt), case.vp, case.rho,
                            case.coefs, case.wavelet[:nt], case.shots[:nshots], case.f0)
     plan = swd.Plan(sub.grid, sub.coefs, sub.vp, sub.rho, device=0, tile=tile)
+    if cfg is not None:
+        plan.set_steps_per_launch(cfg[0])
+        plan.set_tile_config(*cfg[1:])
     plan.set_groups(group, group)
     plan.set_streams(streams)
     shots = swd.ShotSet(sub.shots)
@@ -1086,6 +1097,8 @@ def main():
     ap.add_argument("--nt-iso", type=int, default=400,
                     help="time steps of the isolated and HBM-resident backward-launch measurements")
     ap.add_argument("--tile", type=int, default=0, help="fused tile width (0 = tuned on the first steps)")
+    ap.add_argument("--steps-per-launch", type=int, default=0, choices=(0, 1, 2),
+                    help="time steps per fused launch of the RTM loops (0 = the tuner picks one or two)")
     ap.add_argument("--path", choices=("fused", "batched"), default="fused")
     ap.add_argument("--groups", type=int, nargs=2, default=(0, 0), metavar=("FWD", "BWD"),
                     help="shots per fused RTM forward/backward loop (0 = sized to the cache)")
@@ -1150,6 +1163,7 @@ def main():
     plan = swd.Plan(g, case.coefs, case.vp, case.rho, device=0, path=path, tile=a.tile)
     plan.set_groups(*a.groups)
     plan.set_streams(a.streams)
+    plan.set_steps_per_launch(a.steps_per_launch)
     shots = swd.ShotSet(case.shots)
     # observed data on the true model, up-going integration as the adjoint source (RC:603-637)
     plan_true = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0)
@@ -1203,8 +1217,14 @@ def main():
     nframe = 2 * g.Lc * (g.nx + g.nz - 2 * g.Lc)
     bf, bb, mf, mb = rtm_bytes(g, nrec, nframe)
     st = plan.stats()
+    cfg_b = launch_config(st)
+    spl = st["steps_per_launch"]
+    # launches per step: the groups of the batch / the steps a launch advances (a two-step loop runs its first
+    # step and the remainder of its step count modulo 4 one step per launch); per launch the algorithmic bytes
+    # are the per-step bytes of its shots times the steps it advances, plus the model once
     bwd_launches_per_step = st["bwd_launches"] / (g.nt - 1)
-    group = -(-a.batch // int(round(bwd_launches_per_step)))   # the largest backward group of the run
+    ngroups = max(1, int(round(bwd_launches_per_step * spl)))
+    group = -(-a.batch // ngroups)   # the largest backward group of the run
     # (1) overlapped: the timed region's backward phase over its launches (groups on a.streams streams)
     ov_ms = bwd_ms / a.steps / max(1, st["bwd_launches"])
     ov_bytes = (bb * a.batch + mb * bwd_launches_per_step) / bwd_launches_per_step
@@ -1213,15 +1233,15 @@ def main():
     fwd_launches_per_step = st["fwd_launches"] / g.nt
     fwd_achieved = (bf * a.batch + mf * fwd_launches_per_step) / fwd_launches_per_step / (fwd_launch_ms * 1e-3) / 1e9
     # (2) isolated: one group of that size alone on one stream, graph-replayed launches back to back
-    iso_ms, _ = backward_launch_ms(swd, torch, case, adj, group, group, 1, a.nt_iso, a.tile)
-    iso_bytes = bb * group + mb
+    iso_ms, iso_lps = backward_launch_ms(swd, torch, case, adj, group, group, 1, a.nt_iso, a.tile, cfg_b)
+    iso_bytes = (bb * group + mb * iso_lps) / iso_lps
     iso_achieved = iso_bytes / (iso_ms * 1e-3) / 1e9
     log(f"bench: isolated {group}-shot backward launch {iso_ms * 1e3:.2f} us, {iso_achieved:.0f} GB/s")
     # (3) HBM-resident: every shot of the batch in one launch on one stream, a working set twice the
     # last-level cache (32 MB a shot at c2)
-    res_ms, _ = backward_launch_ms(swd, torch, case, adj, a.batch, a.batch, 1, a.nt_iso, a.tile)
+    res_ms, res_lps = backward_launch_ms(swd, torch, case, adj, a.batch, a.batch, 1, a.nt_iso, a.tile, cfg_b)
     res_set = 4.0 * (9 * g.nx * g.nz + 6 * g.ntx * g.ntz + 6 * (g.ntx * g.ntz - g.nx * g.nz)) * a.batch
-    res_bytes = bb * a.batch + mb
+    res_bytes = (bb * a.batch + mb * res_lps) / res_lps
     res_achieved = res_bytes / (res_ms * 1e-3) / 1e9
     log(f"bench: {a.batch}-shot backward launch ({res_set / 2**20:.0f} MiB working set) {res_ms * 1e3:.1f} us, "
         f"{res_achieved:.0f} GB/s")
@@ -1250,11 +1270,19 @@ def main():
         log(f"bench: cpu baseline {cpu['value']:.3f} Gcell/s ({cpu['sample']})")
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "rtm_fused_c2_traffic.json")
+    tname = "rtm_fused2_c2_traffic.json" if spl == 2 else "rtm_fused_c2_traffic.json"
+    tfile = os.path.join(ROOT, "profiles", tname)
     if a.path == "fused" and os.path.exists(tfile) and (g.nx, g.nz, g.pml, g.Lc) == (845, 502, 40, 4):
         with open(tfile) as f:
             t = json.load(f)
-        traffic = t["l2_miss_bytes_per_shot_step"]["backward"] * group
+        traffic = t["l2_miss_bytes_per_shot_step"]["backward"] * group / iso_lps   # per launch
+    if spl == 2:
+        kname = (f"k_fstep2 paired backward (reconstruction + receiver + imaging), two steps per launch, "
+                 f"{cfg_b[1]} x {cfg_b[2]}-row tiles, {abs(cfg_b[3])} wavefronts"
+                 f"{' in split launches' if cfg_b[3] < 0 else ''}, {group}-shot group alone on one stream")
+    else:
+        kname = (f"k_fstep paired backward step (reconstruction + receiver + imaging), {group}-shot "
+                 "group alone on one stream")
 
     out = {
         "metric": "Gcell-updates/s and batched shots/s vs plain GPU path at <=1e-5 rel L2",
@@ -1276,16 +1304,20 @@ def main():
                    "plain_path": plain,
                    "speedup_vs_plain_shots_per_s": (shots_per_s / plain["shots_per_s"]) if plain else None,
                    "fwd_ms_per_step": fwd_ms / a.steps, "bwd_ms_per_step": bwd_ms / a.steps,
-                   "fwd_achieved_GBs": fwd_achieved, "image_finite": finite},
+                   "fwd_achieved_GBs": fwd_achieved, "image_finite": finite,
+                   "backward_launch": dict(zip(("steps_per_launch", "tile_x", "tile_rows", "waves"), cfg_b)),
+                   "forward_launches_per_step": st["fwd_launches"] / g.nt},
         "roofline": {"bound": "hbm", "achieved": iso_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": iso_achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_note": "FETCH_SIZE x2 + WRITE_SIZE per launch (profiles/rtm_fused_c2_traffic.json): "
+                     "traffic_note": f"FETCH_SIZE x2 + WRITE_SIZE per launch (profiles/{tname}): "
                                      "L2-miss bytes, which include last-level-cache hits, so an upper bound on "
                                      "HBM bytes" if traffic else None,
                      "copy_GBs": copy_gbs, "frac_of_copy": iso_achieved / copy_gbs,
-                     "kernel": (f"k_fstep paired backward step (reconstruction + receiver + imaging), {group}-shot "
-                                "group alone on one stream") if a.path == "fused" else
-                               "RTM backward step (4 sub-step kernels)",
+                     "kernel": kname if a.path == "fused" else "RTM backward step (4 sub-step kernels)",
+                     "steps_per_launch": 1.0 / iso_lps,
+                     "bytes_note": "algorithmic bytes of a shot-step times the steps a launch advances, the "
+                                   "model once per launch (a two-step launch reads and writes the state once "
+                                   "for its two steps)",
                      "bytes_per_launch": iso_bytes, "avg_launch_ms": iso_ms,
                      "timing": f"HIP events on the plan's stream around {a.nt_iso - 1} graph-replayed launches of one "
                                f"{group}-shot group with no other work in flight: launch duration plus launch gap",

@@ -101,7 +101,7 @@ typedef struct swd_stats {
     long long waves;        /* shot waves of the last call (see swd_plan_set_memory_limit) */
     int tile_x, tile_rows, tile_waves; /* tile of the last fused loop: columns, rows, wavefronts-per-SIMD bound (0: none) */
     int steps_per_launch;   /* time steps per launch of the last fused loop (1 or 2) */
-    int tile_group_waves;   /* wavefronts per workgroup of the last fused loop */
+    int tile_group_waves;   /* wavefronts per workgroup of the last fused loop (negative: two-step split launches) */
 } swd_stats;
 
 /* ---------------- errors and devices ---------------- */

@@ -521,13 +521,16 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
     int ocm = -1;
     if (!SRC && I.colmap && oin) ocm = ldgi(I.colmap, ox);
     float4 k0 = f2zero(), im = f2zero(), is = f2zero(), ir = f2zero();
-    if (SRC && oin) k0 = ld4((gcfl *)g.K0, off(ox, oz));
     const bool doimg = NEG && I.image && oin && ocolint;
-    if (doimg) {
-        im = ld4((gcfl *)I.image, off(ox, oz));
-        is = ld4((gcfl *)I.illum_s, off(ox, oz));
-        ir = ld4((gcfl *)I.illum_r, off(ox, oz));
-    }
+    auto own_loads = [&]() {
+        if (SRC && oin) k0 = ld4((gcfl *)g.K0, off(ox, oz));
+        if (doimg) {
+            im = ld4((gcfl *)I.image, off(ox, oz));
+            is = ld4((gcfl *)I.illum_s, off(ox, oz));
+            ir = ld4((gcfl *)I.illum_r, off(ox, oz));
+        }
+    };
+
 #pragma unroll
     for (int r = 0; r < G::RP0; r++) {
         const int i = tid + NT * r;
@@ -578,13 +581,17 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
         sVZ[G::sz(c, q)] = v;
     }
     __syncthreads();
+    // the own quad's K0 and image accumulators, first needed by the pressure loop below (which
+    // takes the own quad last), loaded here rather than with the others to keep registers free
+    own_loads();
 
     // ---- step 1, pressure (level t+1) on the P1 region
     float4 p1 = f2zero(), qx1 = f2zero(), qz1 = f2zero();
 #pragma unroll
-    for (int r = 0; r < G::RP; r++) {
+    for (int rr = 0; rr < G::RP; rr++) {
+        const int r = G::RP - 1 - rr;   // the own quad (round 0) last
         const int i = tid + NT * r;
-        if (!((r + 1) * NT <= G::NP || i < G::NP)) break;
+        if (!((r + 1) * NT <= G::NP || i < G::NP)) continue;
         int c, q;
         f2_p<TX, TQ>(i, c, q);
         const int x = X0 + c, z = Z0 + 4 * q;
@@ -633,7 +640,8 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
         if (!inb(x, z)) continue;
         float4 psn = f2zero();
         float ah = 0.f;
-        const float4 v = f2_vx_quad<FULL, MODE, PNQ>(g, x, z, &sP[G::sp(c + 1, q)], vxv[r], bxv[r],
+        const float4 bx2 = ld4((gcfl *)g.Bx, off(x, z));   // reloaded (L2): fewer registers held across the steps
+        const float4 v = f2_vx_quad<FULL, MODE, PNQ>(g, x, z, &sP[G::sp(c + 1, q)], vxv[r], bx2,
                                                      CPML ? psxv[r] : f2zero(), psn, ah);
         sVX[G::sx(c, q)] = v;
         if (c >= 0 && c < TX) {
@@ -658,7 +666,8 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
         f2_window(&sP[G::sp(c, q - 1)], a);
         float4 psn = f2zero();
         bool anyps = false;
-        const float4 v = f2_vz_quad<FULL, MODE>(g, x, z, a, vzv[r], bzv[r], CPML ? pszv[r] : f2zero(), psn, anyps);
+        const float4 bz2 = ld4((gcfl *)g.Bz, off(x, z));
+        const float4 v = f2_vz_quad<FULL, MODE>(g, x, z, a, vzv[r], bz2, CPML ? pszv[r] : f2zero(), psn, anyps);
         sVZ[G::sz(c, q)] = v;
         if (q >= 0 && q < TQ) {
             if (SRC) {
@@ -720,7 +729,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
 #define SWD_F2_WAVES_FAST 4
 #endif
 #ifndef SWD_F2_WAVES_FRAME
-#define SWD_F2_WAVES_FRAME 1
+#define SWD_F2_WAVES_FRAME 4   // the paired kernel over every tile: spills a few dwords, 6% faster than three
 #endif
 template <int TX, int TQ, int NW, bool PAIR, int SEL>
 __global__ __launch_bounds__(64 * NW, SEL == 1 ? (PAIR ? SWD_F2_WAVES_FAST : SWD_F2_WAVES_FWD) : (PAIR ? SWD_F2_WAVES_FRAME : SWD_F2_WAVES_FWD)) void k_fstep2(

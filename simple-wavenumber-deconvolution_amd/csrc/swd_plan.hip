@@ -393,7 +393,9 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     P.stats.tile_rows = 4 * fstep_cfg_tq(cfg);
     P.stats.tile_waves = fstep_cfg_two(cfg) ? 0 : fstep_cfg_wb(cfg);
     P.stats.steps_per_launch = fstep_cfg_two(cfg) ? 2 : 1;
-    P.stats.tile_group_waves = fstep_cfg_two(cfg) ? fstep_cfg_wb(cfg) : fstep_threads_of(cfg) / 64;
+    P.stats.tile_group_waves = !fstep_cfg_two(cfg)      ? fstep_threads_of(cfg) / 64
+                               : (cfg & kFstepSplit) ? -fstep_cfg_wb(cfg)
+                                                     : fstep_cfg_wb(cfg);
     // the rest: with two steps per launch, the remainder modulo 4 one step at a time, then pairs
     auto rest = [&]() -> int {
         if (fstep_cfg_two(cfg)) {

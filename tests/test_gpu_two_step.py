@@ -44,7 +44,7 @@ def run_both(c, shape, nshots):
         st = plan.stats()
         assert st["steps_per_launch"] == steps
         if shape and steps == 2:
-            assert (st["tile_x"], st["tile_rows"], st["tile_group_waves"]) == shape[:2] + (abs(shape[2]),)
+            assert (st["tile_x"], st["tile_rows"], st["tile_group_waves"]) == shape
         r = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
         st = plan.stats()
         assert st["steps_per_launch"] == steps
@@ -130,3 +130,29 @@ def test_two_step_tuner_and_groups():
     with pytest.raises(swd.SwdError):
         plan.set_steps_per_launch(3)
     plan.close()
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_two_step_frame_sources_and_receivers(shape):
+    """The AVSPRTM driver test's geometry (80 x 60 interior, pml 10): three shots
+    with their sources on the interior frame's top row, receivers on the left
+    and right frame columns, a constant adjoint source; bit for bit against one
+    step per launch.  (A register-bounded build of the 16 x 64 configuration
+    that spilled gave wrong source illuminations here.)"""
+    nx, nz, pml, nt = 80, 60, 10, 41
+    c = configs.small(nx=nx, nz=nz, pml=pml, Lc=4, nt=nt, nshots=3)
+    pm = c.pmlc
+    rz = np.arange(pm, pm + nz, 2)
+    rx = np.where(np.arange(rz.size) < rz.size // 2, pm, nx + pm - 1)
+    c.shots = [(pm + 10 + 20 * k, pm, rx, rz) for k in range(3)]
+    adj = np.full((3 * rz.size, nt), 1e-3, np.float32)
+    print(f"case two-step frame geometry {shape}", flush=True)
+    res = {}
+    for steps in (1, 2):
+        plan = plan_of(c, steps, shape if steps == 2 else None)
+        res[steps] = plan.rtm(c.shots, c.wavelet, adj, want_gather=True)
+        assert plan.stats()["steps_per_launch"] == steps
+        plan.close()
+    for name, a, b in zip(("image", "illum_src", "illum_rcv", "gather"), res[1], res[2]):
+        assert np.isfinite(b).all(), name
+        assert np.array_equal(a, b), name
