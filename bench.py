@@ -1306,7 +1306,8 @@ def main():
                    "fwd_ms_per_step": fwd_ms / a.steps, "bwd_ms_per_step": bwd_ms / a.steps,
                    "fwd_achieved_GBs": fwd_achieved, "image_finite": finite,
                    "backward_launch": dict(zip(("steps_per_launch", "tile_x", "tile_rows", "waves"), cfg_b)),
-                   "forward_launches_per_step": st["fwd_launches"] / g.nt},
+                   "forward_launch": {"steps_per_launch": st["fwd_steps_per_launch"], "tile_x": st["fwd_tile_x"],
+                                      "tile_rows": st["fwd_tile_rows"], "waves": st["fwd_tile_group_waves"]}},
         "roofline": {"bound": "hbm", "achieved": iso_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": iso_achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_note": f"FETCH_SIZE x2 + WRITE_SIZE per launch (profiles/{tname}): "

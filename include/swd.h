@@ -102,6 +102,7 @@ typedef struct swd_stats {
     int tile_x, tile_rows, tile_waves; /* tile of the last fused loop: columns, rows, wavefronts-per-SIMD bound (0: none) */
     int steps_per_launch;   /* time steps per launch of the last fused loop (1 or 2) */
     int tile_group_waves;   /* wavefronts per workgroup of the last fused loop (negative: two-step split launches) */
+    int fwd_tile_x, fwd_tile_rows, fwd_tile_group_waves, fwd_steps_per_launch; /* the same of the last fused forward loop */
 } swd_stats;
 
 /* ---------------- errors and devices ---------------- */

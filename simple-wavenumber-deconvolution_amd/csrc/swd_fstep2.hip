@@ -33,7 +33,10 @@ hipError_t launch_fstep2(const KGrid &g, const KStep *insts, int ninst, int it, 
     const int nzb = (g.ntz + 4 * tq - 1) / (4 * tq), nxb = (g.ntx + tx - 1) / tx;
     const int4 fr = f2_fast_rect(g, tx, tq, nxb, nzb);
     const int nfast = fr.y * fr.w, nframe = nzb * nxb - nfast;
-    const int b0 = (n * nzb * nxb + 7) / 8 * 8, b1 = (n * nfast + 7) / 8 * 8, b2 = (n * nframe + 7) / 8 * 8;
+    // single launch: the forward gives each XCD a share of the frame tiles and one of the fast
+    // tiles (k_fstep2, SEL 0), the paired backward whole tile columns
+    const int b0 = pair ? (n * nzb * nxb + 7) / 8 * 8 : 8 * ((n * nframe + 7) / 8 + (n * nfast + 7) / 8);
+    const int b1 = (n * nfast + 7) / 8 * 8, b2 = (n * nframe + 7) / 8 * 8;
 #define L2K(TXV, TQV, NWV, PV, SV, NB)                                                                                 \
     hipLaunchKernelGGL((k_fstep2<TXV, TQV, NWV, PV, SV>), dim3(NB), dim3(64 * NWV), 0, s, g, insts, n, nzb, nxb, fr, it, ph)
 #define L2P(TXV, TQV, NWV, PV)                                                                                         \

@@ -396,6 +396,12 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     P.stats.tile_group_waves = !fstep_cfg_two(cfg)      ? fstep_threads_of(cfg) / 64
                                : (cfg & kFstepSplit) ? -fstep_cfg_wb(cfg)
                                                      : fstep_cfg_wb(cfg);
+    if (!pair) {
+        P.stats.fwd_tile_x = P.stats.tile_x;
+        P.stats.fwd_tile_rows = P.stats.tile_rows;
+        P.stats.fwd_tile_group_waves = P.stats.tile_group_waves;
+        P.stats.fwd_steps_per_launch = P.stats.steps_per_launch;
+    }
     // the rest: with two steps per launch, the remainder modulo 4 one step at a time, then pairs
     auto rest = [&]() -> int {
         if (fstep_cfg_two(cfg)) {

@@ -68,7 +68,9 @@ class Stats(C.Structure):
                 ("fwd_cells", C.c_longlong), ("bwd_cells", C.c_longlong),
                 ("ws_bytes", C.c_longlong), ("ckpt_bytes", C.c_longlong), ("full_bytes", C.c_longlong),
                 ("waves", C.c_longlong), ("tile_x", C.c_int), ("tile_rows", C.c_int), ("tile_waves", C.c_int),
-                ("steps_per_launch", C.c_int), ("tile_group_waves", C.c_int)]
+                ("steps_per_launch", C.c_int), ("tile_group_waves", C.c_int),
+                ("fwd_tile_x", C.c_int), ("fwd_tile_rows", C.c_int), ("fwd_tile_group_waves", C.c_int),
+                ("fwd_steps_per_launch", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
