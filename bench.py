@@ -1008,8 +1008,10 @@ def bench_c5(a):
     per_phase["gradient"]["ckpt_GB"] = stats["gradient"]["ckpt_bytes"] / 1e9
     per_phase["gradient"]["full_storage_GB"] = stats["gradient"]["full_bytes"] / 1e9
     # the dominant kernel (the paired RTM backward step, one-shot groups at c5) alone on one stream
-    iso_ms, _ = backward_launch_ms(swd, torch, case, adj, 1, 1, 1, a.nt_iso, a.tile)
-    iso = (bb + mb) / (iso_ms * 1e-3) / 1e9
+    cfg_b = launch_config(stats["rtm"])
+    iso_ms, iso_lps = backward_launch_ms(swd, torch, case, adj, 1, 1, 1, a.nt_iso, a.tile, cfg_b)
+    iso_bytes = (bb + mb * iso_lps) / iso_lps   # a launch advances 1 / iso_lps steps; the model once
+    iso = iso_bytes / (iso_ms * 1e-3) / 1e9
     log(f"bench: isolated 1-shot backward launch {iso_ms * 1e3:.1f} us, {iso:.0f} GB/s")
     plain = None
     if not a.no_plain:
@@ -1076,8 +1078,12 @@ def bench_c5(a):
                    "grad_finite": bool(torch.isfinite(d_grad).all().item())},
         "roofline": {"bound": "hbm", "achieved": iso, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": iso / HBM_PEAK_GBS,
                      "traffic": None, "copy_GBs": copy_gbs, "frac_of_copy": iso / copy_gbs,
-                     "kernel": "k_fstep paired RTM backward step, one-shot group alone on one stream",
-                     "bytes_per_launch": bb + mb, "avg_launch_ms": iso_ms,
+                     "kernel": (f"{'k_fstep2' if cfg_b[0] == 2 else 'k_fstep'} paired RTM backward "
+                                f"({cfg_b[0]} step(s) per launch, {cfg_b[1]} x {cfg_b[2]}-row tiles), one-shot group "
+                                "alone on one stream"),
+                     "steps_per_launch": 1.0 / iso_lps, "rtm_backward_launch": dict(zip(
+                         ("steps_per_launch", "tile_x", "tile_rows", "waves"), cfg_b)),
+                     "bytes_per_launch": iso_bytes, "avg_launch_ms": iso_ms,
                      "timing": f"HIP events on the plan's stream around {a.nt_iso - 1} graph-replayed launches"},
         "cpu_baseline": cpu,
     }

@@ -181,7 +181,7 @@ int afor_dev(Plan *P, int nshots, const swd_shot *shots, const float *wavelet, c
                 }
             }
         SWD_TRY(upload_steps(*P, hs, o_inst));
-        SWD_TRY(fused_forward_loop(*P, at<KStep>(*P, o_inst), 2 * nshots, false));
+        SWD_TRY(fused_forward_loop(*P, at<KStep>(*P, o_inst), 2 * nshots, true));
     }
     std::vector<KInst> h(fused ? 0 : 2 * nshots);
     for (int k = 0; !fused && k < nshots; k++)

@@ -31,11 +31,11 @@
 // 0, 1): after a pair, level L is again in slot L & 1 and the pressure CPML
 // memories in pvx, pvz, as the one-step kernel expects.
 //
-// Modes: FWD (forward with source, receivers, strips), and the paired RTM
-// backward: SRC (reconstruction with wavelet subtraction and frame restore)
-// then RCV (receiver back-propagation with hard replacement and the imaging
-// condition).  Not here: AFor capture, dp/dt output, angle gathers, Born, the
-// first reconstruction step (which keeps the forward's velocities).
+// Modes: FWD (forward with source, receivers, strips, AFor column capture),
+// and the paired RTM backward: SRC (reconstruction with wavelet subtraction
+// and frame restore) then RCV (receiver back-propagation with hard replacement
+// and the imaging condition).  Not here: dp/dt output, angle gathers, Born,
+// the first reconstruction step (which keeps the forward's velocities).
 #pragma once
 #include <algorithm>
 
@@ -290,11 +290,13 @@ __device__ __forceinline__ float4 f2_vz_quad(const KGrid &g, int x, int z, const
 // source; RCV: the hard replacement (rec: whether the receivers of this cell
 // are replaced here).  qxo/qzo in, qxn/qzn out: the CPML memories (frame
 // tiles).  Returns false where the reconstruction leaves the quad unchanged.
+// srcnear: the source may lie in the quad (a workgroup-uniform test of the
+// tile's region; false spares the per-quad comparison).
 template <bool FULL, int MODE, int XNQ>
 __device__ __forceinline__ bool f2_p_quad(const KGrid &g, const KStep &I, int x, int z, int lv, const float4 *vxa,
                                           const float (&vzw)[12], float4 pold, float4 K, float4 K0, float4 qxo,
                                           float4 qzo, float4 &pv, float4 &dp, float4 &qxn, float4 &qzn, float &axo,
-                                          bool &anyz)
+                                          bool &anyz, bool srcnear)
 {
     constexpr int LC = 4;
     pv = pold;
@@ -306,7 +308,7 @@ __device__ __forceinline__ bool f2_p_quad(const KGrid &g, const KStep &I, int x,
     if (MODE == KSTEP_SRC) {
         const bool colint = FULL || (x >= g.pmlc && x < g.ntx - g.pmlc);
         const bool rowint = FULL || (z + 3 >= g.pmlc && z < g.ntz - g.pmlc);
-        const bool hassrc = x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4;
+        const bool hassrc = srcnear && x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4;
         if (!(colint && rowint) && !hassrc) return false;
         float rst[4];
         int fr[4];
@@ -386,7 +388,8 @@ __device__ __forceinline__ bool f2_p_quad(const KGrid &g, const KStep &I, int x,
             }
         }
     }
-    if (!NEG && I.wavelet && x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4) f2_add_at(pv, I.s_iz - z, ldg(I.wavelet, lv));   // RK:175-178
+    if (!NEG && srcnear && I.wavelet && x == I.s_ix && I.s_iz >= z && I.s_iz < z + 4)
+        f2_add_at(pv, I.s_iz - z, ldg(I.wavelet, lv));   // RK:175-178
     return true;
 }
 
@@ -406,6 +409,19 @@ __device__ __forceinline__ void f2_receivers(const KGrid &g, const KStep &I, int
             else if (I.gather) ((gfl *)I.gather)[go] = elc(pv, k);
         }
     }
+}
+
+// AFor column capture of one own quad at level lv (FK:324-327, 529-532): the pressure of a
+// capture column into its kz-transform row, real or imaginary part by the instance.
+template <bool FULL>
+__device__ __forceinline__ void f2_capture(const KGrid &g, const KStep &I, int x, int z, int lv, float4 pv)
+{
+    const int cc = ldgi(I.colcap, x);
+    if (cc < 0) return;
+    gfl *dst = (gfl *)reinterpret_cast<float *>(I.capture + (static_cast<size_t>(lv) * I.ncap + cc) * I.nfftz + z);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (FULL || z + k < g.ntz) dst[2 * k + I.capcomp] = elc(pv, k);
 }
 
 // Imaging condition of one own quad at one level (RK:785-817).
@@ -435,7 +451,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
     using G = T2<TX, TQ, NW>;
     constexpr int NT = G::NT, PNQ = G::PNQ, XNQ = G::XNQ, ZNQ = G::ZNQ;
     constexpr bool SRC = MODE == KSTEP_SRC, NEG = MODE == KSTEP_RCV, CPML = !FULL && !SRC;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane0 = tid & 63;
     const int ntx = g.ntx, ntz = g.ntz;
     const unsigned ldz = static_cast<unsigned>(g.ldz);
     const int dir = MODE == KSTEP_FWD ? 1 : -1;
@@ -520,6 +536,18 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
     const bool ocolint = FULL || (ox >= g.pmlc && ox < ntx - g.pmlc);
     int ocm = -1;
     if (!SRC && I.colmap && oin) ocm = ldgi(I.colmap, ox);
+    // whether the source or a receiver column lies in the pressure region (columns -7 .. TX+6, rows
+    // -8 .. 4TQ+7): uniform tests that spare the others the per-quad checks
+    const bool srcnear = I.s_ix >= X0 - 7 && I.s_ix < X0 + TX + 7 && I.s_iz >= Z0 - 8 && I.s_iz < Z0 + 4 * TQ + 8;
+    bool rcvnear = false;
+    if (NEG && I.colmap) {
+        bool hit = false;
+        for (int j = lane0; j < TX + 14; j += 64) {
+            const int xc = X0 - 7 + j;
+            if (xc >= 0 && xc < ntx) hit |= ldgi(I.colmap, xc) >= 0;
+        }
+        rcvnear = __ballot(hit) != 0;
+    }
     float4 k0 = f2zero(), im = f2zero(), is = f2zero(), ir = f2zero();
     const bool doimg = NEG && I.image && oin && ocolint;
     auto own_loads = [&]() {
@@ -607,7 +635,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
             if (z < g.fz0 || z + 3 > g.fz1) qzo = ld4(PVZr, off(x, z));
         }
         const bool wr = f2_p_quad<FULL, MODE, XNQ>(g, I, x, z, lv1, &sVX[G::sx(c, q)], w, pold, kv[r], k0, qxo, qzo, pv,
-                                                   dp, qxn, qzn, axo, anyz);
+                                                   dp, qxn, qzn, axo, anyz, srcnear);
         if (i < G::PCORE) {   // the own quad (core = item tid)
             if (SRC) dps[0] = dp;
             if (!SRC) f2_receivers<FULL, MODE>(g, I, ocm, z, lv1, pv);
@@ -619,10 +647,11 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
                     if (f >= 0) ((gfl *)I.strips)[static_cast<size_t>(lv1) * g.nframe + f] = elc(pv, k);   // RK:200-219
                 }
             }
+            if (MODE == KSTEP_FWD && I.capture) f2_capture<FULL>(g, I, x, z, lv1, pv);
             p1 = pv;
             qx1 = qxn;
             qz1 = qzn;
-        } else if (NEG && I.colmap) {
+        } else if (NEG && rcvnear) {
             f2_receivers<FULL, MODE>(g, I, ldgi(I.colmap, x), z, lv1, pv);
         }
         if (wr) sP[G::sp(c, q)] = pv;
@@ -690,7 +719,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
         float axo;
         bool anyz;
         const bool wr = f2_p_quad<FULL, MODE, XNQ>(g, I, x, z, lv2, &sVX[G::sx(oc, oq)], w, p1, kv[0], k0, qx1, qz1,
-                                                   pv, dp, qxn, qzn, axo, anyz);
+                                                   pv, dp, qxn, qzn, axo, anyz, srcnear);
         if (SRC) dps[1] = dp;
         if (!SRC) f2_receivers<FULL, MODE>(g, I, ocm, z, lv2, pv);
         if (wr) st4(Pw, off(x, z), pv);
@@ -705,6 +734,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
                 if (f >= 0) ((gfl *)I.strips)[static_cast<size_t>(lv2) * g.nframe + f] = elc(pv, k);
             }
         }
+        if (MODE == KSTEP_FWD && I.capture) f2_capture<FULL>(g, I, x, z, lv2, pv);
         if (doimg) {
             f2_image<FULL>(g, z, dps[1], pv, im, is, ir);
             st4((gfl *)I.image, off(x, z), im);

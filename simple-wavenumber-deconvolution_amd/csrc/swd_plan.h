@@ -231,8 +231,8 @@ int upload_steps(Plan &P, const std::vector<KStep> &h, size_t o);
 // it is tuned here: each candidate runs one warm-up unit, then kTry timed
 // units of the loop's own steps, and the fastest per step runs the rest (the
 // results do not depend on the configuration).  two: the steps may run two
-// per launch (swd_fstep2.h; the caller's instances have no AFor capture and
-// no dp/dt output, there is no hook, and consecutive steps follow each other
+// per launch (swd_fstep2.h; the caller's instances have no dp/dt output,
+// there is no hook, and consecutive steps follow each other
 // in its): the two-step configurations join the candidates (or alone run the
 // loop, if the plan asks for two steps per launch).  A two-step schedule runs
 // the first reconstruction step (first_it) and the remainder of the step
@@ -287,7 +287,7 @@ int forward_loop(Plan &P, const KInst *d_inst, int per_shot, int nshots);
 // Forward time loop of the fused path: n independent wavefield instances in
 // groups whose per-step working set stays in the last-level cache, the groups
 // taking the plan's streams in turn (as the RTM forward).  two: the instances
-// may run two steps per launch (no AFor capture; see fused_steps).
+// may run two steps per launch (see fused_steps).
 int fused_forward_loop(Plan &P, const KStep *d, int n, bool two);
 
 int total_traces(int nshots, const swd_shot *shots);

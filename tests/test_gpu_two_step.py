@@ -156,3 +156,31 @@ def test_two_step_frame_sources_and_receivers(shape):
     for name, a, b in zip(("image", "illum_src", "illum_rcv", "gather"), res[1], res[2]):
         assert np.isfinite(b).all(), name
         assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_two_step_afor(shape):
+    """AFor (p and its Hilbert companion per shot, column capture for the kz
+    split) two steps per launch: gathers and up/down gathers equal one step per
+    launch bit for bit and the oracle's within 1e-5; odd nt."""
+    c = configs.small(nx=150, nz=110, pml=10, Lc=4, nt=203, nshots=2, dup=True)
+    print(f"case two-step afor {shape}", flush=True)
+    nfftz = swd.nfft(c.grid.ntz)
+    kz, _ = swd.kz_taper(nfftz, c.grid.dz)
+    hw = swd.hilbert1d(c.wavelet)
+    res = {}
+    for steps in (1, 2):
+        plan = plan_of(c, steps, shape if steps == 2 else None)
+        res[steps] = plan.afor(c.shots, c.wavelet, hw, kz)
+        assert plan.stats()["steps_per_launch"] == steps
+        plan.close()
+    for name, a, b in zip(("gather", "up", "down"), res[1], res[2]):
+        assert np.array_equal(a, b), name
+    row = 0
+    for k in range(2):
+        og, ou, od = oracle_problem(c, k).afor(hw, kz)
+        n = og.shape[0]
+        for name, a, ref in (("gather", res[2][0], og), ("up", res[2][1], ou), ("down", res[2][2], od)):
+            e32, _ = report(f"two-step afor {shape} shot {k} {name}", a[row:row + n], ref)
+            assert e32 <= TOL
+        row += n
