@@ -579,7 +579,9 @@ def bench_afor(a):
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     up0 = d[1, :nrec].cpu().numpy()
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "afor_c2_traffic.json")
+    # the counter figures are per wavefield-step of the one-step kernel, or of the two-step kernel
+    tfile = os.path.join(ROOT, "profiles", "afor2_c2_traffic.json" if st["fwd_steps_per_launch"] == 2
+                         else "afor_c2_traffic.json")
     if os.path.exists(tfile) and (g.nx, g.nz, g.pml, g.Lc) == (845, 502, 40, 4):
         with open(tfile) as f:
             traffic = json.load(f)["l2_miss_bytes_per_wavefield_step"] * 2 * a.batch / launches_per_step
@@ -1013,6 +1015,15 @@ def bench_c5(a):
     iso_bytes = (bb + mb * iso_lps) / iso_lps   # a launch advances 1 / iso_lps steps; the model once
     iso = iso_bytes / (iso_ms * 1e-3) / 1e9
     log(f"bench: isolated 1-shot backward launch {iso_ms * 1e3:.1f} us, {iso:.0f} GB/s")
+    # HBM-resident: a 4-shot group alone on one stream, a per-step working set (119 MB a shot) about twice the
+    # 256 MB last-level cache
+    res_n = min(4, nshots)
+    res_ms, res_lps = backward_launch_ms(swd, torch, case, adj, res_n, res_n, 1, a.nt_iso, a.tile, cfg_b)
+    res_set = 4.0 * (9 * g.nx * g.nz + 6 * g.ntx * g.ntz + 6 * (g.ntx * g.ntz - g.nx * g.nz)) * res_n
+    res_bytes = (bb * res_n + mb * res_lps) / res_lps
+    res = res_bytes / (res_ms * 1e-3) / 1e9
+    log(f"bench: {res_n}-shot backward launch ({res_set / 2**20:.0f} MiB working set) {res_ms * 1e3:.1f} us, "
+        f"{res:.0f} GB/s")
     plain = None
     if not a.no_plain:
         one = swd.ShotSet(case.shots[:1])
@@ -1084,7 +1095,12 @@ def bench_c5(a):
                      "steps_per_launch": 1.0 / iso_lps, "rtm_backward_launch": dict(zip(
                          ("steps_per_launch", "tile_x", "tile_rows", "waves"), cfg_b)),
                      "bytes_per_launch": iso_bytes, "avg_launch_ms": iso_ms,
-                     "timing": f"HIP events on the plan's stream around {a.nt_iso - 1} graph-replayed launches"},
+                     "timing": f"HIP events on the plan's stream around {a.nt_iso - 1} graph-replayed launches",
+                     "hbm_resident": {"achieved": res, "frac": res / HBM_PEAK_GBS, "shots_per_launch": res_n,
+                                      "working_set_MiB": res_set / 2**20, "bytes_per_launch": res_bytes,
+                                      "avg_launch_ms": res_ms,
+                                      "timing": f"{res_n}-shot group alone on one stream, {a.nt_iso - 1} steps: a "
+                                                "per-step working set about twice the 256 MB last-level cache"}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -27,7 +27,8 @@ int fstep2_cfg_of_shape(int tx, int tq)
 hipError_t launch_fstep2(const KGrid &g, const KStep *insts, int ninst, int it, int ph, int cfg, bool pair, hipStream_t s)
 {
     const int tx = fstep_cfg_tx(cfg), tq = fstep_cfg_tq(cfg), nw = fstep_cfg_wb(cfg);
-    if (g.Lc != 4 || !fstep2_cfg_supported(cfg) || (ph != 0 && ph != 1)) return hipErrorInvalidValue;
+    if (g.Lc != 4 || !fstep2_cfg_supported(cfg) || (ph != 0 && ph != 1) || g.ldz >= (1 << 23))
+        return hipErrorInvalidValue;
     const bool split = (cfg & kFstepSplit) != 0;
     const int n = pair ? ninst / 2 : ninst;
     const int nzb = (g.ntz + 4 * tq - 1) / (4 * tq), nxb = (g.ntx + tx - 1) / tx;

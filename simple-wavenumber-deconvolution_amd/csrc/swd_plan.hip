@@ -295,7 +295,8 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
 {
     if (!st) st = P.stream;
     // two steps per launch: the forward goes up in time, the paired backward down, one step at a time
-    two = two && !hook && !cig && !born && P.g.Lc == 4 && P.steps_per_launch != 1;
+    // (the two-step kernel forms tile-relative offsets with a 24-bit multiply: column strides below 2^23)
+    two = two && !hook && !cig && !born && P.g.Lc == 4 && P.steps_per_launch != 1 && P.ldz < (1 << 23);
     const int dir = pair ? -1 : 1;
     for (size_t j = 1; two && j < its.size(); j++)
         if (its[j] != its[j - 1] + dir) two = false;

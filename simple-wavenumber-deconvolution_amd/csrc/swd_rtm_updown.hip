@@ -127,7 +127,7 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
         for (size_t g = 0; g + 1 < gbf.size(); g++) {
             const int k0 = gbf[g], n = gbf[g + 1] - gbf[g];
             SWD_TRY(fused_steps(P, at<KStep>(P, o_fwd) + 2 * k0, 2 * n, steps_up(nt), -1, P.tuned_fwd, &launches, false,
-                                rtm_group_stream(P, static_cast<int>(g))));
+                                rtm_group_stream(P, static_cast<int>(g)), nullptr, false, false, true));
         }
         SWD_TRY(join_streams(P));
         P.stats.fwd_ms = stats_end(P, 0);
