@@ -33,9 +33,11 @@
 //
 // Modes: FWD (forward with source, receivers, strips, AFor column capture),
 // and the paired RTM backward: SRC (reconstruction with wavelet subtraction
-// and frame restore) then RCV (receiver back-propagation with hard replacement
-// and the imaging condition).  Not here: dp/dt output, angle gathers, Born,
-// the first reconstruction step (which keeps the forward's velocities).
+// and frame restore; dp/dt of both levels stored for the up/down imaging if
+// asked) then RCV (receiver back-propagation with hard replacement and the
+// imaging condition; the first level's pressure stored if asked).  Not here:
+// angle gathers, Born, the first reconstruction step (which keeps the
+// forward's velocities).
 #pragma once
 #include <algorithm>
 
@@ -477,6 +479,10 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
     const unsigned obase = static_cast<unsigned>(X0) * ldz + static_cast<unsigned>(Z0);
     auto offc = [&](int c, int q) { return obase + static_cast<unsigned>(__mul24(c, static_cast<int>(ldz)) + 4 * q); };
     auto off = [&](int x, int z) { return offc(x - X0, (z - Z0) >> 2); };
+    // dp/dt is stored where the reconstruction's pressure update runs (interior columns, quads reaching the interior rows)
+    auto dpint = [&](int x, int z) {
+        return FULL || (x >= g.pmlc && x < ntx - g.pmlc && z + 3 >= g.pmlc && z < ntz - g.pmlc);
+    };
 
     // ---- loads: staged p (level t), old velocities and model of every updated cell
     float4 st[G::RP0];
@@ -637,7 +643,10 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
         const bool wr = f2_p_quad<FULL, MODE, XNQ>(g, I, x, z, lv1, &sVX[G::sx(c, q)], w, pold, kv[r], k0, qxo, qzo, pv,
                                                    dp, qxn, qzn, axo, anyz, srcnear);
         if (i < G::PCORE) {   // the own quad (core = item tid)
-            if (SRC) dps[0] = dp;
+            if (SRC) {
+                dps[0] = dp;
+                if (I.dpdt && wr && dpint(x, z)) st4((gfl *)I.dpdt, off(x, z), dp);   // for the up/down imaging
+            }
             if (!SRC) f2_receivers<FULL, MODE>(g, I, ocm, z, lv1, pv);
             if (doimg) f2_image<FULL>(g, z, dps[0], pv, im, is, ir);
             if (MODE == KSTEP_FWD && !FULL && I.strips) {
@@ -648,6 +657,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
                 }
             }
             if (MODE == KSTEP_FWD && I.capture) f2_capture<FULL>(g, I, x, z, lv1, pv);
+            if (NEG && I.pmid) st4((gfl *)I.pmid, off(x, z), pv);
             p1 = pv;
             qx1 = qxn;
             qz1 = qzn;
@@ -720,7 +730,10 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
         bool anyz;
         const bool wr = f2_p_quad<FULL, MODE, XNQ>(g, I, x, z, lv2, &sVX[G::sx(oc, oq)], w, p1, kv[0], k0, qx1, qz1,
                                                    pv, dp, qxn, qzn, axo, anyz, srcnear);
-        if (SRC) dps[1] = dp;
+        if (SRC) {
+            dps[1] = dp;
+            if (I.dpdt_b && wr && dpint(x, z)) st4((gfl *)I.dpdt_b, off(x, z), dp);
+        }
         if (!SRC) f2_receivers<FULL, MODE>(g, I, ocm, z, lv2, pv);
         if (wr) st4(Pw, off(x, z), pv);
         if (CPML) {

@@ -177,6 +177,9 @@ constexpr double kCacheBudgetFwd = 128.0 * (1 << 20);
 // the graph cache.  The launch counts count it beside the fused step.
 struct StepHook {
     std::function<int(int, hipStream_t)> run;
+    // after a two-step launch of steps it and it-1 (the paired backward), ph its place in
+    // its pair: queues the work of both steps (two launches); unset: no two-step launches
+    std::function<int(int, int, hipStream_t)> run2;
     std::vector<long long> key;
 };
 

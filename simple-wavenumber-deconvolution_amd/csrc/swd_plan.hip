@@ -296,7 +296,8 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     if (!st) st = P.stream;
     // two steps per launch: the forward goes up in time, the paired backward down, one step at a time
     // (the two-step kernel forms tile-relative offsets with a 24-bit multiply: column strides below 2^23)
-    two = two && !hook && !cig && !born && P.g.Lc == 4 && P.steps_per_launch != 1 && P.ldz < (1 << 23);
+    two = two && (!hook || hook->run2) && !cig && !born && P.g.Lc == 4 && P.steps_per_launch != 1 &&
+          P.ldz < (1 << 23);
     const int dir = pair ? -1 : 1;
     for (size_t j = 1; two && j < its.size(); j++)
         if (its[j] != its[j - 1] + dir) two = false;
@@ -347,10 +348,16 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     };
     // four steps as a pair of two-step launches (slot parity as before afterwards)
     auto quad = [&](int c) -> int {
-        HIP_TRY(launch_fstep2(P.kg, d, n, its[k], 0, c, pair, st));
-        HIP_TRY(launch_fstep2(P.kg, d, n, its[k + 2], 1, c, pair, st));
+        for (int ph = 0; ph < 2; ph++) {
+            const int it = its[k + 2 * ph];
+            HIP_TRY(launch_fstep2(P.kg, d, n, it, ph, c, pair, st));
+            nl++;
+            if (hook) {
+                SWD_TRY(hook->run2(it, ph, st));
+                nl += 2;
+            }
+        }
         k += 4;
-        nl += 2;
         return SWD_OK;
     };
     auto unit = [&](int c) -> int { return fstep_cfg_two(c) ? quad(c) : one(c); };
@@ -421,8 +428,10 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         if (hook) gkey.insert(gkey.end(), hook->key.begin(), hook->key.end());
         // launches of the rest, counted here since a replay does not record them
         const size_t left = its.size() - k;
-        const long long nrest = fstep_cfg_two(cfg) ? static_cast<long long>(left % 4 + left / 4 * 2)
-                                                   : static_cast<long long>(left) * (hook ? 2 : 1);
+        const long long h = hook ? 1 : 0;
+        const long long nrest = fstep_cfg_two(cfg) ? static_cast<long long>(left % 4) * (1 + h) +
+                                                         static_cast<long long>(left / 4) * 2 * (1 + 2 * h)
+                                                   : static_cast<long long>(left) * (1 + h);
         const long long before = nl;   // the first step's and the tuner's launches
         SWD_TRY(graph_run(P, gkey, rest, st));
         *launches += before + nrest;

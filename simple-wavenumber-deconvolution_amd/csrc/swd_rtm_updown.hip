@@ -24,7 +24,9 @@ std::vector<float2> kz_twiddles(int nfftz)
 
 // Wavefield arrays of one shot of the up/down RTM: four wavefields (12 arrays
 // each), then du, ud, uu, dd, image, illum_s, illum_r and the two dp/dt.
-constexpr int kUdImages = 9;
+// du, ud, uu, dd, image, illum_s, illum_r, the two reconstructions' dp/dt, their second
+// levels and the two receiver fields' first-level pressures of two-step launches
+constexpr int kUdImages = 13;
 constexpr int kUdPerShot = 4 * kStepArrays + kUdImages;
 
 // Up/down-separated RTM of a batch on the fused kernels.  The forward runs two
@@ -46,7 +48,9 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
     const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
     const size_t o_fwd = c.add<KStep>(2 * nshots);
     const size_t o_bwd = c.add<KStep>(4 * nshots);
-    const size_t o_ud = c.add<KUpDown>(nshots);
+    // up/down descriptors: one step per launch; two steps, first level; two steps, second level
+    // after the first and after the second launch of a pair
+    const size_t o_ud = c.add<KUpDown>(4 * nshots);
     const size_t o_wav = c.add<float>(2 * nt);
     const size_t o_tw = c.add<float2>(nfftz / 2);
     const size_t o_coef = c.add<float>(nfftz);
@@ -62,7 +66,7 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
     HIP_TRY(hipMemcpyAsync(P.ws + o_coef, kzcoef, sizeof(float) * nfftz, hipMemcpyHostToDevice, P.stream));
     zero_fields(P, at<float>(P, o_f), kUdPerShot * nf * nshots);
     std::vector<KStep> hf(2 * nshots), hb(4 * nshots);
-    std::vector<KUpDown> hu(nshots);
+    std::vector<KUpDown> hu(4 * nshots);
     for (int k = 0; k < nshots; k++) {
         float *base = at<float>(P, o_f) + kUdPerShot * nf * k;
         float *im = base + 4 * kStepArrays * nf;   // du, ud, uu, dd, image, illum_s, illum_r, dp/dt, dp_h/dt
@@ -87,30 +91,39 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
                 R.adj = (h ? hadj : adj) + static_cast<size_t>(b.toff[k]) * nt;
             }
             S.dpdt = im + (7 + h) * nf;
+            S.dpdt_b = im + (9 + h) * nf;
+            R.pmid = im + (11 + h) * nf;
             if (h == 0 && plain) {
                 R.image = im + 4 * nf;
                 R.illum_s = im + 5 * nf;
                 R.illum_r = im + 6 * nf;
             }
         }
-        KUpDown &U = hu[k];
-        U.ds = im + 7 * nf;
-        U.dh = im + 8 * nf;
-        for (int q = 0; q < 2; q++) {
-            U.pr[q] = hb[4 * k + 1].p[q];
-            U.ph[q] = hb[4 * k + 3].p[q];
+        for (int v = 0; v < 4; v++) {
+            // k_updown of step it reads p of slot it & 1: one step per launch (v 0) and the second level of
+            // the second launch of a pair (v 3) find it there; the first launch of a pair (v 2) wrote its
+            // second level into the other slot; the first level of a two-step launch (v 1) is in pmid
+            KUpDown &U = hu[v * nshots + k];
+            const bool second = v >= 2;
+            U.ds = im + (second ? 9 : 7) * nf;
+            U.dh = im + (second ? 10 : 8) * nf;
+            for (int q = 0; q < 2; q++) {
+                const int sq = v == 2 ? 1 - q : q;
+                U.pr[q] = v == 1 ? im + 11 * nf : hb[4 * k + 1].p[sq];
+                U.ph[q] = v == 1 ? im + 12 * nf : hb[4 * k + 3].p[sq];
+            }
+            U.du = im;
+            U.ud = im + nf;
+            U.uu = both ? im + 2 * nf : nullptr;
+            U.dd = both ? im + 3 * nf : nullptr;
+            U.tw = at<float2>(P, o_tw);
+            U.coef = at<float>(P, o_coef);
+            U.ld = P.ldz;
         }
-        U.du = im;
-        U.ud = im + nf;
-        U.uu = both ? im + 2 * nf : nullptr;
-        U.dd = both ? im + 3 * nf : nullptr;
-        U.tw = at<float2>(P, o_tw);
-        U.coef = at<float>(P, o_coef);
-        U.ld = P.ldz;
     }
     SWD_TRY(upload_steps(P, hf, o_fwd));
     SWD_TRY(upload_steps(P, hb, o_bwd));
-    HIP_TRY(hipMemcpyAsync(P.ws + o_ud, hu.data(), sizeof(KUpDown) * nshots, hipMemcpyHostToDevice, P.stream));
+    HIP_TRY(hipMemcpyAsync(P.ws + o_ud, hu.data(), sizeof(KUpDown) * hu.size(), hipMemcpyHostToDevice, P.stream));
     // two wavefields per shot in both phases: the backward takes half the
     // plain RTM's shots per group; the forward keeps the plain RTM's count
     // (c2, 16 shots, nt 1000, three streams: one-shot forward groups 157 ms,
@@ -155,9 +168,15 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
             HIP_TRY(launch_updown(P.kg, du, n, nfftz, it, st));
             return SWD_OK;
         };
+        // a two-step launch of steps it and it-1: the first level, then the second
+        hook.run2 = [&P, du, n, nshots, nfftz](int it, int ph, hipStream_t st) -> int {
+            HIP_TRY(launch_updown(P.kg, du + nshots, n, nfftz, it, st));
+            HIP_TRY(launch_updown(P.kg, du + (ph == 0 ? 2 : 3) * nshots, n, nfftz, it - 1, st));
+            return SWD_OK;
+        };
         hook.key = {2, reinterpret_cast<long long>(du), nfftz};
         SWD_TRY(fused_steps(P, at<KStep>(P, o_bwd) + 4 * k0, 4 * n, its, nt - 2, P.tuned_bwd, &launches, true,
-                            rtm_group_stream(P, static_cast<int>(g)), &hook));
+                            rtm_group_stream(P, static_cast<int>(g)), &hook, false, false, true));
     }
     SWD_TRY(join_streams(P));
     P.stats.bwd_ms = stats_end(P, 1);

@@ -184,3 +184,32 @@ def test_two_step_afor(shape):
             e32, _ = report(f"two-step afor {shape} shot {k} {name}", a[row:row + n], ref)
             assert e32 <= TOL
         row += n
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 4), (64, 32, -8), (16, 64, 4)])
+@pytest.mark.parametrize("nt", [161, 164])
+def test_two_step_updown(shape, nt):
+    """Up/down-separated RTM two steps per launch (dp/dt of both levels and the
+    receiver pressure of the first level stored for the kz split after each
+    level): du, ud, uu, dd and the plain image equal one step per launch bit for
+    bit, with two shots in groups on two streams."""
+    c = configs.small(nx=150, nz=110, pml=10, Lc=4, nt=nt, nshots=2)
+    s_ix, s_iz, rx, rz = c.shots[1]
+    c.shots[1] = (c.pmlc + 1, c.pmlc + 40, rx, rz)
+    print(f"case two-step updown {shape} nt={nt}", flush=True)
+    outs = [oracle_problem(c, k).forward()[0] for k in range(2)]
+    adj = np.concatenate([O.adjoint_source(d, c.grid.dt) for d in outs])
+    hadj = swd.hilbert_traces(adj)
+    hw = swd.hilbert1d(c.wavelet)
+    nfftz = swd.nfft(c.grid.ntz)
+    kz, _ = swd.kz_taper(nfftz, c.grid.dz)
+    res = {}
+    for steps in (1, 2):
+        plan = plan_of(c, steps, shape if steps == 2 else None)
+        plan.set_groups(1, 1)
+        res[steps] = plan.rtm_updown(c.shots, c.wavelet, hw, adj, hadj, kz, want_same_dir=True, want_image=True)
+        assert plan.stats()["steps_per_launch"] == steps
+        plan.close()
+    for name in res[1]:
+        assert np.isfinite(res[2][name]).all(), name
+        assert np.array_equal(res[1][name], res[2][name]), name
