@@ -75,6 +75,10 @@ hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2
                         const KCapCol *cols, const int2 *traces, float *up, float *down, int nt, hipStream_t s);
 // up/down imaging of step it for nshot shots (KUpDown each) over the interior columns
 hipError_t launch_updown(const KGrid &g, const KUpDown *shots, int nshot, int nfftz, int it, hipStream_t s);
+// the same for steps ita then itb in one launch (descriptors a and b of one
+// shot share its images; the interior rows at most nfftz)
+hipError_t launch_updown2(const KGrid &g, const KUpDown *a, const KUpDown *b, int nshot, int nfftz, int ita, int itb,
+                          hipStream_t s);
 // angle-gather update of step it for nshot shots (KCig each) over the interior cells;
 // w1, w2: the bin weights exp(-1/18), exp(-4/18) rounded to float; thr: the
 // cosine below which no bin is in reach (cig_bins)

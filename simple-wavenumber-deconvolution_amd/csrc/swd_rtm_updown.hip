@@ -168,10 +168,9 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
             HIP_TRY(launch_updown(P.kg, du, n, nfftz, it, st));
             return SWD_OK;
         };
-        // a two-step launch of steps it and it-1: the first level, then the second
+        // a two-step launch of steps it and it-1: both levels in one launch, the first level's products first
         hook.run2 = [&P, du, n, nshots, nfftz](int it, int ph, hipStream_t st) -> int {
-            HIP_TRY(launch_updown(P.kg, du + nshots, n, nfftz, it, st));
-            HIP_TRY(launch_updown(P.kg, du + (ph == 0 ? 2 : 3) * nshots, n, nfftz, it - 1, st));
+            HIP_TRY(launch_updown2(P.kg, du + nshots, du + (ph == 0 ? 2 : 3) * nshots, n, nfftz, it, it - 1, st));
             return SWD_OK;
         };
         hook.key = {2, reinterpret_cast<long long>(du), nfftz};

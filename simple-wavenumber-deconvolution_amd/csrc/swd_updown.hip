@@ -22,19 +22,16 @@
 
 namespace {
 
+// kz split of one column of one shot at step it: the analytic source column
+// in L[0, NP), the receiver's in L[NP, 2 NP), both split into (Re up, Re
+// down) in natural order of the rows z0 + i.
 template <int LOGN, int V>
-__global__ __launch_bounds__(2 * (1 << LOGN) / V) void k_updown(KGrid g, const KUpDown *__restrict__ shots, int it)
+__device__ __forceinline__ void ud_split(const KGrid &g, const KUpDown &U, int it, size_t c0, int z0, int nzw, float2 *lds)
 {
     constexpr int N = 1 << LOGN, T = N / V, NP = N + N / 16;
-    __shared__ float2 lds[2 * NP];
-    const int col = g.pmlc + blockIdx.x;
-    const KUpDown &U = shots[blockIdx.y];
     const int f = threadIdx.x / T, t = threadIdx.x - f * T;   // field 0: source, 1: receiver
     float2 *L = lds + f * NP;
-    const size_t c0 = static_cast<size_t>(col) * U.ld;
-    // the whole padded column when it fits in N, else its interior rows
-    const int z0 = N >= g.ntz ? 0 : g.pmlc, nzw = z0 ? g.nz : g.ntz;
-    const int slot = it & 1, ntz = g.ntz;
+    const int slot = it & 1;
     const float *re = (f ? U.pr[slot] : U.ds) + c0 + z0;
     const float *im = (f ? U.ph[slot] : U.dh) + c0 + z0;
     const float2 *__restrict__ tw = U.tw;
@@ -51,18 +48,82 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / V) void k_updown(KGrid g, const K
     kz_pack<N, V>(v, t, L, U.coef, 0.5f / static_cast<float>(N));   // 1/(2N): a power of two, exact
     first_pass<N, V>(v, t, L);
     passes<N, V, V, true>(v, t, L, tw);
+}
+
+template <int LOGN, int V>
+__global__ __launch_bounds__(2 * (1 << LOGN) / V) void k_updown(KGrid g, const KUpDown *__restrict__ shots, int it)
+{
+    constexpr int N = 1 << LOGN, T = N / V, NP = N + N / 16;
+    __shared__ float2 lds[2 * NP];
+    const int col = g.pmlc + blockIdx.x;
+    const KUpDown &U = shots[blockIdx.y];
+    const size_t c0 = static_cast<size_t>(col) * U.ld;
+    // the whole padded column when it fits in N, else its interior rows
+    const int z0 = N >= g.ntz ? 0 : g.pmlc, nzw = z0 ? g.nz : g.ntz;
+    ud_split<LOGN, V>(g, U, it, c0, z0, nzw, lds);
 
     // imaging of the interior rows
     const float2 *S = lds, *Rv = lds + NP;
     float *du = U.du + c0, *ud = U.ud + c0;
     float *uu = U.uu ? U.uu + c0 : nullptr, *dd = U.dd ? U.dd + c0 : nullptr;
-    for (int iz = g.pmlc + static_cast<int>(threadIdx.x); iz < ntz - g.pmlc; iz += 2 * T) {
+    for (int iz = g.pmlc + static_cast<int>(threadIdx.x); iz < g.ntz - g.pmlc; iz += 2 * T) {
         const float2 s = S[pad(iz - z0)], r = Rv[pad(iz - z0)];
         du[iz] = __builtin_fmaf(s.y, r.x, du[iz]);
         ud[iz] = __builtin_fmaf(s.x, r.y, ud[iz]);
         if (uu) {
             uu[iz] = __builtin_fmaf(s.x, r.x, uu[iz]);
             dd[iz] = __builtin_fmaf(s.y, r.y, dd[iz]);
+        }
+    }
+}
+
+// Two steps of one shot's column in one workgroup: step ita (descriptors
+// A), then step itb (descriptors B, the same images).  The images take
+// level a's products and then level b's, the order of two k_updown launches,
+// read and written once.  The interior rows are at most N (V/2 rows a thread).
+template <int LOGN, int V>
+__global__ __launch_bounds__(2 * (1 << LOGN) / V) void k_updown2(KGrid g, const KUpDown *__restrict__ sa,
+                                                                 const KUpDown *__restrict__ sb, int ita, int itb)
+{
+    constexpr int N = 1 << LOGN, T = N / V, NP = N + N / 16, NR = V / 2;
+    __shared__ float2 lds[2 * NP];
+    const int col = g.pmlc + blockIdx.x;
+    const KUpDown &A = sa[blockIdx.y], &B = sb[blockIdx.y];
+    const size_t c0 = static_cast<size_t>(col) * A.ld;
+    const int z0 = N >= g.ntz ? 0 : g.pmlc, nzw = z0 ? g.nz : g.ntz;
+    ud_split<LOGN, V>(g, A, ita, c0, z0, nzw, lds);
+
+    const float2 *S = lds, *Rv = lds + NP;
+    float *du = A.du + c0, *ud = A.ud + c0;
+    float *uu = A.uu ? A.uu + c0 : nullptr, *dd = A.dd ? A.dd + c0 : nullptr;
+    const int iz0 = g.pmlc + static_cast<int>(threadIdx.x), iz1 = g.ntz - g.pmlc;
+    float a_du[NR], a_ud[NR], a_uu[NR], a_dd[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int iz = iz0 + r * 2 * T;
+        if (iz < iz1) {
+            const float2 s = S[pad(iz - z0)], q = Rv[pad(iz - z0)];
+            a_du[r] = __builtin_fmaf(s.y, q.x, du[iz]);
+            a_ud[r] = __builtin_fmaf(s.x, q.y, ud[iz]);
+            if (uu) {
+                a_uu[r] = __builtin_fmaf(s.x, q.x, uu[iz]);
+                a_dd[r] = __builtin_fmaf(s.y, q.y, dd[iz]);
+            }
+        }
+    }
+    __syncthreads();   // level a's spectra read before level b's first pass overwrites them
+    ud_split<LOGN, V>(g, B, itb, c0, z0, nzw, lds);
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int iz = iz0 + r * 2 * T;
+        if (iz < iz1) {
+            const float2 s = S[pad(iz - z0)], q = Rv[pad(iz - z0)];
+            du[iz] = __builtin_fmaf(s.y, q.x, a_du[r]);
+            ud[iz] = __builtin_fmaf(s.x, q.y, a_ud[r]);
+            if (uu) {
+                uu[iz] = __builtin_fmaf(s.x, q.x, a_uu[r]);
+                dd[iz] = __builtin_fmaf(s.y, q.y, a_dd[r]);
+            }
         }
     }
 }
@@ -75,6 +136,24 @@ hipError_t launch_updown(const KGrid &g, const KUpDown *shots, int nshot, int nf
 #define UD(LOGN, V)                                                                                   \
     case 1 << LOGN:                                                                                       \
         hipLaunchKernelGGL((k_updown<LOGN, V>), grid, dim3(2 * (1 << LOGN) / V), 0, s, g, shots, it); \
+        break;
+    switch (nfftz) {
+        UD(7, 8) UD(8, 8) UD(9, 8) UD(10, 16) UD(11, 16) UD(12, 16)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef UD
+    return hipGetLastError();
+}
+
+hipError_t launch_updown2(const KGrid &g, const KUpDown *a, const KUpDown *b, int nshot, int nfftz, int ita, int itb,
+                          hipStream_t s)
+{
+    if (g.nz > nfftz) return hipErrorInvalidValue;
+    const dim3 grid(g.nx, nshot);
+#define UD(LOGN, V)                                                                                                \
+    case 1 << LOGN:                                                                                                    \
+        hipLaunchKernelGGL((k_updown2<LOGN, V>), grid, dim3(2 * (1 << LOGN) / V), 0, s, g, a, b, ita, itb);            \
         break;
     switch (nfftz) {
         UD(7, 8) UD(8, 8) UD(9, 8) UD(10, 16) UD(11, 16) UD(12, 16)
