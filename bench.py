@@ -1011,14 +1011,15 @@ def bench_c5(a):
     per_phase["gradient"]["full_storage_GB"] = stats["gradient"]["full_bytes"] / 1e9
     # the dominant kernel (the paired RTM backward step, one-shot groups at c5) alone on one stream
     cfg_b = launch_config(stats["rtm"])
-    iso_ms, iso_lps = backward_launch_ms(swd, torch, case, adj, 1, 1, 1, a.nt_iso, a.tile, cfg_b)
+    nt_iso = min(a.nt_iso, a.nt_c5)
+    iso_ms, iso_lps = backward_launch_ms(swd, torch, case, adj, 1, 1, 1, nt_iso, a.tile, cfg_b)
     iso_bytes = (bb + mb * iso_lps) / iso_lps   # a launch advances 1 / iso_lps steps; the model once
     iso = iso_bytes / (iso_ms * 1e-3) / 1e9
     log(f"bench: isolated 1-shot backward launch {iso_ms * 1e3:.1f} us, {iso:.0f} GB/s")
     # HBM-resident: a 4-shot group alone on one stream, a per-step working set (119 MB a shot) about twice the
     # 256 MB last-level cache
     res_n = min(4, nshots)
-    res_ms, res_lps = backward_launch_ms(swd, torch, case, adj, res_n, res_n, 1, a.nt_iso, a.tile, cfg_b)
+    res_ms, res_lps = backward_launch_ms(swd, torch, case, adj, res_n, res_n, 1, nt_iso, a.tile, cfg_b)
     res_set = 4.0 * (9 * g.nx * g.nz + 6 * g.ntx * g.ntz + 6 * (g.ntx * g.ntz - g.nx * g.nz)) * res_n
     res_bytes = (bb * res_n + mb * res_lps) / res_lps
     res = res_bytes / (res_ms * 1e-3) / 1e9
@@ -1095,11 +1096,11 @@ def bench_c5(a):
                      "steps_per_launch": 1.0 / iso_lps, "rtm_backward_launch": dict(zip(
                          ("steps_per_launch", "tile_x", "tile_rows", "waves"), cfg_b)),
                      "bytes_per_launch": iso_bytes, "avg_launch_ms": iso_ms,
-                     "timing": f"HIP events on the plan's stream around {a.nt_iso - 1} graph-replayed launches",
+                     "timing": f"HIP events on the plan's stream around {nt_iso - 1} graph-replayed launches",
                      "hbm_resident": {"achieved": res, "frac": res / HBM_PEAK_GBS, "shots_per_launch": res_n,
                                       "working_set_MiB": res_set / 2**20, "bytes_per_launch": res_bytes,
                                       "avg_launch_ms": res_ms,
-                                      "timing": f"{res_n}-shot group alone on one stream, {a.nt_iso - 1} steps: a "
+                                      "timing": f"{res_n}-shot group alone on one stream, {nt_iso - 1} steps: a "
                                                 "per-step working set about twice the 256 MB last-level cache"}},
         "cpu_baseline": cpu,
     }
@@ -1255,13 +1256,14 @@ def main():
     fwd_launches_per_step = st["fwd_launches"] / g.nt
     fwd_achieved = (bf * a.batch + mf * fwd_launches_per_step) / fwd_launches_per_step / (fwd_launch_ms * 1e-3) / 1e9
     # (2) isolated: one group of that size alone on one stream, graph-replayed launches back to back
-    iso_ms, iso_lps = backward_launch_ms(swd, torch, case, adj, group, group, 1, a.nt_iso, a.tile, cfg_b)
+    nt_iso = min(a.nt_iso, a.nt)
+    iso_ms, iso_lps = backward_launch_ms(swd, torch, case, adj, group, group, 1, nt_iso, a.tile, cfg_b)
     iso_bytes = (bb * group + mb * iso_lps) / iso_lps
     iso_achieved = iso_bytes / (iso_ms * 1e-3) / 1e9
     log(f"bench: isolated {group}-shot backward launch {iso_ms * 1e3:.2f} us, {iso_achieved:.0f} GB/s")
     # (3) HBM-resident: every shot of the batch in one launch on one stream, a working set twice the
     # last-level cache (32 MB a shot at c2)
-    res_ms, res_lps = backward_launch_ms(swd, torch, case, adj, a.batch, a.batch, 1, a.nt_iso, a.tile, cfg_b)
+    res_ms, res_lps = backward_launch_ms(swd, torch, case, adj, a.batch, a.batch, 1, nt_iso, a.tile, cfg_b)
     res_set = 4.0 * (9 * g.nx * g.nz + 6 * g.ntx * g.ntz + 6 * (g.ntx * g.ntz - g.nx * g.nz)) * a.batch
     res_bytes = (bb * a.batch + mb * res_lps) / res_lps
     res_achieved = res_bytes / (res_ms * 1e-3) / 1e9
@@ -1342,7 +1344,7 @@ def main():
                                    "model once per launch (a two-step launch reads and writes the state once "
                                    "for its two steps)",
                      "bytes_per_launch": iso_bytes, "avg_launch_ms": iso_ms,
-                     "timing": f"HIP events on the plan's stream around {a.nt_iso - 1} graph-replayed launches of one "
+                     "timing": f"HIP events on the plan's stream around {nt_iso - 1} graph-replayed launches of one "
                                f"{group}-shot group with no other work in flight: launch duration plus launch gap",
                      "overlapped": {"achieved": ov_achieved, "frac": ov_achieved / HBM_PEAK_GBS,
                                     "bytes_per_launch": ov_bytes, "avg_launch_ms": ov_ms,
@@ -1353,7 +1355,7 @@ def main():
                      "hbm_resident": {"achieved": res_achieved, "frac": res_achieved / HBM_PEAK_GBS,
                                       "shots_per_launch": a.batch, "working_set_MiB": res_set / 2**20,
                                       "bytes_per_launch": res_bytes, "avg_launch_ms": res_ms,
-                                      "timing": f"{a.batch}-shot group alone on one stream, {a.nt_iso - 1} "
+                                      "timing": f"{a.batch}-shot group alone on one stream, {nt_iso - 1} "
                                                 "launches: a per-step working set twice the 256 MB last-level cache"},
                      "forward_overlapped_GBs": fwd_achieved},
         "cpu_baseline": cpu,

@@ -7,6 +7,19 @@ using namespace swd_rt;
 // ------------------------------------------------------------------ Born
 namespace {
 
+// Snapshot of the fused path's background state: array j (p, vx, vz, psx,
+// psz in the slot that holds the state, pvx, pvz) of shot k, src[per k + j],
+// to dst + kstride k + nf j, in float4s (nf a multiple of 4).
+__global__ __launch_bounds__(256) void k_snap(float4 *__restrict__ dst, const float4 *const *__restrict__ src, int per,
+                                              size_t nf4, size_t kstride4)
+{
+    const int a = blockIdx.y, k = a / per, j = a - k * per;
+    const float4 *__restrict__ s = src[a];
+    float4 *__restrict__ t = dst + kstride4 * k + nf4 * j;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < nf4; i += static_cast<size_t>(gridDim.x) * 256)
+        t[i] = s[i];
+}
+
 // Born forward of a batch: m on the device [nshots][nx][nz], d on the device
 // [sum nrec][nt].  Each shot runs the background wavefield and the scattered
 // wavefield as two instances of the same launches.
@@ -142,6 +155,12 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
     const bool tiled = P.path == SWD_PATH_FUSED;
     const size_t per_a = kFieldArrays + (tiled ? 4 : 2);
     const size_t o_f = c.add<float>((kFieldArrays + per_a) * nf * nshots);
+    // fused path: the checkpoint pass runs the background as KStep instances
+    // (kStepArrays per shot, in the fields' storage before the adjoint
+    // starts) and snapshots the slot that holds the state
+    static_assert(kStepArrays <= 2 * kFieldArrays + 4, "the fused background fits the fields' storage");
+    const size_t o_step = c.add<KStep>(tiled ? nshots : 1);
+    const size_t o_snapsrc = c.add<float *>(tiled ? 2 * kFieldArrays * nshots : 1);
     SWD_TRY(ensure_ws(P, c.off));
     SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
@@ -187,11 +206,60 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
     // checkpoint pass
     SWD_TRY(upload_insts(P, h, o_inst));
     stats_begin(P, 0);
-    for (int s = 0; s < nseg; s++) {
-        const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
-        for (int k = 0; k < nshots; k++)
-            HIP_TRY(hipMemcpyAsync(snap(k, s), h[k].p, sizeof(float) * kFieldArrays * nf, hipMemcpyDeviceToDevice, P.stream));
-        SWD_TRY(steps(t0, t1));
+    if (tiled) {
+        // the fused forward (two steps per launch where the plan allows); the
+        // state before step t0 is in slot (t0 + 1) & 1 after whole loops
+        float *fs = at<float>(P, o_f);
+        zero_fields(P, fs, kStepArrays * nf * nshots);
+        std::vector<KStep> hs(nshots);
+        std::vector<float *> src(2 * kFieldArrays * nshots);
+        for (int k = 0; k < nshots; k++) {
+            KStep &S = hs[k];
+            std::memset(&S, 0, sizeof S);
+            S.mode = KSTEP_FWD;
+            bind_step(S, fs + kStepArrays * nf * k, nf);
+            S.wavelet = at<float>(P, o_wav);
+            S.s_ix = shots[k].s_ix;
+            S.s_iz = shots[k].s_iz;
+            for (int c2 = 0; c2 < 2; c2++) {
+                float **q = src.data() + kFieldArrays * (nshots * c2 + k);
+                q[0] = S.p[c2]; q[1] = S.vx[c2]; q[2] = S.vz[c2]; q[3] = S.psx[c2]; q[4] = S.psz[c2];
+                q[5] = S.pvx; q[6] = S.pvz;
+            }
+        }
+        SWD_TRY(upload_steps(P, hs, o_step));
+        HIP_TRY(hipMemcpyAsync(P.ws + o_snapsrc, src.data(), sizeof(float *) * src.size(), hipMemcpyHostToDevice, P.stream));
+        const float4 *const *dsrc = reinterpret_cast<const float4 *const *>(at<float *>(P, o_snapsrc));
+        const size_t nf4 = nf / 4;
+        const dim3 sg(static_cast<unsigned>(std::min<size_t>((nf4 + 255) / 256, 1024)), kFieldArrays * nshots);
+        // each segment is its own loop: no graph is worth capturing for one replay
+        const bool graphs = P.graphs;
+        P.graphs = false;
+        int rc = SWD_OK;
+        for (int s = 0; s < nseg && rc == SWD_OK; s++) {
+            const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
+            hipLaunchKernelGGL(k_snap, sg, dim3(256), 0, P.stream, reinterpret_cast<float4 *>(snap(0, s)),
+                               dsrc + kFieldArrays * nshots * ((t0 + 1) & 1), kFieldArrays, nf4,
+                               kFieldArrays * nf4 * static_cast<size_t>(nseg));
+            if (hipGetLastError() != hipSuccess) rc = swd_fail(SWD_E_DEVICE, "born adjoint: snapshot launch failed");
+            std::vector<int> its;
+            for (int it = t0; it < t1; it++) its.push_back(it);
+            if (rc == SWD_OK)
+                rc = fused_steps(P, at<KStep>(P, o_step), nshots, its, -1, P.tuned_fwd, &fl, false, P.stream, nullptr,
+                                 false, false, true);
+        }
+        P.graphs = graphs;
+        SWD_TRY(rc);
+        // the adjoint fields start at zero; the background fields are restored from the snapshots
+        zero_fields(P, at<float>(P, o_f), (kFieldArrays + per_a) * nf * nshots);
+    } else {
+        for (int s = 0; s < nseg; s++) {
+            const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
+            for (int k = 0; k < nshots; k++)
+                HIP_TRY(hipMemcpyAsync(snap(k, s), h[k].p, sizeof(float) * kFieldArrays * nf, hipMemcpyDeviceToDevice,
+                                       P.stream));
+            SWD_TRY(steps(t0, t1));
+        }
     }
     // reverse segments: recompute the background sums, then the adjoint steps
     for (int s = nseg - 1; s >= 0; s--) {

@@ -394,7 +394,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
             tuned[tkey] = cfg;
             tune_store(key, cfg);
         } else {
-            cfg = only2 ? kFstep2Default : kFstepDefault;   // too few steps to time the candidates
+            cfg = two ? kFstep2Default : kFstepDefault;   // too few steps to time the candidates
         }
     }
     P.stats.tile_x = fstep_cfg_tx(cfg);

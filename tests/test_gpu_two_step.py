@@ -213,3 +213,35 @@ def test_two_step_updown(shape, nt):
     for name in res[1]:
         assert np.isfinite(res[2][name]).all(), name
         assert np.array_equal(res[1][name], res[2][name]), name
+
+
+@pytest.mark.parametrize("nsnap", [7, 9])
+def test_two_step_born_adjoint_checkpoints(nsnap):
+    """The Born adjoint's checkpoint pass on the fused forward (two steps per
+    launch, snapshots taken from the slot that holds the state): the gradient
+    equals the one-step fused pass's and the batched path's bit for bit, with
+    segments of even (38) and odd (29) length, and the oracle's within 1e-5."""
+    c = configs.small(nshots=3, dup=True, nt=261, Lc=4)
+    print(f"case two-step born adjoint checkpoints nsnap={nsnap}", flush=True)
+    rng = np.random.default_rng(80 + nsnap)
+    ntr = sum(sh[2].size for sh in c.shots)
+    d = rng.standard_normal((ntr, c.grid.nt)).astype(np.float32)
+    res = {}
+    for steps in (1, 2):
+        plan = plan_of(c, steps)
+        res[steps] = plan.born_adjoint(c.shots, c.wavelet, d, nsnap=nsnap)
+        plan.close()
+    batched = swd.Plan(c.grid, c.coefs, c.vp, c.rho, device=0, path=swd.SWD_PATH_BATCHED)
+    gb = batched.born_adjoint(c.shots, c.wavelet, d, nsnap=nsnap)
+    batched.close()
+    assert np.isfinite(res[2]).all()
+    assert np.array_equal(res[1], res[2])
+    assert np.array_equal(res[2], gb)
+    og = np.zeros_like(gb, dtype=np.float64)
+    row = 0
+    for k in range(3):
+        n = c.shots[k][2].size
+        og += oracle_problem(c, k).born_adjoint(d[row:row + n]).astype(np.float64)
+        row += n
+    e32, _ = report(f"two-step born adjoint nsnap={nsnap}", res[2], og)
+    assert e32 <= TOL
