@@ -25,15 +25,32 @@ namespace {
 
 constexpr int kTX = 16, kNZ = 64;
 
+// The workgroup's tile: instances fastest, then z tiles, then x tiles, each
+// XCD taking a contiguous eighth of that order (workgroups are dealt to the
+// eight XCDs in turn), so a tile's halo neighbours and the model tile the
+// instances share are mostly read through the same L2.  False for the padding
+// workgroups of the last eighth.
+__device__ __forceinline__ bool adj_tile(const KGrid &g, int ninst, int &inst, int &X0, int &Z0)
+{
+    const int nzb = (g.ntz + kNZ - 1) / kNZ, nxb = (g.ntx + kTX - 1) / kTX;
+    const int L = xcd_block();
+    if (L >= ninst * nzb * nxb) return false;
+    inst = L % ninst;
+    Z0 = (L / ninst) % nzb * kNZ;
+    X0 = L / (ninst * nzb) * kTX;
+    return true;
+}
+
 template <int LC>
-__global__ __launch_bounds__(256) void k_adj_pv(KGrid g, const KInst *__restrict__ insts, int it)
+__global__ __launch_bounds__(256) void k_adj_pv(KGrid g, const KInst *__restrict__ insts, int ninst, int it)
 {
     constexpr int XL = LC - 1, NXE = kTX + 2 * LC - 1;   // ex: columns X0-XL .. X0+kTX+LC-1
     constexpr int ZL = LC - 1, NZE = kNZ + 2 * LC - 1;   // ez: rows Z0-ZL .. Z0+kNZ+LC-1
     __shared__ float sEX[NXE * kNZ];
     __shared__ float sEZ[kTX * NZE];
-    const KInst &I = insts[blockIdx.z];
-    const int X0 = blockIdx.y * kTX, Z0 = blockIdx.x * kNZ;
+    int inst, X0, Z0;
+    if (!adj_tile(g, ninst, inst, X0, Z0)) return;
+    const KInst &I = insts[inst];
     const int ntx = g.ntx, ntz = g.ntz;
     const size_t ldz = static_cast<size_t>(g.ldz);
     const bool odd = it & 1;
@@ -95,14 +112,15 @@ __global__ __launch_bounds__(256) void k_adj_pv(KGrid g, const KInst *__restrict
 }
 
 template <int LC>
-__global__ __launch_bounds__(256) void k_adj_vp(KGrid g, const KInst *__restrict__ insts, int it, int first)
+__global__ __launch_bounds__(256) void k_adj_vp(KGrid g, const KInst *__restrict__ insts, int ninst, int it, int first)
 {
     constexpr int XL = LC, NXE = kTX + 2 * LC - 1;   // ex: columns X0-LC .. X0+kTX+LC-2
     constexpr int ZL = LC, NZE = kNZ + 2 * LC - 1;   // ez: rows Z0-LC .. Z0+kNZ+LC-2
     __shared__ float sEX[NXE * kNZ];
     __shared__ float sEZ[kTX * NZE];
-    const KInst &I = insts[blockIdx.z];
-    const int X0 = blockIdx.y * kTX, Z0 = blockIdx.x * kNZ;
+    int inst, X0, Z0;
+    if (!adj_tile(g, ninst, inst, X0, Z0)) return;
+    const KInst &I = insts[inst];
     const int ntx = g.ntx, ntz = g.ntz;
     const size_t ldz = static_cast<size_t>(g.ldz);
     if (!first) {
@@ -184,10 +202,11 @@ __global__ __launch_bounds__(256) void k_adj_vp(KGrid g, const KInst *__restrict
 
 hipError_t launch_adjoint_step_tiled(const KGrid &g, const KInst *insts, int ninst, int it, bool first, hipStream_t s)
 {
-    const dim3 gr((g.ntz + kNZ - 1) / kNZ, (g.ntx + kTX - 1) / kTX, ninst);
+    const long long nblk = static_cast<long long>(ninst) * ((g.ntz + kNZ - 1) / kNZ) * ((g.ntx + kTX - 1) / kTX);
+    const dim3 gr(static_cast<unsigned>((nblk + 7) / 8 * 8));
     if (!first) {
-        DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_adj_pv<L>), gr, dim3(256), 0, s, g, insts, it));
+        DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_adj_pv<L>), gr, dim3(256), 0, s, g, insts, ninst, it));
     }
-    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_adj_vp<L>), gr, dim3(256), 0, s, g, insts, it, first ? 1 : 0));
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_adj_vp<L>), gr, dim3(256), 0, s, g, insts, ninst, it, first ? 1 : 0));
     return hipGetLastError();
 }
