@@ -367,10 +367,13 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         // time kTry units of every candidate after a warm-up unit of each, on
         // the loop's own steps, and keep the choice for later loops
         constexpr int kTry = 5;
+        // the first candidate is the reference another must beat by 10%: the
+        // two-step default where two steps per launch apply (faster in every
+        // loop measured), else the one-step default
         std::vector<int> cand;
-        if (!only2) cand = fstep_candidates(P.g.Lc, pair, cig, born, false);
-        if (two)
-            for (int c : fstep2_candidates()) cand.push_back(c);
+        if (two) cand = fstep2_candidates();
+        if (!only2)
+            for (int c : fstep_candidates(P.g.Lc, pair, cig, born, false)) cand.push_back(c);
         size_t need = 1;
         for (int c : cand) need += unit_steps(c) * (kTry + 1);
         if (its.size() - k >= need) {
