@@ -7,17 +7,19 @@ using namespace swd_rt;
 // ------------------------------------------------------------------ Born
 namespace {
 
-// Snapshot of the fused path's background state: array j (p, vx, vz, psx,
-// psz in the slot that holds the state, pvx, pvz) of shot k, src[per k + j],
-// to dst + kstride k + nf j, in float4s (nf a multiple of 4).
-__global__ __launch_bounds__(256) void k_snap(float4 *__restrict__ dst, const float4 *const *__restrict__ src, int per,
-                                              size_t nf4, size_t kstride4)
+// Snapshots of the fused path's background state: array j (p, vx, vz, psx,
+// psz in the slot that holds the state, pvx, pvz) of shot k, slots[per k + j],
+// and its snapshot at snap + kstride k + nf j, in float4s (nf a multiple of
+// 4); save copies the state to the snapshot, else the snapshot back.
+__global__ __launch_bounds__(256) void k_snap(float4 *__restrict__ snap, float4 *const *__restrict__ slots, int per,
+                                              size_t nf4, size_t kstride4, int save)
 {
     const int a = blockIdx.y, k = a / per, j = a - k * per;
-    const float4 *__restrict__ s = src[a];
-    float4 *__restrict__ t = dst + kstride4 * k + nf4 * j;
+    float4 *st = slots[a], *sn = snap + kstride4 * k + nf4 * j;
+    const float4 *__restrict__ from = save ? st : sn;
+    float4 *__restrict__ to = save ? sn : st;
     for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < nf4; i += static_cast<size_t>(gridDim.x) * 256)
-        t[i] = s[i];
+        to[i] = from[i];
 }
 
 // Born forward of a batch: m on the device [nshots][nx][nz], d on the device
@@ -141,6 +143,13 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
     const size_t ng = static_cast<size_t>(b.ntr) * nt;
     const int seg = nsnap > 0 ? (nt + nsnap - 1) / nsnap : static_cast<int>(std::ceil(std::sqrt(static_cast<double>(nt))));
     const int nseg = (nt + seg - 1) / seg;
+    // Fused path: the background runs as KStep instances (both slots), on
+    // the two-step forward where the plan allows; the batched path keeps one
+    // KInst state.  The adjoint fields carry ex, ez scratch, on the fused path
+    // the second slots of pvx, pvz, psx, psz (the tiled adjoint step).
+    const bool fused = P.path == SWD_PATH_FUSED;
+    const bool two = fused && P.g.Lc == 4 && P.steps_per_launch != 1 && P.ldz < (1 << 23);
+    const size_t per_b = fused ? kStepArrays : kFieldArrays, per_a = kFieldArrays + (fused ? 4 : 2);
     Carver c;
     const size_t o_ints = c.add<int>(b.ints.size());
     const size_t o_pairs = c.add<int2>(std::max<size_t>(1, b.dup_pairs.size()));
@@ -150,17 +159,9 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
     const size_t o_grad = c.add<float>(nint * nshots);
     const size_t o_s0 = c.add<float>(nint * seg * nshots);
     const size_t o_snap = c.add<float>(kFieldArrays * nf * static_cast<size_t>(nseg) * nshots);
-    // adjoint fields + ex, ez scratch (the tiled step of the fused path: the
-    // second slots of pvx, pvz, psx, psz)
-    const bool tiled = P.path == SWD_PATH_FUSED;
-    const size_t per_a = kFieldArrays + (tiled ? 4 : 2);
-    const size_t o_f = c.add<float>((kFieldArrays + per_a) * nf * nshots);
-    // fused path: the checkpoint pass runs the background as KStep instances
-    // (kStepArrays per shot, in the fields' storage before the adjoint
-    // starts) and snapshots the slot that holds the state
-    static_assert(kStepArrays <= 2 * kFieldArrays + 4, "the fused background fits the fields' storage");
-    const size_t o_step = c.add<KStep>(tiled ? nshots : 1);
-    const size_t o_snapsrc = c.add<float *>(tiled ? 2 * kFieldArrays * nshots : 1);
+    const size_t o_f = c.add<float>((per_b + per_a) * nf * nshots);
+    const size_t o_step = c.add<KStep>(fused ? nshots : 1);
+    const size_t o_slots = c.add<float *>(fused ? 2 * kFieldArrays * nshots : 1);
     SWD_TRY(ensure_ws(P, c.off));
     SWD_TRY(upload_batch(P, b, o_ints, o_pairs));
     HIP_TRY(hipMemcpyAsync(P.ws + o_wav, wavelet, sizeof(float) * nt, hipMemcpyHostToDevice, P.stream));
@@ -168,19 +169,38 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
     float *dsum = at<float>(P, o_d);
     HIP_TRY(hipMemcpyAsync(dsum, d, sizeof(float) * ng, hipMemcpyDeviceToDevice, P.stream));
     HIP_TRY(launch_sum_rows(dsum, d, at<int2>(P, o_pairs), static_cast<int>(b.dup_pairs.size()), nt, P.stream));
-    zero_fields(P, at<float>(P, o_f), (kFieldArrays + per_a) * nf * nshots);
+    zero_fields(P, at<float>(P, o_f), (per_b + per_a) * nf * nshots);
     zero_fields(P, at<float>(P, o_grad), nint * nshots);
     std::vector<KInst> h(2 * nshots);
-    float *fb = at<float>(P, o_f), *fa = fb + kFieldArrays * nf * nshots;
+    std::vector<KStep> hs(fused ? nshots : 0);
+    float *fb = at<float>(P, o_f), *fa = fb + per_b * nf * nshots;
+    // the batched kernels' view of the fused background: the arrays of slot c
+    auto view = [&](int k, int cs) {
+        const KStep &S = hs[k];
+        KInst &B = h[k];
+        B.p = S.p[cs]; B.vx = S.vx[cs]; B.vz = S.vz[cs]; B.psx = S.psx[cs]; B.psz = S.psz[cs];
+        B.pvx = S.pvx; B.pvz = S.pvz;
+    };
     for (int k = 0; k < nshots; k++) {
         KInst &B = h[k], &A = h[nshots + k];
         std::memset(&B, 0, sizeof B);
         std::memset(&A, 0, sizeof A);
-        bind_fields(B, fb + kFieldArrays * nf * k, nf);
+        if (fused) {
+            KStep &S = hs[k];
+            std::memset(&S, 0, sizeof S);
+            S.mode = KSTEP_FWD;
+            bind_step(S, fb + kStepArrays * nf * k, nf);
+            S.wavelet = at<float>(P, o_wav);
+            S.s_ix = shots[k].s_ix;
+            S.s_iz = shots[k].s_iz;
+            view(k, 0);
+        } else {
+            bind_fields(B, fb + kFieldArrays * nf * k, nf);
+        }
         bind_fields(A, fa + per_a * nf * k, nf);
         A.ex = A.p + kFieldArrays * nf;
         A.ez = A.ex + nf;
-        if (tiled) {
+        if (fused) {
             A.e2 = A.ez + nf;
             A.e3 = A.e2 + nf;
         }
@@ -196,6 +216,7 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
         A.s0 = at<float>(P, o_s0) + nint * seg * k;
     }
     const KInst *dB = at<KInst>(P, o_inst), *dA = dB + nshots;
+    const KStep *dS = at<KStep>(P, o_step);
     const KGrid &g = P.kg;
     long long fl = 0, bl = 0;
     // a snapshot holds the background state before the first step of a segment
@@ -203,83 +224,104 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
         return at<float>(P, o_snap) + kFieldArrays * nf * (static_cast<size_t>(nseg) * k + s);
     };
     auto steps = [&](int it0, int it1) -> int { return batched_forward_steps(g, dB, nshots, it0, it1, P.stream, &fl); };
+    // fused path: the state before step t0 is in slot (t0 + 1) & 1 after whole
+    // loops; k_snap moves it to or from the KInst-layout snapshot of segment s
+    float4 *const *dslots = at<float4 *>(P, o_slots);
+    auto snap_copy = [&](int s, int t0, bool save) -> int {
+        const size_t nf4 = nf / 4;
+        const dim3 sg(static_cast<unsigned>(std::min<size_t>((nf4 + 255) / 256, 1024)), kFieldArrays * nshots);
+        hipLaunchKernelGGL(k_snap, sg, dim3(256), 0, P.stream, reinterpret_cast<float4 *>(snap(0, s)),
+                           dslots + kFieldArrays * nshots * ((t0 + 1) & 1), kFieldArrays, nf4,
+                           kFieldArrays * nf4 * static_cast<size_t>(nseg), save ? 1 : 0);
+        HIP_TRY(hipGetLastError());
+        return SWD_OK;
+    };
+    if (fused) {
+        std::vector<float *> slots(2 * kFieldArrays * nshots);
+        for (int k = 0; k < nshots; k++)
+            for (int cs = 0; cs < 2; cs++) {
+                const KStep &S = hs[k];
+                float **q = slots.data() + kFieldArrays * (nshots * cs + k);
+                q[0] = S.p[cs]; q[1] = S.vx[cs]; q[2] = S.vz[cs]; q[3] = S.psx[cs]; q[4] = S.psz[cs];
+                q[5] = S.pvx; q[6] = S.pvz;
+            }
+        HIP_TRY(hipMemcpyAsync(P.ws + o_slots, slots.data(), sizeof(float *) * slots.size(), hipMemcpyHostToDevice,
+                               P.stream));
+        SWD_TRY(upload_steps(P, hs, o_step));
+    }
+    // the two-step configuration of the recompute pass: the forward loops' pick for this many
+    // instances, else the default
+    int cfg2 = kFstep2Default;
+    // each segment is its own loop: no graph is worth capturing for one replay
+    const bool graphs = P.graphs;
+    P.graphs = false;
+    int rc = SWD_OK;
     // checkpoint pass
     SWD_TRY(upload_insts(P, h, o_inst));
     stats_begin(P, 0);
-    if (tiled) {
-        // the fused forward (two steps per launch where the plan allows); the
-        // state before step t0 is in slot (t0 + 1) & 1 after whole loops
-        float *fs = at<float>(P, o_f);
-        zero_fields(P, fs, kStepArrays * nf * nshots);
-        std::vector<KStep> hs(nshots);
-        std::vector<float *> src(2 * kFieldArrays * nshots);
-        for (int k = 0; k < nshots; k++) {
-            KStep &S = hs[k];
-            std::memset(&S, 0, sizeof S);
-            S.mode = KSTEP_FWD;
-            bind_step(S, fs + kStepArrays * nf * k, nf);
-            S.wavelet = at<float>(P, o_wav);
-            S.s_ix = shots[k].s_ix;
-            S.s_iz = shots[k].s_iz;
-            for (int c2 = 0; c2 < 2; c2++) {
-                float **q = src.data() + kFieldArrays * (nshots * c2 + k);
-                q[0] = S.p[c2]; q[1] = S.vx[c2]; q[2] = S.vz[c2]; q[3] = S.psx[c2]; q[4] = S.psz[c2];
-                q[5] = S.pvx; q[6] = S.pvz;
-            }
-        }
-        SWD_TRY(upload_steps(P, hs, o_step));
-        HIP_TRY(hipMemcpyAsync(P.ws + o_snapsrc, src.data(), sizeof(float *) * src.size(), hipMemcpyHostToDevice, P.stream));
-        const float4 *const *dsrc = reinterpret_cast<const float4 *const *>(at<float *>(P, o_snapsrc));
-        const size_t nf4 = nf / 4;
-        const dim3 sg(static_cast<unsigned>(std::min<size_t>((nf4 + 255) / 256, 1024)), kFieldArrays * nshots);
-        // each segment is its own loop: no graph is worth capturing for one replay
-        const bool graphs = P.graphs;
-        P.graphs = false;
-        int rc = SWD_OK;
-        for (int s = 0; s < nseg && rc == SWD_OK; s++) {
-            const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
-            hipLaunchKernelGGL(k_snap, sg, dim3(256), 0, P.stream, reinterpret_cast<float4 *>(snap(0, s)),
-                               dsrc + kFieldArrays * nshots * ((t0 + 1) & 1), kFieldArrays, nf4,
-                               kFieldArrays * nf4 * static_cast<size_t>(nseg));
-            if (hipGetLastError() != hipSuccess) rc = swd_fail(SWD_E_DEVICE, "born adjoint: snapshot launch failed");
+    for (int s = 0; s < nseg && rc == SWD_OK; s++) {
+        const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
+        if (fused) {
+            rc = snap_copy(s, t0, true);
             std::vector<int> its;
             for (int it = t0; it < t1; it++) its.push_back(it);
             if (rc == SWD_OK)
-                rc = fused_steps(P, at<KStep>(P, o_step), nshots, its, -1, P.tuned_fwd, &fl, false, P.stream, nullptr,
-                                 false, false, true);
-        }
-        P.graphs = graphs;
-        SWD_TRY(rc);
-        // the adjoint fields start at zero; the background fields are restored from the snapshots
-        zero_fields(P, at<float>(P, o_f), (kFieldArrays + per_a) * nf * nshots);
-    } else {
-        for (int s = 0; s < nseg; s++) {
-            const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
-            for (int k = 0; k < nshots; k++)
-                HIP_TRY(hipMemcpyAsync(snap(k, s), h[k].p, sizeof(float) * kFieldArrays * nf, hipMemcpyDeviceToDevice,
-                                       P.stream));
-            SWD_TRY(steps(t0, t1));
+                rc = fused_steps(P, dS, nshots, its, -1, P.tuned_fwd, &fl, false, P.stream, nullptr, false, false, true);
+        } else {
+            for (int k = 0; k < nshots && rc == SWD_OK; k++)
+                if (hipMemcpyAsync(snap(k, s), h[k].p, sizeof(float) * kFieldArrays * nf, hipMemcpyDeviceToDevice,
+                                   P.stream) != hipSuccess)
+                    rc = swd_fail(SWD_E_DEVICE, "born adjoint: snapshot copy failed");
+            if (rc == SWD_OK) rc = steps(t0, t1);
         }
     }
-    // reverse segments: recompute the background sums, then the adjoint steps
-    for (int s = nseg - 1; s >= 0; s--) {
+    if (fused && rc == SWD_OK) {
+        const auto f = P.tuned_fwd.find(4 * nshots + (P.steps_per_launch == 2 ? 2 : 1));
+        if (f != P.tuned_fwd.end() && fstep_cfg_two(f->second)) cfg2 = f->second;
+    }
+    // reverse segments: recompute the background sums, then the adjoint steps.  On
+    // the fused path whole pairs of two-step launches come first (the state keeps
+    // its slot after four steps), then the rest on the batched kernels over the
+    // slot that holds the state.
+    for (int s = nseg - 1; s >= 0 && rc == SWD_OK; s--) {
         const int t0 = s * seg, t1 = std::min(nt, t0 + seg);
+        const int nq = two ? (t1 - t0) / 4 : 0;
+        if (fused) {
+            rc = snap_copy(s, t0, false);
+            for (int k = 0; k < nshots; k++) {
+                hs[k].s0_out = at<float>(P, o_s0) + nint * seg * k;
+                hs[k].s0_t0 = t0;
+                view(k, (t0 + 1) & 1);
+            }
+            if (rc == SWD_OK && nq > 0) rc = upload_steps(P, hs, o_step);
+            for (int q = 0; q < nq && rc == SWD_OK; q++)
+                for (int ph = 0; ph < 2 && rc == SWD_OK; ph++) {
+                    if (launch_fstep2(g, dS, nshots, t0 + 4 * q + 2 * ph, ph, cfg2, false, P.stream) != hipSuccess)
+                        rc = swd_fail(SWD_E_DEVICE, "born adjoint: two-step recompute launch failed");
+                    fl++;
+                }
+        } else {
+            for (int k = 0; k < nshots && rc == SWD_OK; k++)
+                if (hipMemcpyAsync(h[k].p, snap(k, s), sizeof(float) * kFieldArrays * nf, hipMemcpyDeviceToDevice,
+                                   P.stream) != hipSuccess)
+                    rc = swd_fail(SWD_E_DEVICE, "born adjoint: snapshot restore failed");
+        }
         for (int k = 0; k < nshots; k++) {
-            HIP_TRY(hipMemcpyAsync(h[k].p, snap(k, s), sizeof(float) * kFieldArrays * nf, hipMemcpyDeviceToDevice, P.stream));
             h[k].s0_out = at<float>(P, o_s0) + nint * seg * k;
             h[k].s0_t0 = t0;
             h[nshots + k].s0_t0 = t0;
         }
-        SWD_TRY(upload_insts(P, h, o_inst));
-        SWD_TRY(steps(t0, t1));
-        for (int it = t1 - 1; it >= t0; it--) {
-            if (tiled)
-                HIP_TRY(launch_adjoint_step_tiled(g, dA, nshots, it, it == nt - 1, P.stream));
-            else
-                HIP_TRY(launch_adjoint_step(g, dA, nshots, it, it == nt - 1, P.stream));
-            bl += it == nt - 1 ? 1 : (tiled ? 2 : 4);
+        if (rc == SWD_OK) rc = upload_insts(P, h, o_inst);
+        if (rc == SWD_OK) rc = steps(t0 + 4 * nq, t1);
+        for (int it = t1 - 1; it >= t0 && rc == SWD_OK; it--) {
+            const hipError_t e = fused ? launch_adjoint_step_tiled(g, dA, nshots, it, it == nt - 1, P.stream)
+                                       : launch_adjoint_step(g, dA, nshots, it, it == nt - 1, P.stream);
+            if (e != hipSuccess) rc = swd_fail(SWD_E_DEVICE, "born adjoint: adjoint step launch failed");
+            bl += it == nt - 1 ? 1 : (fused ? 2 : 4);
         }
     }
+    P.graphs = graphs;
+    SWD_TRY(rc);
     P.stats.fwd_ms = stats_end(P, 0);
     P.stats.fwd_launches = fl;
     P.stats.bwd_launches = bl;
@@ -323,7 +365,11 @@ extern "C" int swd_born_adjoint_dev(void *plan, int nshots, const swd_shot *shot
     const int nseg = (nt + seg - 1) / seg;
     int ntr_max = 0;
     for (int k = 0; k < nshots; k++) ntr_max = std::max(ntr_max, shots[k].nrec);
-    const double per = 4.0 * (7.0 * nf * nseg + nint * seg + 18.0 * nf + nint + static_cast<double>(ntr_max) * nt);
+    // per shot: snapshots, one segment of background sums, the background and
+    // adjoint fields (kStepArrays + 11 on the fused path, 7 + 9 on the batched),
+    // the gradient and the data
+    const double nfa = P->path == SWD_PATH_FUSED ? kStepArrays + kFieldArrays + 4.0 : 2.0 * kFieldArrays + 2.0;
+    const double per = 4.0 * (7.0 * nf * nseg + nint * seg + nfa * nf + nint + static_cast<double>(ntr_max) * nt);
     int wave = nshots;
     SWD_TRY(wave_shots(*P, nshots, per, 4.0 * nt * 4, wave));
     // Waves also hold their per-step working set (both wavefields, the

@@ -366,6 +366,7 @@ __device__ __forceinline__ bool f2_p_quad(const KGrid &g, const KStep &I, int x,
                 for (int ic = 0; ic < LC; ic++) dvz = __builtin_fmaf(g.rc[ic] * (vzw[4 + k + ic] - vzw[4 + k - ic - 1]), idz, dvz);
                 const float S = elc(dvx, k) + dvz;
                 el(pv, k) = __builtin_fmaf(elc(K, k), S, elc(pold, k));
+                if (MODE == KSTEP_FWD) el(dp, k) = S;
             }
         } else {
             const float ax = ldg(g.ax, x), bx = ldg(g.bx, x);
@@ -387,6 +388,7 @@ __device__ __forceinline__ bool f2_p_quad(const KGrid &g, const KStep &I, int x,
                 const float S = ((elc(dvx, k) + qx) + dvz) + qz;
                 const float np = __builtin_fmaf(elc(K, k), S, elc(pold, k));
                 el(pv, k) = in ? np : elc(pold, k);
+                if (MODE == KSTEP_FWD) el(dp, k) = S;
             }
         }
     }
@@ -410,6 +412,20 @@ __device__ __forceinline__ void f2_receivers(const KGrid &g, const KStep &I, int
             if (MODE == KSTEP_RCV) el(pv, k) = ((gcfl *)I.adj)[go];
             else if (I.gather) ((gfl *)I.gather)[go] = elc(pv, k);
         }
+    }
+}
+
+// The divergence sums S of one own quad's interior cells at level lv (FWD
+// with s0_out: the Born adjoint's background sums, as the batched k_p stores them).
+template <bool FULL>
+__device__ __forceinline__ void f2_s0(const KGrid &g, const KStep &I, int x, int z, int lv, float4 S)
+{
+    if (!FULL && (x < g.pmlc || x >= g.ntx - g.pmlc)) return;
+    gfl *dst = (gfl *)I.s0_out + static_cast<size_t>(lv - I.s0_t0) * g.nx * g.nz + static_cast<size_t>(x - g.pmlc) * g.nz;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int zk = z + k;
+        if (FULL || (zk >= g.pmlc && zk < g.ntz - g.pmlc)) dst[zk - g.pmlc] = elc(S, k);
     }
 }
 
@@ -657,6 +673,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
                 }
             }
             if (MODE == KSTEP_FWD && I.capture) f2_capture<FULL>(g, I, x, z, lv1, pv);
+            if (MODE == KSTEP_FWD && I.s0_out) f2_s0<FULL>(g, I, x, z, lv1, dp);
             if (NEG && I.pmid) st4((gfl *)I.pmid, off(x, z), pv);
             p1 = pv;
             qx1 = qxn;
@@ -748,6 +765,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
             }
         }
         if (MODE == KSTEP_FWD && I.capture) f2_capture<FULL>(g, I, x, z, lv2, pv);
+        if (MODE == KSTEP_FWD && I.s0_out) f2_s0<FULL>(g, I, x, z, lv2, dp);
         if (doimg) {
             f2_image<FULL>(g, z, dps[1], pv, im, is, ir);
             st4((gfl *)I.image, off(x, z), im);

@@ -75,6 +75,8 @@ struct KStep {
     float cw1, cw2;              // bin weights exp(-1/18), exp(-4/18) rounded to float
     float cthr;                  // cosine below which no bin is in reach
     const float *born_m;         // Born scattered field (paired after its background): dK/K [nx][nz], or null
+    float *s0_out;               // FWD: the divergence sum S of the pressure update on the interior cells,
+    int s0_t0;                   //   [it - s0_t0][nx][nz] (the Born adjoint's background sums), or null
 };
 
 // One shot of the angle-gather update that follows a backward step: the
