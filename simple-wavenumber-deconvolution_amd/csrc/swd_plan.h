@@ -178,8 +178,9 @@ constexpr double kCacheBudgetFwd = 128.0 * (1 << 20);
 struct StepHook {
     std::function<int(int, hipStream_t)> run;
     // after a two-step launch of steps it and it-1 (the paired backward), ph its place in
-    // its pair: queues the work of both steps (one launch); unset: no two-step launches
+    // its pair: queues the work of both steps (launches2 launches); unset: no two-step launches
     std::function<int(int, int, hipStream_t)> run2;
+    int launches2 = 1;
     std::vector<long long> key;
 };
 

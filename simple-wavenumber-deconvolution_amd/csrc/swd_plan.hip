@@ -354,7 +354,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
             nl++;
             if (hook) {
                 SWD_TRY(hook->run2(it, ph, st));
-                nl++;
+                nl += hook->launches2;
             }
         }
         k += 4;
@@ -433,7 +433,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         const size_t left = its.size() - k;
         const long long h = hook ? 1 : 0;
         const long long nrest = fstep_cfg_two(cfg) ? static_cast<long long>(left % 4) * (1 + h) +
-                                                         static_cast<long long>(left / 4) * 2 * (1 + h)
+                                                         static_cast<long long>(left / 4) * 2 * (1 + (hook ? hook->launches2 : 0))
                                                    : static_cast<long long>(left) * (1 + h);
         const long long before = nl;   // the first step's and the tuner's launches
         SWD_TRY(graph_run(P, gkey, rest, st));

@@ -173,6 +173,7 @@ int rtm_updown_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b,
             HIP_TRY(launch_updown2(P.kg, du + nshots, du + (ph == 0 ? 2 : 3) * nshots, n, nfftz, it, it - 1, st));
             return SWD_OK;
         };
+        hook.launches2 = updown2_launches(nfftz);
         hook.key = {2, reinterpret_cast<long long>(du), nfftz};
         SWD_TRY(fused_steps(P, at<KStep>(P, o_bwd) + 4 * k0, 4 * n, its, nt - 2, P.tuned_bwd, &launches, true,
                             rtm_group_stream(P, static_cast<int>(g)), &hook, false, false, true));

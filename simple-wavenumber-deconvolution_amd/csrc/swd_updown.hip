@@ -150,13 +150,21 @@ hipError_t launch_updown2(const KGrid &g, const KUpDown *a, const KUpDown *b, in
                           hipStream_t s)
 {
     if (g.nz > nfftz) return hipErrorInvalidValue;
+    // From 1024 points (16 values a thread) the two-level kernel holds about
+    // 190 VGPRs, two wavefronts per SIMD: c5 (nfftz 1024), 54.2 us a launch
+    // against 2 x 23.4 us for two one-level launches.  Up to 512 points it
+    // keeps four wavefronts per SIMD and is the faster one.
+    if (updown2_launches(nfftz) == 2) {
+        const hipError_t e = launch_updown(g, a, nshot, nfftz, ita, s);
+        return e != hipSuccess ? e : launch_updown(g, b, nshot, nfftz, itb, s);
+    }
     const dim3 grid(g.nx, nshot);
 #define UD(LOGN, V)                                                                                                \
     case 1 << LOGN:                                                                                                    \
         hipLaunchKernelGGL((k_updown2<LOGN, V>), grid, dim3(2 * (1 << LOGN) / V), 0, s, g, a, b, ita, itb);            \
         break;
     switch (nfftz) {
-        UD(7, 8) UD(8, 8) UD(9, 8) UD(10, 16) UD(11, 16) UD(12, 16)
+        UD(7, 8) UD(8, 8) UD(9, 8)
     default:
         return hipErrorInvalidValue;
     }
