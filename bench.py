@@ -541,6 +541,7 @@ def bench_afor(a):
     plan = swd.Plan(g, case.coefs, case.vp_true, case.rho, device=0, tile=a.tile)
     plan.set_groups(*a.groups)
     plan.set_streams(a.streams)
+    plan.set_steps_per_launch(a.steps_per_launch)
     shots = swd.ShotSet(case.shots)
     nfftz = swd.nfft(g.ntz)
     kz, _ = swd.kz_taper(nfftz, g.dz)
@@ -643,12 +644,16 @@ def bench_afor(a):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
-                     "kernel": "k_fstep (forward step of a group of p and Hilbert wavefields with capture)",
+                     "kernel": (f"k_fstep2 (two forward steps per launch, {st['fwd_tile_x']} x {st['fwd_tile_rows']}-row "
+                                f"tiles, of a group of p and Hilbert wavefields with capture)"
+                                if st["fwd_steps_per_launch"] == 2 else
+                                "k_fstep (forward step of a group of p and Hilbert wavefields with capture)"),
+                     "steps_per_launch": st["fwd_steps_per_launch"],
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": launch_ms,
                      "timing": "HIP events on the plan's stream around the wavefield loop (joining the other "
                                "streams) / launches: the interval between launch completions",
-                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per wavefield-step, "
-                                       "profiles/afor_c2_16shots.md" if traffic else None},
+                     "traffic_source": f"rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per wavefield-step, "
+                                       f"profiles/{os.path.basename(tfile)}" if traffic else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

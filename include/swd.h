@@ -176,9 +176,9 @@ int swd_plan_set_tile(void *plan, int tile_x);
  * reference launches three kernels per step, RK:1000-1045): 1, 2 (two steps
  * of every wavefield per launch, temporal blocking: the state, CPML memories
  * and image accumulators are read and written once per two steps), or 0 (the
- * default): the tuner times both kinds with the tile configurations and keeps
- * two steps per launch on 32 x 32-row tiles unless another configuration is
- * more than 10% faster (a loop too short to time them keeps that default).
+ * default): two steps per launch where they apply, the tuner timing their
+ * tile shapes and keeping 32 x 32 rows unless another is more than 10% faster
+ * (a loop too short to time them keeps 32 x 32).
  * With 2 and an asked tile configuration, its shape runs two steps per launch
  * where the two-step kernel has it (32 x 32, 16 x 64, 64 x 32, 32 x 64 rows),
  * else 32 x 32.  Two steps per launch need Lc 4; the backward of the angle

@@ -323,9 +323,10 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         const auto f = tuned.find(tkey);
         if (f != tuned.end()) cfg = f->second;
     }
+    // a pick from the plan or the disk cache: of the kind the tuner would time
     auto usable = [&](int c) {
-        return fstep_cfg_two(c) ? two && fstep2_cfg_supported(c)
-                                : !only2 && fstep_cfg_supported(P.g.Lc, pair, cig, born, false, c);
+        return two ? fstep_cfg_two(c) && fstep2_cfg_supported(c)
+                   : !fstep_cfg_two(c) && fstep_cfg_supported(P.g.Lc, pair, cig, born, false, c);
     };
     if (cfg == 0) {
         key = tune_key(P, mode.c_str(), n);
@@ -367,13 +368,12 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         // time kTry units of every candidate after a warm-up unit of each, on
         // the loop's own steps, and keep the choice for later loops
         constexpr int kTry = 5;
-        // the first candidate is the reference another must beat by 10%: the
-        // two-step default where two steps per launch apply (faster in every
-        // loop measured), else the one-step default
-        std::vector<int> cand;
-        if (two) cand = fstep2_candidates();
-        if (!only2)
-            for (int c : fstep_candidates(P.g.Lc, pair, cig, born, false)) cand.push_back(c);
+        // Where two steps per launch apply (faster in every loop measured) the
+        // candidates are the two-step shapes, else the one-step ones; the first
+        // is the reference another must beat by 10%.  (With both kinds timed, a
+        // one-step shape could win the short overlapped timing by 10% and lose
+        // the run: AFor at c2, 470 against 424 ms.)
+        const std::vector<int> cand = two ? fstep2_candidates() : fstep_candidates(P.g.Lc, pair, cig, born, false);
         size_t need = 1;
         for (int c : cand) need += unit_steps(c) * (kTry + 1);
         if (its.size() - k >= need) {
