@@ -1016,6 +1016,12 @@ def bench_c5(a):
     per_phase["gradient"]["full_storage_GB"] = stats["gradient"]["full_bytes"] / 1e9
     # the dominant kernel (the paired RTM backward step, one-shot groups at c5) alone on one stream
     cfg_b = launch_config(stats["rtm"])
+    # its counter traffic per shot-step, measured for two steps per launch on 32 x 32-row tiles
+    c5_traffic = None
+    tfile = os.path.join(ROOT, "profiles", "rtm_fused2_c5_traffic.json")
+    if tuple(cfg_b) == (2, 32, 32, 4) and os.path.exists(tfile) and (g.nx, g.nz, g.Lc) == (1690, 1001, 4):
+        with open(tfile) as f:
+            c5_traffic = json.load(f)["l2_miss_bytes_per_shot_step"]
     nt_iso = min(a.nt_iso, a.nt_c5)
     iso_ms, iso_lps = backward_launch_ms(swd, torch, case, adj, 1, 1, 1, nt_iso, a.tile, cfg_b)
     iso_bytes = (bb + mb * iso_lps) / iso_lps   # a launch advances 1 / iso_lps steps; the model once
@@ -1094,7 +1100,11 @@ def bench_c5(a):
                    else None,
                    "grad_finite": bool(torch.isfinite(d_grad).all().item())},
         "roofline": {"bound": "hbm", "achieved": iso, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": iso / HBM_PEAK_GBS,
-                     "traffic": None, "copy_GBs": copy_gbs, "frac_of_copy": iso / copy_gbs,
+                     "traffic": c5_traffic["backward"] / iso_lps if c5_traffic else None,
+                     "traffic_note": ("FETCH_SIZE x2 + WRITE_SIZE per launch (profiles/rtm_fused2_c5_traffic.json): "
+                                      "L2-miss bytes, which include last-level-cache hits, so an upper bound on HBM "
+                                      "bytes") if c5_traffic else None,
+                     "copy_GBs": copy_gbs, "frac_of_copy": iso / copy_gbs,
                      "kernel": (f"{'k_fstep2' if cfg_b[0] == 2 else 'k_fstep'} paired RTM backward "
                                 f"({cfg_b[0]} step(s) per launch, {cfg_b[1]} x {cfg_b[2]}-row tiles), one-shot group "
                                 "alone on one stream"),
@@ -1104,6 +1114,8 @@ def bench_c5(a):
                      "timing": f"HIP events on the plan's stream around {nt_iso - 1} graph-replayed launches",
                      "hbm_resident": {"achieved": res, "frac": res / HBM_PEAK_GBS, "shots_per_launch": res_n,
                                       "working_set_MiB": res_set / 2**20, "bytes_per_launch": res_bytes,
+                                      "traffic": (c5_traffic["backward_4shot_group"] * res_n / res_lps
+                                                  if c5_traffic and res_n == 4 else None),
                                       "avg_launch_ms": res_ms,
                                       "timing": f"{res_n}-shot group alone on one stream, {nt_iso - 1} steps: a "
                                                 "per-step working set about twice the 256 MB last-level cache"}},
