@@ -22,11 +22,24 @@
 
 namespace {
 
+// The twiddle half table e^{-2 pi i k/N}, k < N/2, staged into LDS once per
+// workgroup: every pass then reads it from LDS instead of waiting on L2 (the
+// kernel runs one workgroup's dependent chain of passes; the launches of the
+// backward loop leave every workgroup resident at once).  Visible to the
+// first twiddled pass through the first pass's barrier.
+template <int LOGN, int V>
+__device__ __forceinline__ void stage_twiddles(const float2 *__restrict__ tw, float2 *stw)
+{
+    constexpr int N = 1 << LOGN;
+    for (int i = threadIdx.x; i < N / 2; i += 2 * N / V) stw[i] = tw[i];
+}
+
 // kz split of one column of one shot at step it: the analytic source column
 // in L[0, NP), the receiver's in L[NP, 2 NP), both split into (Re up, Re
-// down) in natural order of the rows z0 + i.
+// down) in natural order of the rows z0 + i; tw: the twiddles in LDS.
 template <int LOGN, int V>
-__device__ __forceinline__ void ud_split(const KGrid &g, const KUpDown &U, int it, size_t c0, int z0, int nzw, float2 *lds)
+__device__ __forceinline__ void ud_split(const KGrid &g, const KUpDown &U, int it, size_t c0, int z0, int nzw, float2 *lds,
+                                         const float2 *tw)
 {
     constexpr int N = 1 << LOGN, T = N / V, NP = N + N / 16;
     const int f = threadIdx.x / T, t = threadIdx.x - f * T;   // field 0: source, 1: receiver
@@ -34,7 +47,6 @@ __device__ __forceinline__ void ud_split(const KGrid &g, const KUpDown &U, int i
     const int slot = it & 1;
     const float *re = (f ? U.pr[slot] : U.ds) + c0 + z0;
     const float *im = (f ? U.ph[slot] : U.dh) + c0 + z0;
-    const float2 *__restrict__ tw = U.tw;
 
     // transform of the column, zero padded to N, split, inverse transforms
     float2 v[V];
@@ -55,12 +67,14 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / V) void k_updown(KGrid g, const K
 {
     constexpr int N = 1 << LOGN, T = N / V, NP = N + N / 16;
     __shared__ float2 lds[2 * NP];
+    __shared__ float2 stw[N / 2];
     const int col = g.pmlc + blockIdx.x;
     const KUpDown &U = shots[blockIdx.y];
     const size_t c0 = static_cast<size_t>(col) * U.ld;
     // the whole padded column when it fits in N, else its interior rows
     const int z0 = N >= g.ntz ? 0 : g.pmlc, nzw = z0 ? g.nz : g.ntz;
-    ud_split<LOGN, V>(g, U, it, c0, z0, nzw, lds);
+    stage_twiddles<LOGN, V>(U.tw, stw);
+    ud_split<LOGN, V>(g, U, it, c0, z0, nzw, lds, stw);
 
     // imaging of the interior rows
     const float2 *S = lds, *Rv = lds + NP;
@@ -87,11 +101,13 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / V) void k_updown2(KGrid g, const 
 {
     constexpr int N = 1 << LOGN, T = N / V, NP = N + N / 16, NR = V / 2;
     __shared__ float2 lds[2 * NP];
+    __shared__ float2 stw[N / 2];
     const int col = g.pmlc + blockIdx.x;
     const KUpDown &A = sa[blockIdx.y], &B = sb[blockIdx.y];
     const size_t c0 = static_cast<size_t>(col) * A.ld;
     const int z0 = N >= g.ntz ? 0 : g.pmlc, nzw = z0 ? g.nz : g.ntz;
-    ud_split<LOGN, V>(g, A, ita, c0, z0, nzw, lds);
+    stage_twiddles<LOGN, V>(A.tw, stw);
+    ud_split<LOGN, V>(g, A, ita, c0, z0, nzw, lds, stw);
 
     const float2 *S = lds, *Rv = lds + NP;
     float *du = A.du + c0, *ud = A.ud + c0;
@@ -112,7 +128,7 @@ __global__ __launch_bounds__(2 * (1 << LOGN) / V) void k_updown2(KGrid g, const 
         }
     }
     __syncthreads();   // level a's spectra read before level b's first pass overwrites them
-    ud_split<LOGN, V>(g, B, itb, c0, z0, nzw, lds);
+    ud_split<LOGN, V>(g, B, itb, c0, z0, nzw, lds, stw);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
         const int iz = iz0 + r * 2 * T;
