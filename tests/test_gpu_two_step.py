@@ -215,12 +215,15 @@ def test_two_step_updown(shape, nt):
         assert np.array_equal(res[1][name], res[2][name]), name
 
 
-@pytest.mark.parametrize("nsnap", [7, 9])
+@pytest.mark.parametrize("nsnap", [7, 9, 65, 87])
 def test_two_step_born_adjoint_checkpoints(nsnap):
-    """The Born adjoint's checkpoint pass on the fused forward (two steps per
-    launch, snapshots taken from the slot that holds the state): the gradient
-    equals the one-step fused pass's and the batched path's bit for bit, with
-    segments of even (38) and odd (29) length, and the oracle's within 1e-5."""
+    """The Born adjoint's background passes on the fused forward (two steps per
+    launch, snapshots taken from and restored into the slot that holds the
+    state, the recompute's sums from both levels of a launch and from the
+    batched kernels for a segment's leftover steps): the gradient equals the
+    one-step fused passes' and the batched path's bit for bit, with segments of
+    38, 29, 5 (one pair of launches and one leftover step) and 3 steps (no
+    pair), and the oracle's within 1e-5."""
     c = configs.small(nshots=3, dup=True, nt=261, Lc=4)
     print(f"case two-step born adjoint checkpoints nsnap={nsnap}", flush=True)
     rng = np.random.default_rng(80 + nsnap)
