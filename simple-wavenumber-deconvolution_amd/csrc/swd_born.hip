@@ -276,7 +276,7 @@ int born_adjoint_dev(Plan &P, int nshots, const swd_shot *shots, const float *wa
         }
     }
     if (fused && rc == SWD_OK) {
-        const auto f = P.tuned_fwd.find(4 * nshots + (P.steps_per_launch == 2 ? 2 : 1));
+        const auto f = P.tuned_fwd.find(4 * nshots + 2);   // fused_steps' key of a two-step loop
         if (f != P.tuned_fwd.end() && fstep_cfg_two(f->second)) cfg2 = f->second;
     }
     // reverse segments: recompute the background sums, then the adjoint steps.  On

@@ -301,10 +301,11 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     const int dir = pair ? -1 : 1;
     for (size_t j = 1; two && j < its.size(); j++)
         if (its[j] != its[j - 1] + dir) two = false;
+    // steps = 2 asked: an asked tile runs two steps per launch in its shape (or the nearest)
     const bool only2 = two && P.steps_per_launch == 2;
-    const int variant = two ? (only2 ? 2 : 1) : 0;
-    const std::string mode = std::string(born ? "born" : cig ? "cig" : pair ? "pair" : "fwd") +
-                             (variant == 2 ? " two" : variant == 1 ? " one+two" : "");
+    // the tuner's kind: two-step shapes wherever two steps per launch apply
+    const int variant = two ? 2 : 0;
+    const std::string mode = std::string(born ? "born" : cig ? "cig" : pair ? "pair" : "fwd") + (two ? " two" : "");
     int cfg = P.tile_cfg;
     if (cfg != 0) {
         const int tx = fstep_cfg_tx(cfg), tq = fstep_cfg_tq(cfg);
