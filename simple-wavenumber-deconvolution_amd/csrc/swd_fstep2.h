@@ -35,9 +35,11 @@
 // and the paired RTM backward: SRC (reconstruction with wavelet subtraction
 // and frame restore; dp/dt of both levels stored for the up/down imaging if
 // asked) then RCV (receiver back-propagation with hard replacement and the
-// imaging condition; the first level's pressure stored if asked).  Not here:
-// angle gathers, Born, the first reconstruction step (which keeps the
-// forward's velocities).
+// imaging condition; the first level's pressure stored if asked).  The MID
+// variant of the paired kernel also stores both fields' first level (p and
+// velocities) for the angle gathers, which k_cig bins after the launch.  Not
+// here: Born, the first reconstruction step (which keeps the forward's
+// velocities).
 #pragma once
 #include <algorithm>
 
@@ -461,7 +463,7 @@ __device__ __forceinline__ void f2_image(const KGrid &g, int z, float4 dp, float
 
 // Two steps of one wavefield on one tile.  dps: dp/dt of the own quad at the
 // two levels (SRC writes them, RCV images with them).
-template <int TX, int TQ, int NW, bool FULL, int MODE>
+template <int TX, int TQ, int NW, bool FULL, int MODE, bool MID = false>
 __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0, int Z0, int it, int ph,
                                          float4 *__restrict__ sP, float4 *__restrict__ sVX, float4 *__restrict__ sVZ,
                                          float4 (&dps)[2])
@@ -684,6 +686,15 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
         if (wr) sP[G::sp(c, q)] = pv;
     }
     __syncthreads();
+    if constexpr (MID) {
+        // the middle level for the angle gathers: the own quad's first-level velocities
+        // (still in LDS) and, for the reconstruction, its first-level pressure
+        if (oin) {
+            if (SRC) st4((gfl *)I.pmid, off(ox, oz), sP[G::sp(oc, oq)]);
+            st4((gfl *)I.vxmid, off(ox, oz), sVX[G::sx(oc, oq)]);
+            st4((gfl *)I.vzmid, off(ox, oz), sVZ[G::sz(oc, oq)]);
+        }
+    }
 
     // ---- step 2, velocities (level t+3/2) on the core regions; own cells stored
 #pragma unroll
@@ -792,7 +803,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
 #ifndef SWD_F2_WAVES_FRAME
 #define SWD_F2_WAVES_FRAME 4   // the paired kernel over every tile: spills a few dwords, 6% faster than three
 #endif
-template <int TX, int TQ, int NW, bool PAIR, int SEL>
+template <int TX, int TQ, int NW, bool PAIR, int SEL, bool MID = false>
 __global__ __launch_bounds__(64 * NW, SEL == 1 ? (PAIR ? SWD_F2_WAVES_FAST : SWD_F2_WAVES_FWD) : (PAIR ? SWD_F2_WAVES_FRAME : SWD_F2_WAVES_FWD)) void k_fstep2(
     KGrid g, const KStep *__restrict__ insts, int ninst, int nzb, int nxb, int4 fr, int it, int ph)
 {
@@ -865,12 +876,12 @@ __global__ __launch_bounds__(64 * NW, SEL == 1 ? (PAIR ? SWD_F2_WAVES_FAST : SWD
     const bool any = X0 + TX > g.pmlc && X0 < g.ntx - g.pmlc && Z0 + rows > g.pmlc && Z0 < g.ntz - g.pmlc;
     const bool src = S.s_ix >= X0 && S.s_ix < X0 + TX && S.s_iz >= Z0 && S.s_iz < Z0 + rows;
     if (any || src) {
-        if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_SRC>(g, S, X0, Z0, it, ph, sP, sVX, sVZ, dps);
-        else f2_field<TX, TQ, NW, SEL == 1, KSTEP_SRC>(g, S, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+        if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_SRC, MID>(g, S, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+        else f2_field<TX, TQ, NW, SEL == 1, KSTEP_SRC, MID>(g, S, X0, Z0, it, ph, sP, sVX, sVZ, dps);
         __syncthreads();   // LDS is staged again for the receiver wavefield
     }
-    if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_RCV>(g, Rv, X0, Z0, it, ph, sP, sVX, sVZ, dps);
-    else f2_field<TX, TQ, NW, SEL == 1, KSTEP_RCV>(g, Rv, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+    if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_RCV, MID>(g, Rv, X0, Z0, it, ph, sP, sVX, sVZ, dps);
+    else f2_field<TX, TQ, NW, SEL == 1, KSTEP_RCV, MID>(g, Rv, X0, Z0, it, ph, sP, sVX, sVZ, dps);
 }
 
 // The fast rectangle of a grid for TX x 4TQ tiles (empty: y = w = 0): a tile

@@ -69,7 +69,10 @@ struct KStep {
     float *image, *illum_s, *illum_r;   // RCV (paired with the SRC of the same shot)
     float *dpdt;                 // SRC: dp/dt of step it stored on the interior [ntx][ldz], or null
     float *dpdt_b;               // SRC, two steps per launch: dp/dt of the second level (dpdt holds the first)
-    float *pmid;                 // RCV, two steps per launch: p of the first level, own cells, or null
+    float *pmid;                 // RCV, two steps per launch: p of the first level, own cells, or null (and
+                                 //   SRC in the middle-level variant: p of the first level, own cells)
+    float *vxmid, *vzmid;        // SRC, RCV in the middle-level variant of the two-step kernel (angle
+                                 //   gathers): the velocities of the first level, own cells
     float *cig;                  // RCV, angle gathers in the paired launch: [nx*nz][thetan], or null
     int thetan;
     float cw1, cw2;              // bin weights exp(-1/18), exp(-4/18) rounded to float

@@ -181,6 +181,8 @@ struct StepHook {
     // its pair: queues the work of both steps (launches2 launches); unset: no two-step launches
     std::function<int(int, int, hipStream_t)> run2;
     int launches2 = 1;
+    // the two-step launches are the paired kernel's middle-level variant (default configuration)
+    bool mid = false;
     std::vector<long long> key;
 };
 

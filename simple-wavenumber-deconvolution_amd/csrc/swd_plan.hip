@@ -318,6 +318,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         else if (!fstep_cfg_supported(P.g.Lc, pair, cig, born, false, cfg))
             cfg = fstep_cfg(tx, 16, 0);   // the asked width with the variant's 64-row tiles
     }
+    if (two && hook && hook->mid) cfg = kFstep2Default;   // the only shape of the middle-level variant
     const int tkey = 4 * n + variant;
     std::string key;
     if (cfg == 0) {
@@ -352,7 +353,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     auto quad = [&](int c) -> int {
         for (int ph = 0; ph < 2; ph++) {
             const int it = its[k + 2 * ph];
-            HIP_TRY(launch_fstep2(P.kg, d, n, it, ph, c, pair, st));
+            HIP_TRY(launch_fstep2(P.kg, d, n, it, ph, c, pair, st, hook && hook->mid));
             nl++;
             if (hook) {
                 SWD_TRY(hook->run2(it, ph, st));

@@ -41,7 +41,14 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     const size_t o_f = c.add<float>(kStepArrays * nf * 2 * nshots);   // source side, then receiver side
     const size_t nint = static_cast<size_t>(P.g.nx) * P.g.nz;
     const size_t n_ring = cig && cig->delt > 1 ? 2 * static_cast<size_t>(cig->delt) * nint : 0;
-    const size_t o_kcig = cig ? c.add<KCig>(nshots) : 0;
+    // Angle gathers with delt 1 and two steps per launch: the paired kernel's middle-level
+    // variant stores both fields' first level, and k_cig bins the launch's two steps from
+    // that and from the slots (three descriptor tables a shot: the slots as after a pair of
+    // launches, the slots swapped as after the first launch of a pair, the middle level).
+    const bool two_cig = cig && cig->delt == 1 && P.g.Lc == 4 && P.steps_per_launch != 1 && P.ldz < (1 << 23) &&
+                         (P.tile_cfg == 0 || P.steps_per_launch == 2) && nt >= 3;
+    const size_t o_kcig = cig ? c.add<KCig>((two_cig ? 3 : 1) * nshots) : 0;
+    const size_t o_mid = c.add<float>(two_cig ? 6 * nf * nshots : 1);
     const size_t o_cig = cig ? c.add<float>(nint * cig->thetan * nshots) : 0;
     const size_t o_ring = cig ? c.add<float>(std::max<size_t>(1, n_ring * nshots)) : 0;
     SWD_TRY(ensure_ws(P, c.off));
@@ -86,6 +93,17 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     // then only adds step 0 after the loop.  Averaging over delt > 1 steps
     // runs k_cig after every launch.
     const bool fuse_cig = cig && cig->delt == 1;
+    if (two_cig)
+        for (int k = 0; k < nshots; k++) {
+            KStep &S = hb[2 * k], &R = hb[2 * k + 1];
+            float *mid = at<float>(P, o_mid) + 6 * nf * k;
+            S.pmid = mid;
+            S.vxmid = mid + nf;
+            S.vzmid = mid + 2 * nf;
+            R.pmid = mid + 3 * nf;
+            R.vxmid = mid + 4 * nf;
+            R.vzmid = mid + 5 * nf;
+        }
     // A bin needs half-angle <= thetan + 1.5 degrees, that is cos(full angle)
     // >= cos((thetan + 1.5) pi / 90); the test uses thetan + 2 and a margin far
     // above the float error of its estimate.
@@ -106,22 +124,25 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     if (cig) {
         zero_fields(P, at<float>(P, o_cig), nint * cig->thetan * nshots);
         if (n_ring) zero_fields(P, at<float>(P, o_ring), n_ring * nshots);
-        std::vector<KCig> hk(nshots);
+        std::vector<KCig> hk((two_cig ? 3 : 1) * nshots);
         for (int k = 0; k < nshots; k++) {
             const KStep &S = hb[2 * k], &R = hb[2 * k + 1];
-            KCig &K = hk[k];
-            for (int q = 0; q < 2; q++) {
-                K.ps[q] = S.p[q];
-                K.vxs[q] = S.vx[q];
-                K.vzs[q] = S.vz[q];
-                K.pr[q] = R.p[q];
-                K.vxr[q] = R.vx[q];
-                K.vzr[q] = R.vz[q];
+            for (int v = 0; v < (two_cig ? 3 : 1); v++) {
+                KCig &K = hk[v * nshots + k];
+                for (int q = 0; q < 2; q++) {
+                    const int u = v == 1 ? 1 - q : q;
+                    K.ps[q] = v == 2 ? S.pmid : S.p[u];
+                    K.vxs[q] = v == 2 ? S.vxmid : S.vx[u];
+                    K.vzs[q] = v == 2 ? S.vzmid : S.vz[u];
+                    K.pr[q] = v == 2 ? R.pmid : R.p[u];
+                    K.vxr[q] = v == 2 ? R.vxmid : R.vx[u];
+                    K.vzr[q] = v == 2 ? R.vzmid : R.vz[u];
+                }
+                K.ring = n_ring ? at<float>(P, o_ring) + n_ring * k : nullptr;
+                K.cig = at<float>(P, o_cig) + nint * cig->thetan * k;
             }
-            K.ring = n_ring ? at<float>(P, o_ring) + n_ring * k : nullptr;
-            K.cig = at<float>(P, o_cig) + nint * cig->thetan * k;
         }
-        HIP_TRY(hipMemcpyAsync(P.ws + o_kcig, hk.data(), sizeof(KCig) * nshots, hipMemcpyHostToDevice, P.stream));
+        HIP_TRY(hipMemcpyAsync(P.ws + o_kcig, hk.data(), sizeof(KCig) * hk.size(), hipMemcpyHostToDevice, P.stream));
     }
     // Shots run in groups whose per-step working set stays in the last-level
     // cache; every shot keeps its strips and final state, so the groups of the
@@ -155,6 +176,9 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
         HIP_TRY(hipMemcpyAsync(F.p[oth], F.p[fin], sizeof(float) * nf, hipMemcpyDeviceToDevice, P.stream));
         HIP_TRY(hipMemcpyAsync(F.vx[oth], F.vx[fin], sizeof(float) * nf, hipMemcpyDeviceToDevice, P.stream));
         HIP_TRY(hipMemcpyAsync(F.vz[oth], F.vz[fin], sizeof(float) * nf, hipMemcpyDeviceToDevice, P.stream));
+        // the middle level's reconstructed pressure outside the interior too (its stencil reads there)
+        if (two_cig)
+            HIP_TRY(hipMemcpyAsync(hb[2 * k].pmid, F.p[fin], sizeof(float) * nf, hipMemcpyDeviceToDevice, P.stream));
     }
     // backward: it = nt-2 .. 0 (RK:1073), then the imaging of step 0
     std::vector<int> its;
@@ -167,6 +191,34 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
         const int k0 = gbb[g], n = gbb[g + 1] - gbb[g];
         const KStep *d = at<KStep>(P, o_bwd) + 2 * k0;
         const hipStream_t gs = group_stream(static_cast<int>(g));
+        if (two_cig) {
+            // the first launch bins step nt-1 itself (it keeps the forward's velocities), k_cig
+            // then every step after its launch: step it from the slots after a one-step
+            // launch, steps it (middle level) and it-1 after a two-step launch (it, it-1)
+            const KCig *k1 = at<KCig>(P, o_kcig) + k0, *ksw = k1 + nshots, *kmid = k1 + 2 * nshots;
+            const int thetan = cig->thetan;
+            HIP_TRY(launch_fstep(P.kg, d, 2 * n, nt - 2, true, kFstepDefault, true, true, false, gs));
+            HIP_TRY(launch_cig(P.kg, k1, n, thetan, 1, w1, w2, cthr, nt - 2, gs));
+            launches += 2;
+            std::vector<int> its2(its.begin() + 1, its.end());
+            StepHook hook;
+            hook.mid = true;
+            hook.launches2 = 2;
+            hook.run = [&P, k1, n, thetan, w1, w2, cthr](int it, hipStream_t st) -> int {
+                HIP_TRY(launch_cig(P.kg, k1, n, thetan, 1, w1, w2, cthr, it, st));
+                return SWD_OK;
+            };
+            hook.run2 = [&P, k1, ksw, kmid, n, thetan, w1, w2, cthr](int it, int ph, hipStream_t st) -> int {
+                HIP_TRY(launch_cig(P.kg, kmid, n, thetan, 1, w1, w2, cthr, it, st));
+                HIP_TRY(launch_cig(P.kg, ph ? k1 : ksw, n, thetan, 1, w1, w2, cthr, it - 1, st));
+                return SWD_OK;
+            };
+            hook.key = {4, reinterpret_cast<long long>(k1), thetan};
+            if (!its2.empty())
+                SWD_TRY(fused_steps(P, d, 2 * n, its2, nt - 2, P.tuned_bwd, &launches, true, gs, &hook, false, false,
+                                    true));
+            continue;
+        }
         if (fuse_cig) {
             SWD_TRY(fused_steps(P, d, 2 * n, its, nt - 2, P.tuned_cig, &launches, true, gs, nullptr, true));
             HIP_TRY(launch_cig(P.kg, at<KCig>(P, o_kcig) + k0, n, cig->thetan, 1, w1, w2, cthr, 0, gs));   // step 0

@@ -248,3 +248,39 @@ def test_two_step_born_adjoint_checkpoints(nsnap):
         row += n
     e32, _ = report(f"two-step born adjoint nsnap={nsnap}", res[2], og)
     assert e32 <= TOL
+
+
+@pytest.mark.parametrize("nt", [161, 162, 163, 164])
+def test_two_step_angle_gathers(nt):
+    """Angle gathers (delt 1) with the backward two steps per launch: the
+    paired kernel's middle-level variant stores both fields' first level and
+    k_cig bins the two steps of each launch (the first launch bins step nt-1
+    itself).  Gathers, image and illuminations equal one step per launch bit
+    for bit, for every remainder of the step count modulo 4, two shots (one
+    source on the interior frame) on two streams, and the oracle's."""
+    c = configs.small(nx=150, nz=110, pml=10, Lc=4, nt=nt, nshots=2)
+    s_ix, s_iz, rx, rz = c.shots[1]
+    c.shots[1] = (c.pmlc + 1, c.pmlc + 40, rx, rz)
+    print(f"case two-step angle gathers nt={nt}", flush=True)
+    outs = [oracle_problem(c, k).forward(strips=True, final=True) for k in range(2)]
+    adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
+    res = {}
+    for steps in (1, 2):
+        plan = plan_of(c, steps)
+        plan.set_groups(1, 1)
+        plan.set_streams(2)
+        res[steps] = plan.rtm_cig(c.shots, c.wavelet, adj, 10, 1)
+        assert plan.stats()["steps_per_launch"] == steps
+        plan.close()
+    for name, a, b in zip(("image", "illum_src", "illum_rcv", "cig"), res[1], res[2]):
+        assert np.isfinite(b).all(), name
+        assert np.array_equal(a, b), name
+    assert np.abs(res[2][3]).max() > 0
+    row = 0
+    for k in range(2):
+        d, st, fs = outs[k]
+        n = d.shape[0]
+        o = oracle_problem(c, k).rtm_cig(fs, st, adj[row:row + n], 10, 1)
+        assert np.array_equal(res[2][3][k], o[3]), f"shot {k} cig"
+        assert np.array_equal(res[2][0][k], o[0]), f"shot {k} image"
+        row += n
