@@ -64,7 +64,44 @@ __global__ __launch_bounds__(256) void k_cig(KGrid g, const KCig *__restrict__ s
     cig_bins(pre, pst, ps * pr, thetan, w1, w2, thr, K.cig + static_cast<size_t>(c) * thetan);
 }
 
+// Two levels of one cell in one launch (delt 1): step ita from the descriptors
+// a, then step itb from b (the same gathers), the order of two k_cig
+// launches; the model's Bx, Bz read once.
+template <int LC>
+__global__ __launch_bounds__(256) void k_cig2(KGrid g, const KCig *__restrict__ sa, const KCig *__restrict__ sb,
+                                              int thetan, float w1, float w2, float thr, int ita, int itb)
+{
+    const int nint = g.nx * g.nz;
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
+    if (c >= nint) return;
+    const int ix = g.pmlc + c / g.nz, iz = g.pmlc + c % g.nz;
+    const size_t ld = static_cast<size_t>(g.ldz), o = static_cast<size_t>(ix) * ld + iz;
+    const float bx = g.Bx[o], bz = g.Bz[o];
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+        const KCig &K = l ? sb[blockIdx.y] : sa[blockIdx.y];
+        const int it = l ? itb : ita, s = it & 1, s1 = (it + 1) & 1;
+        const float *P = K.ps[s];
+        const float ps = P[o];
+        const float vxs = __builtin_fmaf(-bx, dplus<LC>(P, o, ld, g.rc, g.idx), K.vxs[s1][o]);
+        const float vzs = __builtin_fmaf(-bz, dplus<LC>(P, o, 1, g.rc, g.idz), K.vzs[s1][o]);
+        const float pr = K.pr[s][o], vxr = K.vxr[s][o], vzr = K.vzr[s][o];
+        float F, a, b;
+        cig_products(ps, vxs, vzs, pr, vxr, vzr, F, a, b);
+        if (!cig_may_bin(F, a, b, thr)) continue;
+        cig_bins(F, cig_norm(a, b), ps * pr, thetan, w1, w2, thr, K.cig + static_cast<size_t>(c) * thetan);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_cig2(const KGrid &g, const KCig *a, const KCig *b, int nshot, int thetan, float w1, float w2, float thr,
+                       int ita, int itb, hipStream_t s)
+{
+    const dim3 grid(((g.nx * g.nz + 255) / 256 + 7) / 8 * 8, nshot);
+    DISPATCH_LC(g.Lc, hipLaunchKernelGGL((k_cig2<L>), grid, dim3(256), 0, s, g, a, b, thetan, w1, w2, thr, ita, itb));
+    return hipGetLastError();
+}
 
 hipError_t launch_cig(const KGrid &g, const KCig *shots, int nshot, int thetan, int delt, float w1, float w2, float thr,
                       int it, hipStream_t s)

@@ -90,3 +90,7 @@ constexpr int updown2_launches(int nfftz) { return nfftz >= 1024 ? 2 : 1; }
 // cosine below which no bin is in reach (cig_bins)
 hipError_t launch_cig(const KGrid &g, const KCig *shots, int nshot, int thetan, int delt, float w1, float w2, float thr,
                       int it, hipStream_t s);
+// delt 1: steps ita (descriptors a) then itb (descriptors b) in one launch, the
+// order of two launch_cig calls
+hipError_t launch_cig2(const KGrid &g, const KCig *a, const KCig *b, int nshot, int thetan, float w1, float w2, float thr,
+                       int ita, int itb, hipStream_t s);

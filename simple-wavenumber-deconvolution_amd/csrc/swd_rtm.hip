@@ -194,7 +194,8 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
         if (two_cig) {
             // the first launch bins step nt-1 itself (it keeps the forward's velocities), k_cig
             // then every step after its launch: step it from the slots after a one-step
-            // launch, steps it (middle level) and it-1 after a two-step launch (it, it-1)
+            // launch, steps it (middle level) and it-1 after a two-step launch (it, it-1),
+            // both in one k_cig2 launch
             const KCig *k1 = at<KCig>(P, o_kcig) + k0, *ksw = k1 + nshots, *kmid = k1 + 2 * nshots;
             const int thetan = cig->thetan;
             HIP_TRY(launch_fstep(P.kg, d, 2 * n, nt - 2, true, kFstepDefault, true, true, false, gs));
@@ -203,14 +204,13 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
             std::vector<int> its2(its.begin() + 1, its.end());
             StepHook hook;
             hook.mid = true;
-            hook.launches2 = 2;
+            hook.launches2 = 1;   // one k_cig2 for both levels
             hook.run = [&P, k1, n, thetan, w1, w2, cthr](int it, hipStream_t st) -> int {
                 HIP_TRY(launch_cig(P.kg, k1, n, thetan, 1, w1, w2, cthr, it, st));
                 return SWD_OK;
             };
             hook.run2 = [&P, k1, ksw, kmid, n, thetan, w1, w2, cthr](int it, int ph, hipStream_t st) -> int {
-                HIP_TRY(launch_cig(P.kg, kmid, n, thetan, 1, w1, w2, cthr, it, st));
-                HIP_TRY(launch_cig(P.kg, ph ? k1 : ksw, n, thetan, 1, w1, w2, cthr, it - 1, st));
+                HIP_TRY(launch_cig2(P.kg, kmid, ph ? k1 : ksw, n, thetan, w1, w2, cthr, it, it - 1, st));
                 return SWD_OK;
             };
             hook.key = {4, reinterpret_cast<long long>(k1), thetan};
