@@ -183,9 +183,8 @@ int swd_plan_set_tile(void *plan, int tile_x);
  * where the two-step kernel has it (32 x 32, 16 x 64, 64 x 32, 32 x 64 rows),
  * else 32 x 32.  Two steps per launch need Lc 4; they cover the forward
  * loops (with the Born adjoint's background passes) and the paired backward
- * of the RTM, the up/down RTM and the angle gathers with delt 1; the angle
- * gathers with delt > 1 and the Born forward pair keep one step per launch.
- * Results do not depend on it. */
+ * of the RTM, the up/down RTM and the angle-gather RTM; the Born forward pair
+ * keeps one step per launch.  Results do not depend on it. */
 int swd_plan_set_steps_per_launch(void *plan, int steps);
 /* Group sizes of the fused loops: the RTM forward and backward run their
  * batch in groups of at most fwd_shots and bwd_shots shots, and swd_forward,

@@ -41,11 +41,11 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
     const size_t o_f = c.add<float>(kStepArrays * nf * 2 * nshots);   // source side, then receiver side
     const size_t nint = static_cast<size_t>(P.g.nx) * P.g.nz;
     const size_t n_ring = cig && cig->delt > 1 ? 2 * static_cast<size_t>(cig->delt) * nint : 0;
-    // Angle gathers with delt 1 and two steps per launch: the paired kernel's middle-level
-    // variant stores both fields' first level, and k_cig bins the launch's two steps from
-    // that and from the slots (three descriptor tables a shot: the slots as after a pair of
-    // launches, the slots swapped as after the first launch of a pair, the middle level).
-    const bool two_cig = cig && cig->delt == 1 && P.g.Lc == 4 && P.steps_per_launch != 1 && P.ldz < (1 << 23) &&
+    // Angle gathers with two steps per launch: the paired kernel's middle-level variant
+    // stores both fields' first level, and k_cig bins the launch's two steps from that and
+    // from the slots (three descriptor tables a shot: the slots as after a pair of launches,
+    // the slots swapped as after the first launch of a pair, the middle level).
+    const bool two_cig = cig && P.g.Lc == 4 && P.steps_per_launch != 1 && P.ldz < (1 << 23) &&
                          (P.tile_cfg == 0 || P.steps_per_launch == 2) && nt >= 3;
     const size_t o_kcig = cig ? c.add<KCig>((two_cig ? 3 : 1) * nshots) : 0;
     const size_t o_mid = c.add<float>(two_cig ? 6 * nf * nshots : 1);
@@ -192,28 +192,36 @@ int rtm_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, co
         const KStep *d = at<KStep>(P, o_bwd) + 2 * k0;
         const hipStream_t gs = group_stream(static_cast<int>(g));
         if (two_cig) {
-            // the first launch bins step nt-1 itself (it keeps the forward's velocities), k_cig
-            // then every step after its launch: step it from the slots after a one-step
-            // launch, steps it (middle level) and it-1 after a two-step launch (it, it-1),
-            // both in one k_cig2 launch
+            // k_cig bins every step after its launch: step it from the slots after a one-step
+            // launch, steps it (middle level) and it-1 after a two-step launch (it, it-1), with
+            // delt 1 both in one k_cig2 launch.  With delt 1 the first launch bins step nt-1
+            // itself (it keeps the forward's velocities; with delt > 1 that step is not binned).
             const KCig *k1 = at<KCig>(P, o_kcig) + k0, *ksw = k1 + nshots, *kmid = k1 + 2 * nshots;
-            const int thetan = cig->thetan;
-            HIP_TRY(launch_fstep(P.kg, d, 2 * n, nt - 2, true, kFstepDefault, true, true, false, gs));
-            HIP_TRY(launch_cig(P.kg, k1, n, thetan, 1, w1, w2, cthr, nt - 2, gs));
-            launches += 2;
-            std::vector<int> its2(its.begin() + 1, its.end());
+            const int thetan = cig->thetan, delt = cig->delt;
+            std::vector<int> its2 = its;
+            if (delt == 1) {
+                HIP_TRY(launch_fstep(P.kg, d, 2 * n, nt - 2, true, kFstepDefault, true, true, false, gs));
+                HIP_TRY(launch_cig(P.kg, k1, n, thetan, 1, w1, w2, cthr, nt - 2, gs));
+                launches += 2;
+                its2.erase(its2.begin());
+            }
             StepHook hook;
             hook.mid = true;
-            hook.launches2 = 1;   // one k_cig2 for both levels
-            hook.run = [&P, k1, n, thetan, w1, w2, cthr](int it, hipStream_t st) -> int {
-                HIP_TRY(launch_cig(P.kg, k1, n, thetan, 1, w1, w2, cthr, it, st));
+            hook.launches2 = delt == 1 ? 1 : 2;
+            hook.run = [&P, k1, n, thetan, delt, w1, w2, cthr](int it, hipStream_t st) -> int {
+                HIP_TRY(launch_cig(P.kg, k1, n, thetan, delt, w1, w2, cthr, it, st));
                 return SWD_OK;
             };
-            hook.run2 = [&P, k1, ksw, kmid, n, thetan, w1, w2, cthr](int it, int ph, hipStream_t st) -> int {
-                HIP_TRY(launch_cig2(P.kg, kmid, ph ? k1 : ksw, n, thetan, w1, w2, cthr, it, it - 1, st));
+            hook.run2 = [&P, k1, ksw, kmid, n, thetan, delt, w1, w2, cthr](int it, int ph, hipStream_t st) -> int {
+                if (delt == 1) {
+                    HIP_TRY(launch_cig2(P.kg, kmid, ph ? k1 : ksw, n, thetan, w1, w2, cthr, it, it - 1, st));
+                } else {
+                    HIP_TRY(launch_cig(P.kg, kmid, n, thetan, delt, w1, w2, cthr, it, st));
+                    HIP_TRY(launch_cig(P.kg, ph ? k1 : ksw, n, thetan, delt, w1, w2, cthr, it - 1, st));
+                }
                 return SWD_OK;
             };
-            hook.key = {4, reinterpret_cast<long long>(k1), thetan};
+            hook.key = {4, reinterpret_cast<long long>(k1), thetan, delt};
             if (!its2.empty())
                 SWD_TRY(fused_steps(P, d, 2 * n, its2, nt - 2, P.tuned_bwd, &launches, true, gs, &hook, false, false,
                                     true));

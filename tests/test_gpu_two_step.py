@@ -250,18 +250,21 @@ def test_two_step_born_adjoint_checkpoints(nsnap):
     assert e32 <= TOL
 
 
+@pytest.mark.parametrize("delt", [1, 3])
 @pytest.mark.parametrize("nt", [161, 162, 163, 164])
-def test_two_step_angle_gathers(nt):
-    """Angle gathers (delt 1) with the backward two steps per launch: the
-    paired kernel's middle-level variant stores both fields' first level and
-    k_cig bins the two steps of each launch (the first launch bins step nt-1
-    itself).  Gathers, image and illuminations equal one step per launch bit
-    for bit, for every remainder of the step count modulo 4, two shots (one
-    source on the interior frame) on two streams, and the oracle's."""
+def test_two_step_angle_gathers(nt, delt):
+    """Angle gathers with the backward two steps per launch: the paired
+    kernel's middle-level variant stores both fields' first level and k_cig
+    bins the two steps of each launch (delt 1: one k_cig2 launch, the first
+    launch binning step nt-1 itself; delt 3: time-averaged over a ring).
+    Gathers, image and illuminations equal one step per launch bit for bit,
+    for every remainder of the step count modulo 4, two shots (one source on
+    the interior frame) on two streams, and the oracle's."""
     c = configs.small(nx=150, nz=110, pml=10, Lc=4, nt=nt, nshots=2)
     s_ix, s_iz, rx, rz = c.shots[1]
     c.shots[1] = (c.pmlc + 1, c.pmlc + 40, rx, rz)
-    print(f"case two-step angle gathers nt={nt}", flush=True)
+    thetan = 10 if delt == 1 else 30
+    print(f"case two-step angle gathers nt={nt} delt={delt}", flush=True)
     outs = [oracle_problem(c, k).forward(strips=True, final=True) for k in range(2)]
     adj = np.concatenate([O.adjoint_source(o[0], c.grid.dt) for o in outs])
     res = {}
@@ -269,7 +272,7 @@ def test_two_step_angle_gathers(nt):
         plan = plan_of(c, steps)
         plan.set_groups(1, 1)
         plan.set_streams(2)
-        res[steps] = plan.rtm_cig(c.shots, c.wavelet, adj, 10, 1)
+        res[steps] = plan.rtm_cig(c.shots, c.wavelet, adj, thetan, delt)
         assert plan.stats()["steps_per_launch"] == steps
         plan.close()
     for name, a, b in zip(("image", "illum_src", "illum_rcv", "cig"), res[1], res[2]):
@@ -280,7 +283,7 @@ def test_two_step_angle_gathers(nt):
     for k in range(2):
         d, st, fs = outs[k]
         n = d.shape[0]
-        o = oracle_problem(c, k).rtm_cig(fs, st, adj[row:row + n], 10, 1)
+        o = oracle_problem(c, k).rtm_cig(fs, st, adj[row:row + n], thetan, delt)
         assert np.array_equal(res[2][3][k], o[3]), f"shot {k} cig"
         assert np.array_equal(res[2][0][k], o[0]), f"shot {k} image"
         row += n
