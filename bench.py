@@ -807,6 +807,7 @@ def bench_multi(a):
     log(f"bench: c4 multi-grid forward, {len(cases)} grids, {sum(len(c.shots) for c in cases)} shots, "
         f"nt {a.nt_multi}")
     plans = [swd.Plan(c.grid, c.coefs, c.vp, c.rho, device=0) for c in cases]
+    plans[0].set_steps_per_launch(a.steps_per_launch)
     sets = [swd.ShotSet(c.shots) for c in cases]
     wavs = [c.wavelet for c in cases]
     cells = sum(c.grid.ntx * c.grid.ntz * c.grid.nt * len(c.shots) for c in cases)
@@ -886,7 +887,9 @@ def bench_multi(a):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
-                     "kernel": "k_fstep_multi (forward step of every grid's shots in one launch)",
+                     "kernel": ("k_fstep2_multi (two forward steps of every grid's shots in one launch, 32 x "
+                                "32-row tiles)" if plans[0].stats()["steps_per_launch"] == 2 else
+                                "k_fstep_multi (forward step of every grid's shots in one launch)"),
                      "bytes_per_launch": bytes_per_step, "avg_launch_ms": launch_ms,
                      "timing": "HIP events on the first plan's stream around the time loop / nt launches"},
         "cpu_baseline": cpu,

@@ -182,7 +182,8 @@ int swd_plan_set_tile(void *plan, int tile_x);
  * With 2 and an asked tile configuration, its shape runs two steps per launch
  * where the two-step kernel has it (32 x 32, 16 x 64, 64 x 32, 32 x 64 rows),
  * else 32 x 32.  Two steps per launch need Lc 4; they cover the forward
- * loops (with the Born adjoint's background passes) and the paired backward
+ * loops (with the Born adjoint's background passes and swd_multi_forward,
+ * whose tile is then 32 x 32 rows) and the paired backward
  * of the RTM, the up/down RTM and the angle-gather RTM; the Born forward pair
  * keeps one step per launch.  Results do not depend on it. */
 int swd_plan_set_steps_per_launch(void *plan, int steps);

@@ -884,6 +884,27 @@ __global__ __launch_bounds__(64 * NW, SEL == 1 ? (PAIR ? SWD_F2_WAVES_FAST : SWD
     else f2_field<TX, TQ, NW, SEL == 1, KSTEP_RCV, MID>(g, Rv, X0, Z0, it, ph, sP, sVX, sVZ, dps);
 }
 
+// Multi-grid form of the forward (each instance on its own grid, KStep::grid):
+// btab[b] = (instance, tile column, tile row, fast) for 32 x 32-row tiles, four
+// wavefronts, the fast flag computed on the host from the instance's grid.
+__global__ __launch_bounds__(256, SWD_F2_WAVES_FWD) void k_fstep2_multi(const KStep *__restrict__ insts,
+                                                                         const int4 *__restrict__ btab, int nblk,
+                                                                         int it, int ph)
+{
+    constexpr int TX = 32, TQ = 8, NW = 4;
+    using G = T2<TX, TQ, NW>;
+    __shared__ float4 sP[G::PNC * G::PNQ];
+    __shared__ float4 sVX[G::XNC * G::XNQ];
+    __shared__ float4 sVZ[G::ZNC * G::ZNQ];
+    if (static_cast<int>(blockIdx.x) >= nblk) return;
+    const int4 e = btab[blockIdx.x];
+    const KStep &I = insts[e.x];
+    const KGrid &g = *I.grid;
+    float4 dps[2] = {f2zero(), f2zero()};
+    if (e.w) f2_field<TX, TQ, NW, true, KSTEP_FWD>(g, I, e.y * TX, e.z * 4 * TQ, it, ph, sP, sVX, sVZ, dps);
+    else f2_field<TX, TQ, NW, false, KSTEP_FWD>(g, I, e.y * TX, e.z * 4 * TQ, it, ph, sP, sVX, sVZ, dps);
+}
+
 // The fast rectangle of a grid for TX x 4TQ tiles (empty: y = w = 0): a tile
 // is fast when its staged region (columns X0-14 .. X0+TX+13, rows Z0-16 ..
 // Z0+4TQ+15) lies inside the damping-free box.

@@ -76,3 +76,12 @@ hipError_t launch_fstep2(const KGrid &g, const KStep *insts, int ninst, int it, 
 #undef L2K
     return hipGetLastError();
 }
+
+int4 fstep2_fast_rect(const KGrid &g, int tx, int tq, int nxb, int nzb) { return f2_fast_rect(g, tx, tq, nxb, nzb); }
+
+hipError_t launch_fstep2_multi(const KStep *insts, const int4 *btab, int nblk, int it, int ph, hipStream_t s)
+{
+    if (ph != 0 && ph != 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fstep2_multi, dim3(nblk), dim3(256), 0, s, insts, btab, nblk, it, ph);
+    return hipGetLastError();
+}

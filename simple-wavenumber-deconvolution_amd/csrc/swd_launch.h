@@ -75,6 +75,12 @@ hipError_t launch_fstep(const KGrid &g, const KStep *insts, int ninst, int it, b
                         bool born, hipStream_t s);
 // multi-grid forward step: btab[nblk] = (instance, tile numbered for cfg's tiles)
 hipError_t launch_fstep_multi(int Lc, int cfg, const KStep *insts, const int2 *btab, int nblk, int it, hipStream_t s);
+// Two steps of the multi-grid forward (Lc 4, every grid's column stride below 2^23):
+// btab[nblk] = (instance, tile column, tile row, fast) of 32 x 32-row tiles, the
+// fast flag from fstep2_fast_rect (the rectangle of tile columns x .. x+y-1 and
+// tile rows z .. z+w-1 whose staged regions avoid damping and frame cells).
+int4 fstep2_fast_rect(const KGrid &g, int tx, int tq, int nxb, int nzb);
+hipError_t launch_fstep2_multi(const KStep *insts, const int4 *btab, int nblk, int it, int ph, hipStream_t s);
 hipError_t launch_kzsep(const float2 *capture, int ncap, int nfftz, const float2 *tw, const float *coef,
                         const KCapCol *cols, const int2 *traces, float *up, float *down, int nt, hipStream_t s);
 // up/down imaging of step it for nshot shots (KUpDown each) over the interior columns

@@ -125,11 +125,36 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     if (!tune) cand.assign(1, cfg);
     std::vector<BTab> tabs;
     for (int c : cand) tabs.push_back(make_btab(c));
+    // Two steps per launch (Lc 4, as fused_steps decides: not with one step or an asked one-step
+    // tile): each group's leftover steps modulo 4 on the one-step kernel, the rest in pairs of
+    // k_fstep2_multi launches over 32 x 32-row tiles; its table per group, per segment, the
+    // grid's tiles z-fastest with the shots of a tile next to each other and the fast flag.
+    bool two = P0.g.Lc == 4 && P0.steps_per_launch != 1 && (P0.tile_cfg == 0 || P0.steps_per_launch == 2);
+    for (int i = 0; i < nplans; i++) two = two && Ps[i]->ldz < (1 << 23);
+    std::vector<int4> t2;
+    std::vector<size_t> off2;
+    std::vector<int> cnt2;
+    if (two)
+        for (const auto &G : groups) {
+            off2.push_back(t2.size());
+            for (const Seg &sg : G) {
+                const Plan &P = *Ps[sg.plan];
+                const int nzb = (P.ntz + 31) / 32, nxb = (P.ntx + 31) / 32;
+                const int4 fr = fstep2_fast_rect(P.kg, 32, 8, nxb, nzb);
+                for (int t = 0; t < nzb * nxb; t++) {
+                    const int xb = t / nzb, zb = t % nzb;
+                    const int fast = xb >= fr.x && xb < fr.x + fr.y && zb >= fr.z && zb < fr.z + fr.w;
+                    for (int q = sg.k0; q < sg.k1; q++) t2.push_back(make_int4(inst0[sg.plan] + q, xb, zb, fast));
+                }
+            }
+            cnt2.push_back(static_cast<int>(t2.size() - off2.back()));
+        }
     Carver ca;
     const size_t oa_grid = ca.add<KGrid>(nplans);
     const size_t oa_inst = ca.add<KStep>(h.size());
     std::vector<size_t> oa_btab;
     for (const BTab &B : tabs) oa_btab.push_back(ca.add<int2>(B.t.size()));
+    const size_t oa_btab2 = ca.add<int4>(std::max<size_t>(1, t2.size()));
     SWD_TRY(ensure_aux(P0, ca.off));
     const KGrid *d_grids = reinterpret_cast<const KGrid *>(P0.aux + oa_grid);
     for (int i = 0; i < nplans; i++) {
@@ -158,6 +183,8 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     for (size_t c = 0; c < tabs.size(); c++)
         HIP_TRY(hipMemcpyAsync(P0.aux + oa_btab[c], tabs[c].t.data(), sizeof(int2) * tabs[c].t.size(),
                                hipMemcpyHostToDevice, P0.stream));
+    if (two)
+        HIP_TRY(hipMemcpyAsync(P0.aux + oa_btab2, t2.data(), sizeof(int4) * t2.size(), hipMemcpyHostToDevice, P0.stream));
     const KStep *d_inst = reinterpret_cast<const KStep *>(P0.aux + oa_inst);
     stats_begin(P0, 0);
     SWD_TRY(fork_streams(P0));
@@ -189,22 +216,30 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     while (cand[ci] != cfg) ci++;
     const BTab &B = tabs[ci];
     const int2 *d_btab = reinterpret_cast<const int2 *>(P0.aux + oa_btab[ci]);
-    P0.stats.tile_x = fstep_cfg_tx(cfg);
-    P0.stats.tile_rows = 4 * fstep_cfg_tq(cfg);
-    P0.stats.tile_waves = fstep_cfg_wb(cfg);
-    P0.stats.steps_per_launch = 1;
-    P0.stats.tile_group_waves = fstep_threads_of(cfg) / 64;
+    const int4 *d_btab2 = reinterpret_cast<const int4 *>(P0.aux + oa_btab2);
+    P0.stats.tile_x = two ? 32 : fstep_cfg_tx(cfg);
+    P0.stats.tile_rows = two ? 32 : 4 * fstep_cfg_tq(cfg);
+    P0.stats.tile_waves = two ? 0 : fstep_cfg_wb(cfg);
+    P0.stats.steps_per_launch = two ? 2 : 1;
+    P0.stats.tile_group_waves = two ? 4 : fstep_threads_of(cfg) / 64;
+    long long nlaunch = 0;
     for (size_t gi = 0; gi < groups.size(); gi++) {
         const hipStream_t st = rtm_group_stream(P0, static_cast<int>(gi));
         const int first = gi == 0 ? it0 : 0;
+        const int r = two ? (nt - first) % 4 : nt - first;   // one-step launches
+        nlaunch += r + (nt - first - r) / 2;
         auto record = [&]() -> int {
-            for (int it = first; it < nt; it++)
+            int it = first;
+            for (; it < first + r; it++)
                 HIP_TRY(launch_fstep_multi(P0.g.Lc, cfg, d_inst, d_btab + B.off[gi], B.cnt[gi], it, st));
+            for (; it < nt; it += 4)
+                for (int ph = 0; ph < 2; ph++)
+                    HIP_TRY(launch_fstep2_multi(d_inst, d_btab2 + off2[gi], cnt2[gi], it + 2 * ph, ph, st));
             return SWD_OK;
         };
         if (P0.graphs) {
             std::vector<long long> key = {2, reinterpret_cast<long long>(P0.aux), static_cast<long long>(h.size()),
-                                          static_cast<long long>(B.off[gi]), B.cnt[gi], nt, first, cfg};
+                                          static_cast<long long>(B.off[gi]), B.cnt[gi], nt, first, cfg, two ? 1 : 0};
             for (int i = 0; i < nplans; i++) key.push_back(reinterpret_cast<long long>(Ps[i]->ws));
             SWD_TRY(graph_run(P0, key, record, st));
         } else {
@@ -213,7 +248,7 @@ extern "C" int swd_multi_forward(int nplans, void *const *plans, const int *nsho
     }
     SWD_TRY(join_streams(P0));
     P0.stats.fwd_ms = stats_end(P0, 0);
-    P0.stats.fwd_launches = static_cast<long long>(nt) * static_cast<long long>(groups.size());
+    P0.stats.fwd_launches = nlaunch + it0;
     long long cells = 0;
     for (int i = 0; i < nplans; i++) cells += static_cast<long long>(Ps[i]->ntx) * Ps[i]->ntz * nt * nshots[i];
     P0.stats.fwd_cells = cells;

@@ -39,11 +39,18 @@ def test_multi_forward_c4_matches_single_plans_and_oracle(tmp_path, monkeypatch)
     got = swd.multi_forward(plans, [c.shots for c in cases], [c.wavelet for c in cases])
     t_multi = time.perf_counter() - t0
     st = plans[0].stats()
-    print(f"c4 tuned tile {st['tile_x']} x {st['tile_rows']} rows, bound {st['tile_waves']}", flush=True)
-    assert st["tile_x"] == 16 and st["tile_rows"] in (32, 64, 128) and st["tile_waves"] in (0, 5)
+    print(f"c4 tile {st['tile_x']} x {st['tile_rows']} rows, bound {st['tile_waves']}, "
+          f"{st['steps_per_launch']} steps per launch", flush=True)
     rec = [ln for ln in cache.read_text().splitlines() if " multi " in ln]
     print(rec, flush=True)
-    assert len(rec) == 1 and rec[0].endswith(str(16 | (st["tile_rows"] // 4) << 8 | st["tile_waves"] << 16))
+    assert len(rec) == 1
+    picked = int(rec[0].split()[-1])
+    assert picked & 0xff == 16 and 4 * ((picked >> 8) & 0xff) in (32, 64, 128) and (picked >> 16) & 0xff in (0, 5)
+    if st["steps_per_launch"] == 2:
+        # two steps per launch on 32 x 32-row tiles, the tuned one-step tile for the leftover steps
+        assert (st["tile_x"], st["tile_rows"]) == (32, 32)
+    else:
+        assert picked == st["tile_x"] | (st["tile_rows"] // 4) << 8 | st["tile_waves"] << 16
     t0 = time.perf_counter()
     ref = [p.forward(c.shots, c.wavelet) for p, c in zip(plans, cases)]
     t_single = time.perf_counter() - t0
@@ -123,3 +130,24 @@ def test_rtm_graph_replay_matches_eager():
         pg.forward(c.shots, c.wavelet)   # other captured loops in between
     pe.close()
     pg.close()
+
+
+@pytest.mark.parametrize("nt", [121, 123])
+def test_multi_forward_two_steps_per_launch(nt):
+    """The multi-grid forward two steps per launch (k_fstep2_multi over 32 x
+    32-row tiles, the leftover steps on the one-step kernel) gives the one-step
+    multi-grid launch's gathers bit for bit, on two streams."""
+    cases = configs.c4(nt=nt)
+    print(f"case c4 nt {nt}: two steps per launch against one", flush=True)
+    res = {}
+    for steps in (1, 2):
+        plans = plans_of(cases)
+        plans[0].set_tile(0)
+        plans[0].set_steps_per_launch(steps)
+        res[steps] = swd.multi_forward(plans, [c.shots for c in cases], [c.wavelet for c in cases])
+        assert plans[0].stats()["steps_per_launch"] == steps
+        for p in plans:
+            p.close()
+    for k, (a, b) in enumerate(zip(res[1], res[2])):
+        assert np.array_equal(a, b), k
+
