@@ -27,8 +27,8 @@ __global__ __launch_bounds__(256) void k_snap(float4 *__restrict__ snap, float4 
 // wavefield as two instances of the same launches.
 // Born forward on the fused path: the background (2k) and scattered (2k+1)
 // fields of shot k in one workgroup per tile, the Born source from the
-// background's divergence sums in registers; shots in cache-sized groups on
-// the plan's streams.
+// background's divergence sums in registers (two steps per launch where the
+// plan allows); shots in cache-sized groups on the plan's streams.
 int born_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, const float *wavelet, const float *m,
                    float *d)
 {
@@ -73,7 +73,7 @@ int born_dev_fused(Plan &P, int nshots, const swd_shot *shots, const Batch &b, c
     for (size_t i = 0; i + 1 < bounds.size(); i++)
         SWD_TRY(fused_steps(P, at<KStep>(P, o_inst) + 2 * bounds[i], 2 * (bounds[i + 1] - bounds[i]), steps_up(nt), -1,
                             P.tuned_born, &launches, true, rtm_group_stream(P, static_cast<int>(i)), nullptr, false,
-                            true));
+                            true, true));
     SWD_TRY(join_streams(P));
     P.stats.fwd_ms = stats_end(P, 0);
     P.stats.fwd_launches = launches;

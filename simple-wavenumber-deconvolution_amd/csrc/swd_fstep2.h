@@ -296,11 +296,14 @@ __device__ __forceinline__ float4 f2_vz_quad(const KGrid &g, int x, int z, const
 // tiles).  Returns false where the reconstruction leaves the quad unchanged.
 // srcnear: the source may lie in the quad (a workgroup-uniform test of the
 // tile's region; false spares the per-quad comparison).
-template <bool FULL, int MODE, int XNQ>
+// BRN 2 (the Born pair's scattered field): sbv holds the background's sums S
+// of the same quad and level; the Born source m S is added to the own sum on
+// interior cells, in the one-step pair kernel's operation order.
+template <bool FULL, int MODE, int XNQ, int BRN = 0>
 __device__ __forceinline__ bool f2_p_quad(const KGrid &g, const KStep &I, int x, int z, int lv, const float4 *vxa,
                                           const float (&vzw)[12], float4 pold, float4 K, float4 K0, float4 qxo,
                                           float4 qzo, float4 &pv, float4 &dp, float4 &qxn, float4 &qzn, float &axo,
-                                          bool &anyz, bool srcnear)
+                                          bool &anyz, bool srcnear, float4 sbv = make_float4(0.f, 0.f, 0.f, 0.f))
 {
     constexpr int LC = 4;
     pv = pold;
@@ -366,9 +369,12 @@ __device__ __forceinline__ bool f2_p_quad(const KGrid &g, const KStep &I, int x,
                 float dvz = 0.f;
 #pragma unroll
                 for (int ic = 0; ic < LC; ic++) dvz = __builtin_fmaf(g.rc[ic] * (vzw[4 + k + ic] - vzw[4 + k - ic - 1]), idz, dvz);
-                const float S = elc(dvx, k) + dvz;
-                el(pv, k) = __builtin_fmaf(elc(K, k), S, elc(pold, k));
+                float S = elc(dvx, k) + dvz;
                 if (MODE == KSTEP_FWD) el(dp, k) = S;
+                if constexpr (BRN == 2)   // the tile is interior
+                    S = __builtin_fmaf(ldg(I.born_m, static_cast<unsigned>((x - g.pmlc) * g.nz + (z + k - g.pmlc))),
+                                       elc(sbv, k), S);
+                el(pv, k) = __builtin_fmaf(elc(K, k), S, elc(pold, k));
             }
         } else {
             const float ax = ldg(g.ax, x), bx = ldg(g.bx, x);
@@ -387,10 +393,14 @@ __device__ __forceinline__ bool f2_p_quad(const KGrid &g, const KStep &I, int x,
                 const float qz = az != 0.f ? __builtin_fmaf(bz, elc(qzo, k), az * dvz) : 0.f;
                 el(qxn, k) = (in && ax != 0.f) ? qx : elc(qxo, k);
                 el(qzn, k) = (in && az != 0.f) ? qz : elc(qzo, k);
-                const float S = ((elc(dvx, k) + qx) + dvz) + qz;
+                float S = ((elc(dvx, k) + qx) + dvz) + qz;
+                if (MODE == KSTEP_FWD) el(dp, k) = S;
+                if constexpr (BRN == 2)
+                    if (x >= g.pmlc && x < g.ntx - g.pmlc && zk >= g.pmlc && zk < g.ntz - g.pmlc)
+                        S = __builtin_fmaf(ldg(I.born_m, static_cast<unsigned>((x - g.pmlc) * g.nz + (zk - g.pmlc))),
+                                           elc(sbv, k), S);
                 const float np = __builtin_fmaf(elc(K, k), S, elc(pold, k));
                 el(pv, k) = in ? np : elc(pold, k);
-                if (MODE == KSTEP_FWD) el(dp, k) = S;
             }
         }
     }
@@ -463,10 +473,13 @@ __device__ __forceinline__ void f2_image(const KGrid &g, int z, float4 dp, float
 
 // Two steps of one wavefield on one tile.  dps: dp/dt of the own quad at the
 // two levels (SRC writes them, RCV images with them).
-template <int TX, int TQ, int NW, bool FULL, int MODE, bool MID = false>
+// BRN: 0, or the Born pair's background (1: its sums of the first level's P1
+// items and the second level's own quad kept in sb) and scattered field (2:
+// adds m S from sb); the same thread handles an item in both fields.
+template <int TX, int TQ, int NW, bool FULL, int MODE, bool MID = false, int BRN = 0>
 __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0, int Z0, int it, int ph,
                                          float4 *__restrict__ sP, float4 *__restrict__ sVX, float4 *__restrict__ sVZ,
-                                         float4 (&dps)[2])
+                                         float4 (&dps)[2], float4 *sb = nullptr)
 {
     using G = T2<TX, TQ, NW>;
     constexpr int NT = G::NT, PNQ = G::PNQ, XNQ = G::XNQ, ZNQ = G::ZNQ;
@@ -658,8 +671,10 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
             if (x < g.fx0 || x > g.fx1) qxo = ld4(PVXr, off(x, z));
             if (z < g.fz0 || z + 3 > g.fz1) qzo = ld4(PVZr, off(x, z));
         }
-        const bool wr = f2_p_quad<FULL, MODE, XNQ>(g, I, x, z, lv1, &sVX[G::sx(c, q)], w, pold, kv[r], k0, qxo, qzo, pv,
-                                                   dp, qxn, qzn, axo, anyz, srcnear);
+        const bool wr = f2_p_quad<FULL, MODE, XNQ, BRN>(g, I, x, z, lv1, &sVX[G::sx(c, q)], w, pold, kv[r], k0, qxo,
+                                                        qzo, pv, dp, qxn, qzn, axo, anyz, srcnear,
+                                                        BRN == 2 ? sb[r] : f2zero());
+        if constexpr (BRN == 1) sb[r] = dp;
         if (i < G::PCORE) {   // the own quad (core = item tid)
             if (SRC) {
                 dps[0] = dp;
@@ -756,8 +771,10 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
         float4 pv, dp = f2zero(), qxn, qzn;
         float axo;
         bool anyz;
-        const bool wr = f2_p_quad<FULL, MODE, XNQ>(g, I, x, z, lv2, &sVX[G::sx(oc, oq)], w, p1, kv[0], k0, qx1, qz1,
-                                                   pv, dp, qxn, qzn, axo, anyz, srcnear);
+        const bool wr = f2_p_quad<FULL, MODE, XNQ, BRN>(g, I, x, z, lv2, &sVX[G::sx(oc, oq)], w, p1, kv[0], k0, qx1,
+                                                        qz1, pv, dp, qxn, qzn, axo, anyz, srcnear,
+                                                        BRN == 2 ? sb[G::RP] : f2zero());
+        if constexpr (BRN == 1) sb[G::RP] = dp;
         if (SRC) {
             dps[1] = dp;
             if (I.dpdt_b && wr && dpint(x, z)) st4((gfl *)I.dpdt_b, off(x, z), dp);
@@ -803,7 +820,7 @@ __device__ __forceinline__ void f2_field(const KGrid &g, const KStep &I, int X0,
 #ifndef SWD_F2_WAVES_FRAME
 #define SWD_F2_WAVES_FRAME 4   // the paired kernel over every tile: spills a few dwords, 6% faster than three
 #endif
-template <int TX, int TQ, int NW, bool PAIR, int SEL, bool MID = false>
+template <int TX, int TQ, int NW, bool PAIR, int SEL, bool MID = false, bool BORN = false>
 __global__ __launch_bounds__(64 * NW, SEL == 1 ? (PAIR ? SWD_F2_WAVES_FAST : SWD_F2_WAVES_FWD) : (PAIR ? SWD_F2_WAVES_FRAME : SWD_F2_WAVES_FWD)) void k_fstep2(
     KGrid g, const KStep *__restrict__ insts, int ninst, int nzb, int nxb, int4 fr, int it, int ph)
 {
@@ -868,6 +885,18 @@ __global__ __launch_bounds__(64 * NW, SEL == 1 ? (PAIR ? SWD_F2_WAVES_FAST : SWD
     if (!PAIR) {
         if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_FWD>(g, insts[inst], X0, Z0, it, ph, sP, sVX, sVZ, dps);
         else f2_field<TX, TQ, NW, SEL == 1, KSTEP_FWD>(g, insts[inst], X0, Z0, it, ph, sP, sVX, sVZ, dps);
+        return;
+    }
+    if constexpr (BORN) {
+        // Born pair: the background (2k), then the scattered field (2k+1) with the background's
+        // sums of both levels in registers
+        const KStep &B = insts[2 * inst], &U = insts[2 * inst + 1];
+        float4 sb[T2<TX, TQ, NW>::RP + 1];
+        if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_FWD, false, 1>(g, B, X0, Z0, it, ph, sP, sVX, sVZ, dps, sb);
+        else f2_field<TX, TQ, NW, SEL == 1, KSTEP_FWD, false, 1>(g, B, X0, Z0, it, ph, sP, sVX, sVZ, dps, sb);
+        __syncthreads();   // LDS is staged again for the scattered field
+        if (full) f2_field<TX, TQ, NW, SEL != 2, KSTEP_FWD, false, 2>(g, U, X0, Z0, it, ph, sP, sVX, sVZ, dps, sb);
+        else f2_field<TX, TQ, NW, SEL == 1, KSTEP_FWD, false, 2>(g, U, X0, Z0, it, ph, sP, sVX, sVZ, dps, sb);
         return;
     }
     const KStep &S = insts[2 * inst], &Rv = insts[2 * inst + 1];

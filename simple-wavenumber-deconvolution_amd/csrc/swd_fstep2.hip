@@ -25,19 +25,23 @@ int fstep2_cfg_of_shape(int tx, int tq)
 // One launch over every tile, or (split) the fast tiles then the frame tiles:
 // both read slot sa and write the other, so their order does not matter.
 hipError_t launch_fstep2(const KGrid &g, const KStep *insts, int ninst, int it, int ph, int cfg, bool pair, hipStream_t s,
-                         bool mid)
+                         bool mid, bool born)
 {
     const int tx = fstep_cfg_tx(cfg), tq = fstep_cfg_tq(cfg), nw = fstep_cfg_wb(cfg);
     if (g.Lc != 4 || !fstep2_cfg_supported(cfg) || (ph != 0 && ph != 1) || g.ldz >= (1 << 23))
         return hipErrorInvalidValue;
-    if (mid) {
-        // the middle-level variant: the paired kernel in the default configuration only
-        if (!pair || cfg != kFstep2Default) return hipErrorInvalidValue;
+    if (mid || born) {
+        // the middle-level and Born variants: the paired kernel in the default configuration only
+        if (!pair || cfg != kFstep2Default || (mid && born)) return hipErrorInvalidValue;
         const int nzb = (g.ntz + 4 * 8 - 1) / (4 * 8), nxb = (g.ntx + 32 - 1) / 32;
         const int4 fr = f2_fast_rect(g, 32, 8, nxb, nzb);
         const int nb = (ninst / 2 * nzb * nxb + 7) / 8 * 8;
-        hipLaunchKernelGGL((k_fstep2<32, 8, 4, true, 0, true>), dim3(nb), dim3(64 * 4), 0, s, g, insts, ninst / 2, nzb, nxb,
-                           fr, it, ph);
+        if (mid)
+            hipLaunchKernelGGL((k_fstep2<32, 8, 4, true, 0, true>), dim3(nb), dim3(64 * 4), 0, s, g, insts, ninst / 2, nzb,
+                               nxb, fr, it, ph);
+        else
+            hipLaunchKernelGGL((k_fstep2<32, 8, 4, true, 0, false, true>), dim3(nb), dim3(64 * 4), 0, s, g, insts,
+                               ninst / 2, nzb, nxb, fr, it, ph);
         return hipGetLastError();
     }
     const bool split = (cfg & kFstepSplit) != 0;

@@ -50,13 +50,15 @@ std::vector<int> fstep2_candidates();
 int fstep2_cfg_of_shape(int tx, int tq);
 // Two steps (it, it+1 forward; it, it-1 for the paired backward) of every
 // instance in one launch, Lc 4, with the column capture, dp/dt and first-level
-// outputs but no angle binning or Born; ph: the launch's place in its pair
-// (launches come in pairs, see swd_fstep2.h).  Not for the first
-// reconstruction step.
+// outputs but no angle binning; ph: the launch's place in its pair (launches
+// come in pairs, see swd_fstep2.h).  Not for the first reconstruction step.
 // mid: the paired kernel's middle-level variant (kFstep2Default only), which also
 // stores both fields' first level (pmid, vxmid, vzmid) for the angle gathers.
+// born: the Born pair (instances 2k and 2k+1: the background and the scattered
+// field of shot k, FWD; kFstep2Default only), the background's sums kept in
+// registers for the scattered field's Born source.
 hipError_t launch_fstep2(const KGrid &g, const KStep *insts, int ninst, int it, int ph, int cfg, bool pair, hipStream_t s,
-                         bool mid = false);
+                         bool mid = false, bool born = false);
 // The configurations a fused launch supports: 64-row tiles 16, 32 or 64 wide
 // for every variant and Lc; at Lc 4, for the forward and the paired RTM
 // backward (not the angle-gather or Born variants), also 32- and 128-row tiles

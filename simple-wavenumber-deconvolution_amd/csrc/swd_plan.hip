@@ -296,9 +296,9 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     if (!st) st = P.stream;
     // two steps per launch: the forward goes up in time, the paired backward down, one step at a time
     // (the two-step kernel forms tile-relative offsets with a 24-bit multiply: column strides below 2^23)
-    two = two && (!hook || hook->run2) && !cig && !born && P.g.Lc == 4 && P.steps_per_launch != 1 &&
-          P.ldz < (1 << 23);
-    const int dir = pair ? -1 : 1;
+    two = two && (!hook || hook->run2) && !cig && P.g.Lc == 4 && P.steps_per_launch != 1 && P.ldz < (1 << 23) &&
+          (!born || pair);
+    const int dir = pair && !born ? -1 : 1;   // the Born pair is two forward fields
     for (size_t j = 1; two && j < its.size(); j++)
         if (its[j] != its[j - 1] + dir) two = false;
     // steps = 2 asked: an asked tile runs two steps per launch in its shape (or the nearest)
@@ -318,7 +318,8 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
         else if (!fstep_cfg_supported(P.g.Lc, pair, cig, born, false, cfg))
             cfg = fstep_cfg(tx, 16, 0);   // the asked width with the variant's 64-row tiles
     }
-    if (two && hook && hook->mid) cfg = kFstep2Default;   // the only shape of the middle-level variant
+    // the middle-level and Born variants have one shape (an asked one-step tile keeps one step)
+    if (two && ((hook && hook->mid) || born) && (cfg == 0 || fstep_cfg_two(cfg))) cfg = kFstep2Default;
     const int tkey = 4 * n + variant;
     std::string key;
     if (cfg == 0) {
@@ -353,7 +354,7 @@ int fused_steps(Plan &P, const KStep *d, int n, const std::vector<int> &its, int
     auto quad = [&](int c) -> int {
         for (int ph = 0; ph < 2; ph++) {
             const int it = its[k + 2 * ph];
-            HIP_TRY(launch_fstep2(P.kg, d, n, it, ph, c, pair, st, hook && hook->mid));
+            HIP_TRY(launch_fstep2(P.kg, d, n, it, ph, c, pair, st, hook && hook->mid, born));
             nl++;
             if (hook) {
                 SWD_TRY(hook->run2(it, ph, st));

@@ -215,6 +215,37 @@ def test_two_step_updown(shape, nt):
         assert np.array_equal(res[1][name], res[2][name]), name
 
 
+@pytest.mark.parametrize("nt", [260, 261, 262, 263])
+def test_two_step_born_forward(nt):
+    """The fused Born pair two steps per launch (the background's two levels,
+    then the scattered field's two in the same workgroup, the Born source of
+    each level carried in registers): the gathers equal the one-step pairs'
+    bit for bit at every nt mod 4, in one-shot groups on two streams, and the
+    oracle's within 1e-5, with receivers sharing nodes."""
+    c = configs.small(nshots=3, dup=True, nt=nt, Lc=4)
+    print(f"case two-step born forward nt={nt}", flush=True)
+    rng = np.random.default_rng(90 + nt)
+    nint = c.grid.nx * c.grid.nz
+    m = rng.standard_normal(3 * nint).astype(np.float32)
+    res = {}
+    for steps in (1, 2):
+        plan = plan_of(c, steps)
+        plan.set_groups(2, 0)
+        plan.set_streams(2)
+        res[steps] = plan.born(c.shots, c.wavelet, m)
+        assert plan.stats()["steps_per_launch"] == steps
+        plan.close()
+    assert np.isfinite(res[2]).all()
+    assert np.array_equal(res[1], res[2])
+    row = 0
+    for k in range(3):
+        od = oracle_problem(c, k).born(m[k * nint:(k + 1) * nint])
+        n = od.shape[0]
+        e32, _ = report(f"two-step born nt={nt} shot {k}", res[2][row:row + n], od)
+        assert e32 <= TOL
+        row += n
+
+
 @pytest.mark.parametrize("nsnap", [7, 9, 65, 87])
 def test_two_step_born_adjoint_checkpoints(nsnap):
     """The Born adjoint's background passes on the fused forward (two steps per
