@@ -700,7 +700,7 @@ def bench_born(a):
         log(f"bench: warmup {i + 1}/{a.warmup} done")
     torch.cuda.synchronize()
     t_f = t_a = f_ms = 0.0
-    f_launches = 0
+    f_launches = f_spl = 0
     for i in range(a.steps):
         t0 = time.perf_counter()
         fwd()
@@ -709,6 +709,7 @@ def bench_born(a):
         st = plan.stats()
         f_ms += st["fwd_ms"]
         f_launches = st["fwd_launches"]
+        f_spl = st["steps_per_launch"]
         adj()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
@@ -727,7 +728,8 @@ def bench_born(a):
     launch_ms = f_ms / a.steps / max(1, f_launches)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "born_c5_traffic.json")
+    # the counter figures are per shot-step of the one-step pair, or of the two-step pair
+    tfile = os.path.join(ROOT, "profiles", "born2_c5_traffic.json" if f_spl == 2 else "born_c5_traffic.json")
     if os.path.exists(tfile) and (g.nx, g.nz, g.pml, g.Lc) == (1690, 1001, 60, 4):
         with open(tfile) as f:
             traffic = json.load(f)["l2_miss_bytes_per_shot_step"] * nshots / launches_per_step
@@ -781,8 +783,12 @@ def bench_born(a):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
-                     "kernel": "k_fstep Born pair (fused Born forward step: background and scattered field of "
-                               "each shot of a group in one workgroup per tile)",
+                     "kernel": ("k_fstep2 Born pair (fused Born forward, two steps per launch: the background's "
+                                "two levels, then the scattered field's, of each shot of a group in one workgroup "
+                                "per tile)" if f_spl == 2 else
+                                "k_fstep Born pair (fused Born forward step: background and scattered field of "
+                                "each shot of a group in one workgroup per tile)"),
+                     "steps_per_launch": f_spl,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": launch_ms,
                      "launches_per_step": launches_per_step,
                      "timing": "HIP events on the plan's stream around the Born forward loop (joining the other "
