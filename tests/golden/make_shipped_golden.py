@@ -1,0 +1,47 @@
+"""Writes shipped_rtm.npz: the reference's shipped true model and the RTM image
+it shipped for it, for the end-to-end comparison in test_gpu_shipped_image.py.
+
+The reference ships its RTM driver's parameter file (parameter.txt: 845 x 702
+cells of 10 m, pml 16, Lc 5, 7000 steps of 1 ms, 20 Hz, 210 surface shots
+every 40 m from 50 m, two borehole receiver lines to 5000 m depth, smoothing
+window 5), the true model it names (input/acc_vp.bin, float32 [nx][nz]) and,
+as the deconvolution's input, the migration image of the observed data
+(kkdecon_acoustic/input/migration_pp.bin, float32 [nx][nz], written at
+RC:779 after the edge zeroing of RC:754-768 with Lc 5), both kept exact.  The parameter values are read with
+the library's own parser of that file.
+
+Run from the repository root in a container that has the reference at
+/root/reference:  python3 tests/golden/make_shipped_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "simple-wavenumber-deconvolution_amd"))
+import swd  # noqa: E402
+
+REF = "/root/reference"
+
+
+def main():
+    par = os.path.join(REF, "parameter.txt")
+    vp_f = os.path.join(REF, "input", "acc_vp.bin")
+    mig_f = os.path.join(REF, "kkdecon_acoustic", "input", "migration_pp.bin")
+    p = swd.read_parameters(par)
+    nx, nz = p["nx"], p["nz"]
+    vp = np.fromfile(vp_f, np.float32).reshape(nx, nz)
+    mig = np.fromfile(mig_f, np.float32).reshape(nx, nz)
+    meta = {"params": p,
+            "md5": {os.path.relpath(f, REF): hashlib.md5(open(f, "rb").read()).hexdigest()
+                    for f in (par, vp_f, mig_f)}}
+    np.savez_compressed(os.path.join(HERE, "shipped_rtm.npz"), acc_vp=vp, migration_pp=mig,
+                        meta=np.array(json.dumps(meta)))
+    print(json.dumps(meta, indent=1))
+
+
+if __name__ == "__main__":
+    main()

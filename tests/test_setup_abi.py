@@ -183,3 +183,19 @@ def test_postprocessing_matches_oracle():
     rho = np.full(ntx * ntz, 1000.0, np.float32)
     np.testing.assert_array_equal(swd.scatterers(rho, vp, ntx, ntz, pmlc, 11),
                                   O.scatterers(rho, vp, ntx, ntz, pmlc, 11))
+
+
+def test_edge_zero_matches_the_shipped_image():
+    """The reference's shipped migration image (kkdecon_acoustic/input/
+    migration_pp.bin, 845 x 702, Lc 5) is zero exactly on the edges that
+    swd_edge_zero clears (RC:754-768: ix <= 3, ix >= nx - Lc, iz < 7,
+    iz >= nz - Lc), and nowhere inside them."""
+    import json
+    import os
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "shipped_rtm.npz"))
+    p = json.loads(str(z["meta"]))["params"]
+    nx, nz, Lc = p["nx"], p["nz"], p["Lc"]
+    ref = z["migration_pp"]
+    assert ref.shape == (nx, nz) and ref.dtype == np.float32
+    cleared = swd.edge_zero(np.ones(nx * nz, np.float32), nx, nz, Lc).reshape(nx, nz) == 0
+    np.testing.assert_array_equal(ref == 0, cleared)
