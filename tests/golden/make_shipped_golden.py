@@ -7,7 +7,8 @@ every 40 m from 50 m, two borehole receiver lines to 5000 m depth, smoothing
 window 5), the true model it names (input/acc_vp.bin, float32 [nx][nz]) and,
 as the deconvolution's input, the migration image of the observed data
 (kkdecon_acoustic/input/migration_pp.bin, float32 [nx][nz], written at
-RC:779 after the edge zeroing of RC:754-768 with Lc 5), both kept exact.  The parameter values are read with
+RC:779 after the edge zeroing of RC:754-768 with Lc 5) and the image of the
+point-scatterer run (migration_psf.bin, RC:777), all kept exact.  The parameter values are read with
 the library's own parser of that file.
 
 Run from the repository root in a container that has the reference at
@@ -31,14 +32,16 @@ def main():
     par = os.path.join(REF, "parameter.txt")
     vp_f = os.path.join(REF, "input", "acc_vp.bin")
     mig_f = os.path.join(REF, "kkdecon_acoustic", "input", "migration_pp.bin")
+    psf_f = os.path.join(REF, "kkdecon_acoustic", "input", "migration_psf.bin")
     p = swd.read_parameters(par)
     nx, nz = p["nx"], p["nz"]
     vp = np.fromfile(vp_f, np.float32).reshape(nx, nz)
     mig = np.fromfile(mig_f, np.float32).reshape(nx, nz)
+    psf = np.fromfile(psf_f, np.float32).reshape(nx, nz)
     meta = {"params": p,
             "md5": {os.path.relpath(f, REF): hashlib.md5(open(f, "rb").read()).hexdigest()
-                    for f in (par, vp_f, mig_f)}}
-    np.savez_compressed(os.path.join(HERE, "shipped_rtm.npz"), acc_vp=vp, migration_pp=mig,
+                    for f in (par, vp_f, mig_f, psf_f)}}
+    np.savez_compressed(os.path.join(HERE, "shipped_rtm.npz"), acc_vp=vp, migration_pp=mig, migration_psf=psf,
                         meta=np.array(json.dumps(meta)))
     print(json.dumps(meta, indent=1))
 

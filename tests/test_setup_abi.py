@@ -195,7 +195,33 @@ def test_edge_zero_matches_the_shipped_image():
     z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "shipped_rtm.npz"))
     p = json.loads(str(z["meta"]))["params"]
     nx, nz, Lc = p["nx"], p["nz"], p["Lc"]
-    ref = z["migration_pp"]
-    assert ref.shape == (nx, nz) and ref.dtype == np.float32
     cleared = swd.edge_zero(np.ones(nx * nz, np.float32), nx, nz, Lc).reshape(nx, nz) == 0
-    np.testing.assert_array_equal(ref == 0, cleared)
+    for kind in ("pp", "psf"):
+        ref = z["migration_" + kind]
+        assert ref.shape == (nx, nz) and ref.dtype == np.float32
+        np.testing.assert_array_equal(ref == 0, cleared)
+
+
+def test_shipped_psf_lattice_period_is_the_scatterer_window():
+    """The shipped PSF image (migration_psf.bin) repeats with period 31 along x
+    and z, the lattice swd_scatterers lays with swin 31 (FC:697-705); the
+    shipped parameter file's value on that line is 51."""
+    import json
+    import os
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "shipped_rtm.npz"))
+    p = json.loads(str(z["meta"]))["params"]
+    a = z["migration_psf"].astype(np.float64)
+    a -= a.mean()
+    ac = np.fft.irfft2(np.abs(np.fft.rfft2(a)) ** 2, a.shape)
+    lag_x = 2 + int(np.argmax(ac[2:80, 0]))
+    lag_z = 2 + int(np.argmax(ac[0, 2:80]))
+    assert (lag_x, lag_z) == (31, 31) and p["swin"] == 51
+    pml, Lc = p["pml"], p["Lc"]
+    pmlc = pml + Lc
+    ntx, ntz = p["nx"] + 2 * pmlc, p["nz"] + 2 * pmlc
+    rho = np.full(ntx * ntz, 1000.0, np.float32)
+    vp = np.full(ntx * ntz, 2000.0, np.float32)
+    r = swd.scatterers(rho, vp, ntx, ntz, pmlc, 31).reshape(ntx, ntz)
+    xs, zs = np.nonzero(r != 1000.0)
+    assert xs.size > 0
+    assert set(np.diff(np.unique(xs))) == {31} and set(np.diff(np.unique(zs))) == {31}
