@@ -91,8 +91,8 @@ hipError_t launch_updown(const KGrid &g, const KUpDown *shots, int nshot, int nf
 // shot share its images; the interior rows at most nfftz)
 hipError_t launch_updown2(const KGrid &g, const KUpDown *a, const KUpDown *b, int nshot, int nfftz, int ita, int itb,
                           hipStream_t s);
-// the kernel launches launch_updown2 makes (two from 1024 points)
-constexpr int updown2_launches(int nfftz) { return nfftz >= 1024 ? 2 : 1; }
+// the kernel launches launch_updown2 makes (two from 2048 points)
+constexpr int updown2_launches(int nfftz) { return nfftz >= 2048 ? 2 : 1; }
 // angle-gather update of step it for nshot shots (KCig each) over the interior cells;
 // w1, w2: the bin weights exp(-1/18), exp(-4/18) rounded to float; thr: the
 // cosine below which no bin is in reach (cig_bins)

@@ -166,10 +166,12 @@ hipError_t launch_updown2(const KGrid &g, const KUpDown *a, const KUpDown *b, in
                           hipStream_t s)
 {
     if (g.nz > nfftz) return hipErrorInvalidValue;
-    // From 1024 points (16 values a thread) the two-level kernel holds about
-    // 190 VGPRs, two wavefronts per SIMD: c5 (nfftz 1024), 54.2 us a launch
-    // against 2 x 23.4 us for two one-level launches.  Up to 512 points it
-    // keeps four wavefronts per SIMD and is the faster one.
+    // The two-level kernel takes 8 values a thread at every size: at 1024
+    // points 16 values a thread held about 190 VGPRs (two wavefronts per SIMD;
+    // c5, 54.2 us a launch against 2 x 23.4 us for two one-level launches),
+    // 8 hold 104 (four wavefronts of 256-thread workgroups; c2 up/down RTM over
+    // whole columns at nfftz 1024: backward 1198-1206 ms against 1280-1289 ms
+    // with two one-level launches).  From 2048 points two one-level launches.
     if (updown2_launches(nfftz) == 2) {
         const hipError_t e = launch_updown(g, a, nshot, nfftz, ita, s);
         return e != hipSuccess ? e : launch_updown(g, b, nshot, nfftz, itb, s);
@@ -180,7 +182,7 @@ hipError_t launch_updown2(const KGrid &g, const KUpDown *a, const KUpDown *b, in
         hipLaunchKernelGGL((k_updown2<LOGN, V>), grid, dim3(2 * (1 << LOGN) / V), 0, s, g, a, b, ita, itb);            \
         break;
     switch (nfftz) {
-        UD(7, 8) UD(8, 8) UD(9, 8)
+        UD(7, 8) UD(8, 8) UD(9, 8) UD(10, 8)
     default:
         return hipErrorInvalidValue;
     }
