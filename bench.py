@@ -71,8 +71,8 @@ def build_case(nshots, nt):
 
 def host_cpu():
     """(threads to use, description of the host's CPUs): the threads OMP_NUM_THREADS
-    grants (the CPU share a GPU box gives one GPU), else every CPU this process
-    may run on."""
+    grants (a host shared between GPUs may give each GPU's process a share of its
+    CPUs), else every CPU this process may run on."""
     avail = len(os.sched_getaffinity(0))
     try:
         threads = min(avail, int(os.environ.get("OMP_NUM_THREADS", "")))
